@@ -1,0 +1,235 @@
+"""Benchmark: GRE's superstep on one MI355X.
+
+Headline (the `value`): FP32 PageRank, 20 iterations per step, on a directed
+R-MAT graph (scale 26, edge factor 16, seed 6) resident in HBM as CSR + CSC;
+GTEPS = stored edges x iterations / time.  The same JSON line carries:
+  roofline      the pull tile kernel's algorithmic bytes / its mean launch time
+                (HIP events around each launch, on the library stream);
+  cpu_baseline  GRE's scatter/combine/apply superstep restated in C with 12
+                worker threads and 8192 striped locks, timed on this host on a
+                bounded sample (a few PageRank iterations of the same graph);
+  bfs / msbfs   16-source BFS on the symmetrised scale-26 graph (seed 6):
+                plain direction-optimising BFS per source vs one 64-bit-mask
+                multi-source launch (Graph500 GTEPS);
+  ppr           32 personalised vectors in one batched launch sequence and the
+                adjoint on the same batch, vs the plain single-vector path.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS + % HBM roofline, R-MAT s26; batched/adjoint vs plain path, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--bfs-sources", type=int, default=16)
+    ap.add_argument("--ppr-k", type=int, default=32)
+    ap.add_argument("--no-extra", action="store_true", help="headline PageRank only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    return ap.parse_args()
+
+
+def pmc_traffic(scale):
+    """HBM bytes per pull-tile launch from the committed rocprofv3 --pmc summary, if present."""
+    f = os.path.join(ROOT, "profiles", f"pull_tile_pagerank_f32_s{scale}_pmc.txt")
+    if not os.path.exists(f):
+        return None
+    vals = {}
+    for line in open(f):
+        if "=" in line and line.strip().startswith("traffic_bytes_per_launch"):
+            vals["t"] = float(line.split("=")[1])
+    return vals.get("t")
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} (WORLD_SIZE={world}) is not supported "
+              "in this build: the superstep is single-device and no collective path exists.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from graphine_amd import native as N
+    N.init(0)
+    S, iters = a.scale, a.iters
+
+    # ---------------- headline: FP32 PageRank on directed R-MAT --------------
+    t0 = time.time()
+    g = N.Graph.rmat(S, 16, 6, False)
+    log(f"directed R-MAT scale {S}: V={g.nV} E={g.nE} built in {time.time() - t0:.1f}s")
+    pr = N.PageRank(g, N.F32)
+    pr.run(0.85, 2)                          # autotunes the tile size, warms up
+    for _ in range(a.warmup):
+        pr.run(0.85, iters)
+    N.sync()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        pr.run(0.85, iters)
+    N.sync()
+    dt = time.perf_counter() - t1
+    ms_step = 1e3 * dt / a.steps
+    gteps = g.nE * iters * a.steps / dt / 1e9
+    log(f"pagerank f32: {ms_step:.2f} ms/step, {gteps:.2f} GTEPS")
+
+    # per-kernel launch times (HIP events on the library stream), separate pass
+    N.prof_reset()
+    N.prof_enable(True)
+    pr.run(0.85, iters)
+    N.sync()
+    n_tile, ms_tile = N.prof_get("pull_tile")
+    n_fix, ms_fix = N.prof_get("pull_fixup")
+    N.prof_enable(False)
+    V, E = g.nV, g.nE
+    o, s_ = 8, 4
+    bytes_iter = o * (V + 1) + (4 + s_) * E + (3 * s_ + 4) * V
+    avg_tile_s = ms_tile / max(n_tile, 1) / 1e3
+    achieved = bytes_iter / avg_tile_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(S),
+                "kernel": "k_pull_tile (PageRank f32 iteration)",
+                "alg_bytes_per_launch": bytes_iter, "avg_launch_ms": round(avg_tile_s * 1e3, 4),
+                "fixup_avg_ms": round(ms_fix / max(n_fix, 1), 4)}
+    log(f"pull_tile avg {avg_tile_s * 1e3:.3f} ms -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
+
+    # ---------------- CPU baseline (bounded sample, same graph) ---------------
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+        d = g.download(csc=False)
+        og = type("G", (), {})()
+        og.nV, og.off, og.idx, og.outdeg = g.nV, d["off"], d["idx"], d["outdeg"]
+        nthreads = 12
+        _, secs = oracle.gre_pagerank_mt(og, nthreads=nthreads, iters=a.cpu_iters)
+        cg = E * a.cpu_iters / secs / 1e9
+        cpu = {"value": round(cg, 4), "unit": "GTEPS", "cores": nthreads, "kind": "port",
+               "sample": f"{a.cpu_iters} PageRank iterations of the same scale-{S} graph, GRE-style push "
+                         f"superstep (12 workers, 8192 striped locks), {secs:.1f}s"}
+        log(f"cpu baseline: {cg:.3f} GTEPS in {secs:.1f}s")
+        del d
+
+    extra = {}
+    if not a.no_extra:
+        # ---------------- batched PPR + adjoint vs plain --------------------
+        k = a.ppr_k
+        rng = np.random.default_rng(6)
+        P = np.zeros((V, k), np.float32)
+        for j in range(k):
+            P[rng.choice(V, 100, replace=False), j] = 1.0 / 100
+        tg = rng.choice(V, 1024, replace=False).astype(np.uint32)
+        bp = N.PageRank(g, N.F32, k, weighted=True)
+        bp.set_edge_weights(None, seed=6)
+        bp.set_personalization(P)
+        del P
+        bp.run(0.85, 2)
+        N.sync()
+        t1 = time.perf_counter()
+        bp.run(0.85, iters)
+        N.sync()
+        t_bf = time.perf_counter() - t1
+        bp.adjoint(tg, 0.85, 2)
+        N.sync()
+        t1 = time.perf_counter()
+        bp.adjoint(tg, 0.85, iters)
+        N.sync()
+        t_ba = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        bp.gradients(0.85, want_p=False)
+        t_bg = time.perf_counter() - t1
+        del bp
+        sp = N.PageRank(g, N.F32, 1, weighted=True)
+        sp.set_edge_weights(None, seed=6)
+        p1 = np.zeros(V, np.float32)
+        p1[rng.choice(V, 100, replace=False)] = 1.0 / 100
+        sp.set_personalization(p1)
+        sp.run(0.85, 2)
+        N.sync()
+        t1 = time.perf_counter()
+        sp.run(0.85, iters)
+        N.sync()
+        t_sf = time.perf_counter() - t1
+        sp.adjoint(tg, 0.85, 2)
+        t1 = time.perf_counter()
+        sp.adjoint(tg, 0.85, iters)
+        N.sync()
+        t_sa = time.perf_counter() - t1
+        del sp
+        vi_b = k * iters / t_bf
+        vi_s = iters / t_sf
+        extra["ppr"] = {
+            "k": k, "iters": iters, "weights": "lognormal(0,1) row-normalised, seed 6",
+            "batched_forward_ms": round(1e3 * t_bf, 2), "batched_adjoint_ms": round(1e3 * t_ba, 2),
+            "batched_edge_grad_ms": round(1e3 * t_bg, 2),
+            "plain_forward_ms": round(1e3 * t_sf, 2), "plain_adjoint_ms": round(1e3 * t_sa, 2),
+            "batched_vector_iters_per_s": round(vi_b, 1), "plain_vector_iters_per_s": round(vi_s, 1),
+            "batched_vs_plain": round(vi_b / vi_s, 2),
+            "adjoint_vs_forward_batched": round(t_bf / t_ba, 3),
+            "batched_gteps_equiv": round(E * k * iters / t_bf / 1e9, 2),
+        }
+        log("ppr:", extra["ppr"])
+        del g, pr
+        # ---------------- BFS on the symmetrised graph ----------------------
+        t0 = time.time()
+        gs = N.Graph.rmat(S, 16, 6, True)
+        log(f"symmetrised R-MAT scale {S}: V={gs.nV} E={gs.nE} built in {time.time() - t0:.1f}s")
+        lev, _, _ = N.bfs(gs, int(np.argmax(gs.download(csc=False)["outdeg"])))
+        reached = np.nonzero(lev >= 0)[0]
+        srcs = rng.choice(reached, a.bfs_sources, replace=False).astype(np.uint32)
+        N.bfs(gs, int(srcs[0]), want_levels=False)
+        rates = []
+        for s in srcs:
+            N.sync()
+            t1 = time.perf_counter()
+            _, er, _ = N.bfs(gs, int(s), want_levels=False)
+            N.sync()
+            rates.append((er / 2) / (time.perf_counter() - t1) / 1e9)
+        hm = len(rates) / sum(1.0 / r for r in rates)
+        extra["bfs"] = {"sources": len(srcs), "graph": f"R-MAT scale {S} symmetrised, seed 6",
+                        "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
+                        "gteps_max": round(max(rates), 3)}
+        N.sync()
+        t1 = time.perf_counter()
+        _, ems = N.msbfs(gs, srcs[:64], want_levels=False)
+        N.sync()
+        t_ms = time.perf_counter() - t1
+        extra["msbfs"] = {"sources": int(min(64, len(srcs))), "gteps": round((ems / 2) / t_ms / 1e9, 3),
+                          "ms": round(1e3 * t_ms, 2),
+                          "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
+        log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
+
+    out = {
+        "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"PageRank FP32, {iters} iterations per step, directed R-MAT scale {S}, "
+                               f"edge factor 16, seed 6, CSR+CSC int32 ids in HBM",
+                   "vertices": V, "edges": E, "iterations_per_step": iters, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

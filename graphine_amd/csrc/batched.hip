@@ -1,0 +1,531 @@
+// Batched personalised PageRank (k vectors in one launch sequence), its
+// adjoint and the batched edge gradient.
+//
+// Vectors are stored vertex-major (nV x k), so one neighbour gather fetches
+// k contiguous values.  A group of G = pow2ceil(k) lanes walks one tile of
+// kTileG adjacency entries: lanes load G neighbour ids / weights at a time
+// (coalesced), issue G independent k-wide gathers, then fold them into a
+// per-lane running sum, flushing at row boundaries.  Rows crossing a tile
+// boundary leave partial vectors that a fix-up pass adds in tile order, so
+// results are deterministic.  Per-vector FP64 reductions (dangling mass,
+// p.lambda, L1 residual) are fixed-order block partials.
+#include "pr.hpp"
+
+namespace gre {
+namespace {
+
+constexpr int kTileG = 1024;   // adjacency entries per group tile
+constexpr int kBlk = 256;
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kBlk - 1) / kBlk;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+int pow2ceil(int k) {
+  int g = 1;
+  while (g < k) g <<= 1;
+  return g;
+}
+
+template <typename TT>
+struct BFwd {  // forward: gather x (CSC), finalise ranks
+  const TT* xin; TT* xout; const float* w; const TT* p; const int32_t* outdeg;
+  const double* scal;  // [2k]: dang_j at scal[2j]
+  double damp; int k;
+  __device__ TT gather(uint64_t e, uint32_t u, int j) const { return (TT)w[e] * xin[(uint64_t)u * k + j]; }
+  __device__ void finalize(uint32_t r, int j, TT s, double& a0, double& a1) const {
+    const uint64_t o = (uint64_t)r * k + j;
+    const TT pv = p[o];
+    const TT y = (TT)damp * (s + (TT)scal[2 * j] * pv) + (TT)(1.0 - damp) * pv;
+    const TT old = xin[o];
+    xout[o] = y;
+    if (outdeg[r] == 0) a0 += (double)y;
+    a1 += fabs((double)y - (double)old);
+  }
+};
+
+template <typename TT>
+struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda
+  const TT* lin; TT* lout; const float* w; const TT* c; const TT* p; const int32_t* outdeg; TT* wl;
+  const double* scal;  // [2k]: p.lambda_j at scal[2j]
+  double damp; int k;
+  __device__ TT gather(uint64_t e, uint32_t v, int j) const { return (TT)w[e] * lin[(uint64_t)v * k + j]; }
+  __device__ void finalize(uint32_t u, int j, TT s, double& a0, double& a1) const {
+    const uint64_t o = (uint64_t)u * k + j;
+    wl[o] = s;
+    const TT y = c[o] + (TT)damp * (s + (outdeg[u] == 0 ? (TT)scal[2 * j] : (TT)0));
+    const TT old = lin[o];
+    lout[o] = y;
+    a0 += (double)p[o] * (double)y;
+    a1 += fabs((double)y - (double)old);
+  }
+};
+
+// Group-walk tile kernel.  Group size G (lanes), groups per block kBlk/G.
+template <int G, class M, typename TT>
+__global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                                const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
+                                                uint64_t ntiles, TT* __restrict__ carryP, TT* __restrict__ carryO,
+                                                double* __restrict__ part, uint64_t nblk, M m) {
+  constexpr int NG = kBlk / G;
+  __shared__ double red0[NG][G], red1[NG][G];
+  const int gl = threadIdx.x % G, gi = threadIdx.x / G;
+  const int k = m.k;
+  const bool act = gl < k;
+  const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
+  double a0 = 0.0, a1 = 0.0;
+  if (t < ntiles) {
+    const uint64_t e0 = t * (uint64_t)kTileG;
+    const uint64_t e1 = e0 + kTileG < E ? e0 + kTileG : E;
+    const uint32_t rb = tile_rb[t];
+    const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
+    uint32_t r;
+    if (rb > 0 && e1 > e0 && off[rb] > e0) r = rb - 1; else r = rb;
+    uint64_t rbeg = r < V ? off[r] : E;
+    uint64_t rend = r < V ? off[r + 1] : E;
+    TT acc = (TT)0;
+    for (uint64_t cb = e0; cb < e1; cb += G) {
+      const uint64_t my = cb + gl;
+      const uint32_t myu = my < e1 ? idx[my] : 0u;
+      TT g[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const uint32_t u = __shfl(myu, q, G);
+        g[q] = (act && cb + q < e1) ? m.gather(cb + q, u, gl) : (TT)0;
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const uint64_t e = cb + q;
+        if (e >= e1) break;
+        while (e >= rend) {  // row r ends before e: flush it
+          if (rbeg < e0) { if (act) carryP[t * k + gl] = acc; }
+          else if (act) m.finalize(r, gl, acc, a0, a1);
+          acc = (TT)0;
+          ++r;
+          rbeg = off[r];
+          rend = off[r + 1];
+        }
+        acc += g[q];
+      }
+    }
+    // rows still open at the tile end, plus empty rows that start in the tile
+    while (r < re) {
+      if (rbeg < e0) { if (act) carryP[t * k + gl] = acc; }
+      else if (rend > e1) { if (act) carryO[t * k + gl] = acc; }
+      else if (act) m.finalize(r, gl, acc, a0, a1);
+      acc = (TT)0;
+      ++r;
+      if (r < V) { rbeg = off[r]; rend = off[r + 1]; }
+    }
+  }
+  red0[gi][gl] = a0;
+  red1[gi][gl] = a1;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)k) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = 0; q < NG; ++q) { s0 += red0[q][threadIdx.x]; s1 += red1[q][threadIdx.x]; }
+    part[(uint64_t)(2 * threadIdx.x) * nblk + blockIdx.x] = s0;
+    part[(uint64_t)(2 * threadIdx.x + 1) * nblk + blockIdx.x] = s1;
+  }
+}
+
+template <int G, class M, typename TT>
+__global__ __launch_bounds__(kBlk) void k_bfix(const uint64_t* __restrict__ off, const uint32_t* __restrict__ tile_rb,
+                                               uint64_t E, uint32_t V, uint64_t ntiles,
+                                               const TT* __restrict__ carryP, const TT* __restrict__ carryO,
+                                               double* __restrict__ part, uint64_t nblk, M m) {
+  constexpr int NG = kBlk / G;
+  __shared__ double red0[NG][G], red1[NG][G];
+  const int gl = threadIdx.x % G, gi = threadIdx.x / G;
+  const int k = m.k;
+  const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
+  double a0 = 0.0, a1 = 0.0;
+  if (t + 1 < ntiles && gl < k) {
+    const uint32_t rb = tile_rb[t], re = tile_rb[t + 1];
+    const uint64_t e1 = (t + 1) * (uint64_t)kTileG;
+    if (re > rb && off[re] > e1) {
+      const uint32_t r = re - 1;
+      const uint64_t end = off[r + 1];
+      TT s = carryO[t * k + gl];
+      for (uint64_t q = t + 1; q * kTileG < end; ++q) s += carryP[q * k + gl];
+      m.finalize(r, gl, s, a0, a1);
+    }
+  }
+  red0[gi][gl] = a0;
+  red1[gi][gl] = a1;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)k) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = 0; q < NG; ++q) { s0 += red0[q][threadIdx.x]; s1 += red1[q][threadIdx.x]; }
+    part[(uint64_t)(2 * threadIdx.x) * nblk + blockIdx.x] = s0;
+    part[(uint64_t)(2 * threadIdx.x + 1) * nblk + blockIdx.x] = s1;
+  }
+}
+
+// out[o] = sum of a[o*na + i] + b[o*nb + i] (one block per output, fixed order)
+__global__ void k_reduce_rows(const double* __restrict__ a, uint64_t na, const double* __restrict__ b, uint64_t nb,
+                              double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int o = blockIdx.x;
+  double s = 0.0;
+  for (uint64_t i = threadIdx.x; i < na; i += blockDim.x) s += a[(uint64_t)o * na + i];
+  for (uint64_t i = threadIdx.x; i < nb; i += blockDim.x) s += b[(uint64_t)o * nb + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[o] = sh[0];
+}
+
+// init: x = p (forward) or lambda = c (adjoint); a0_j = sum over dangling of x_j
+// (forward) or p_j . c_j (adjoint).  part is [2k][nblk].
+template <typename TT>
+__global__ void k_binit(uint32_t nV, int k, const TT* __restrict__ src, const TT* __restrict__ pw,
+                        const int32_t* __restrict__ outdeg, TT* __restrict__ dst, int adjoint,
+                        double* __restrict__ part, uint64_t nblk) {
+  __shared__ double sh[kBlk];
+  // thread layout: lanes over j (k <= 64), rows strided
+  const int G = 64;
+  const int j = threadIdx.x % G, rl = threadIdx.x / G;
+  double a0 = 0.0;
+  if (j < k) {
+    for (uint64_t v = (uint64_t)blockIdx.x * (kBlk / G) + rl; v < nV; v += (uint64_t)gridDim.x * (kBlk / G)) {
+      const uint64_t o = v * k + j;
+      const TT s = src[o];
+      dst[o] = s;
+      if (adjoint) a0 += (double)pw[o] * (double)s;
+      else if (outdeg[v] == 0) a0 += (double)s;
+    }
+  }
+  sh[threadIdx.x] = a0;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)k) {
+    double s = 0.0;
+    for (int q = 0; q < kBlk / G; ++q) s += sh[q * G + threadIdx.x];
+    part[(uint64_t)(2 * threadIdx.x) * nblk + blockIdx.x] = s;
+    part[(uint64_t)(2 * threadIdx.x + 1) * nblk + blockIdx.x] = 0.0;
+  }
+}
+
+template <typename TT>
+__global__ void k_btargets(const uint32_t* __restrict__ tg, int64_t nt, int k, TT* __restrict__ c) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nt * k; i += (int64_t)gridDim.x * blockDim.x)
+    c[(uint64_t)tg[i / k] * k + (i % k)] = (TT)1;
+}
+
+// batched edge gradient: one group per tile of CSR entries
+template <int G, typename TT>
+__global__ __launch_bounds__(kBlk) void k_bgrad(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                                const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
+                                                uint64_t ntiles, int k, const double* __restrict__ rowsum,
+                                                const TT* __restrict__ x, const TT* __restrict__ lam,
+                                                const TT* __restrict__ wl, double damp, float* __restrict__ gW,
+                                                float* __restrict__ graw) {
+  constexpr int NG = kBlk / G;
+  const int gl = threadIdx.x % G, gi = threadIdx.x / G;
+  const bool act = gl < k;
+  const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
+  if (t >= ntiles) return;
+  const uint64_t e0 = t * (uint64_t)kTileG;
+  const uint64_t e1 = e0 + kTileG < E ? e0 + kTileG : E;
+  if (e1 <= e0) return;
+  const uint32_t rb = tile_rb[t];
+  uint32_t r = (rb > 0 && off[rb] > e0) ? rb - 1 : rb;
+  uint64_t rend = off[r + 1];
+  while (rend <= e0) { ++r; rend = off[r + 1]; }
+  double xu = act ? (double)x[(uint64_t)r * k + gl] : 0.0;
+  double xwl = act ? xu * (double)wl[(uint64_t)r * k + gl] : 0.0;
+  for (int o = G / 2; o > 0; o >>= 1) xwl += __shfl_xor(xwl, o, G);
+  for (uint64_t cb = e0; cb < e1; cb += G) {
+    const uint64_t my = cb + gl;
+    const uint32_t myv = my < e1 ? idx[my] : 0u;
+    float outW = 0.f, outR = 0.f;
+    for (int q = 0; q < G; ++q) {
+      const uint64_t e = cb + q;
+      if (e >= e1) break;
+      while (e >= rend) {
+        ++r;
+        rend = off[r + 1];
+        xu = act ? (double)x[(uint64_t)r * k + gl] : 0.0;
+        xwl = act ? xu * (double)wl[(uint64_t)r * k + gl] : 0.0;
+        for (int o = G / 2; o > 0; o >>= 1) xwl += __shfl_xor(xwl, o, G);
+      }
+      const uint32_t v = __shfl(myv, q, G);
+      double d = act ? xu * (double)lam[(uint64_t)v * k + gl] : 0.0;
+      for (int o = G / 2; o > 0; o >>= 1) d += __shfl_xor(d, o, G);
+      if (gl == q) {
+        outW = (float)(damp * d);
+        const double rs = rowsum[r];
+        outR = rs > 0.0 ? (float)(damp * (d - xwl) / rs) : 0.f;
+      }
+    }
+    if (my < e1) {
+      gW[my] = outW;
+      if (graw) graw[my] = outR;
+    }
+  }
+}
+
+template <typename TT>
+__global__ void k_bgrad_p(uint32_t nV, int k, const TT* __restrict__ lam, const double* __restrict__ dang, double damp,
+                          TT* __restrict__ gp) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * k;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(i % k);
+    gp[i] = (TT)(((1.0 - damp) + damp * dang[j]) * (double)lam[i]);
+  }
+}
+
+size_t tsz(const gre_pr* P) { return P->dtype == GRE_F64 ? 8 : 4; }
+
+struct BPlan {
+  const uint64_t* off; const uint32_t* idx; const uint32_t* trb;
+  uint64_t E, ntiles; uint32_t V;
+  unsigned nblk, nfix;
+};
+
+BPlan make_plan(gre_pr* P, bool csc) {
+  gre_graph* g = P->g;
+  BPlan b;
+  b.off = (csc ? g->coff : g->off).as<uint64_t>();
+  b.idx = (csc ? g->cidx : g->idx).as<uint32_t>();
+  b.trb = graph_tiles(g, csc, kTileG, &b.ntiles);
+  b.E = g->nE;
+  b.V = g->nV;
+  const int G = pow2ceil(P->k);
+  const int NG = kBlk / G;
+  b.nblk = (unsigned)((b.ntiles + NG - 1) / NG);
+  b.nfix = b.nblk;
+  BatchState& bs = P->bstate;
+  const size_t vec = tsz(P) * (size_t)P->k;
+  if (bs.ntiles < b.ntiles) {
+    bs.carryP.alloc(vec * b.ntiles);
+    bs.carryO.alloc(vec * b.ntiles);
+    bs.ntiles = b.ntiles;
+  }
+  const size_t need = sizeof(double) * 2 * (size_t)P->k * b.nblk;
+  if (bs.part.bytes < need) bs.part.alloc(need);
+  if (bs.part2.bytes < need) bs.part2.alloc(need);
+  return b;
+}
+
+void ensure_bscal(gre_pr* P, int iters) {
+  BatchState& bs = P->bstate;
+  if (bs.scal_cap >= iters + 2) return;
+  bs.scal_cap = iters + 2;
+  bs.scal.alloc(sizeof(double) * 2 * P->k * (size_t)bs.scal_cap);
+}
+
+template <int G, class M, typename TT>
+void bpass(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s) {
+  BatchState& bs = P->bstate;
+  {
+    ProfScope ps("batch_walk");
+    hipLaunchKernelGGL((k_bwalk<G, M, TT>), dim3(b.nblk), dim3(kBlk), 0, s, b.off, b.idx, b.trb, b.E, b.V, b.ntiles,
+                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), bs.part.as<double>(), (uint64_t)b.nblk, m);
+  }
+  {
+    ProfScope ps("batch_fixup");
+    hipLaunchKernelGGL((k_bfix<G, M, TT>), dim3(b.nfix), dim3(kBlk), 0, s, b.off, b.trb, b.E, b.V, b.ntiles,
+                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), bs.part2.as<double>(), (uint64_t)b.nfix, m);
+  }
+  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * P->k), dim3(256), 0, s, bs.part.as<double>(), (uint64_t)b.nblk,
+                     bs.part2.as<double>(), (uint64_t)b.nfix, scal_out);
+  GRE_HIP(hipGetLastError());
+}
+
+template <class M, typename TT>
+void bpass_g(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s) {
+  switch (pow2ceil(P->k)) {
+    case 1: bpass<1, M, TT>(b, P, m, scal_out, s); break;
+    case 2: bpass<2, M, TT>(b, P, m, scal_out, s); break;
+    case 4: bpass<4, M, TT>(b, P, m, scal_out, s); break;
+    case 8: bpass<8, M, TT>(b, P, m, scal_out, s); break;
+    case 16: bpass<16, M, TT>(b, P, m, scal_out, s); break;
+    case 32: bpass<32, M, TT>(b, P, m, scal_out, s); break;
+    default: bpass<64, M, TT>(b, P, m, scal_out, s); break;
+  }
+}
+
+// max over vectors of the L1 residual (scal layout: [a0_0, a1_0, a0_1, a1_1, ...])
+double max_resid(const gre_pr* P, const double* dev, hipStream_t s) {
+  std::vector<double> h(2 * P->k);
+  GRE_HIP(hipMemcpyAsync(h.data(), dev, sizeof(double) * 2 * P->k, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  double m = 0.0;
+  for (int j = 0; j < P->k; ++j) m = h[2 * j + 1] > m ? h[2 * j + 1] : m;
+  return m;
+}
+
+template <typename TT>
+double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
+  hipStream_t s = stream();
+  const BPlan b = make_plan(P, true);
+  ensure_bscal(P, iters);
+  BatchState& bs = P->bstate;
+  const int k = P->k;
+  double* sc = bs.scal.as<double>();
+  const unsigned gb = gcap((uint64_t)P->g->nV * 64);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
+  P->cur = 0;
+  hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->p.as<TT>(), P->p.as<TT>(),
+                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), 0, tmp, (uint64_t)gb);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+  GRE_HIP(hipGetLastError());
+  int it;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    BFwd<TT> m;
+    m.xin = P->x[P->cur].as<TT>();
+    m.xout = P->x[P->cur ^ 1].as<TT>();
+    m.w = P->wcsc.as<float>();
+    m.p = P->p.as<TT>();
+    m.outdeg = P->g->outdeg.as<int32_t>();
+    m.scal = sc + 2 * k * it;  // a0_j at index 2j
+    m.damp = damp;
+    m.k = k;
+    bpass_g<BFwd<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s);
+    P->cur ^= 1;
+    if (tol > 0.0) {
+      res = max_resid(P, sc + 2 * k * (it + 1), s);
+      if (res < tol) { ++it; break; }
+    }
+  }
+  if (tol <= 0.0 && iters > 0) res = max_resid(P, sc + 2 * k * iters, s);
+  GRE_HIP(hipStreamSynchronize(s));
+  if (done) *done = it;
+  return res;
+}
+
+template <typename TT>
+double badjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
+  hipStream_t s = stream();
+  const BPlan b = make_plan(P, false);
+  ensure_bscal(P, iters);
+  BatchState& bs = P->bstate;
+  const int k = P->k;
+  double* sc = bs.scal.as<double>();
+  const unsigned gb = gcap((uint64_t)P->g->nV * 64);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
+  P->lcur = 0;
+  hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->c.as<TT>(), P->p.as<TT>(),
+                     P->g->outdeg.as<int32_t>(), P->lam[0].as<TT>(), 1, tmp, (uint64_t)gb);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+  GRE_HIP(hipGetLastError());
+  int it;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    BAdj<TT> m;
+    m.lin = P->lam[P->lcur].as<TT>();
+    m.lout = P->lam[P->lcur ^ 1].as<TT>();
+    m.w = P->wcsr.as<float>();
+    m.c = P->c.as<TT>();
+    m.p = P->p.as<TT>();
+    m.outdeg = P->g->outdeg.as<int32_t>();
+    m.wl = P->wl.as<TT>();
+    m.scal = sc + 2 * k * it;
+    m.damp = damp;
+    m.k = k;
+    bpass_g<BAdj<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s);
+    P->lcur ^= 1;
+    if (tol > 0.0) {
+      res = max_resid(P, sc + 2 * k * (it + 1), s);
+      if (res < tol) { ++it; break; }
+    }
+  }
+  if (tol <= 0.0 && iters > 0) res = max_resid(P, sc + 2 * k * iters, s);
+  GRE_HIP(hipStreamSynchronize(s));
+  if (done) *done = it;
+  return res;
+}
+
+template <typename TT>
+void bgrad(gre_pr* P, double damp, float* gW_h, float* graw_h, void* gp_h) {
+  hipStream_t s = stream();
+  gre_graph* g = P->g;
+  const int k = P->k;
+  const TT* x = P->x[P->cur].as<TT>();
+  const TT* lam = P->lam[P->lcur].as<TT>();
+  const unsigned gb = gcap((uint64_t)g->nV * 64);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
+  double* dang = static_cast<double*>(scratch(4, sizeof(double) * 2 * k));
+  DevBuf tmpv;
+  tmpv.alloc(tsz(P) * (size_t)g->nV * k);
+  hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, g->nV, k, x, x, g->outdeg.as<int32_t>(), tmpv.as<TT>(), 0,
+                     tmp, (uint64_t)gb);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dang);
+  if (gp_h) {
+    // dang_j sits at index 2j: pass a strided view via a compacting copy
+    std::vector<double> h(2 * k), dj(k);
+    GRE_HIP(hipMemcpyAsync(h.data(), dang, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) dj[j] = h[2 * j];
+    GRE_HIP(hipMemcpyAsync(dang, dj.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_bgrad_p<TT>, dim3(gcap((uint64_t)g->nV * k)), dim3(kBlk), 0, s, g->nV, k, lam, dang, damp,
+                       tmpv.as<TT>());
+    GRE_HIP(hipMemcpyAsync(gp_h, tmpv.p, tsz(P) * (size_t)g->nV * k, hipMemcpyDeviceToHost, s));
+  }
+  if ((gW_h || graw_h) && g->nE > 0) {
+    uint64_t nt = 0;
+    const uint32_t* trb = graph_tiles(g, false, kTileG, &nt);
+    DevBuf dgW, dgraw;
+    dgW.alloc(4 * (size_t)g->nE);
+    if (graw_h) dgraw.alloc(4 * (size_t)g->nE);
+    const int G = pow2ceil(k);
+    const unsigned nb = (unsigned)((nt + kBlk / G - 1) / (kBlk / G));
+    {
+      ProfScope ps("batch_edge_grad");
+#define GRE_BGRAD(GG)                                                                                          \
+  hipLaunchKernelGGL((k_bgrad<GG, TT>), dim3(nb), dim3(kBlk), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), \
+                     trb, g->nE, g->nV, nt, k, P->rowsum.as<double>(), x, lam, P->wl.as<TT>(), damp,            \
+                     dgW.as<float>(), graw_h ? dgraw.as<float>() : nullptr)
+      switch (G) {
+        case 1: GRE_BGRAD(1); break;
+        case 2: GRE_BGRAD(2); break;
+        case 4: GRE_BGRAD(4); break;
+        case 8: GRE_BGRAD(8); break;
+        case 16: GRE_BGRAD(16); break;
+        case 32: GRE_BGRAD(32); break;
+        default: GRE_BGRAD(64); break;
+      }
+#undef GRE_BGRAD
+    }
+    GRE_HIP(hipGetLastError());
+    if (gW_h) GRE_HIP(hipMemcpyAsync(gW_h, dgW.p, 4 * g->nE, hipMemcpyDeviceToHost, s));
+    if (graw_h) GRE_HIP(hipMemcpyAsync(graw_h, dgraw.p, 4 * g->nE, hipMemcpyDeviceToHost, s));
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+double batched_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
+  return P->dtype == GRE_F64 ? bforward<double>(P, damp, iters, tol, done) : bforward<float>(P, damp, iters, tol, done);
+}
+
+double batched_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
+  return P->dtype == GRE_F64 ? badjoint<double>(P, damp, iters, tol, done) : badjoint<float>(P, damp, iters, tol, done);
+}
+
+void batched_targets(gre_pr* P, const uint32_t* targets_dev, int64_t nt) {
+  hipStream_t s = stream();
+  if (P->dtype == GRE_F64)
+    hipLaunchKernelGGL(k_btargets<double>, dim3(gcap((uint64_t)nt * P->k)), dim3(kBlk), 0, s, targets_dev, nt, P->k,
+                       P->c.as<double>());
+  else
+    hipLaunchKernelGGL(k_btargets<float>, dim3(gcap((uint64_t)nt * P->k)), dim3(kBlk), 0, s, targets_dev, nt, P->k,
+                       P->c.as<float>());
+  GRE_HIP(hipGetLastError());
+}
+
+void batched_gradients(gre_pr* P, double damp, float* gW, float* graw, void* gp) {
+  if (P->dtype == GRE_F64) bgrad<double>(P, damp, gW, graw, gp);
+  else bgrad<float>(P, damp, gW, graw, gp);
+}
+
+}  // namespace gre

@@ -1,0 +1,772 @@
+// GRE's BSP superstep on the device: SynchronousEngine<VC>::run<dir>()
+// (include/Engine/Engine.hpp:180-406 master, :422-814 workers) at one process.
+//
+// Superstep k, as GRE runs it once the two MPI sync phases are no-ops:
+//   compute  every vertex u in the current frontier queue scatters along its
+//            Out / In / InOut edges; each message is combined into the
+//            destination's combine data, and a true combine() marks the
+//            destination in nextSet (Engine.hpp:592-625, :1385-1389);
+//   halt     assert_to_halt(u) clears u from the currSet bitmap (:617-621);
+//   apply    every vertex marked in nextSet runs apply(); a true result sets it
+//            in currSet (:757-775);
+//   rebuild  the frontier queue is rebuilt from the whole currSet bitmap, which
+//            persists across supersteps (:777-798);
+//   vote     the active count goes back to the host, which stops at zero or at
+//            max_nSupersteps (:386-400).
+// Here: compute = one edge-balanced push kernel (frontier degrees are
+// prefix-summed so every thread gets the same number of edges, and a hub's
+// edges spread over many waves); combine = one atomic per message (atomicAdd,
+// atomicMin, first-writer CAS), which plays the role of GRE's 8192 striped
+// spin-locks; apply + halt + count are one thread-per-vertex kernel that
+// rebuilds the bitmap words with wave ballots; rebuild = block scan + write.
+#include "graph.hpp"
+
+#include <chrono>
+#include <type_traits>
+
+namespace gre {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kVB = 4096;           // vertices per block in apply/count/write
+constexpr int kEB = kB * 8;         // edges per block in the push expand
+
+struct EngCtx {
+  void* S;
+  void* C;
+  void* X;
+  const void* aux;
+  int32_t level;
+  const int32_t* outdeg;
+  const uint32_t* w_csr;            // SSSP weights, CSR order
+};
+
+struct VtxState { int32_t level; int32_t numPaths; };  // BC.hpp:12-17
+
+// ---- vertex programs --------------------------------------------------------
+// init / scatter+combine / apply restate include/Kernels/*.hpp; scatter+combine
+// returns combine()'s result for the message of edge (u -> v).
+template <int P> struct Prog;
+
+template <> struct Prog<GRE_PROG_PAGERANK> {            // PageRank.hpp:18-126
+  static constexpr bool kHalt = false;                  // assert_to_halt -> false
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    double* pr = (double*)c.X; double* ps = (double*)c.S;
+    pr[v] = 0.15;
+    const int32_t d = c.outdeg[v];
+    if (d > 0) { ps[v] = pr[v] / (double)d; return true; }
+    return false;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    atomic_add_f64((double*)c.C + v, ((const double*)c.S)[u]);
+    return true;
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    double* pr = (double*)c.X; double* cm = (double*)c.C;
+    pr[v] = 0.15 + (1.0 - 0.15) * cm[v];
+    cm[v] = 0.0;
+    return false;
+  }
+};
+
+template <> struct Prog<GRE_PROG_PAGERANK_DELTA> {      // PageRank.hpp:129-234
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) { return Prog<GRE_PROG_PAGERANK>::init(c, v); }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t e) {
+    return Prog<GRE_PROG_PAGERANK>::combine(c, u, v, e);
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    double* pr = (double*)c.X; double* cm = (double*)c.C; double* ps = (double*)c.S;
+    const double delta = (1.0 - 0.15) * cm[v];
+    pr[v] += delta;
+    cm[v] = 0.0;
+    const int32_t d = c.outdeg[v];
+    if (delta > 1e-5 && d > 0) { ps[v] = delta / (double)d; return true; }
+    return false;
+  }
+};
+
+template <> struct Prog<GRE_PROG_BFS> {                 // BFS.hpp:18-86
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    uint64_t* pred = (uint64_t*)c.X; uint64_t* ps = (uint64_t*)c.S;
+    pred[v] = v;   // get_vid(lv) == lv at one process
+    ps[v] = pred[v];
+    return true;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    unsigned long long* cm = (unsigned long long*)c.C;
+    if (cm[v] != ~0ULL) return false;
+    const unsigned long long m = ((const unsigned long long*)c.S)[u];
+    return atomicCAS(cm + v, ~0ULL, m) == ~0ULL;
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    uint64_t* pred = (uint64_t*)c.X; uint64_t* cm = (uint64_t*)c.C; uint64_t* ps = (uint64_t*)c.S;
+    if (pred[v] == ~0ULL) { pred[v] = cm[v]; ps[v] = pred[v]; return true; }
+    return false;
+  }
+};
+
+template <> struct Prog<GRE_PROG_CC> {                  // CC.hpp:18-125
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    uint64_t* cc = (uint64_t*)c.X; uint64_t* cs = (uint64_t*)c.S; uint64_t* cm = (uint64_t*)c.C;
+    cc[v] = v; cs[v] = cc[v]; cm[v] = cc[v];
+    return true;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    unsigned long long* cm = (unsigned long long*)c.C;
+    const unsigned long long m = ((const unsigned long long*)c.S)[u];
+    if (!(cm[v] > m)) return false;
+    return m < atomicMin(cm + v, m);
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    uint64_t* cc = (uint64_t*)c.X; uint64_t* cm = (uint64_t*)c.C; uint64_t* cs = (uint64_t*)c.S;
+    if (cm[v] >= cc[v]) return false;
+    cc[v] = cm[v]; cs[v] = cc[v];
+    return true;
+  }
+};
+
+template <> struct Prog<GRE_PROG_SSSP> {                // SSSP.hpp:19-96 (states = C = X)
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    uint32_t* st = (uint32_t*)c.C; uint32_t* last = (uint32_t*)c.S;
+    st[v] = 0; last[v] = st[v];
+    return true;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t epos) {
+    uint32_t* st = (uint32_t*)c.C;
+    const uint32_t d = ((const uint32_t*)c.S)[u] + c.w_csr[epos];
+    if (!(d < st[v])) return false;
+    return d < atomicMin(st + v, d);
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    ((uint32_t*)c.S)[v] = ((uint32_t*)c.C)[v];
+    return true;
+  }
+};
+
+template <> struct Prog<GRE_PROG_BC_FORWARD> {          // BC.hpp:22-95
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    VtxState* s = (VtxState*)c.X; int32_t* np = (int32_t*)c.S;
+    s[v].numPaths = 1; s[v].level = 0; np[v] = s[v].numPaths;
+    return true;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    atomicAdd((int32_t*)c.C + v, ((const int32_t*)c.S)[u]);   // int32, wraps like GRE's int
+    return true;
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    VtxState* s = (VtxState*)c.X; int32_t* sum = (int32_t*)c.C; int32_t* np = (int32_t*)c.S;
+    if (s[v].level < 0) { s[v].level = c.level; s[v].numPaths = sum[v]; np[v] = s[v].numPaths; return true; }
+    return false;
+  }
+};
+
+template <> struct Prog<GRE_PROG_BC_BACKWARD> {         // BC.hpp:97-168
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    const VtxState* bs = (const VtxState*)c.aux; double* pw = (double*)c.S; double* dep = (double*)c.X;
+    if (bs[v].level == c.level) { pw[v] = (1.0 + dep[v]) / (double)bs[v].numPaths; return true; }
+    return false;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    atomic_add_f64((double*)c.C + v, ((const double*)c.S)[u]);
+    return true;
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    const VtxState* bs = (const VtxState*)c.aux; double* sum = (double*)c.C; double* dep = (double*)c.X;
+    if (bs[v].level == c.level - 1) dep[v] = sum[v] * (double)bs[v].numPaths;
+    return false;
+  }
+};
+
+template <> struct Prog<GRE_PROG_COUNTIN> {             // CountIn.hpp:18-106
+  static constexpr bool kHalt = true;
+  __device__ static bool init(const EngCtx& c, uint32_t v) {
+    ((int32_t*)c.X)[v] = 0; ((int32_t*)c.S)[v] = 1;
+    return true;
+  }
+  __device__ static bool combine(const EngCtx& c, uint32_t u, uint32_t v, uint64_t) {
+    atomicAdd((int32_t*)c.C + v, ((const int32_t*)c.S)[u]);
+    return true;
+  }
+  __device__ static bool apply(const EngCtx& c, uint32_t v) {
+    ((int32_t*)c.X)[v] = ((int32_t*)c.C)[v];
+    return false;
+  }
+};
+
+// ---- frontier plumbing ------------------------------------------------------
+struct Adj {
+  const uint64_t* off; const uint32_t* idx;     // CSR
+  const uint64_t* coff; const uint32_t* cidx;   // CSC
+  const uint64_t* ceid;                         // CSC -> CSR position
+  int dir;
+};
+
+__device__ inline uint64_t degree(const Adj& a, uint32_t v) {
+  uint64_t d = 0;
+  if (a.dir != GRE_DIR_IN) d += a.off[v + 1] - a.off[v];
+  if (a.dir != GRE_DIR_OUT) d += a.coff[v + 1] - a.coff[v];
+  return d;
+}
+
+__device__ inline bool bit_test(const uint32_t* b, uint32_t v) { return (b[v >> 5] >> (v & 31)) & 1u; }
+
+template <int P>
+__global__ void k_init_all(EngCtx c, uint32_t nV, uint32_t nvpad, uint32_t* __restrict__ curr) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nvpad;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const bool on = v < nV && Prog<P>::init(c, (uint32_t)v);
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+      curr[v >> 5] = (uint32_t)m;
+      curr[(v >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+  }
+}
+
+// master pre-insert of the sources, then init() on each (Engine.hpp:252-257, :459-471)
+template <int P>
+__global__ void k_init_src(EngCtx c, const uint32_t* __restrict__ src, int64_t n, uint32_t* __restrict__ curr) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int64_t i = 0; i < n; ++i) curr[src[i] >> 5] |= 1u << (src[i] & 31);
+  for (int64_t i = 0; i < n; ++i)
+    if (Prog<P>::init(c, src[i])) curr[src[i] >> 5] |= 1u << (src[i] & 31);
+}
+
+// per block of kVB vertices: number of set bits and their degree sum
+__device__ inline void block_count_write(uint32_t cnt, uint64_t deg, uint32_t* __restrict__ bcnt,
+                                         uint64_t* __restrict__ bdeg) {
+  __shared__ uint32_t sc[kB / 64];
+  __shared__ uint64_t sd[kB / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, 64);
+    deg += __shfl_xor(deg, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { sc[threadIdx.x >> 6] = cnt; sd[threadIdx.x >> 6] = deg; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t c = 0; uint64_t d = 0;
+    for (int w = 0; w < kB / 64; ++w) { c += sc[w]; d += sd[w]; }
+    bcnt[blockIdx.x] = c;
+    bdeg[blockIdx.x] = d;
+  }
+}
+
+__global__ void k_count(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, uint32_t* __restrict__ bcnt,
+                        uint64_t* __restrict__ bdeg) {
+  const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
+  uint32_t cnt = 0;
+  uint64_t deg = 0;
+  for (int i = threadIdx.x; i < kVB; i += kB) {
+    const uint64_t v = v0 + i;
+    if (v < nV && bit_test(curr, (uint32_t)v)) { ++cnt; deg += degree(a, (uint32_t)v); }
+  }
+  block_count_write(cnt, deg, bcnt, bdeg);
+}
+
+// apply + halt + count: new curr bit = (kHalt ? 0 : curr bit) | (next bit && apply(v))
+template <int P>
+__global__ void k_apply(EngCtx c, const uint32_t* __restrict__ next, uint32_t* __restrict__ curr, uint32_t nV, Adj a,
+                        uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg) {
+  const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
+  uint32_t cnt = 0;
+  uint64_t deg = 0;
+  for (int i = threadIdx.x; i < kVB; i += kB) {
+    const uint64_t v = v0 + i;
+    bool on = false;
+    if (v < nV) {
+      on = Prog<P>::kHalt ? false : bit_test(curr, (uint32_t)v);
+      if (bit_test(next, (uint32_t)v) && Prog<P>::apply(c, (uint32_t)v)) on = true;
+    }
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+      curr[v >> 5] = (uint32_t)m;
+      curr[(v >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+    if (on) { ++cnt; deg += degree(a, (uint32_t)v); }
+  }
+  block_count_write(cnt, deg, bcnt, bdeg);
+}
+
+// exclusive scan of block (count, degree) pairs; totals at [nb]
+__global__ void k_scan_blocks(uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg, uint32_t nb) {
+  __shared__ uint32_t sc[1024];
+  __shared__ uint64_t sd[1024];
+  uint32_t carry_c = 0;
+  uint64_t carry_d = 0;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t c = i < nb ? bcnt[i] : 0;
+    const uint64_t d = i < nb ? bdeg[i] : 0;
+    sc[threadIdx.x] = c;
+    sd[threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      uint32_t tc = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+      uint64_t td = threadIdx.x >= (unsigned)o ? sd[threadIdx.x - o] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += tc;
+      sd[threadIdx.x] += td;
+      __syncthreads();
+    }
+    if (i < nb) { bcnt[i] = carry_c + sc[threadIdx.x] - c; bdeg[i] = carry_d + sd[threadIdx.x] - d; }
+    carry_c += sc[1023];
+    carry_d += sd[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { bcnt[nb] = carry_c; bdeg[nb] = carry_d; }
+}
+
+// queue[pos] = v, qpos[pos] = edge prefix; qpos[total] = total edges
+__global__ void k_write(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, const uint32_t* __restrict__ bcnt,
+                        const uint64_t* __restrict__ bdeg, uint32_t nb, uint32_t* __restrict__ queue,
+                        uint64_t* __restrict__ qpos) {
+  __shared__ uint32_t wc[kB / 64];
+  __shared__ uint64_t wd[kB / 64];
+  const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
+  uint32_t base_c = bcnt[blockIdx.x];
+  uint64_t base_d = bdeg[blockIdx.x];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i0 = 0; i0 < kVB; i0 += kB) {
+    const uint64_t v = v0 + i0 + threadIdx.x;
+    const bool on = v < nV && bit_test(curr, (uint32_t)v);
+    const uint64_t d = on ? degree(a, (uint32_t)v) : 0;
+    // wave-inclusive scans of (on, d)
+    uint32_t c = on ? 1u : 0u;
+    uint64_t dd = d;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t tc = __shfl_up(c, o, 64);
+      const uint64_t td = __shfl_up(dd, o, 64);
+      if (lane >= o) { c += tc; dd += td; }
+    }
+    if (lane == 63) { wc[w] = c; wd[w] = dd; }
+    __syncthreads();
+    uint32_t pc = base_c;
+    uint64_t pd = base_d;
+    for (int q = 0; q < w; ++q) { pc += wc[q]; pd += wd[q]; }
+    if (on) {
+      const uint32_t pos = pc + c - 1;
+      queue[pos] = (uint32_t)v;
+      qpos[pos] = pd + dd - d;
+    }
+    uint32_t tc = 0; uint64_t td = 0;
+    for (int q = 0; q < kB / 64; ++q) { tc += wc[q]; td += wd[q]; }
+    base_c += tc;
+    base_d += td;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) qpos[bcnt[nb]] = bdeg[nb];
+}
+
+// edge-balanced push: virtual edge k of the frontier -> (u, v, CSR position)
+template <int P>
+__global__ __launch_bounds__(kB) void k_expand(EngCtx c, Adj a, const uint32_t* __restrict__ queue,
+                                               const uint64_t* __restrict__ qpos, uint32_t qn, uint64_t T,
+                                               uint32_t* __restrict__ next) {
+  __shared__ uint32_t range[2];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
+  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+  if (threadIdx.x < 2) {
+    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+    uint32_t lo = 0, hi = qn;  // last i with qpos[i] <= key
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= key) lo = mid; else hi = mid;
+    }
+    range[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint32_t i0 = range[0], i1 = range[1];
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kB) {
+    uint32_t lo = i0, hi = i1 + 1;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= k) lo = mid; else hi = mid;
+    }
+    const uint32_t u = queue[lo];
+    uint64_t j = k - qpos[lo];
+    uint32_t v;
+    uint64_t epos;
+    const uint64_t dout = a.dir == GRE_DIR_IN ? 0 : a.off[u + 1] - a.off[u];
+    if (j < dout) {
+      epos = a.off[u] + j;
+      v = a.idx[epos];
+    } else {
+      const uint64_t p = a.coff[u] + (j - dout);
+      v = a.cidx[p];
+      epos = a.ceid ? a.ceid[p] : p;
+    }
+    if (Prog<P>::combine(c, u, v, epos)) {
+      const uint32_t bit = 1u << (v & 31);
+      if (!(next[v >> 5] & bit)) atomicOr(next + (v >> 5), bit);
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, uint64_t n, T v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kB - 1) / kB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+}  // namespace
+}  // namespace gre
+
+struct gre_engine {
+  gre_graph* g = nullptr;
+  int prog = 0;
+  int64_t max_ss = 1024;
+  gre::DevBuf S, C, X, AUX;
+  size_t es[4] = {0, 0, 0, 0};
+  bool c_is_x = false;  // SSSP: combine and vertex data are one array
+  gre::DevBuf curr, next, queue, qpos, bcnt, bdeg;
+  uint32_t nwords = 0, nb = 0;
+  int32_t level = 1;
+  int dir = GRE_DIR_OUT;
+  bool begun = false;
+  int64_t superstep = 0;
+  uint32_t qn = 0;
+  uint64_t qedges = 0;
+  double times[5] = {0, 0, 0, 0, 0};
+};
+
+namespace gre {
+namespace {
+
+EngCtx make_ctx(gre_engine* e) {
+  EngCtx c;
+  c.S = e->S.p;
+  c.C = e->c_is_x ? e->X.p : e->C.p;
+  c.X = e->X.p;
+  c.aux = e->AUX.p;
+  c.level = e->level;
+  c.outdeg = e->g->outdeg.as<int32_t>();
+  c.w_csr = e->g->w_u32.as<uint32_t>();
+  return c;
+}
+
+Adj make_adj(gre_engine* e) {
+  Adj a;
+  gre_graph* g = e->g;
+  a.off = g->off.as<uint64_t>();
+  a.idx = g->idx.as<uint32_t>();
+  a.coff = g->coff.as<uint64_t>();
+  a.cidx = g->cidx.as<uint32_t>();
+  a.ceid = (e->prog == GRE_PROG_SSSP && e->dir != GRE_DIR_OUT) ? graph_ceid(g) : nullptr;
+  a.dir = e->dir;
+  return a;
+}
+
+template <typename F>
+void dispatch(int prog, F&& f) {
+  switch (prog) {
+    case GRE_PROG_PAGERANK: f(std::integral_constant<int, GRE_PROG_PAGERANK>()); break;
+    case GRE_PROG_PAGERANK_DELTA: f(std::integral_constant<int, GRE_PROG_PAGERANK_DELTA>()); break;
+    case GRE_PROG_BFS: f(std::integral_constant<int, GRE_PROG_BFS>()); break;
+    case GRE_PROG_CC: f(std::integral_constant<int, GRE_PROG_CC>()); break;
+    case GRE_PROG_SSSP: f(std::integral_constant<int, GRE_PROG_SSSP>()); break;
+    case GRE_PROG_BC_FORWARD: f(std::integral_constant<int, GRE_PROG_BC_FORWARD>()); break;
+    case GRE_PROG_BC_BACKWARD: f(std::integral_constant<int, GRE_PROG_BC_BACKWARD>()); break;
+    case GRE_PROG_COUNTIN: f(std::integral_constant<int, GRE_PROG_COUNTIN>()); break;
+    default: throw Error(GRE_DEV_NULL, "unknown vertex program id");
+  }
+}
+
+// Fill the arrays exactly as the program's alloc() leaves them
+// (include/Kernels/*.hpp alloc(), VertexCode.hpp:132-178).
+void default_state(gre_engine* e) {
+  hipStream_t s = stream();
+  const uint64_t n = e->g->nV;
+  auto fill_u64 = [&](DevBuf& b, uint64_t v) {
+    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(gcap(n)), dim3(kB), 0, s, b.as<uint64_t>(), n, v);
+  };
+  auto zero = [&](DevBuf& b) { GRE_HIP(hipMemsetAsync(b.p, 0, b.bytes, s)); };
+  switch (e->prog) {
+    case GRE_PROG_PAGERANK: case GRE_PROG_PAGERANK_DELTA:
+    case GRE_PROG_BC_BACKWARD: case GRE_PROG_COUNTIN:
+      zero(e->S); zero(e->C); zero(e->X); break;
+    case GRE_PROG_BFS: case GRE_PROG_CC:
+      zero(e->S); fill_u64(e->C, ~0ULL); fill_u64(e->X, ~0ULL); break;
+    case GRE_PROG_SSSP:
+      zero(e->S);
+      hipLaunchKernelGGL(k_fill<uint32_t>, dim3(gcap(n)), dim3(kB), 0, s, e->X.as<uint32_t>(), n, 0xffffffffu);
+      break;
+    case GRE_PROG_BC_FORWARD: {
+      zero(e->S); zero(e->C);
+      // vtxData(): level = -1, numPaths = 0
+      hipLaunchKernelGGL(k_fill<uint64_t>, dim3(gcap(n)), dim3(kB), 0, s, e->X.as<uint64_t>(), n,
+                         (uint64_t)0xffffffffULL);
+      break;
+    }
+  }
+  GRE_HIP(hipGetLastError());
+}
+
+void rebuild_queue(gre_engine* e, const Adj& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb);
+  hipLaunchKernelGGL(k_write, dim3(e->nb), dim3(kB), 0, s, e->curr.as<uint32_t>(), e->g->nV, a,
+                     e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb, e->queue.as<uint32_t>(),
+                     e->qpos.as<uint64_t>());
+  GRE_HIP(hipGetLastError());
+  uint32_t qn = 0;
+  uint64_t qe = 0;
+  GRE_HIP(hipMemcpyAsync(&qn, e->bcnt.as<uint32_t>() + e->nb, 4, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipMemcpyAsync(&qe, e->bdeg.as<uint64_t>() + e->nb, 8, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  e->qn = qn;
+  e->qedges = qe;
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
+  GRE_CHECK(dir >= GRE_DIR_OUT && dir <= GRE_DIR_INOUT, GRE_DEV_NULL, "dir must be Out, In or InOut");
+  GRE_CHECK(nsrc >= 0, GRE_DEV_NULL, "nsrc must be >= 0");
+  GRE_CHECK(e->prog != GRE_PROG_SSSP || e->g->w_u32.p, GRE_DEV_NULL,
+            "SSSP needs edge weights (gre_graph_set_weights_u32)");
+  GRE_CHECK(e->prog != GRE_PROG_BC_BACKWARD || e->AUX.p, GRE_DEV_NULL,
+            "BC backward needs the forward state (GRE_ARR_AUX)");
+  if (srcs)
+    for (int64_t i = 0; i < nsrc; ++i)
+      GRE_CHECK(srcs[i] < e->g->nV, GRE_DEV_NULL, "source id >= nV");
+  hipStream_t s = stream();
+  e->dir = dir;
+  const double t0 = now();
+  // run(): superstep != 0 -> currSet.clear(), nextSet.clear() (Engine.hpp:182-193)
+  GRE_HIP(hipMemsetAsync(e->curr.p, 0, e->curr.bytes, s));
+  GRE_HIP(hipMemsetAsync(e->next.p, 0, e->next.bytes, s));
+  EngCtx c = make_ctx(e);
+  const uint32_t nvpad = e->nwords * 32;
+  if (!srcs) {
+    dispatch(e->prog, [&](auto P) {
+      hipLaunchKernelGGL(k_init_all<decltype(P)::value>, dim3(gcap(nvpad)), dim3(kB), 0, s, c, e->g->nV, nvpad,
+                         e->curr.as<uint32_t>());
+    });
+  } else if (nsrc > 0) {
+    DevBuf ds;
+    ds.alloc(4 * (size_t)nsrc);
+    GRE_HIP(hipMemcpyAsync(ds.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+    dispatch(e->prog, [&](auto P) {
+      hipLaunchKernelGGL(k_init_src<decltype(P)::value>, dim3(1), dim3(64), 0, s, c, ds.as<uint32_t>(), nsrc,
+                         e->curr.as<uint32_t>());
+    });
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  GRE_HIP(hipGetLastError());
+  const Adj a = make_adj(e);
+  hipLaunchKernelGGL(k_count, dim3(e->nb), dim3(kB), 0, s, e->curr.as<uint32_t>(), e->g->nV, a,
+                     e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
+  rebuild_queue(e, a, s);
+  e->superstep = 0;
+  e->begun = true;
+  for (double& t : e->times) t = 0.0;
+  e->times[0] = now() - t0;
+}
+
+int64_t engine_step(gre_engine* e) {
+  GRE_CHECK(e->begun, GRE_DEV_NULL, "call gre_engine_begin first");
+  hipStream_t s = stream();
+  const Adj a = make_adj(e);
+  EngCtx c = make_ctx(e);
+  const double t0 = now();
+  // nextSet.clear() (Engine.hpp:322)
+  hipLaunchKernelGGL(k_zero_u32, dim3(gcap(e->nwords)), dim3(kB), 0, s, e->next.as<uint32_t>(), (uint64_t)e->nwords);
+  if (e->qedges > 0) {
+    const uint64_t nblk = (e->qedges + kEB - 1) / kEB;
+    GRE_CHECK(nblk < 0x7fffffffULL, GRE_DEV_FAILURE, "frontier too large for one launch");
+    ProfScope ps("push_expand");
+    dispatch(e->prog, [&](auto P) {
+      hipLaunchKernelGGL(k_expand<decltype(P)::value>, dim3((unsigned)nblk), dim3(kB), 0, s, c, a,
+                         e->queue.as<uint32_t>(), e->qpos.as<uint64_t>(), e->qn, e->qedges, e->next.as<uint32_t>());
+    });
+    GRE_HIP(hipGetLastError());
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+  const double t1 = now();
+  {
+    ProfScope ps("apply");
+    dispatch(e->prog, [&](auto P) {
+      hipLaunchKernelGGL(k_apply<decltype(P)::value>, dim3(e->nb), dim3(kB), 0, s, c, e->next.as<uint32_t>(),
+                         e->curr.as<uint32_t>(), e->g->nV, a, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
+    });
+  }
+  GRE_HIP(hipGetLastError());
+  rebuild_queue(e, a, s);
+  const double t2 = now();
+  e->times[2] += t1 - t0;
+  e->times[4] += t2 - t1;
+  // the program's reduce() hook: Forward increments currLevel (BC.hpp:40-42)
+  if (e->prog == GRE_PROG_BC_FORWARD) e->level++;
+  return e->qn;
+}
+
+}  // namespace
+}  // namespace gre
+
+using namespace gre;
+
+extern "C" {
+
+int gre_engine_create(gre_engine** out, gre_graph* g, int prog) {
+  return guard([&] {
+    GRE_CHECK(out && g, GRE_DEV_NULL, "null argument");
+    auto* e = new gre_engine();
+    try {
+      e->g = g;
+      e->prog = prog;
+      size_t s = 0, c = 0, x = 0, aux = 0;
+      switch (prog) {
+        case GRE_PROG_PAGERANK: case GRE_PROG_PAGERANK_DELTA: s = c = x = 8; break;
+        case GRE_PROG_BFS: case GRE_PROG_CC: s = c = x = 8; break;
+        case GRE_PROG_SSSP: s = c = x = 4; e->c_is_x = true; break;
+        case GRE_PROG_BC_FORWARD: s = 4; c = 4; x = 8; break;
+        case GRE_PROG_BC_BACKWARD: s = c = x = 8; aux = 8; break;
+        case GRE_PROG_COUNTIN: s = c = x = 4; break;
+        default: throw Error(GRE_DEV_NULL, "unknown vertex program id");
+      }
+      const size_t n = g->nV;
+      e->es[0] = s; e->es[1] = c; e->es[2] = x; e->es[3] = aux;
+      e->S.alloc(s * n);
+      if (!e->c_is_x) e->C.alloc(c * n);
+      e->X.alloc(x * n);
+      if (aux) e->AUX.alloc(aux * n);
+      // bitmaps cover every vertex slot a kVB-vertex block touches, so the
+      // apply/count/write blocks never index past them
+      e->nb = (uint32_t)((n + kVB - 1) / kVB);
+      e->nwords = e->nb * (uint32_t)(kVB / 32);
+      e->curr.alloc(4 * (size_t)e->nwords);
+      e->next.alloc(4 * (size_t)e->nwords);
+      e->queue.alloc(4 * n);
+      e->qpos.alloc(8 * (n + 1));
+      e->bcnt.alloc(4 * ((size_t)e->nb + 1));
+      e->bdeg.alloc(8 * ((size_t)e->nb + 1));
+      default_state(e);
+      GRE_HIP(hipStreamSynchronize(stream()));
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+int gre_engine_destroy(gre_engine* e) {
+  return guard([&] {
+    if (!e) return;
+    GRE_HIP(hipStreamSynchronize(stream()));
+    delete e;
+  });
+}
+
+int gre_engine_config_max_supersteps(gre_engine* e, int64_t n) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null engine");
+    e->max_ss = n > 0 ? n : 1024;
+  });
+}
+
+int gre_engine_elem_size(const gre_engine* e, int which, size_t* bytes) {
+  return guard([&] {
+    GRE_CHECK(e && bytes, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(which >= 0 && which <= 3, GRE_DEV_NULL, "bad array id");
+    *bytes = e->es[which];
+  });
+}
+
+static gre::DevBuf& arr(gre_engine* e, int which) {
+  GRE_CHECK(which >= 0 && which <= 3, GRE_DEV_NULL, "bad array id");
+  GRE_CHECK(e->es[which] > 0, GRE_DEV_NULL, "program has no such array");
+  if (which == GRE_ARR_COMBINE && e->c_is_x) return e->X;
+  return which == 0 ? e->S : which == 1 ? e->C : which == 2 ? e->X : e->AUX;
+}
+
+int gre_engine_upload(gre_engine* e, int which, const void* host) {
+  return guard([&] {
+    GRE_CHECK(e && host, GRE_DEV_NULL, "null argument");
+    gre::DevBuf& b = arr(e, which);
+    GRE_HIP(hipMemcpyAsync(b.p, host, e->es[which] * (size_t)e->g->nV, hipMemcpyHostToDevice, stream()));
+    GRE_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int gre_engine_download(gre_engine* e, int which, void* host) {
+  return guard([&] {
+    GRE_CHECK(e && host, GRE_DEV_NULL, "null argument");
+    gre::DevBuf& b = arr(e, which);
+    GRE_HIP(hipMemcpyAsync(host, b.p, e->es[which] * (size_t)e->g->nV, hipMemcpyDeviceToHost, stream()));
+    GRE_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int gre_engine_set_level(gre_engine* e, int32_t level) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null engine");
+    e->level = level;
+  });
+}
+
+int gre_engine_get_level(const gre_engine* e, int32_t* level) {
+  return guard([&] {
+    GRE_CHECK(e && level, GRE_DEV_NULL, "null argument");
+    *level = e->level;
+  });
+}
+
+int gre_engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null engine");
+    engine_begin(e, dir, srcs, nsrc);
+  });
+}
+
+int gre_engine_step(gre_engine* e, int64_t* active) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null engine");
+    const int64_t n = engine_step(e);
+    if (active) *active = n;
+  });
+}
+
+int gre_engine_run(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc, int64_t* counts, int64_t counts_cap,
+                   int64_t* supersteps) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null engine");
+    engine_begin(e, dir, srcs, nsrc);
+    int64_t ss;
+    for (ss = 0; ss < e->max_ss; ++ss) {
+      const int64_t n = engine_step(e);
+      if (counts && ss < counts_cap) counts[ss] = n;
+      if (n == 0) break;
+    }
+    e->superstep = ss;
+    if (supersteps) *supersteps = ss;
+  });
+}
+
+int gre_engine_phase_times(const gre_engine* e, double* t5) {
+  return guard([&] {
+    GRE_CHECK(e && t5, GRE_DEV_NULL, "null argument");
+    for (int i = 0; i < 5; ++i) t5[i] = e->times[i];
+  });
+}
+
+}  // extern "C"

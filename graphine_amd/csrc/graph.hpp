@@ -1,0 +1,47 @@
+// Device-resident graph: CSR (out-edges) + CSC (in-edges), uint64 offsets,
+// uint32 vertex ids.  At one process GRE's local graph holds exactly this
+// (include/Graph/LocalGraph.hpp:23-200): edgeIndex/edgeList and
+// redgeIndex/redgeList/redgeID.
+#pragma once
+#include "common.hpp"
+
+#include <map>
+#include <utility>
+
+struct gre_graph {
+  uint32_t nV = 0;
+  uint64_t nE = 0;
+  gre::DevBuf off, idx;     // CSR
+  gre::DevBuf coff, cidx;   // CSC
+  gre::DevBuf ceid;         // CSC position -> CSR position (uint64), built on demand
+  gre::DevBuf outdeg;       // int32
+  gre::DevBuf w_u32;        // SSSP weights, CSR order (optional)
+  gre::DevBuf w_u32_csc;    // same, CSC order (built on demand)
+  bool rows_sorted = false; // CSR rows sorted by destination (R-MAT, from_csr sorted)
+  // tile row partitions of the pull kernels, keyed by (csc?, tile edges)
+  std::map<std::pair<int, uint32_t>, gre::DevBuf> tiles;
+};
+
+namespace gre {
+// CSC -> CSR position map (built once, cached on the graph).
+const uint64_t* graph_ceid(gre_graph* g);
+// SSSP weights permuted to CSC order (cached).
+const uint32_t* graph_w_csc(gre_graph* g);
+// tile_rb[t] = #rows r < nV with off[r] < t*te, t in [0, ntiles]; ntiles >= 1.
+const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles);
+
+// device helpers
+__device__ inline uint32_t row_of(const uint64_t* __restrict__ off, uint32_t nV, uint64_t e) {
+  // largest r with off[r] <= e (rows with off[r] == off[r+1] are skipped)
+  uint32_t lo = 0, hi = nV;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (off[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// exclusive scan helpers (rocPRIM), sizes in elements
+void scan_excl_u64(const uint64_t* in, uint64_t* out, size_t n, hipStream_t s);
+void scan_excl_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s);
+}  // namespace gre

@@ -1,0 +1,636 @@
+// PageRank / personalised PageRank forward, adjoint and edge gradients.
+//
+// Forward (pull over CSC):  x' = d (W^T x + (g.x) p) + (1-d) p
+//   unweighted: W_uv = 1/outdeg(u), p uniform -> the classic power iteration
+//   (1/N start, uniform teleport, dangling mass spread uniformly);
+// Adjoint (pull over CSR):  lambda' = c + d (W lambda + g (p.lambda))
+// Gradients:  dL/dW_uv = d x_u lambda_v,
+//             dL/dw_uv = d x_u (lambda_v - (W lambda)_u) / rowsum_u,
+//             dL/dp_v  = ((1-d) + d (g.x)) lambda_v.
+// The FP64 reductions (dangling mass g.x, p.lambda, L1 residual) are fixed
+// order block partials, so every run gives the same bits.
+#include "pr.hpp"
+
+#include <cmath>
+
+
+
+namespace gre {
+namespace {
+
+constexpr unsigned kB = 256;
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kB - 1) / kB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+template <typename TT>
+struct ModeFwd {
+  using T = TT;
+  const TT* gin;          // unweighted: contrib ; weighted: x
+  TT* contrib_out;        // unweighted only
+  const TT* xin;          // previous ranks (residual)
+  TT* xout;               // new ranks
+  const float* w;         // normalised weights in CSC order (weighted)
+  const TT* p;            // personalisation or nullptr (uniform)
+  const int32_t* outdeg;
+  const double* scal_in;  // [0] = dangling mass of xin
+  double damp, invn;
+  __device__ TT gather(uint64_t e, uint32_t u) const {
+    if (w) return (TT)w[e] * gin[u];
+    return gin[u];
+  }
+  __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const {
+    const double dang = scal_in[0];
+    const TT pv = p ? p[r] : (TT)invn;
+    const TT y = (TT)damp * (s + (TT)dang * pv) + (TT)(1.0 - damp) * pv;
+    const TT old = xin[r];
+    xout[r] = y;
+    const int32_t d = outdeg[r];
+    if (!w) contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
+    if (d == 0) a0 += (double)y;
+    a1 += fabs((double)y - (double)old);
+  }
+};
+
+template <typename TT>
+struct ModeAdj {
+  using T = TT;
+  const TT* lin;
+  TT* lout;
+  const float* w;         // normalised weights, CSR order (weighted)
+  const TT* c;
+  const TT* p;
+  const int32_t* outdeg;
+  TT* wl;                 // (W lambda)_u
+  const double* scal_in;  // [0] = p.lambda of lin
+  double damp, invn;
+  __device__ TT gather(uint64_t e, uint32_t v) const {
+    if (w) return (TT)w[e] * lin[v];
+    return lin[v];
+  }
+  __device__ void finalize(uint32_t u, TT s, double& a0, double& a1) const {
+    const int32_t d = outdeg[u];
+    if (!w && d > 0) s = s / (TT)d;
+    wl[u] = s;
+    const TT y = c[u] + (TT)damp * (s + (d == 0 ? (TT)scal_in[0] : (TT)0));
+    const TT old = lin[u];
+    lout[u] = y;
+    a0 += (double)(p ? p[u] : (TT)invn) * (double)y;
+    a1 += fabs((double)y - (double)old);
+  }
+};
+
+// row sums of the raw weights (pull over CSR, gather = weight)
+struct ModeRowSum {
+  using T = double;
+  const float* w;
+  double* rowsum;
+  __device__ double gather(uint64_t e, uint32_t) const { return (double)w[e]; }
+  __device__ void finalize(uint32_t r, double s, double&, double&) const { rowsum[r] = s; }
+};
+
+template <typename TT>
+__global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t* __restrict__ outdeg,
+                           TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part) {
+  __shared__ double red[8];
+  double a0 = 0.0, a1 = 0.0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const TT xv = p ? p[v] : (TT)invn;
+    x[v] = xv;
+    const int32_t d = outdeg[v];
+    if (contrib) contrib[v] = d > 0 ? xv / (TT)d : (TT)0;
+    if (d == 0) a0 += (double)xv;
+  }
+  block_sum2(a0, a1, red);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+template <typename TT>
+__global__ void k_adj_init(uint32_t nV, const TT* __restrict__ c, const TT* __restrict__ p, TT* __restrict__ lam,
+                           double invn, double* __restrict__ part) {
+  __shared__ double red[8];
+  double a0 = 0.0, a1 = 0.0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    lam[v] = c[v];
+    a0 += (double)(p ? p[v] : (TT)invn) * (double)c[v];
+  }
+  block_sum2(a0, a1, red);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+__global__ void k_lognormal(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx, uint32_t nV,
+                            uint64_t nE, uint64_t seed, float* __restrict__ w) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nE;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = row_of(off, nV, e), v = idx[e];
+    const uint64_t h1 = mix64(seed * GRE_K0 + (uint64_t)u * GRE_K1 + (uint64_t)v * GRE_K2 + GRE_K3);
+    const uint64_t h2 = mix64(h1 ^ GRE_K4);
+    const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    w[e] = (float)exp(z);
+  }
+}
+
+__global__ void k_normalise(const uint64_t* __restrict__ off, uint32_t nV, uint64_t nE, const float* __restrict__ w,
+                            const double* __restrict__ rowsum, float* __restrict__ wn) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nE;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = row_of(off, nV, e);
+    wn[e] = (float)((double)w[e] / rowsum[u]);
+  }
+}
+
+__global__ void k_permute_f32(const float* __restrict__ in, const uint64_t* __restrict__ perm, uint64_t n,
+                              float* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[perm[i]];
+}
+
+template <typename TT>
+__global__ void k_targets(uint32_t nV, const uint32_t* __restrict__ tg, int64_t nt, TT* __restrict__ c) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nt; i += (int64_t)gridDim.x * blockDim.x)
+    c[tg[i]] = (TT)1;
+}
+
+// gW/graw over CSR; one block per CSR tile, row found inside the tile's range
+template <typename TT>
+__global__ __launch_bounds__(256) void k_edge_grad(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                                   const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
+                                                   uint64_t ntiles, uint32_t te, const float* __restrict__ w,
+                                                   const double* __restrict__ rowsum, const TT* __restrict__ x,
+                                                   const TT* __restrict__ lam, const TT* __restrict__ wl,
+                                                   const int32_t* __restrict__ outdeg, double damp,
+                                                   float* __restrict__ gW, float* __restrict__ graw) {
+  const uint64_t t = blockIdx.x;
+  const uint64_t e0 = t * (uint64_t)te;
+  const uint64_t e1 = e0 + te < E ? e0 + te : E;
+  uint32_t lo0 = tile_rb[t];
+  lo0 = lo0 > 0 ? lo0 - 1 : 0;
+  const uint32_t hi0 = (t + 1 == ntiles) ? V : tile_rb[t + 1];
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    uint32_t lo = lo0, hi = hi0 < V ? hi0 + 1 : V;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const uint32_t u = lo, v = idx[e];
+    const double xu = (double)x[u];
+    const double lv = (double)lam[v];
+    gW[e] = (float)(damp * xu * lv);
+    if (graw) {
+      const double rs = w ? rowsum[u] : (double)outdeg[u];
+      graw[e] = rs > 0.0 ? (float)(damp * xu * (lv - (double)wl[u]) / rs) : 0.f;
+    }
+  }
+}
+
+template <typename TT>
+__global__ void k_grad_p(uint32_t nV, const TT* __restrict__ lam, const double* __restrict__ dang, double damp,
+                         TT* __restrict__ gp) {
+  const double f = (1.0 - damp) + damp * dang[0];
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    gp[v] = (TT)(f * (double)lam[v]);
+}
+
+size_t tsize(int dtype) { return dtype == GRE_F64 ? 8 : 4; }
+
+void ensure_scal(gre_pr* P, int iters) {
+  if (P->scal_cap >= iters + 2) return;
+  P->scal_cap = iters + 2;
+  P->scal.alloc(sizeof(double) * 2 * (size_t)P->scal_cap);
+}
+
+void ensure_plan(gre_pr* P, bool csc, int ipt) {
+  int& have = csc ? P->plan_ipt_csc : P->plan_ipt_csr;
+  if (have == ipt) return;
+  (csc ? P->buf_csc : P->buf_csr).make(csc ? P->plan_csc : P->plan_csr, P->g, csc, ipt, tsize(P->dtype));
+  have = ipt;
+}
+
+template <typename TT>
+ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
+  const int c = P->cur;
+  ModeFwd<TT> m;
+  m.w = P->weighted ? P->wcsc.as<float>() : nullptr;
+  m.gin = P->weighted ? P->x[c].as<TT>() : P->contrib[c].as<TT>();
+  m.contrib_out = P->weighted ? nullptr : P->contrib[c ^ 1].as<TT>();
+  m.xin = P->x[c].as<TT>();
+  m.xout = P->weighted ? P->x[c ^ 1].as<TT>() : P->x[c].as<TT>();
+  m.p = P->has_p ? P->p.as<TT>() : nullptr;
+  m.outdeg = P->g->outdeg.as<int32_t>();
+  m.damp = damp;
+  m.invn = 1.0 / (double)P->g->nV;
+  return m;
+}
+
+// one forward iteration; scal_in/scal_out are device pointers to (acc0, acc1)
+template <typename TT>
+void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, hipStream_t s) {
+  ModeFwd<TT> m = fwd_mode<TT>(P, damp);
+  m.scal_in = scal_in;
+  // in-place x for the unweighted form: residual reads xin before xout writes
+  pull_pass(P->plan_csc, m, scal_out, s);
+  P->cur ^= 1;
+}
+
+template <typename TT>
+void fwd_init(gre_pr* P, hipStream_t s) {
+  const uint32_t nV = P->g->nV;
+  const unsigned gb = gcap(nV);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
+  P->cur = 0;
+  hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
+                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), P->weighted ? nullptr : P->contrib[0].as<TT>(),
+                     1.0 / (double)nV, tmp);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0,
+                     P->scal.as<double>());
+  GRE_HIP(hipGetLastError());
+}
+
+template <typename TT>
+double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
+  hipStream_t s = stream();
+  fwd_init<TT>(P, s);
+  double* sc = P->scal.as<double>();
+  int it = 0;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s);
+    if (tol > 0.0) {
+      double h[2];
+      GRE_HIP(hipMemcpyAsync(h, sc + 2 * (it + 1), sizeof(h), hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      res = h[1];
+      if (res < tol) { ++it; break; }
+    }
+  }
+  if (tol <= 0.0 && iters > 0) {
+    double h[2];
+    GRE_HIP(hipMemcpyAsync(h, sc + 2 * iters, sizeof(h), hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    res = h[1];
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+  if (done) *done = it;
+  return res;
+}
+
+// Autotuner: times each tile size on two forward iterations and keeps the fastest.
+template <typename TT>
+int autotune_forward(gre_pr* P, double damp) {
+  hipStream_t s = stream();
+  const int cands[3] = {4, 8, 16};
+  int best = 8;
+  float best_ms = 1e30f;
+  hipEvent_t a, b;
+  GRE_HIP(hipEventCreate(&a));
+  GRE_HIP(hipEventCreate(&b));
+  ensure_scal(P, 4);
+  for (int ci = 0; ci < 3; ++ci) {
+    ensure_plan(P, true, cands[ci]);
+    fwd_init<TT>(P, s);
+    double* sc = P->scal.as<double>();
+    fwd_iter<TT>(P, damp, sc, sc + 2, s);  // warm
+    GRE_HIP(hipEventRecord(a, s));
+    fwd_iter<TT>(P, damp, sc + 2, sc + 4, s);
+    fwd_iter<TT>(P, damp, sc + 4, sc + 6, s);
+    GRE_HIP(hipEventRecord(b, s));
+    GRE_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GRE_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) { best_ms = ms; best = cands[ci]; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+template <typename TT>
+double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
+  hipStream_t s = stream();
+  const uint32_t nV = P->g->nV;
+  const unsigned gb = gcap(nV);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
+  ensure_plan(P, false, P->ipt);
+  ensure_scal(P, iters);
+  double* sc = P->scal.as<double>();
+  P->lcur = 0;
+  hipLaunchKernelGGL(k_adj_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->c.as<TT>(), P->has_p ? P->p.as<TT>() : nullptr,
+                     P->lam[0].as<TT>(), 1.0 / (double)nV, tmp);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+  GRE_HIP(hipGetLastError());
+  int it = 0;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    ModeAdj<TT> m;
+    m.lin = P->lam[P->lcur].as<TT>();
+    m.lout = P->lam[P->lcur ^ 1].as<TT>();
+    m.w = P->weighted ? P->wcsr.as<float>() : nullptr;
+    m.c = P->c.as<TT>();
+    m.p = P->has_p ? P->p.as<TT>() : nullptr;
+    m.outdeg = P->g->outdeg.as<int32_t>();
+    m.wl = P->wl.as<TT>();
+    m.scal_in = sc + 2 * it;
+    m.damp = damp;
+    m.invn = 1.0 / (double)nV;
+    pull_pass(P->plan_csr, m, sc + 2 * (it + 1), s);
+    P->lcur ^= 1;
+    if (tol > 0.0) {
+      double h[2];
+      GRE_HIP(hipMemcpyAsync(h, sc + 2 * (it + 1), sizeof(h), hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      res = h[1];
+      if (res < tol) { ++it; break; }
+    }
+  }
+  if (tol <= 0.0 && iters > 0) {
+    double h[2];
+    GRE_HIP(hipMemcpyAsync(h, sc + 2 * iters, sizeof(h), hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    res = h[1];
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+  if (done) *done = it;
+  return res;
+}
+
+template <typename TT>
+void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* gp_host) {
+  hipStream_t s = stream();
+  gre_graph* g = P->g;
+  const TT* x = P->x[P->cur].as<TT>();
+  const TT* lam = P->lam[P->lcur].as<TT>();
+  // dangling mass of the final forward, recomputed in fixed order
+  const unsigned gb = gcap(g->nV);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
+  double* dsc = static_cast<double*>(scratch(4, sizeof(double) * 2));
+  DevBuf dummy;
+  dummy.alloc(sizeof(TT) * (size_t)g->nV);
+  hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, g->nV, x, g->outdeg.as<int32_t>(), dummy.as<TT>(),
+                     (TT*)nullptr, 0.0, tmp);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dsc);
+  if (gp_host) {
+    hipLaunchKernelGGL(k_grad_p<TT>, dim3(gb), dim3(kB), 0, s, g->nV, lam, dsc, damp, dummy.as<TT>());
+    GRE_HIP(hipMemcpyAsync(gp_host, dummy.p, sizeof(TT) * g->nV, hipMemcpyDeviceToHost, s));
+  }
+  if (gW_host || graw_host) {
+    const uint32_t te = 2048;
+    uint64_t nt = 0;
+    const uint32_t* trb = graph_tiles(g, false, te, &nt);
+    DevBuf dgW, dgraw;
+    dgW.alloc(4 * (size_t)g->nE);
+    if (graw_host) dgraw.alloc(4 * (size_t)g->nE);
+    if (g->nE > 0) {
+      ProfScope ps("edge_grad");
+      hipLaunchKernelGGL(k_edge_grad<TT>, dim3((unsigned)nt), dim3(256), 0, s, g->off.as<uint64_t>(),
+                         g->idx.as<uint32_t>(), trb, g->nE, g->nV, nt, te,
+                         P->weighted ? P->wcsr.as<float>() : nullptr, P->weighted ? P->rowsum.as<double>() : nullptr,
+                         x, lam, P->wl.as<TT>(), g->outdeg.as<int32_t>(), damp, dgW.as<float>(),
+                         graw_host ? dgraw.as<float>() : nullptr);
+    }
+    GRE_HIP(hipGetLastError());
+    if (gW_host && g->nE) GRE_HIP(hipMemcpyAsync(gW_host, dgW.p, 4 * g->nE, hipMemcpyDeviceToHost, s));
+    if (graw_host && g->nE) GRE_HIP(hipMemcpyAsync(graw_host, dgraw.p, 4 * g->nE, hipMemcpyDeviceToHost, s));
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+}
+
+void alloc_state(gre_pr* P) {
+  const size_t n = (size_t)P->g->nV * P->k;
+  const size_t ts = tsize(P->dtype);
+  P->x[0].alloc(ts * n);
+  if (P->weighted || P->k > 1) P->x[1].alloc(ts * n);
+  if (!P->weighted && P->k == 1) { P->contrib[0].alloc(ts * n); P->contrib[1].alloc(ts * n); }
+}
+
+}  // namespace
+}  // namespace gre
+
+using namespace gre;
+
+extern "C" {
+
+int gre_pr_create(gre_pr** out, gre_graph* g, int dtype, int k, int weighted) {
+  return guard([&] {
+    GRE_CHECK(out && g, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(dtype == GRE_F32 || dtype == GRE_F64, GRE_DEV_NULL, "dtype must be GRE_F32 or GRE_F64");
+    GRE_CHECK(k >= 1 && k <= kMaxBatch, GRE_DEV_NULL, "k must be in [1, 64]");
+    GRE_CHECK(k == 1 || weighted, GRE_DEV_NULL, "batched vectors (k > 1) need weighted = 1");
+    auto* P = new gre_pr();
+    try {
+      P->g = g;
+      P->dtype = dtype;
+      P->k = k;
+      P->weighted = weighted ? 1 : 0;
+      alloc_state(P);
+      if (P->weighted) {
+        // unit raw weights until set: row-normalised 1/outdeg
+        const int rc = gre_pr_set_edge_weights(P, nullptr, 0);
+        GRE_CHECK(rc == 0, rc, gre_last_error());
+      }
+    } catch (...) {
+      delete P;
+      throw;
+    }
+    *out = P;
+  });
+}
+
+int gre_pr_destroy(gre_pr* P) {
+  return guard([&] {
+    if (!P) return;
+    GRE_HIP(hipStreamSynchronize(stream()));
+    delete P;
+  });
+}
+
+int gre_pr_set_schedule(gre_pr* P, int schedule, int tile_items) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(schedule >= 0 && schedule <= 1, GRE_DEV_NULL, "schedule must be 0 (autotune) or 1 (tile)");
+    GRE_CHECK(tile_items == 0 || tile_items == 4 || tile_items == 8 || tile_items == 16, GRE_DEV_NULL,
+              "tile items per thread must be 4, 8 or 16");
+    P->schedule = schedule;
+    P->ipt = schedule == 0 ? 0 : (tile_items ? tile_items : 8);
+  });
+}
+
+int gre_pr_set_edge_weights(gre_pr* P, const float* w, uint64_t seed) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(P->weighted, GRE_DEV_NULL, "handle was created unweighted");
+    gre_graph* g = P->g;
+    hipStream_t s = stream();
+    const uint64_t nE = g->nE;
+    P->wraw.alloc(4 * nE);
+    P->wcsr.alloc(4 * nE);
+    P->wcsc.alloc(4 * nE);
+    P->rowsum.alloc(8 * (size_t)g->nV);
+    if (nE > 0) {
+      if (w) {
+        GRE_HIP(hipMemcpyAsync(P->wraw.p, w, 4 * nE, hipMemcpyHostToDevice, s));
+      } else if (seed == 0) {
+        std::vector<float> ones(nE, 1.0f);
+        GRE_HIP(hipMemcpyAsync(P->wraw.p, ones.data(), 4 * nE, hipMemcpyHostToDevice, s));
+        GRE_HIP(hipStreamSynchronize(s));
+      } else {
+        hipLaunchKernelGGL(k_lognormal, dim3(gcap(nE)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(),
+                           g->nV, nE, seed, P->wraw.as<float>());
+      }
+    }
+    // row sums in FP64 (pull over CSR with gather = weight)
+    {
+      PullPlan pl;
+      PullBuffers pb;
+      pb.make(pl, g, false, 8, sizeof(double));
+      ModeRowSum m{P->wraw.as<float>(), P->rowsum.as<double>()};
+      double* sc = static_cast<double*>(scratch(4, 2 * sizeof(double)));
+      pull_pass(pl, m, sc, s);
+      GRE_HIP(hipStreamSynchronize(s));
+    }
+    if (nE > 0) {
+      hipLaunchKernelGGL(k_normalise, dim3(gcap(nE)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->nV, nE,
+                         P->wraw.as<float>(), P->rowsum.as<double>(), P->wcsr.as<float>());
+      const uint64_t* ceid = graph_ceid(g);
+      hipLaunchKernelGGL(k_permute_f32, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<float>(), ceid, nE,
+                         P->wcsc.as<float>());
+      GRE_HIP(hipGetLastError());
+    }
+    GRE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int gre_pr_get_edge_weights(gre_pr* P, float* w_raw, float* w_norm) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(P->weighted, GRE_DEV_NULL, "handle was created unweighted");
+    hipStream_t s = stream();
+    const uint64_t nE = P->g->nE;
+    if (w_raw && nE) GRE_HIP(hipMemcpyAsync(w_raw, P->wraw.p, 4 * nE, hipMemcpyDeviceToHost, s));
+    if (w_norm && nE) GRE_HIP(hipMemcpyAsync(w_norm, P->wcsr.p, 4 * nE, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int gre_pr_set_personalization(gre_pr* P, const void* pvec) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    if (!pvec) { P->has_p = false; P->p.release(); return; }
+    const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
+    P->p.alloc(bytes);
+    GRE_HIP(hipMemcpyAsync(P->p.p, pvec, bytes, hipMemcpyHostToDevice, stream()));
+    GRE_HIP(hipStreamSynchronize(stream()));
+    P->has_p = true;
+  });
+}
+
+int gre_pr_run(gre_pr* P, double damping, int iters, double tol, int* iters_done, double* residual) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(iters >= 0, GRE_DEV_NULL, "iters must be >= 0");
+    GRE_CHECK(damping >= 0.0 && damping < 1.0, GRE_DEV_NULL, "damping must be in [0,1)");
+    if (P->k > 1) {
+      GRE_CHECK(P->has_p, GRE_DEV_NULL, "batched runs need a personalisation matrix");
+      double r = batched_forward(P, damping, iters, tol, iters_done);
+      if (residual) *residual = r;
+      P->have_fwd = true;
+      return;
+    }
+    if (P->ipt == 0) {
+      P->ipt = P->dtype == GRE_F64 ? autotune_forward<double>(P, damping) : autotune_forward<float>(P, damping);
+    }
+    ensure_plan(P, true, P->ipt);
+    ensure_scal(P, iters);
+    const double r = P->dtype == GRE_F64 ? run_forward<double>(P, damping, iters, tol, iters_done)
+                                         : run_forward<float>(P, damping, iters, tol, iters_done);
+    if (residual) *residual = r;
+    P->have_fwd = true;
+  });
+}
+
+int gre_pr_get_ranks(gre_pr* P, void* host_out) {
+  return guard([&] {
+    GRE_CHECK(P && host_out, GRE_DEV_NULL, "null argument");
+    const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
+    GRE_HIP(hipMemcpyAsync(host_out, P->x[P->cur].p, bytes, hipMemcpyDeviceToHost, stream()));
+    GRE_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int gre_pr_adjoint(gre_pr* P, const uint32_t* targets, int64_t ntgt, double damping, int iters, double tol,
+                   int* iters_done, double* residual) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(P->have_fwd, GRE_DEV_NULL, "run the forward (gre_pr_run) first");
+    GRE_CHECK(ntgt >= 0 && (ntgt == 0 || targets), GRE_DEV_NULL, "bad targets");
+    for (int64_t i = 0; i < ntgt; ++i) GRE_CHECK(targets[i] < P->g->nV, GRE_DEV_NULL, "target id >= nV");
+    hipStream_t s = stream();
+    const size_t n = (size_t)P->g->nV * P->k;
+    const size_t ts = tsize(P->dtype);
+    if (!P->c.p) {
+      P->c.alloc(ts * n);
+      P->lam[0].alloc(ts * n);
+      P->lam[1].alloc(ts * n);
+      P->wl.alloc(ts * n);
+    }
+    GRE_HIP(hipMemsetAsync(P->c.p, 0, ts * n, s));
+    if (ntgt > 0) {
+      DevBuf dt;
+      dt.alloc(4 * (size_t)ntgt);
+      GRE_HIP(hipMemcpyAsync(dt.p, targets, 4 * (size_t)ntgt, hipMemcpyHostToDevice, s));
+      if (P->k == 1) {
+        if (P->dtype == GRE_F64)
+          hipLaunchKernelGGL(k_targets<double>, dim3(gcap(ntgt)), dim3(kB), 0, s, P->g->nV, dt.as<uint32_t>(), ntgt,
+                             P->c.as<double>());
+        else
+          hipLaunchKernelGGL(k_targets<float>, dim3(gcap(ntgt)), dim3(kB), 0, s, P->g->nV, dt.as<uint32_t>(), ntgt,
+                             P->c.as<float>());
+      } else {
+        batched_targets(P, dt.as<uint32_t>(), ntgt);
+      }
+      GRE_HIP(hipGetLastError());
+      GRE_HIP(hipStreamSynchronize(s));
+    }
+    double r;
+    if (P->k > 1) {
+      r = batched_adjoint(P, damping, iters, tol, iters_done);
+    } else {
+      if (P->ipt == 0) P->ipt = 8;
+      r = P->dtype == GRE_F64 ? run_adjoint<double>(P, damping, iters, tol, iters_done)
+                              : run_adjoint<float>(P, damping, iters, tol, iters_done);
+    }
+    if (residual) *residual = r;
+    P->have_adj = true;
+  });
+}
+
+int gre_pr_get_adjoint(gre_pr* P, void* host_out) {
+  return guard([&] {
+    GRE_CHECK(P && host_out, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(P->have_adj, GRE_DEV_NULL, "run gre_pr_adjoint first");
+    const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
+    GRE_HIP(hipMemcpyAsync(host_out, P->lam[P->lcur].p, bytes, hipMemcpyDeviceToHost, stream()));
+    GRE_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int gre_pr_gradients(gre_pr* P, double damping, float* gW, float* graw, void* gp) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(P->have_adj, GRE_DEV_NULL, "run gre_pr_adjoint first");
+    if (P->k > 1) { batched_gradients(P, damping, gW, graw, gp); return; }
+    if (P->dtype == GRE_F64) gradients<double>(P, damping, gW, graw, gp);
+    else gradients<float>(P, damping, gW, graw, gp);
+  });
+}
+
+}  // extern "C"

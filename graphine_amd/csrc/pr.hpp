@@ -1,0 +1,47 @@
+// The PageRank handle shared by the single-vector and batched kernels.
+#pragma once
+#include "pull.hpp"
+
+namespace gre {
+constexpr int kMaxBatch = 64;
+struct BatchState {
+  DevBuf carryP, carryO, part, part2, scal;
+  uint64_t ntiles = 0;
+  int scal_cap = 0;
+};
+}  // namespace gre
+
+struct gre_pr {
+  gre_graph* g = nullptr;
+  int dtype = GRE_F32;
+  int k = 1;
+  int weighted = 0;
+  int ipt = 0;            // 0 = autotune on first run
+  int schedule = 0;
+  bool has_p = false;
+  // forward state
+  gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place
+  gre::DevBuf contrib[2]; // unweighted: x/outdeg ping-pong
+  int cur = 0;
+  gre::DevBuf p;          // personalisation (nV*k), optional
+  gre::DevBuf wraw, wcsr, wcsc, rowsum;  // weights (CSR raw, CSR norm, CSC norm), FP64 row sums
+  // adjoint
+  gre::DevBuf lam[2], c, wl;
+  int lcur = 0;
+  bool have_adj = false;
+  bool have_fwd = false;
+  gre::DevBuf scal;       // [2*(maxit+1)] per-iteration (acc0, acc1)
+  int scal_cap = 0;
+  // plans
+  gre::PullPlan plan_csc, plan_csr;
+  gre::PullBuffers buf_csc, buf_csr;
+  int plan_ipt_csc = 0, plan_ipt_csr = 0;
+  gre::BatchState bstate;
+};
+
+namespace gre {
+double batched_forward(gre_pr* P, double damp, int iters, double tol, int* done);
+double batched_adjoint(gre_pr* P, double damp, int iters, double tol, int* done);
+void batched_targets(gre_pr* P, const uint32_t* targets_dev, int64_t nt);
+void batched_gradients(gre_pr* P, double damp, float* gW, float* graw, void* gp);
+}  // namespace gre

@@ -1,0 +1,608 @@
+// Traversals outside GRE's vertex-program model, used as the plain
+// single-source path and the batched multi-source path:
+//   gre_bfs    direction-optimising BFS (levels): top-down push over CSR with
+//              an edge-balanced frontier, bottom-up pull over CSC with early
+//              exit, switched by frontier/unvisited edge counts;
+//   gre_msbfs  up to 64 BFS sources at once, one 64-bit mask per vertex,
+//              pull over CSC;
+//   gre_bc     Brandes BC, FP64 path counts and dependencies, 64 sources per
+//              chunk as the 64 lanes of a wave, pull over CSC (forward) and
+//              CSR (backward), deterministic sums.
+#include "graph.hpp"
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+namespace gre {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kEB = kB * 8;
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kB - 1) / kB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, uint64_t n, T v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// direction-optimising BFS
+// ---------------------------------------------------------------------------
+__global__ void k_bfs_seed(int32_t* level, uint32_t src, uint32_t* queue, uint32_t* fbits) {
+  level[src] = 0;
+  queue[0] = src;
+  fbits[src >> 5] |= 1u << (src & 31);
+}
+
+__global__ void k_qdeg(const uint32_t* __restrict__ q, uint32_t n, const uint64_t* __restrict__ off,
+                       uint64_t* __restrict__ deg) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x)
+    deg[i] = i < n ? off[q[i] + 1] - off[q[i]] : 0;
+}
+
+// top-down: edge-balanced expansion of the queue; new vertices claim level L+1
+__global__ __launch_bounds__(kB) void k_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                           const uint32_t* __restrict__ q, const uint64_t* __restrict__ qpos,
+                                           uint32_t qn, uint64_t T, int32_t L, int32_t* __restrict__ level,
+                                           uint32_t* __restrict__ nq, uint32_t* __restrict__ ncount,
+                                           uint32_t* __restrict__ nbits, unsigned long long* __restrict__ ndeg) {
+  __shared__ uint32_t range[2];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
+  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+  if (threadIdx.x < 2) {
+    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+    uint32_t lo = 0, hi = qn;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= key) lo = mid; else hi = mid;
+    }
+    range[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint32_t i0 = range[0], i1 = range[1];
+  uint64_t mydeg = 0;
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kB) {
+    uint32_t lo = i0, hi = i1 + 1;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= k) lo = mid; else hi = mid;
+    }
+    const uint32_t u = q[lo];
+    const uint32_t v = idx[off[u] + (k - qpos[lo])];
+    if (level[v] < 0 && atomicCAS(level + v, -1, L + 1) == -1) {
+      const uint32_t p = atomicAdd(ncount, 1u);
+      nq[p] = v;
+      atomicOr(nbits + (v >> 5), 1u << (v & 31));
+      mydeg += off[v + 1] - off[v];
+    }
+  }
+  mydeg = wave_sum(mydeg);
+  if ((threadIdx.x & 63) == 0 && mydeg) atomicAdd(ndeg, (unsigned long long)mydeg);
+}
+
+// bottom-up: every unvisited vertex looks for a parent in the frontier bitmap
+__global__ __launch_bounds__(kB) void k_bu(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
+                                           const uint64_t* __restrict__ off, uint32_t nV, int32_t L,
+                                           const uint32_t* __restrict__ fbits, int32_t* __restrict__ level,
+                                           uint32_t* __restrict__ nbits, uint32_t* __restrict__ ncount,
+                                           unsigned long long* __restrict__ ndeg) {
+  uint32_t cnt = 0;
+  uint64_t dsum = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    if (level[v] >= 0) continue;
+    const uint64_t b = coff[v], e = coff[v + 1];
+    for (uint64_t p = b; p < e; ++p) {
+      const uint32_t w = cidx[p];
+      if ((fbits[w >> 5] >> (w & 31)) & 1u) {
+        level[v] = L + 1;
+        atomicOr(nbits + (v >> 5), 1u << (v & 31));
+        ++cnt;
+        dsum += off[v + 1] - off[v];
+        break;
+      }
+    }
+  }
+  cnt = wave_sum(cnt);
+  dsum = wave_sum(dsum);
+  if ((threadIdx.x & 63) == 0) {
+    if (cnt) atomicAdd(ncount, cnt);
+    if (dsum) atomicAdd(ndeg, (unsigned long long)dsum);
+  }
+}
+
+// bitmap -> queue (order within the queue does not matter to BFS levels)
+__global__ void k_bits_to_queue(const uint32_t* __restrict__ bits, uint32_t nwords, uint32_t* __restrict__ q,
+                                uint32_t* __restrict__ qn) {
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t m = bits[w];
+    if (!m) continue;
+    const uint32_t p = atomicAdd(qn, (uint32_t)__popc(m));
+    uint32_t i = 0;
+    while (m) {
+      const int b = __ffs(m) - 1;
+      q[p + i++] = (uint32_t)(w * 32 + b);
+      m &= m - 1;
+    }
+  }
+}
+
+__global__ void k_reached_edges(const int32_t* __restrict__ level, const uint64_t* __restrict__ off, uint32_t nV,
+                                unsigned long long* __restrict__ out, int32_t* __restrict__ depth) {
+  uint64_t s = 0;
+  int32_t dmax = -1;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    if (level[v] >= 0) { s += off[v + 1] - off[v]; dmax = level[v] > dmax ? level[v] : dmax; }
+  }
+  s = wave_sum(s);
+  for (int o = 32; o > 0; o >>= 1) { const int32_t t = __shfl_xor(dmax, o, 64); dmax = t > dmax ? t : dmax; }
+  if ((threadIdx.x & 63) == 0) {
+    if (s) atomicAdd(out, (unsigned long long)s);
+    atomicMax(depth, dmax);
+  }
+}
+
+struct BfsBufs {
+  DevBuf q[2], qdeg, qpos, bits[2], cnt, deg;
+  uint32_t nV = 0;
+};
+BfsBufs& bfs_bufs(uint32_t nV) {
+  static BfsBufs& b = *new BfsBufs();  // lives until the process ends
+  if (b.nV < nV || !b.q[0].p) {
+    b.q[0].alloc(4 * (size_t)nV);
+    b.q[1].alloc(4 * (size_t)nV);
+    b.qpos.alloc(8 * ((size_t)nV + 1));
+    b.qdeg.alloc(8 * ((size_t)nV + 1));
+    const size_t nw = ((size_t)nV + 63) / 64 * 2;
+    b.bits[0].alloc(4 * nw);
+    b.bits[1].alloc(4 * nw);
+    b.cnt.alloc(64);
+    b.deg.alloc(64);
+    b.nV = nV;
+  }
+  return b;
+}
+
+void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, int32_t* depth_out) {
+  hipStream_t s = stream();
+  const uint32_t nV = g->nV;
+  BfsBufs& b = bfs_bufs(nV);
+  const uint32_t nwords = (uint32_t)(((uint64_t)nV + 63) / 64 * 2);
+  const uint64_t* off = g->off.as<uint64_t>();
+  const uint64_t* coff = g->coff.as<uint64_t>();
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap(nV)), dim3(kB), 0, s, level, (uint64_t)nV, -1);
+  GRE_HIP(hipMemsetAsync(b.bits[0].p, 0, 4 * (size_t)nwords, s));
+  hipLaunchKernelGGL(k_bfs_seed, dim3(1), dim3(1), 0, s, level, src, b.q[0].as<uint32_t>(), b.bits[0].as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+  uint64_t dsrc = 0;
+  {
+    uint64_t o2[2];
+    GRE_HIP(hipMemcpyAsync(o2, off + src, 16, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    dsrc = o2[1] - o2[0];
+  }
+  uint32_t qn = 1;
+  uint64_t mf = dsrc;                 // edges out of the frontier
+  uint64_t mu = g->nE - dsrc;         // edges of unvisited vertices (approx.)
+  int cur = 0;
+  bool topdown = true;
+  int32_t L = 0;
+  bool queue_valid = true;            // q[cur] holds the frontier
+  const double alpha = 14.0, beta = 24.0;
+  uint32_t prev_n = 0;
+  while (qn > 0) {
+    const int nxt = cur ^ 1;
+    if (topdown && (double)mf > (double)mu / alpha && qn > prev_n) topdown = false;
+    else if (!topdown && (double)qn < (double)nV / beta && qn < prev_n) topdown = true;
+    GRE_HIP(hipMemsetAsync(b.bits[nxt].p, 0, 4 * (size_t)nwords, s));
+    GRE_HIP(hipMemsetAsync(b.cnt.p, 0, 8, s));
+    GRE_HIP(hipMemsetAsync(b.deg.p, 0, 8, s));
+    if (topdown) {
+      if (!queue_valid) {
+        GRE_HIP(hipMemsetAsync(b.cnt.p, 0, 4, s));
+        hipLaunchKernelGGL(k_bits_to_queue, dim3(gcap(nwords)), dim3(kB), 0, s, b.bits[cur].as<uint32_t>(), nwords,
+                           b.q[cur].as<uint32_t>(), b.cnt.as<uint32_t>());
+        GRE_HIP(hipMemsetAsync(b.cnt.p, 0, 4, s));
+      }
+      hipLaunchKernelGGL(k_qdeg, dim3(gcap((uint64_t)qn + 1)), dim3(kB), 0, s, b.q[cur].as<uint32_t>(), qn, off,
+                         b.qdeg.as<uint64_t>());
+      scan_excl_u64(b.qdeg.as<uint64_t>(), b.qpos.as<uint64_t>(), (size_t)qn + 1, s);
+      uint64_t T = 0;
+      GRE_HIP(hipMemcpyAsync(&T, b.qpos.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      if (T > 0) {
+        ProfScope ps("bfs_topdown");
+        hipLaunchKernelGGL(k_td, dim3((unsigned)((T + kEB - 1) / kEB)), dim3(kB), 0, s, off, g->idx.as<uint32_t>(),
+                           b.q[cur].as<uint32_t>(), b.qpos.as<uint64_t>(), qn, T, L, level, b.q[nxt].as<uint32_t>(),
+                           b.cnt.as<uint32_t>(), b.bits[nxt].as<uint32_t>(), (unsigned long long*)b.deg.p);
+      }
+      queue_valid = true;
+    } else {
+      ProfScope ps("bfs_bottomup");
+      hipLaunchKernelGGL(k_bu, dim3(gcap(nV)), dim3(kB), 0, s, coff, g->cidx.as<uint32_t>(), off, nV, L,
+                         b.bits[cur].as<uint32_t>(), level, b.bits[nxt].as<uint32_t>(), b.cnt.as<uint32_t>(),
+                         (unsigned long long*)b.deg.p);
+      queue_valid = false;
+    }
+    GRE_HIP(hipGetLastError());
+    uint32_t nn = 0;
+    uint64_t nd = 0;
+    GRE_HIP(hipMemcpyAsync(&nn, b.cnt.p, 4, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipMemcpyAsync(&nd, b.deg.p, 8, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    prev_n = qn;
+    qn = nn;
+    mf = nd;
+    mu = mu > nd ? mu - nd : 0;
+    cur = nxt;
+    ++L;
+  }
+  DevBuf tmp;
+  tmp.alloc(16);
+  GRE_HIP(hipMemsetAsync(tmp.p, 0, 8, s));
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(1), 0, s, (int32_t*)((char*)tmp.p + 8), (uint64_t)1, -1);
+  hipLaunchKernelGGL(k_reached_edges, dim3(gcap(nV)), dim3(kB), 0, s, level, off, nV, (unsigned long long*)tmp.p,
+                     (int32_t*)((char*)tmp.p + 8));
+  GRE_HIP(hipGetLastError());
+  uint64_t h[2];
+  GRE_HIP(hipMemcpyAsync(h, tmp.p, 16, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  if (edges_out) *edges_out = h[0];
+  if (depth_out) *depth_out = (int32_t)(h[1] & 0xffffffffULL);
+}
+
+// ---------------------------------------------------------------------------
+// multi-source BFS: 64 sources as bits of one mask per vertex
+// ---------------------------------------------------------------------------
+__global__ void k_ms_seed(const uint32_t* __restrict__ src, int n, unsigned long long* vis, unsigned long long* fr,
+                          int32_t* lev, uint32_t nV) {
+  for (int i = 0; i < n; ++i) {
+    vis[src[i]] |= 1ULL << i;
+    fr[src[i]] |= 1ULL << i;
+    if (lev) lev[(uint64_t)i * nV + src[i]] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_ms_level(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
+                                                 uint32_t nV, unsigned long long full,
+                                                 const unsigned long long* __restrict__ fr,
+                                                 unsigned long long* __restrict__ nfr,
+                                                 unsigned long long* __restrict__ vis, int32_t* __restrict__ lev,
+                                                 int32_t L, unsigned int* __restrict__ any) {
+  bool changed = false;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long seen = vis[v];
+    unsigned long long m = 0;
+    if (seen != full) {
+      const uint64_t b = coff[v], e = coff[v + 1];
+      for (uint64_t p = b; p < e; ++p) {
+        m |= fr[cidx[p]];
+        if ((m | seen) == full) break;
+      }
+      m &= ~seen;
+    }
+    nfr[v] = m;
+    if (m) {
+      vis[v] = seen | m;
+      changed = true;
+      if (lev) {
+        unsigned long long t = m;
+        while (t) {
+          const int i = __ffsll(t) - 1;
+          lev[(uint64_t)i * nV + v] = L + 1;
+          t &= t - 1;
+        }
+      }
+    }
+  }
+  if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
+}
+
+__global__ void k_ms_edges(const unsigned long long* __restrict__ vis, const uint64_t* __restrict__ off, uint32_t nV,
+                           unsigned long long* __restrict__ out) {
+  uint64_t s = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x)
+    s += (off[v + 1] - off[v]) * (uint64_t)__popcll(vis[v]);
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------------------
+// Brandes BC, 64 sources per chunk, one wave per vertex (lane = source)
+// ---------------------------------------------------------------------------
+// forward level L -> L+1: sigma_v[j] = sum over in-neighbours w at level L of sigma_w[j]
+__global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
+                                               uint32_t nV, int nsrc, const unsigned long long* __restrict__ fr,
+                                               unsigned long long* __restrict__ nfr,
+                                               unsigned long long* __restrict__ vis, double* __restrict__ sigma,
+                                               uint8_t* __restrict__ lev, int32_t L, unsigned int* __restrict__ any) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  bool changed = false;
+  for (uint64_t v = wid; v < nV; v += nw) {
+    const unsigned long long seen = vis[v];
+    if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
+    double acc = 0.0;
+    unsigned long long got = 0;
+    const uint64_t b = coff[v], e = coff[v + 1];
+    for (uint64_t p = b; p < e; ++p) {
+      const uint32_t w = cidx[p];
+      const unsigned long long f = fr[w] & ~seen;
+      if (!f) continue;
+      got |= f;
+      if ((f >> lane) & 1ULL) acc += sigma[(uint64_t)w * 64 + lane];
+    }
+    if (lane == 0) nfr[v] = got;
+    if (got) {
+      if ((got >> lane) & 1ULL) {
+        sigma[(uint64_t)v * 64 + lane] = acc;
+        lev[(uint64_t)v * 64 + lane] = (uint8_t)(L + 1);
+      }
+      if (lane == 0) vis[v] = seen | got;
+      changed = true;
+    }
+  }
+  if (__any(changed) && lane == 0) atomicOr(any, 1u);
+}
+
+// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of (1+delta_w[j])/sigma_w[j]
+__global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                               uint32_t nV, const double* __restrict__ sigma,
+                                               double* __restrict__ delta, const uint8_t* __restrict__ lev,
+                                               const unsigned long long* __restrict__ atl, int32_t L) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t v = wid; v < nV; v += nw) {
+    const unsigned long long a = atl[v];
+    if (!a) continue;
+    const bool mine = (a >> lane) & 1ULL;
+    double acc = 0.0;
+    const uint64_t b = off[v], e = off[v + 1];
+    for (uint64_t p = b; p < e; ++p) {
+      const uint32_t w = idx[p];
+      const uint64_t o = (uint64_t)w * 64 + lane;
+      if (mine && lev[o] == (uint8_t)(L + 1)) acc += (1.0 + delta[o]) / sigma[o];
+    }
+    if (mine) delta[(uint64_t)v * 64 + lane] = sigma[(uint64_t)v * 64 + lane] * acc;
+  }
+}
+
+// mask of lanes whose level is L
+__global__ void k_bc_mask(uint32_t nV, const uint8_t* __restrict__ lev, int32_t L, unsigned long long* __restrict__ m) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t v = wid; v < nV; v += nw) {
+    const unsigned long long b = __ballot(lev[(uint64_t)v * 64 + lane] == (uint8_t)L);
+    if (lane == 0) m[v] = b;
+  }
+}
+
+// bc[v] += sum over lanes of delta (source itself excluded); optional copies out
+__global__ void k_bc_accum(uint32_t nV, int nsrc, const uint32_t* __restrict__ src, const double* __restrict__ delta,
+                           double* __restrict__ bc) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t v = wid; v < nV; v += nw) {
+    double d = (lane < nsrc && src[lane] != v) ? delta[(uint64_t)v * 64 + lane] : 0.0;
+    d = wave_sum(d);
+    if (lane == 0) bc[v] += d;
+  }
+}
+
+__global__ void k_bc_init(uint32_t nV, const uint32_t* __restrict__ src, int nsrc, double* __restrict__ sigma,
+                          double* __restrict__ delta, uint8_t* __restrict__ lev, unsigned long long* __restrict__ vis,
+                          unsigned long long* __restrict__ fr) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * 64;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    sigma[i] = 0.0;
+    delta[i] = 0.0;
+    lev[i] = 0xff;
+    if ((i & 63) == 0) { vis[i >> 6] = 0; fr[i >> 6] = 0; }
+  }
+}
+
+__global__ void k_bc_seed(const uint32_t* __restrict__ src, int nsrc, double* sigma, uint8_t* lev,
+                          unsigned long long* vis, unsigned long long* fr) {
+  for (int j = 0; j < nsrc; ++j) {
+    const uint64_t o = (uint64_t)src[j] * 64 + j;
+    sigma[o] = 1.0;
+    lev[o] = 0;
+    vis[src[j]] |= 1ULL << j;
+    fr[src[j]] |= 1ULL << j;
+  }
+}
+
+// lane-major (v*64 + j) -> source-major (j*nV + v)
+template <typename T, typename U>
+__global__ void k_bc_export(uint32_t nV, int nsrc, const T* __restrict__ in, U* __restrict__ out, bool lev8) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * nsrc;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t j = i / nV, v = i % nV;
+    const T x = in[v * 64 + j];
+    if (lev8) out[i] = (U)(x == (T)0xff ? -1 : (int)x);
+    else out[i] = (U)x;
+  }
+}
+
+}  // namespace
+}  // namespace gre
+
+using namespace gre;
+
+extern "C" {
+
+int gre_bfs_dev(gre_graph* g, uint32_t src, int32_t* level_dev, uint64_t* edges_out, int32_t* depth_out) {
+  return guard([&] {
+    GRE_CHECK(g, GRE_DEV_NULL, "null graph");
+    GRE_CHECK(src < g->nV, GRE_DEV_NULL, "source id >= nV");
+    static DevBuf& own = *new DevBuf();  // lives until the process ends
+    int32_t* lev = level_dev;
+    if (!lev) {
+      if (own.bytes < 4 * (size_t)g->nV) own.alloc(4 * (size_t)g->nV);
+      lev = own.as<int32_t>();
+    }
+    bfs_run(g, src, lev, edges_out, depth_out);
+  });
+}
+
+int gre_bfs(gre_graph* g, uint32_t src, int32_t* level_out, uint64_t* edges_out, int32_t* depth_out) {
+  return guard([&] {
+    GRE_CHECK(g, GRE_DEV_NULL, "null graph");
+    GRE_CHECK(src < g->nV, GRE_DEV_NULL, "source id >= nV");
+    DevBuf lev;
+    lev.alloc(4 * (size_t)g->nV);
+    bfs_run(g, src, lev.as<int32_t>(), edges_out, depth_out);
+    if (level_out) {
+      GRE_HIP(hipMemcpyAsync(level_out, lev.p, 4 * (size_t)g->nV, hipMemcpyDeviceToHost, stream()));
+      GRE_HIP(hipStreamSynchronize(stream()));
+    }
+  });
+}
+
+int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, uint64_t* edges_out) {
+  return guard([&] {
+    GRE_CHECK(g && srcs, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(nsrc >= 1 && nsrc <= 64, GRE_DEV_NULL, "nsrc must be in [1, 64]");
+    for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
+    hipStream_t s = stream();
+    const uint32_t nV = g->nV;
+    DevBuf vis, fr[2], lev, dsrc, flag;
+    vis.alloc(8 * (size_t)nV);
+    fr[0].alloc(8 * (size_t)nV);
+    fr[1].alloc(8 * (size_t)nV);
+    dsrc.alloc(4 * (size_t)nsrc);
+    flag.alloc(16);
+    if (level_out) {
+      lev.alloc(4 * (size_t)nV * nsrc);
+      hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
+                         (uint64_t)nV * nsrc, -1);
+    }
+    GRE_HIP(hipMemsetAsync(vis.p, 0, vis.bytes, s));
+    GRE_HIP(hipMemsetAsync(fr[0].p, 0, fr[0].bytes, s));
+    GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(1), 0, s, dsrc.as<uint32_t>(), nsrc, (unsigned long long*)vis.p,
+                       (unsigned long long*)fr[0].p, level_out ? lev.as<int32_t>() : nullptr, nV);
+    const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+    int cur = 0;
+    for (int32_t L = 0;; ++L) {
+      GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
+      {
+        ProfScope ps("msbfs_level");
+        hipLaunchKernelGGL(k_ms_level, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
+                           nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[cur ^ 1].p,
+                           (unsigned long long*)vis.p, level_out ? lev.as<int32_t>() : nullptr, L,
+                           flag.as<unsigned int>());
+      }
+      GRE_HIP(hipGetLastError());
+      uint32_t any = 0;
+      GRE_HIP(hipMemcpyAsync(&any, flag.p, 4, hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      cur ^= 1;
+      if (!any) break;
+      GRE_CHECK(L < (int32_t)nV + 1, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+    }
+    if (edges_out) {
+      GRE_HIP(hipMemsetAsync(flag.p, 0, 8, s));
+      hipLaunchKernelGGL(k_ms_edges, dim3(gcap(nV)), dim3(kB), 0, s, (const unsigned long long*)vis.p,
+                         g->off.as<uint64_t>(), nV, (unsigned long long*)flag.p);
+      GRE_HIP(hipMemcpyAsync(edges_out, flag.p, 8, hipMemcpyDeviceToHost, s));
+    }
+    if (level_out)
+      GRE_HIP(hipMemcpyAsync(level_out, lev.p, 4 * (size_t)nV * nsrc, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double* sigma_out, double* delta_out,
+           int32_t* level_out) {
+  return guard([&] {
+    GRE_CHECK(g && srcs && nsrc >= 1, GRE_DEV_NULL, "bad argument");
+    for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
+    hipStream_t s = stream();
+    const uint32_t nV = g->nV;
+    const uint64_t n64 = (uint64_t)nV * 64;
+    DevBuf sigma, delta, lev, vis, fr[2], msk, bc, dsrc, flag;
+    sigma.alloc(8 * n64);
+    delta.alloc(8 * n64);
+    lev.alloc(n64);
+    vis.alloc(8 * (size_t)nV);
+    fr[0].alloc(8 * (size_t)nV);
+    fr[1].alloc(8 * (size_t)nV);
+    msk.alloc(8 * (size_t)nV);
+    bc.alloc(8 * (size_t)nV);
+    dsrc.alloc(4 * (size_t)nsrc);
+    flag.alloc(8);
+    GRE_HIP(hipMemsetAsync(bc.p, 0, bc.bytes, s));
+    GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+    const unsigned gw = (unsigned)std::min<uint64_t>(((uint64_t)nV * 64 + kB - 1) / kB, 1u << 20);
+    for (int c0 = 0; c0 < nsrc; c0 += 64) {
+      const int nc = std::min(64, nsrc - c0);
+      const uint32_t* cs = dsrc.as<uint32_t>() + c0;
+      hipLaunchKernelGGL(k_bc_init, dim3(gcap(n64)), dim3(kB), 0, s, nV, cs, nc, sigma.as<double>(), delta.as<double>(),
+                         lev.as<uint8_t>(), (unsigned long long*)vis.p, (unsigned long long*)fr[0].p);
+      hipLaunchKernelGGL(k_bc_seed, dim3(1), dim3(1), 0, s, cs, nc, sigma.as<double>(), lev.as<uint8_t>(),
+                         (unsigned long long*)vis.p, (unsigned long long*)fr[0].p);
+      GRE_HIP(hipGetLastError());
+      int cur = 0;
+      int32_t L = 0;
+      for (;; ++L) {
+        GRE_CHECK(L < 250, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
+        GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
+        {
+          ProfScope ps("bc_forward");
+          hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), nV, nc,
+                             (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[cur ^ 1].p,
+                             (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
+                             flag.as<unsigned int>());
+        }
+        GRE_HIP(hipGetLastError());
+        uint32_t any = 0;
+        GRE_HIP(hipMemcpyAsync(&any, flag.p, 4, hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        cur ^= 1;
+        if (!any) break;
+      }
+      // deepest level reached is L; dependencies from L-1 down to 0
+      for (int32_t l = L - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l, (unsigned long long*)msk.p);
+        ProfScope ps("bc_backward");
+        hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), nV,
+                           sigma.as<double>(), delta.as<double>(), lev.as<uint8_t>(), (const unsigned long long*)msk.p,
+                           l);
+      }
+      hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, nV, nc, cs, delta.as<double>(), bc.as<double>());
+      GRE_HIP(hipGetLastError());
+      if (sigma_out || delta_out || level_out) {
+        DevBuf tmp;
+        tmp.alloc(8 * (size_t)nV * nc);
+        auto dl = [&](auto* dev, auto* host, bool lev8, size_t esz) {
+          using T = std::remove_pointer_t<decltype(dev)>;
+          using U = std::remove_pointer_t<decltype(host)>;
+          hipLaunchKernelGGL((k_bc_export<T, U>), dim3(gcap((uint64_t)nV * nc)), dim3(kB), 0, s, nV, nc, dev,
+                             (U*)tmp.p, lev8);
+          GRE_HIP(hipMemcpyAsync(host + (size_t)c0 * nV, tmp.p, esz * (size_t)nV * nc, hipMemcpyDeviceToHost, s));
+          GRE_HIP(hipStreamSynchronize(s));
+        };
+        if (sigma_out) dl(sigma.as<double>(), sigma_out, false, 8);
+        if (delta_out) dl(delta.as<double>(), delta_out, false, 8);
+        if (level_out) dl(lev.as<uint8_t>(), level_out, true, 4);
+      }
+    }
+    if (bc_out) GRE_HIP(hipMemcpyAsync(bc_out, bc.p, 8 * (size_t)nV, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

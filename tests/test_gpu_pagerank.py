@@ -1,0 +1,200 @@
+"""GPU parity: PageRank / personalised PageRank forward, adjoint and gradients
+through the C ABI, against the oracle (FP64) and finite differences.
+
+Tolerances (relative L1 unless stated): FP64 device vs FP64 oracle 1e-12;
+FP32 device vs FP64 oracle 1e-6 (the stated bar for PageRank); gradients in
+FP32 storage 1e-5.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l1(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+@pytest.mark.parametrize("tile_items", [4, 8, 16])
+def test_pagerank_f64_matches_oracle(N, O, tile_items):
+    g = N.Graph.rmat(16, 16, 1, False)
+    og = O.rmat_graph(16, 16, 1, False)
+    pr = N.PageRank(g, N.F64)
+    pr.set_schedule(1, tile_items)
+    it, res = pr.run(0.85, 20)
+    x = pr.ranks()
+    ref = O.pagerank(og, 0.85, 20)
+    assert it == 20
+    assert rel_l1(x, ref) < 1e-12
+    assert abs(x.sum() - 1.0) < 1e-12
+
+
+def test_pagerank_f32_within_1e6_of_f64(N, O):
+    g = N.Graph.rmat(18, 16, 6, False)
+    og = O.rmat_graph(18, 16, 6, False)
+    pr = N.PageRank(g, N.F32)
+    pr.run(0.85, 20)
+    ref = O.pagerank(og, 0.85, 20)
+    assert rel_l1(pr.ranks(), ref) < 1e-6
+
+
+def test_pagerank_autotuned_is_deterministic(N, O):
+    g = N.Graph.rmat(15, 16, 3, False)
+    pr = N.PageRank(g, N.F64)
+    pr.run(0.85, 20)
+    a = pr.ranks()
+    pr.run(0.85, 20)
+    b = pr.ranks()
+    assert np.array_equal(a, b)
+
+
+def test_pagerank_edge_cases(N, O):
+    # a graph with isolated vertices, a long hub row that spans many tiles, and empty tail rows
+    n = 5000
+    hub_src = np.zeros(4000, np.uint32)
+    hub_dst = np.arange(1, 4001, dtype=np.uint32)
+    into_hub = np.arange(1, 3001, dtype=np.uint32)
+    src = np.concatenate([hub_src, into_hub, np.array([10, 11], np.uint32)])
+    dst = np.concatenate([hub_dst, np.zeros(3000, np.uint32), np.array([11, 10], np.uint32)])
+    g = N.Graph.from_edges(n, src, dst)
+    og = O.graph_from_edges(n, src, dst)
+    for ti in (4, 8, 16):
+        pr = N.PageRank(g, N.F64)
+        pr.set_schedule(1, ti)
+        pr.run(0.85, 20)
+        assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
+
+
+def ppr_problem(O, og, seed=5, nseeds=50):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(og.nV)
+    p[rng.choice(og.nV, nseeds, replace=False)] = 1.0 / nseeds
+    return p
+
+
+def test_weighted_ppr_adjoint_gradients_match_oracle(N, O):
+    scale = 13
+    g = N.Graph.rmat(scale, 16, 5, False)
+    og = O.rmat_graph(scale, 16, 5, False)
+    d = 0.85
+    pr = N.PageRank(g, N.F64, 1, weighted=True)
+    pr.set_edge_weights(None, seed=5)
+    wraw, wnorm = pr.edge_weights()
+    # device lognormal weights follow the oracle's definition
+    src = np.repeat(np.arange(og.nV), og.outdeg)
+    for e in range(0, og.nE, max(1, og.nE // 50)):
+        assert abs(wraw[e] - O.lognormal_weight(5, int(src[e]), int(og.idx[e]))) <= 1e-6 * wraw[e]
+    wr = wraw.astype(np.float64)
+    rs = np.bincount(src, weights=wr, minlength=og.nV)
+    wn = np.where(rs[src] > 0, wr / rs[src], 0.0)
+    assert np.allclose(wnorm, wn, rtol=1e-6)
+    p = ppr_problem(O, og)
+    pr.set_personalization(p)
+    it, res = pr.run(d, 2000, 1e-13)
+    x = pr.ranks()
+    wc = wnorm.astype(np.float64)[og.ceid.astype(np.int64)]
+    xref, _ = O.ppr(og, wc, p, d, tol=1e-15, maxit=5000)
+    assert rel_l1(x, xref) < 1e-11
+    tg = np.random.default_rng(1).choice(og.nV, 64, replace=False).astype(np.uint32)
+    pr.adjoint(tg, d, 2000, 1e-13)
+    lam = pr.adjoint_vector()
+    c = np.zeros(og.nV)
+    c[tg] = 1.0
+    lref, _ = O.ppr_adjoint(og, wnorm.astype(np.float64), p, c, d, tol=1e-15, maxit=5000)
+    assert rel_l1(lam, lref) < 1e-11
+    gW, graw, gp = pr.gradients(d)
+    oW, oraw = O.ppr_edge_grad(og, wnorm.astype(np.float64), rs, xref, lref, d)
+    assert rel_l1(gW, oW) < 1e-5
+    assert rel_l1(graw, oraw) < 1e-5
+    gpref = ((1 - d) + d * xref[og.outdeg == 0].sum()) * lref
+    assert rel_l1(gp, gpref) < 1e-11
+
+
+def test_ppr_f32_forward_with_f64_residual(N, O):
+    scale = 14
+    g = N.Graph.rmat(scale, 16, 5, False)
+    og = O.rmat_graph(scale, 16, 5, False)
+    pr = N.PageRank(g, N.F32, 1, weighted=True)
+    pr.set_edge_weights(None, seed=5)
+    _, wnorm = pr.edge_weights()
+    p = ppr_problem(O, og)
+    pr.set_personalization(p)
+    it, res = pr.run(0.85, 1000, 1e-7)
+    assert res < 1e-7 and it < 1000
+    wc = wnorm.astype(np.float64)[og.ceid.astype(np.int64)]
+    xref, _ = O.ppr(og, wc, p, 0.85, tol=1e-15, maxit=5000)
+    assert rel_l1(pr.ranks(), xref) < 1e-6
+
+
+def test_gradient_central_differences_f64(N, O):
+    """d(loss)/d(raw edge weight) against central differences of an FP64 forward (h = 1e-3 relative)."""
+    scale = 11
+    g = N.Graph.rmat(scale, 16, 5, False)
+    og = O.rmat_graph(scale, 16, 5, False)
+    d = 0.85
+    pr = N.PageRank(g, N.F64, 1, weighted=True)
+    pr.set_edge_weights(None, seed=5)
+    wraw, _ = pr.edge_weights()
+    p = ppr_problem(O, og, nseeds=20)
+    pr.set_personalization(p)
+    pr.run(d, 5000, 1e-15)
+    tg = np.random.default_rng(2).choice(og.nV, 32, replace=False).astype(np.uint32)
+    pr.adjoint(tg, d, 5000, 1e-15)
+    _, graw, _ = pr.gradients(d)
+    rng = np.random.default_rng(3)
+    edges = rng.choice(og.nE, 16, replace=False)
+    fd_pr = N.PageRank(g, N.F64, 1, weighted=True)
+    fd_pr.set_personalization(p)
+    for e in edges:
+        h = 1e-3 * float(wraw[e])
+        vals = []
+        for sgn in (+1, -1):
+            w2 = wraw.astype(np.float64).copy()
+            w2[e] += sgn * h
+            fd_pr.set_edge_weights(w2.astype(np.float32))
+            fd_pr.run(d, 5000, 1e-15)
+            vals.append(fd_pr.ranks()[tg].sum())
+        # the perturbation is applied to float32 weights: use the float32 step
+        w_hi = np.float32(wraw[e] + h)
+        w_lo = np.float32(wraw[e] - h)
+        fd = (vals[0] - vals[1]) / (float(w_hi) - float(w_lo))
+        assert abs(fd - graw[e]) <= 2e-4 * max(abs(fd), 1e-9) + 1e-9
+
+
+@pytest.mark.parametrize("k,dtype", [(4, "f64"), (32, "f32")])
+def test_batched_ppr_equals_single_vectors(N, O, k, dtype):
+    scale = 13
+    g = N.Graph.rmat(scale, 16, 6, False)
+    og = O.rmat_graph(scale, 16, 6, False)
+    dt = N.F64 if dtype == "f64" else N.F32
+    npd = np.float64 if dtype == "f64" else np.float32
+    rng = np.random.default_rng(7)
+    P = np.zeros((og.nV, k), npd)
+    for j in range(k):
+        P[rng.choice(og.nV, 100, replace=False), j] = 1.0 / 100
+    bp = N.PageRank(g, dt, k, weighted=True)
+    bp.set_edge_weights(None, seed=6)
+    bp.set_personalization(P)
+    bp.run(0.85, 30)
+    X = bp.ranks()
+    tg = rng.choice(og.nV, 128, replace=False).astype(np.uint32)
+    bp.adjoint(tg, 0.85, 30)
+    LAM = bp.adjoint_vector()
+    gW, graw, gp = bp.gradients(0.85)
+    sp = N.PageRank(g, dt, 1, weighted=True)
+    sp.set_edge_weights(None, seed=6)
+    gW_sum = np.zeros(og.nE)
+    graw_sum = np.zeros(og.nE)
+    tol = 1e-12 if dtype == "f64" else 2e-6
+    for j in range(k):
+        sp.set_personalization(np.ascontiguousarray(P[:, j]))
+        sp.run(0.85, 30)
+        assert rel_l1(X[:, j], sp.ranks().astype(np.float64)) < tol
+        sp.adjoint(tg, 0.85, 30)
+        assert rel_l1(LAM[:, j], sp.adjoint_vector().astype(np.float64)) < tol
+        a, b, c = sp.gradients(0.85)
+        gW_sum += a
+        graw_sum += b
+        assert rel_l1(gp[:, j], c.astype(np.float64)) < tol
+    assert rel_l1(gW, gW_sum) < 1e-5
+    assert rel_l1(graw, graw_sum) < 1e-4

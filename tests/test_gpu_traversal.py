@@ -1,0 +1,84 @@
+"""GPU parity: direction-optimising BFS, 64-source bitmask BFS and batched FP64
+Brandes BC through the C ABI, against the oracle.  Levels are bit-exact; BC
+dependencies and path counts agree to 1e-9 relative (path counts are exact
+integers in FP64)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def graphs(N, O, scale, ef, seed, sym=True):
+    return N.Graph.rmat(scale, ef, seed, sym), O.rmat_graph(scale, ef, seed, sym)
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 2), (18, 2)])
+def test_bfs_levels_bit_exact(N, O, scale, seed):
+    g, og = graphs(N, O, scale, 16, seed)
+    rng = np.random.default_rng(seed)
+    lcc = O.cc_labels(og)
+    big = np.bincount(lcc).argmax()
+    cand = np.nonzero(lcc == big)[0]
+    for src in list(rng.choice(cand, 4, replace=False)) + [int(np.argmax(og.outdeg))]:
+        lev, edges, depth = N.bfs(g, int(src))
+        ref = O.bfs_levels(og, int(src))
+        assert np.array_equal(lev, ref)
+        assert edges == int(og.outdeg[ref >= 0].sum())
+        assert depth == ref.max()
+
+
+def test_bfs_directed_and_isolated_source(N, O):
+    g, og = graphs(N, O, 14, 16, 7, sym=False)
+    for src in (0, int(np.nonzero(og.outdeg == 0)[0][0])):
+        lev, edges, depth = N.bfs(g, src)
+        assert np.array_equal(lev, O.bfs_levels(og, src))
+
+
+def test_bfs_graph500_validation_large(N, O):
+    """Size-independent checks at scale 20: tree-edge and |dlevel| <= 1 rules."""
+    g, og = graphs(N, O, 20, 16, 2)
+    src = int(np.argmax(og.outdeg))
+    lev, edges, depth = N.bfs(g, src)
+    srcs = np.repeat(np.arange(og.nV), og.outdeg)
+    lu, lv = lev[srcs], lev[og.idx]
+    both = (lu >= 0) & (lv >= 0)
+    assert np.all(np.abs(lu[both] - lv[both]) <= 1)
+    assert np.all((lu >= 0) == (lv >= 0))              # reached set closed under edges
+    # every reached non-source vertex has a neighbour one level up
+    has_parent = np.zeros(og.nV, bool)
+    np.logical_or.at(has_parent, og.idx[(lv == lu + 1) & (lu >= 0)], True)
+    reached = lev > 0
+    assert np.all(has_parent[reached])
+    assert np.array_equal(lev, O.bfs_levels(og, src))
+
+
+@pytest.mark.parametrize("nsrc", [1, 17, 64])
+def test_msbfs_equals_single_source(N, O, nsrc):
+    g, og = graphs(N, O, 14, 16, 4)
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.choice(og.nV, nsrc, replace=False).astype(np.uint32)
+    lev, edges = N.msbfs(g, srcs)
+    total = 0
+    for i, s in enumerate(srcs):
+        ref = O.bfs_levels(og, int(s))
+        assert np.array_equal(lev[i], ref)
+        total += int(og.outdeg[ref >= 0].sum())
+    assert edges == total
+
+
+@pytest.mark.parametrize("nsrc", [1, 5, 70])
+def test_bc_matches_brandes(N, O, nsrc):
+    g, og = graphs(N, O, 12, 16, 4)
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.choice(og.nV, nsrc, replace=False).astype(np.uint32)
+    bc, sig, dl, lv = N.bc(g, srcs, want_parts=True)
+    acc = np.zeros(og.nV)
+    for i, s in enumerate(srcs):
+        osig, odl, olv = O.brandes_one(og, int(s))
+        assert np.array_equal(lv[i], olv)
+        assert np.allclose(sig[i], osig, rtol=1e-12, atol=0)
+        assert np.allclose(dl[i], odl, rtol=1e-9, atol=1e-12)
+        odl = odl.copy()
+        odl[s] = 0.0
+        acc += odl
+    assert np.abs(bc - acc).sum() <= 1e-9 * np.abs(acc).sum()
