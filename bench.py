@@ -50,6 +50,16 @@ def parse():
     return ap.parse_args()
 
 
+def sample_distinct(rng, pool, k):
+    """k distinct values drawn uniformly from pool (an int range size or an array)."""
+    n = pool if isinstance(pool, int) else len(pool)
+    got = np.unique(rng.integers(0, n, 2 * k + 16))
+    while len(got) < k:
+        got = np.unique(np.concatenate([got, rng.integers(0, n, 2 * k + 16)]))
+    pick = rng.permutation(got)[:k]
+    return pick if isinstance(pool, int) else np.asarray(pool)[pick]
+
+
 def pmc_traffic(scale):
     """HBM bytes per pull-tile launch from the committed rocprofv3 --pmc summary, if present."""
     f = os.path.join(ROOT, "profiles", f"pull_tile_pagerank_f32_s{scale}_pmc.txt")
@@ -136,8 +146,8 @@ def main():
         rng = np.random.default_rng(6)
         P = np.zeros((V, k), np.float32)
         for j in range(k):
-            P[rng.choice(V, 100, replace=False), j] = 1.0 / 100
-        tg = rng.choice(V, 1024, replace=False).astype(np.uint32)
+            P[sample_distinct(rng, V, 100), j] = 1.0 / 100
+        tg = sample_distinct(rng, V, 1024).astype(np.uint32)
         bp = N.PageRank(g, N.F32, k, weighted=True)
         bp.set_edge_weights(None, seed=6)
         bp.set_personalization(P)
@@ -161,7 +171,7 @@ def main():
         sp = N.PageRank(g, N.F32, 1, weighted=True)
         sp.set_edge_weights(None, seed=6)
         p1 = np.zeros(V, np.float32)
-        p1[rng.choice(V, 100, replace=False)] = 1.0 / 100
+        p1[sample_distinct(rng, V, 100)] = 1.0 / 100
         sp.set_personalization(p1)
         sp.run(0.85, 2)
         N.sync()
@@ -193,9 +203,9 @@ def main():
         t0 = time.time()
         gs = N.Graph.rmat(S, 16, 6, True)
         log(f"symmetrised R-MAT scale {S}: V={gs.nV} E={gs.nE} built in {time.time() - t0:.1f}s")
-        lev, _, _ = N.bfs(gs, int(np.argmax(gs.download(csc=False)["outdeg"])))
+        lev, _, _ = N.bfs(gs, int(np.argmax(gs.outdeg())))
         reached = np.nonzero(lev >= 0)[0]
-        srcs = rng.choice(reached, a.bfs_sources, replace=False).astype(np.uint32)
+        srcs = sample_distinct(rng, reached, a.bfs_sources).astype(np.uint32)
         N.bfs(gs, int(srcs[0]), want_levels=False)
         rates = []
         for s in srcs:

@@ -32,7 +32,7 @@ int pow2ceil(int k) {
 
 template <typename TT>
 struct BFwd {  // forward: gather x (CSC), finalise ranks
-  const TT* xin; TT* xout; const float* w; const TT* p; const int32_t* outdeg;
+  const TT* xin; TT* xout; const TT* w; const TT* p; const int32_t* outdeg;
   const double* scal;  // [2k]: dang_j at scal[2j]
   double damp; int k;
   __device__ TT gather(uint64_t e, uint32_t u, int j) const { return (TT)w[e] * xin[(uint64_t)u * k + j]; }
@@ -49,7 +49,7 @@ struct BFwd {  // forward: gather x (CSC), finalise ranks
 
 template <typename TT>
 struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda
-  const TT* lin; TT* lout; const float* w; const TT* c; const TT* p; const int32_t* outdeg; TT* wl;
+  const TT* lin; TT* lout; const TT* w; const TT* c; const TT* p; const int32_t* outdeg; TT* wl;
   const double* scal;  // [2k]: p.lambda_j at scal[2j]
   double damp; int k;
   __device__ TT gather(uint64_t e, uint32_t v, int j) const { return (TT)w[e] * lin[(uint64_t)v * k + j]; }
@@ -372,7 +372,7 @@ double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
   double* sc = bs.scal.as<double>();
   const unsigned gb = gcap((uint64_t)P->g->nV * 64);
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
-  P->cur = 0;
+  P->xcur = 0;
   hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->p.as<TT>(), P->p.as<TT>(),
                      P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), 0, tmp, (uint64_t)gb);
   hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
@@ -381,16 +381,16 @@ double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
   double res = 0.0;
   for (it = 0; it < iters; ++it) {
     BFwd<TT> m;
-    m.xin = P->x[P->cur].as<TT>();
-    m.xout = P->x[P->cur ^ 1].as<TT>();
-    m.w = P->wcsc.as<float>();
+    m.xin = P->x[P->xcur].as<TT>();
+    m.xout = P->x[P->xcur ^ 1].as<TT>();
+    m.w = P->wcsc.as<TT>();
     m.p = P->p.as<TT>();
     m.outdeg = P->g->outdeg.as<int32_t>();
     m.scal = sc + 2 * k * it;  // a0_j at index 2j
     m.damp = damp;
     m.k = k;
     bpass_g<BFwd<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s);
-    P->cur ^= 1;
+    P->xcur ^= 1;
     if (tol > 0.0) {
       res = max_resid(P, sc + 2 * k * (it + 1), s);
       if (res < tol) { ++it; break; }
@@ -423,7 +423,7 @@ double badjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     BAdj<TT> m;
     m.lin = P->lam[P->lcur].as<TT>();
     m.lout = P->lam[P->lcur ^ 1].as<TT>();
-    m.w = P->wcsr.as<float>();
+    m.w = P->wcsr.as<TT>();
     m.c = P->c.as<TT>();
     m.p = P->p.as<TT>();
     m.outdeg = P->g->outdeg.as<int32_t>();
@@ -449,7 +449,7 @@ void bgrad(gre_pr* P, double damp, float* gW_h, float* graw_h, void* gp_h) {
   hipStream_t s = stream();
   gre_graph* g = P->g;
   const int k = P->k;
-  const TT* x = P->x[P->cur].as<TT>();
+  const TT* x = P->x[P->xcur].as<TT>();
   const TT* lam = P->lam[P->lcur].as<TT>();
   const unsigned gb = gcap((uint64_t)g->nV * 64);
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));

@@ -34,7 +34,7 @@ struct ModeFwd {
   TT* contrib_out;        // unweighted only
   const TT* xin;          // previous ranks (residual)
   TT* xout;               // new ranks
-  const float* w;         // normalised weights in CSC order (weighted)
+  const TT* w;            // normalised weights in CSC order (weighted), compute type
   const TT* p;            // personalisation or nullptr (uniform)
   const int32_t* outdeg;
   const double* scal_in;  // [0] = dangling mass of xin
@@ -61,7 +61,7 @@ struct ModeAdj {
   using T = TT;
   const TT* lin;
   TT* lout;
-  const float* w;         // normalised weights, CSR order (weighted)
+  const TT* w;            // normalised weights, CSR order (weighted), compute type
   const TT* c;
   const TT* p;
   const int32_t* outdeg;
@@ -138,17 +138,19 @@ __global__ void k_lognormal(const uint64_t* __restrict__ off, const uint32_t* __
   }
 }
 
+template <typename TT>
 __global__ void k_normalise(const uint64_t* __restrict__ off, uint32_t nV, uint64_t nE, const float* __restrict__ w,
-                            const double* __restrict__ rowsum, float* __restrict__ wn) {
+                            const double* __restrict__ rowsum, TT* __restrict__ wn) {
   for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nE;
        e += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t u = row_of(off, nV, e);
-    wn[e] = (float)((double)w[e] / rowsum[u]);
+    wn[e] = (TT)((double)w[e] / rowsum[u]);
   }
 }
 
-__global__ void k_permute_f32(const float* __restrict__ in, const uint64_t* __restrict__ perm, uint64_t n,
-                              float* __restrict__ out) {
+template <typename TT>
+__global__ void k_permute(const TT* __restrict__ in, const uint64_t* __restrict__ perm, uint64_t n,
+                          TT* __restrict__ out) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = in[perm[i]];
@@ -164,7 +166,7 @@ __global__ void k_targets(uint32_t nV, const uint32_t* __restrict__ tg, int64_t 
 template <typename TT>
 __global__ __launch_bounds__(256) void k_edge_grad(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                                    const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
-                                                   uint64_t ntiles, uint32_t te, const float* __restrict__ w,
+                                                   uint64_t ntiles, uint32_t te, const void* __restrict__ w,
                                                    const double* __restrict__ rowsum, const TT* __restrict__ x,
                                                    const TT* __restrict__ lam, const TT* __restrict__ wl,
                                                    const int32_t* __restrict__ outdeg, double damp,
@@ -218,13 +220,16 @@ void ensure_plan(gre_pr* P, bool csc, int ipt) {
 
 template <typename TT>
 ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
-  const int c = P->cur;
+  // weighted: ranks ping-pong between x[0] and x[1] because they are gathered;
+  // unweighted: ranks stay in x[0] (only their owner row touches them) and the
+  // gathered x/outdeg ping-pongs between contrib[0] and contrib[1]
+  const int xc = P->xcur, cc = P->ccur;
   ModeFwd<TT> m;
-  m.w = P->weighted ? P->wcsc.as<float>() : nullptr;
-  m.gin = P->weighted ? P->x[c].as<TT>() : P->contrib[c].as<TT>();
-  m.contrib_out = P->weighted ? nullptr : P->contrib[c ^ 1].as<TT>();
-  m.xin = P->x[c].as<TT>();
-  m.xout = P->weighted ? P->x[c ^ 1].as<TT>() : P->x[c].as<TT>();
+  m.w = P->weighted ? P->wcsc.as<TT>() : nullptr;
+  m.gin = P->weighted ? P->x[xc].as<TT>() : P->contrib[cc].as<TT>();
+  m.contrib_out = P->weighted ? nullptr : P->contrib[cc ^ 1].as<TT>();
+  m.xin = P->x[xc].as<TT>();
+  m.xout = P->weighted ? P->x[xc ^ 1].as<TT>() : P->x[xc].as<TT>();
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
   m.outdeg = P->g->outdeg.as<int32_t>();
   m.damp = damp;
@@ -239,7 +244,8 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
   m.scal_in = scal_in;
   // in-place x for the unweighted form: residual reads xin before xout writes
   pull_pass(P->plan_csc, m, scal_out, s);
-  P->cur ^= 1;
+  if (P->weighted) P->xcur ^= 1;
+  else P->ccur ^= 1;
 }
 
 template <typename TT>
@@ -247,7 +253,8 @@ void fwd_init(gre_pr* P, hipStream_t s) {
   const uint32_t nV = P->g->nV;
   const unsigned gb = gcap(nV);
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
-  P->cur = 0;
+  P->xcur = 0;
+  P->ccur = 0;
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
                      P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), P->weighted ? nullptr : P->contrib[0].as<TT>(),
                      1.0 / (double)nV, tmp);
@@ -334,7 +341,7 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     ModeAdj<TT> m;
     m.lin = P->lam[P->lcur].as<TT>();
     m.lout = P->lam[P->lcur ^ 1].as<TT>();
-    m.w = P->weighted ? P->wcsr.as<float>() : nullptr;
+    m.w = P->weighted ? P->wcsr.as<TT>() : nullptr;
     m.c = P->c.as<TT>();
     m.p = P->has_p ? P->p.as<TT>() : nullptr;
     m.outdeg = P->g->outdeg.as<int32_t>();
@@ -367,7 +374,7 @@ template <typename TT>
 void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* gp_host) {
   hipStream_t s = stream();
   gre_graph* g = P->g;
-  const TT* x = P->x[P->cur].as<TT>();
+  const TT* x = P->x[P->xcur].as<TT>();
   const TT* lam = P->lam[P->lcur].as<TT>();
   // dangling mass of the final forward, recomputed in fixed order
   const unsigned gb = gcap(g->nV);
@@ -393,7 +400,7 @@ void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* g
       ProfScope ps("edge_grad");
       hipLaunchKernelGGL(k_edge_grad<TT>, dim3((unsigned)nt), dim3(256), 0, s, g->off.as<uint64_t>(),
                          g->idx.as<uint32_t>(), trb, g->nE, g->nV, nt, te,
-                         P->weighted ? P->wcsr.as<float>() : nullptr, P->weighted ? P->rowsum.as<double>() : nullptr,
+                         P->weighted ? P->wcsr.p : nullptr, P->weighted ? P->rowsum.as<double>() : nullptr,
                          x, lam, P->wl.as<TT>(), g->outdeg.as<int32_t>(), damp, dgW.as<float>(),
                          graw_host ? dgraw.as<float>() : nullptr);
     }
@@ -471,9 +478,11 @@ int gre_pr_set_edge_weights(gre_pr* P, const float* w, uint64_t seed) {
     gre_graph* g = P->g;
     hipStream_t s = stream();
     const uint64_t nE = g->nE;
+    // raw weights are FP32 (the input); normalised weights use the compute type
+    const size_t ts = tsize(P->dtype);
     P->wraw.alloc(4 * nE);
-    P->wcsr.alloc(4 * nE);
-    P->wcsc.alloc(4 * nE);
+    P->wcsr.alloc(ts * nE);
+    P->wcsc.alloc(ts * nE);
     P->rowsum.alloc(8 * (size_t)g->nV);
     if (nE > 0) {
       if (w) {
@@ -498,11 +507,18 @@ int gre_pr_set_edge_weights(gre_pr* P, const float* w, uint64_t seed) {
       GRE_HIP(hipStreamSynchronize(s));
     }
     if (nE > 0) {
-      hipLaunchKernelGGL(k_normalise, dim3(gcap(nE)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->nV, nE,
-                         P->wraw.as<float>(), P->rowsum.as<double>(), P->wcsr.as<float>());
       const uint64_t* ceid = graph_ceid(g);
-      hipLaunchKernelGGL(k_permute_f32, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<float>(), ceid, nE,
-                         P->wcsc.as<float>());
+      if (P->dtype == GRE_F64) {
+        hipLaunchKernelGGL(k_normalise<double>, dim3(gcap(nE)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->nV, nE,
+                           P->wraw.as<float>(), P->rowsum.as<double>(), P->wcsr.as<double>());
+        hipLaunchKernelGGL(k_permute<double>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<double>(), ceid, nE,
+                           P->wcsc.as<double>());
+      } else {
+        hipLaunchKernelGGL(k_normalise<float>, dim3(gcap(nE)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->nV, nE,
+                           P->wraw.as<float>(), P->rowsum.as<double>(), P->wcsr.as<float>());
+        hipLaunchKernelGGL(k_permute<float>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<float>(), ceid, nE,
+                           P->wcsc.as<float>());
+      }
       GRE_HIP(hipGetLastError());
     }
     GRE_HIP(hipStreamSynchronize(s));
@@ -516,7 +532,16 @@ int gre_pr_get_edge_weights(gre_pr* P, float* w_raw, float* w_norm) {
     hipStream_t s = stream();
     const uint64_t nE = P->g->nE;
     if (w_raw && nE) GRE_HIP(hipMemcpyAsync(w_raw, P->wraw.p, 4 * nE, hipMemcpyDeviceToHost, s));
-    if (w_norm && nE) GRE_HIP(hipMemcpyAsync(w_norm, P->wcsr.p, 4 * nE, hipMemcpyDeviceToHost, s));
+    if (w_norm && nE) {
+      if (P->dtype == GRE_F64) {
+        std::vector<double> h(nE);
+        GRE_HIP(hipMemcpyAsync(h.data(), P->wcsr.p, 8 * nE, hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        for (uint64_t e = 0; e < nE; ++e) w_norm[e] = (float)h[e];
+      } else {
+        GRE_HIP(hipMemcpyAsync(w_norm, P->wcsr.p, 4 * nE, hipMemcpyDeviceToHost, s));
+      }
+    }
     GRE_HIP(hipStreamSynchronize(s));
   });
 }
@@ -561,7 +586,7 @@ int gre_pr_get_ranks(gre_pr* P, void* host_out) {
   return guard([&] {
     GRE_CHECK(P && host_out, GRE_DEV_NULL, "null argument");
     const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
-    GRE_HIP(hipMemcpyAsync(host_out, P->x[P->cur].p, bytes, hipMemcpyDeviceToHost, stream()));
+    GRE_HIP(hipMemcpyAsync(host_out, P->x[P->xcur].p, bytes, hipMemcpyDeviceToHost, stream()));
     GRE_HIP(hipStreamSynchronize(stream()));
   });
 }

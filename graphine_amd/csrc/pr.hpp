@@ -22,7 +22,8 @@ struct gre_pr {
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place
   gre::DevBuf contrib[2]; // unweighted: x/outdeg ping-pong
-  int cur = 0;
+  int xcur = 0;           // x[xcur] holds the current ranks (always 0 when unweighted, k == 1)
+  int ccur = 0;           // contrib[ccur] holds the current x/outdeg (unweighted, k == 1)
   gre::DevBuf p;          // personalisation (nV*k), optional
   gre::DevBuf wraw, wcsr, wcsc, rowsum;  // weights (CSR raw, CSR norm, CSC norm), FP64 row sums
   // adjoint

@@ -1,0 +1,211 @@
+// DistributedGraph<GraphInCSR> (include/Graph/DistributedGraph.hpp:20-276 in the
+// GRE tree) at one process on one GPU: every vertex is a master, lvid == vid,
+// there are no scatter/combiner agents, and the local graph is one
+// device-resident CSR + CSC (gre_graph) built at ingress.
+#ifndef GRE_AMD_DISTRIBUTED_GRAPH_HPP
+#define GRE_AMD_DISTRIBUTED_GRAPH_HPP
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Comm/BufferingComm.hpp"
+#include "Comm/CommUnderlying.hpp"
+#include "Graph/GraphIngress.hpp"
+#include "Graph/LocalGraph.hpp"
+#include "Types.hpp"
+#include "Util/Bitmap.hpp"
+#include "Util/BufferPool.hpp"
+#include "Util/Util.hpp"
+#include "gre_dev.h"
+
+namespace GRE {
+namespace Graph {
+
+template <typename LocalGraph>
+class DistributedGraph {
+ public:
+  typedef LocalGraph LocalGraph_t;
+  typedef DistributedGraph<LocalGraph> SELF_t;
+  friend class GraphIngress<SELF_t>;
+
+  DistributedGraph() : isReady_(false), nV_(0), nE_(0), dev_(NULL), edgeProp_(NULL), edgePropBytes_(0) {}
+  ~DistributedGraph() {
+    if (dev_) gre_graph_destroy(dev_);
+  }
+
+  // id translation (one process: every vertex is a master, lvid == vid)
+  lvid_t get_lvid_master(const vid_t vid) const { return (lvid_t)vid; }
+  lvid_t get_lvid_scatter(const vid_t vid) const { return (lvid_t)vid; }
+  lvid_t get_lvid_combiner(const vid_t vid) const { return (lvid_t)vid; }
+  lvid_t get_lvid(const vid_t vid) const { return (lvid_t)vid; }
+  vid_t get_vid(const lvid_t lv) const { return lv < nV_ ? (vid_t)lv : (vid_t)-1; }
+  int get_owner(vid_t) const { return 0; }
+  int is_local(const vid_t vid) const { return vid < nV_; }
+
+  size_t get_begin_lvid_master() { return 0; }
+  size_t get_begin_lvid_agent() { return Util::align(nV_); }
+  size_t get_begin_lvid_scatter() { return Util::align(nV_); }
+  size_t get_begin_lvid_combiner() { return Util::align(nV_); }
+  size_t get_end_lvid_master() { return nV_; }
+  size_t get_end_lvid_agent() { return Util::align(nV_); }
+  size_t get_end_lvid_scatter() { return Util::align(nV_); }
+  size_t get_end_lvid_combiner() { return Util::align(nV_); }
+  size_t get_num_masters() { return nV_; }
+  size_t get_num_agents() { return 0; }
+  size_t get_num_scatters() { return 0; }
+  size_t get_num_combiners() { return 0; }
+  size_t get_num_edges() { return nE_; }
+  size_t get_num_vtx() { return nV_; }
+
+  void load(std::string) {}
+  void save(std::string) {}
+
+  // ingress overloads (DistributedGraph.hpp:136-172)
+  bool ingress(COMM::Underlying& c, Util::BufferPool& b, const std::string prefix) {
+    return ingress<Out>(c, b, prefix);
+  }
+  bool ingress(COMM::Underlying& c, Util::BufferPool& b, const std::string prefix, const std::string format,
+               const std::string method) {
+    return ingress<Out>(c, b, prefix, format, method);
+  }
+  bool ingress(COMM::Underlying& c, Util::BufferPool& b, IngressParameters& p) { return ingress<Out>(c, b, p); }
+  template <Dir dir>
+  bool ingress(COMM::Underlying& c, Util::BufferPool& b, const std::string prefix) {
+    IngressParameters p(prefix);
+    return ingress<dir>(c, b, p);
+  }
+  template <Dir dir>
+  bool ingress(COMM::Underlying& c, Util::BufferPool& b, const std::string prefix, const std::string format,
+               const std::string method) {
+    IngressParameters p(prefix, format, method);
+    return ingress<dir>(c, b, p);
+  }
+  template <Dir dir>
+  bool ingress(COMM::Underlying&, Util::BufferPool&, IngressParameters& p) {
+    GraphIngress<SELF_t> gi(*this);
+    isReady_ = gi.template ingress<dir>(p);
+    return isReady_;
+  }
+
+  // a graph already in memory (CSR arrays or an R-MAT generated on the device)
+  bool fromDevice(gre_graph* g, const std::vector<int>& outdeg) {
+    uint32_t nv = 0;
+    uint64_t ne = 0;
+    if (gre_graph_info(g, &nv, &ne) != 0) return false;
+    if (dev_) gre_graph_destroy(dev_);
+    dev_ = g;
+    nV_ = nv;
+    nE_ = ne;
+    outdegrees_ = outdeg;
+    local_.attach(dev_, nV_, nE_);
+    isReady_ = true;
+    return true;
+  }
+
+  LocalGraph_t& refLocalGraph() { return local_; }
+  size_t getOutdegree(lvid_t lv) { return (size_t)outdegrees_[lv]; }
+
+  // vertex / edge properties (DistributedGraph.hpp:182-240); edge properties
+  // are in local CSR order, one value per edge
+  template <typename T>
+  void loadVertexProperty(T*& vData, const std::string filename) {
+    vData = new T[nV_];
+    std::ifstream in(filename.c_str());
+    if (!in) {
+      std::cerr << "Err in reading vertex peroperty from file" << filename << std::endl;
+      std::exit(-1);
+    }
+    for (size_t v = 0; v < nV_; ++v) in >> vData[v];
+  }
+  template <typename T>
+  void loadEdgeProperty(T*& eData, const std::string filename) {
+    eData = new T[nE_];
+    std::ifstream in(filename.c_str());
+    if (!in) {
+      std::cerr << "Err in reading edge peroperty from file" << filename << std::endl;
+      std::exit(-1);
+    }
+    for (size_t e = 0; e < nE_; ++e) in >> eData[e];
+    registerEdgeProperty(eData);
+  }
+  template <typename T>
+  void loadVertexProperty(T*& vData) {
+    vData = new T[nV_];
+    for (size_t v = 0; v < nV_; ++v) vData[v] = std::rand() % 65535;
+  }
+  template <typename T>
+  void loadEdgeProperty(T*& eData) {
+    eData = new T[nE_];
+    for (size_t e = 0; e < nE_; ++e) eData[e] = std::rand() % 65535;
+    registerEdgeProperty(eData);
+  }
+  template <typename T>
+  void unloadVertexProperty(T*& vData) {
+    delete[] vData;
+    vData = NULL;
+  }
+  template <typename T>
+  void unloadEdgeProperty(T*& eData) {
+    if (eData == edgeProp_) {
+      edgeProp_ = NULL;
+      edgePropBytes_ = 0;
+    }
+    delete[] eData;
+    eData = NULL;
+  }
+
+  // ---- device binding (used by SynchronousEngine) ----------------------------
+  gre_graph* device() {
+    if (!dev_) throw std::runtime_error("DistributedGraph: no graph loaded (call ingress first)");
+    return dev_;
+  }
+  // most recently loaded edge property (SSSP reads its weights from it)
+  const void* edgeProperty() const { return edgeProp_; }
+  size_t edgePropertyBytes() const { return edgePropBytes_; }
+  // vertex data a program left on this graph for a later program (BC Forward -> Backward)
+  void setProgramState(int prog, void* p) { progState_[prog] = p; }
+  void* programState(int prog) const {
+    typename std::map<int, void*>::const_iterator it = progState_.find(prog);
+    return it == progState_.end() ? NULL : it->second;
+  }
+
+  bool adoptEdges(size_t nV, const std::vector<uint32_t>& src, const std::vector<uint32_t>& dst,
+                  const std::vector<int>& outdeg) {
+    gre_graph* g = NULL;
+    const int rc = gre_graph_from_edges(&g, (uint32_t)nV, (uint64_t)src.size(), src.empty() ? NULL : &src[0],
+                                        dst.empty() ? NULL : &dst[0]);
+    if (rc != 0) {
+      std::cerr << "Err: building the device graph failed: " << gre_last_error() << std::endl;
+      return false;
+    }
+    return fromDevice(g, outdeg);
+  }
+
+ private:
+  template <typename T>
+  void registerEdgeProperty(T* p) {
+    edgeProp_ = p;
+    edgePropBytes_ = sizeof(T);
+  }
+
+  bool isReady_;
+  size_t nV_, nE_;
+  gre_graph* dev_;
+  LocalGraph_t local_;
+  std::vector<int> outdegrees_;
+  const void* edgeProp_;
+  size_t edgePropBytes_;
+  std::map<int, void*> progState_;
+
+ public:
+  // GRE keeps per-peer agent maps here; at one process they are empty
+  std::vector<Util::Bitmap*> scatterMaps, rscatterMaps, combinerMaps, rcombinerMaps;
+};
+
+}  // namespace Graph
+}  // namespace GRE
+#endif

@@ -194,6 +194,11 @@ class Graph:
         _ck(lib().gre_graph_download(self.h, _p(off), _p(idx), _p(coff), _p(cidx), _p(ce), _p(od)))
         return dict(off=off, idx=idx, coff=coff, cidx=cidx, ceid=ce, outdeg=od)
 
+    def outdeg(self):
+        od = np.zeros(self.nV, np.int32)
+        _ck(lib().gre_graph_download(self.h, None, None, None, None, None, _p(od)))
+        return od
+
     def set_weights_u32(self, w=None, seed=3):
         if w is not None:
             w = np.ascontiguousarray(w, np.uint32)
@@ -225,8 +230,11 @@ class Engine:
         self.h = C.c_void_p()
         _ck(lib().gre_engine_create(C.byref(self.h), graph.h, prog))
 
+    max_ss = 1024
+
     def config_max_supersteps(self, n):
         _ck(lib().gre_engine_config_max_supersteps(self.h, n))
+        self.max_ss = n if n > 0 else 1024
 
     def _dtype(self, which):
         if which == ARR_AUX:
@@ -256,8 +264,10 @@ class Engine:
         ss = C.c_int64()
         _ck(lib().gre_engine_run(self.h, direction, _p(s), 0 if s is None else len(s), _p(counts),
                                  max_counts, C.byref(ss)))
-        n = min(ss.value + 1, max_counts)
-        return ss.value, counts[:n].copy()
+        # a run that stops at max_supersteps recorded ss counts, one that voted
+        # to halt recorded ss + 1 (the last one is 0)
+        n = ss.value if ss.value >= self.max_ss else ss.value + 1
+        return ss.value, counts[:min(n, max_counts)].copy()
 
     def phase_times(self):
         t = np.zeros(5, np.float64)

@@ -86,13 +86,14 @@ def test_weighted_ppr_adjoint_gradients_match_oracle(N, O):
         assert abs(wraw[e] - O.lognormal_weight(5, int(src[e]), int(og.idx[e]))) <= 1e-6 * wraw[e]
     wr = wraw.astype(np.float64)
     rs = np.bincount(src, weights=wr, minlength=og.nV)
+    # an FP64 run normalises in FP64: w / rowsum, the same division as here
     wn = np.where(rs[src] > 0, wr / rs[src], 0.0)
     assert np.allclose(wnorm, wn, rtol=1e-6)
     p = ppr_problem(O, og)
     pr.set_personalization(p)
     it, res = pr.run(d, 2000, 1e-13)
     x = pr.ranks()
-    wc = wnorm.astype(np.float64)[og.ceid.astype(np.int64)]
+    wc = wn[og.ceid.astype(np.int64)]
     xref, _ = O.ppr(og, wc, p, d, tol=1e-15, maxit=5000)
     assert rel_l1(x, xref) < 1e-11
     tg = np.random.default_rng(1).choice(og.nV, 64, replace=False).astype(np.uint32)
@@ -100,10 +101,10 @@ def test_weighted_ppr_adjoint_gradients_match_oracle(N, O):
     lam = pr.adjoint_vector()
     c = np.zeros(og.nV)
     c[tg] = 1.0
-    lref, _ = O.ppr_adjoint(og, wnorm.astype(np.float64), p, c, d, tol=1e-15, maxit=5000)
+    lref, _ = O.ppr_adjoint(og, wn, p, c, d, tol=1e-15, maxit=5000)
     assert rel_l1(lam, lref) < 1e-11
     gW, graw, gp = pr.gradients(d)
-    oW, oraw = O.ppr_edge_grad(og, wnorm.astype(np.float64), rs, xref, lref, d)
+    oW, oraw = O.ppr_edge_grad(og, wn, rs, xref, lref, d)
     assert rel_l1(gW, oW) < 1e-5
     assert rel_l1(graw, oraw) < 1e-5
     gpref = ((1 - d) + d * xref[og.outdeg == 0].sum()) * lref
