@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="headline PageRank only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the pull-schedule sweep")
     return ap.parse_args()
 
 
@@ -122,6 +123,31 @@ def main():
                 "alg_bytes_per_launch": bytes_iter, "avg_launch_ms": round(avg_tile_s * 1e3, 4),
                 "fixup_avg_ms": round(ms_fix / max(n_fix, 1), 4)}
     log(f"pull_tile avg {avg_tile_s * 1e3:.3f} ms -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
+    sched, tile_items = pr.schedule()
+    roofline["schedule"] = f"{sched}, {tile_items} entries/thread (autotuned)"
+    # achievable streaming bandwidth on this device: a plain copy kernel
+    copy_bytes = 4 << 30
+    ms_copy = N.copy_probe(copy_bytes, 10)
+    roofline["copy_probe_gbs"] = round(2 * copy_bytes / (ms_copy / 1e3) / 1e9, 1)
+    log(f"copy probe: {roofline['copy_probe_gbs']} GB/s")
+    if not a.no_sweep:
+        # every schedule the autotuner chooses from, timed on the same graph
+        sweep = {}
+        for sc in (1, 2, 3, 4):
+            for ti in (4, 8, 16):
+                q = N.PageRank(g, N.F32)
+                q.set_schedule(sc, ti)
+                q.run(0.85, 2)
+                N.prof_reset()
+                N.prof_enable(True)
+                q.run(0.85, 10)
+                N.sync()
+                nt_, mt_ = N.prof_get("pull_tile")
+                N.prof_enable(False)
+                sweep[f"{N.PageRank.SCHEDULES[sc]}/{ti}"] = round(mt_ / max(nt_, 1), 3)
+                del q
+        roofline["schedule_sweep_ms"] = sweep
+        log("schedule sweep (ms per pull_tile launch):", sweep)
 
     # ---------------- CPU baseline (bounded sample, same graph) ---------------
     cpu = None

@@ -296,6 +296,67 @@ const uint32_t* graph_w_csc(gre_graph* g) {
   return g->w_u32_csc.as<uint32_t>();
 }
 
+namespace {
+__global__ void k_iota_u32(uint32_t* p, uint32_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+__global__ void k_outdeg_u32(const int32_t* __restrict__ d, uint32_t n, uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)d[i];
+}
+// cid[order[i]] = i for the nsrc vertices with out-edges, ~0 for the others
+__global__ void k_rank(const uint32_t* __restrict__ order, const uint32_t* __restrict__ deg_sorted, uint32_t n,
+                       uint32_t* __restrict__ cid, uint32_t* __restrict__ nsrc) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const bool has = deg_sorted[i] > 0;
+    cid[order[i]] = has ? (uint32_t)i : 0xffffffffu;
+    if (has && (i + 1 == n || deg_sorted[i + 1] == 0)) *nsrc = (uint32_t)(i + 1);
+  }
+}
+__global__ void k_map_idx(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ cid, uint64_t n,
+                          uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = cid[idx[i]];
+}
+}  // namespace
+
+void graph_compact_sources(gre_graph* g) {
+  if (g->cid.p) return;
+  hipStream_t s = stream();
+  const uint32_t n = g->nV;
+  DevBuf keys, keys_sorted, vals, order, cnt;
+  keys.alloc(4 * (size_t)n);
+  keys_sorted.alloc(4 * (size_t)n);
+  vals.alloc(4 * (size_t)n);
+  order.alloc(4 * (size_t)n);
+  cnt.alloc(8);
+  GRE_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
+  hipLaunchKernelGGL(k_outdeg_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, g->outdeg.as<int32_t>(), n,
+                     keys.as<uint32_t>());
+  hipLaunchKernelGGL(k_iota_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, vals.as<uint32_t>(), n);
+  GRE_HIP(hipGetLastError());
+  // descending out-degree, ties by vertex id (the sort is stable)
+  size_t tb = 0;
+  GRE_HIP(rocprim::radix_sort_pairs_desc(nullptr, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
+                                         vals.as<uint32_t>(), order.as<uint32_t>(), (size_t)n, 0, 32, s));
+  void* tmp = scratch(0, tb);
+  GRE_HIP(rocprim::radix_sort_pairs_desc(tmp, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
+                                         vals.as<uint32_t>(), order.as<uint32_t>(), (size_t)n, 0, 32, s));
+  g->cid.alloc(4 * (size_t)n);
+  hipLaunchKernelGGL(k_rank, dim3(grid_cap(n)), dim3(kBlock), 0, s, order.as<uint32_t>(), keys_sorted.as<uint32_t>(), n,
+                     g->cid.as<uint32_t>(), cnt.as<uint32_t>());
+  g->cidx_c.alloc(4 * (size_t)g->nE);
+  if (g->nE)
+    hipLaunchKernelGGL(k_map_idx, dim3(grid_cap(g->nE)), dim3(kBlock), 0, s, g->cidx.as<uint32_t>(),
+                       g->cid.as<uint32_t>(), g->nE, g->cidx_c.as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+  uint32_t ns = 0;
+  GRE_HIP(hipMemcpyAsync(&ns, cnt.p, 4, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  g->nsrc = ns;
+}
+
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles) {
   uint64_t nt = (g->nE + te - 1) / te;
   if (nt == 0) nt = 1;

@@ -20,6 +20,11 @@ struct gre_graph {
   bool rows_sorted = false; // CSR rows sorted by destination (R-MAT, from_csr sorted)
   // tile row partitions of the pull kernels, keyed by (csc?, tile edges)
   std::map<std::pair<int, uint32_t>, gre::DevBuf> tiles;
+  // hub-sorted compact source ids (built on demand by the pull schedules):
+  // cid[v] = rank of v among vertices with out-edges, by descending out-degree;
+  // cidx_c[p] = cid[cidx[p]] (CSC in compact source ids)
+  gre::DevBuf cid, cidx_c;
+  uint32_t nsrc = 0;
 };
 
 namespace gre {
@@ -29,6 +34,8 @@ const uint64_t* graph_ceid(gre_graph* g);
 const uint32_t* graph_w_csc(gre_graph* g);
 // tile_rb[t] = #rows r < nV with off[r] < t*te, t in [0, ntiles]; ntiles >= 1.
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles);
+// Builds g->cid / g->cidx_c / g->nsrc once (hub-sorted compact source ids).
+void graph_compact_sources(gre_graph* g);
 
 // device helpers
 __device__ inline uint32_t row_of(const uint64_t* __restrict__ off, uint32_t nV, uint64_t e) {

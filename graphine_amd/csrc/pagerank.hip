@@ -37,6 +37,7 @@ struct ModeFwd {
   const TT* w;            // normalised weights in CSC order (weighted), compute type
   const TT* p;            // personalisation or nullptr (uniform)
   const int32_t* outdeg;
+  const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
   __device__ TT gather(uint64_t e, uint32_t u) const {
@@ -50,7 +51,10 @@ struct ModeFwd {
     const TT old = xin[r];
     xout[r] = y;
     const int32_t d = outdeg[r];
-    if (!w) contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
+    if (!w) {
+      if (cid) { if (d > 0) contrib_out[cid[r]] = y / (TT)d; }
+      else contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
+    }
     if (d == 0) a0 += (double)y;
     a1 += fabs((double)y - (double)old);
   }
@@ -95,7 +99,8 @@ struct ModeRowSum {
 
 template <typename TT>
 __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t* __restrict__ outdeg,
-                           TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part) {
+                           TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part,
+                           const uint32_t* __restrict__ cid) {
   __shared__ double red[8];
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
@@ -103,7 +108,10 @@ __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t*
     const TT xv = p ? p[v] : (TT)invn;
     x[v] = xv;
     const int32_t d = outdeg[v];
-    if (contrib) contrib[v] = d > 0 ? xv / (TT)d : (TT)0;
+    if (contrib) {
+      if (cid) { if (d > 0) contrib[cid[v]] = xv / (TT)d; }
+      else contrib[v] = d > 0 ? xv / (TT)d : (TT)0;
+    }
     if (d == 0) a0 += (double)xv;
   }
   block_sum2(a0, a1, red);
@@ -211,11 +219,24 @@ void ensure_scal(gre_pr* P, int iters) {
   P->scal.alloc(sizeof(double) * 2 * (size_t)P->scal_cap);
 }
 
+// The compact schedule applies to the unweighted single-vector forward only:
+// its gathers read contrib[], which then holds one entry per vertex with
+// out-edges, hubs first (graph_compact_sources).
+bool use_compact(const gre_pr* P) { return P->compact && !P->weighted && P->k == 1; }
+
 void ensure_plan(gre_pr* P, bool csc, int ipt) {
   int& have = csc ? P->plan_ipt_csc : P->plan_ipt_csr;
-  if (have == ipt) return;
-  (csc ? P->buf_csc : P->buf_csr).make(csc ? P->plan_csc : P->plan_csr, P->g, csc, ipt, tsize(P->dtype));
-  have = ipt;
+  PullPlan& plan = csc ? P->plan_csc : P->plan_csr;
+  if (have != ipt) {
+    (csc ? P->buf_csc : P->buf_csr).make(plan, P->g, csc, ipt, tsize(P->dtype));
+    have = ipt;
+  }
+  plan.nt = P->nt;
+  plan.idx = (csc ? P->g->cidx : P->g->idx).as<uint32_t>();
+  if (csc && use_compact(P)) {
+    graph_compact_sources(P->g);
+    plan.idx = P->g->cidx_c.as<uint32_t>();
+  }
 }
 
 template <typename TT>
@@ -232,6 +253,7 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
   m.xout = P->weighted ? P->x[xc ^ 1].as<TT>() : P->x[xc].as<TT>();
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
   m.outdeg = P->g->outdeg.as<int32_t>();
+  m.cid = use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr;
   m.damp = damp;
   m.invn = 1.0 / (double)P->g->nV;
   return m;
@@ -257,7 +279,7 @@ void fwd_init(gre_pr* P, hipStream_t s) {
   P->ccur = 0;
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
                      P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), P->weighted ? nullptr : P->contrib[0].as<TT>(),
-                     1.0 / (double)nV, tmp);
+                     1.0 / (double)nV, tmp, use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0,
                      P->scal.as<double>());
   GRE_HIP(hipGetLastError());
@@ -291,34 +313,53 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
   return res;
 }
 
-// Autotuner: times each tile size on two forward iterations and keeps the fastest.
+// Applies schedule code sc (GRE_SCHED_*) and tile size to the handle.
+void apply_schedule(gre_pr* P, int sc, int ipt) {
+  P->schedule = sc;
+  P->ipt = ipt;
+  P->nt = sc == GRE_SCHED_TILE_NT || sc == GRE_SCHED_COMPACT_NT;
+  P->compact = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT;
+}
+
+// Autotuner: times every (schedule, tile size) candidate on two forward
+// iterations after a warm-up one and keeps the fastest.  Candidates: tile
+// sizes of 4/8/16 entries per thread, with and without the non-temporal id
+// stream, and (unweighted single vector) with hub-sorted compact sources.
 template <typename TT>
-int autotune_forward(gre_pr* P, double damp) {
+void autotune_forward(gre_pr* P, double damp) {
   hipStream_t s = stream();
-  const int cands[3] = {4, 8, 16};
-  int best = 8;
+  const int ipts[3] = {4, 8, 16};
+  std::vector<int> scheds = {GRE_SCHED_TILE, GRE_SCHED_TILE_NT};
+  if (!P->weighted && P->k == 1) {
+    scheds.push_back(GRE_SCHED_COMPACT);
+    scheds.push_back(GRE_SCHED_COMPACT_NT);
+  }
+  int best_s = GRE_SCHED_TILE, best_i = 8;
   float best_ms = 1e30f;
   hipEvent_t a, b;
   GRE_HIP(hipEventCreate(&a));
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
-  for (int ci = 0; ci < 3; ++ci) {
-    ensure_plan(P, true, cands[ci]);
-    fwd_init<TT>(P, s);
-    double* sc = P->scal.as<double>();
-    fwd_iter<TT>(P, damp, sc, sc + 2, s);  // warm
-    GRE_HIP(hipEventRecord(a, s));
-    fwd_iter<TT>(P, damp, sc + 2, sc + 4, s);
-    fwd_iter<TT>(P, damp, sc + 4, sc + 6, s);
-    GRE_HIP(hipEventRecord(b, s));
-    GRE_HIP(hipEventSynchronize(b));
-    float ms = 0.f;
-    GRE_HIP(hipEventElapsedTime(&ms, a, b));
-    if (ms < best_ms) { best_ms = ms; best = cands[ci]; }
+  for (int sc : scheds) {
+    for (int ipt : ipts) {
+      apply_schedule(P, sc, ipt);
+      ensure_plan(P, true, ipt);
+      fwd_init<TT>(P, s);
+      double* scal = P->scal.as<double>();
+      fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
+      GRE_HIP(hipEventRecord(a, s));
+      fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
+      fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
+      GRE_HIP(hipEventRecord(b, s));
+      GRE_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      GRE_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best_ms) { best_ms = ms; best_s = sc; best_i = ipt; }
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  return best;
+  apply_schedule(P, best_s, best_i);
 }
 
 template <typename TT>
@@ -383,7 +424,7 @@ void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* g
   DevBuf dummy;
   dummy.alloc(sizeof(TT) * (size_t)g->nV);
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, g->nV, x, g->outdeg.as<int32_t>(), dummy.as<TT>(),
-                     (TT*)nullptr, 0.0, tmp);
+                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dsc);
   if (gp_host) {
     hipLaunchKernelGGL(k_grad_p<TT>, dim3(gb), dim3(kB), 0, s, g->nV, lam, dsc, damp, dummy.as<TT>());
@@ -463,11 +504,20 @@ int gre_pr_destroy(gre_pr* P) {
 int gre_pr_set_schedule(gre_pr* P, int schedule, int tile_items) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    GRE_CHECK(schedule >= 0 && schedule <= 1, GRE_DEV_NULL, "schedule must be 0 (autotune) or 1 (tile)");
+    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_COMPACT_NT, GRE_DEV_NULL,
+              "schedule must be one of GRE_SCHED_*");
     GRE_CHECK(tile_items == 0 || tile_items == 4 || tile_items == 8 || tile_items == 16, GRE_DEV_NULL,
               "tile items per thread must be 4, 8 or 16");
-    P->schedule = schedule;
-    P->ipt = schedule == 0 ? 0 : (tile_items ? tile_items : 8);
+    if (schedule == GRE_SCHED_AUTO) apply_schedule(P, GRE_SCHED_AUTO, 0);
+    else apply_schedule(P, schedule, tile_items ? tile_items : 8);
+  });
+}
+
+int gre_pr_get_schedule(gre_pr* P, int* schedule, int* tile_items) {
+  return guard([&] {
+    GRE_CHECK(P && schedule && tile_items, GRE_DEV_NULL, "null argument");
+    *schedule = P->schedule;
+    *tile_items = P->ipt;
   });
 }
 
@@ -571,7 +621,8 @@ int gre_pr_run(gre_pr* P, double damping, int iters, double tol, int* iters_done
       return;
     }
     if (P->ipt == 0) {
-      P->ipt = P->dtype == GRE_F64 ? autotune_forward<double>(P, damping) : autotune_forward<float>(P, damping);
+      if (P->dtype == GRE_F64) autotune_forward<double>(P, damping);
+      else autotune_forward<float>(P, damping);
     }
     ensure_plan(P, true, P->ipt);
     ensure_scal(P, iters);

@@ -17,7 +17,9 @@ struct gre_pr {
   int k = 1;
   int weighted = 0;
   int ipt = 0;            // 0 = autotune on first run
-  int schedule = 0;
+  int schedule = 0;       // 0 = autotune; else GRE_SCHED_* (gre_dev.h)
+  bool nt = false;        // non-temporal neighbour-id stream
+  bool compact = false;   // hub-sorted compact source ids (unweighted, k == 1)
   bool has_p = false;
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place

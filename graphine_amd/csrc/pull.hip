@@ -41,4 +41,41 @@ void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbyt
   p.part2 = part2.as<double>();
 }
 
+// plain streaming copy (16-byte vectors): the calibration kernel for HBM byte
+// counters and the achievable-bandwidth reference
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
 }  // namespace gre
+
+extern "C" int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy) {
+  using namespace gre;
+  return guard([&] {
+    GRE_CHECK(ms_per_copy && reps > 0 && bytes >= 16, GRE_DEV_NULL, "bad argument");
+    hipStream_t s = stream();
+    const uint64_t n = bytes / 16;
+    DevBuf a, b;
+    a.alloc(16 * n);
+    b.alloc(16 * n);
+    GRE_HIP(hipMemsetAsync(a.p, 1, 16 * n, s));
+    hipEvent_t e0, e1;
+    GRE_HIP(hipEventCreate(&e0));
+    GRE_HIP(hipEventCreate(&e1));
+    const unsigned grid = 256 * 8 * 4;  // 4 blocks of 256 per CU on 256 CUs, grid-stride
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n);
+    GRE_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+      ProfScope ps("copy_probe");
+      hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n);
+    }
+    GRE_HIP(hipEventRecord(e1, s));
+    GRE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GRE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}

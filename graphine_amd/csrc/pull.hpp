@@ -36,7 +36,7 @@ __device__ inline void block_sum2(double& a, double& b, double* sh /*[2*4]*/) {
 //   using T = ...;
 //   __device__ T gather(uint64_t e, uint32_t nbr) const;
 //   __device__ void finalize(uint32_t row, T sum, double& acc0, double& acc1) const;
-template <int IPT, class Mode>
+template <int IPT, bool NT, class Mode>
 __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
@@ -55,12 +55,14 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
   const int tid = threadIdx.x;
 
-  // stage 1: neighbour ids (coalesced), then gathers
+  // stage 1: neighbour ids (coalesced; NT = streamed with a non-temporal hint
+  // so they do not push the gathered values out of the caches), then gathers
   uint32_t nb[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = j * kPullBlock + tid;
-    nb[j] = i < n ? idx[e0 + i] : 0u;
+    if (NT) nb[j] = i < n ? __builtin_nontemporal_load(idx + e0 + i) : 0u;
+    else nb[j] = i < n ? idx[e0 + i] : 0u;
   }
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -158,6 +160,7 @@ struct PullPlan {
   uint64_t E = 0, ntiles = 0;
   uint32_t V = 0;
   int ipt = 8;
+  bool nt = false;        // non-temporal hint on the neighbour-id stream
   void* carryP = nullptr;
   void* carryO = nullptr;
   double* part = nullptr;   // 2*ntiles
@@ -169,8 +172,12 @@ void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream
   using T = typename Mode::T;
   {
     ProfScope ps("pull_tile");
-    hipLaunchKernelGGL((k_pull_tile<IPT, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off, p.idx,
-                       p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, m);
+    if (p.nt)
+      hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
+                         p.idx, p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, m);
+    else
+      hipLaunchKernelGGL((k_pull_tile<IPT, false, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
+                         p.idx, p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, m);
   }
   const unsigned nfb = (unsigned)((p.ntiles + kPullBlock - 1) / kPullBlock);
   {

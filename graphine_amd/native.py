@@ -66,6 +66,8 @@ def _declare(L):
         "gre_prof_reset": ([], i32),
         "gre_prof_get": ([C.c_char_p, C.POINTER(i64), C.POINTER(dbl)], i32),
         "gre_mem_in_use": ([C.POINTER(u64)], i32),
+        "gre_copy_probe": ([u64, i32, C.POINTER(dbl)], i32),
+        "gre_pr_get_schedule": ([vp, C.POINTER(i32), C.POINTER(i32)], i32),
         "gre_graph_from_edges": ([pp, u32, u64, vp, vp], i32),
         "gre_graph_from_csr": ([pp, u32, u64, vp, vp], i32),
         "gre_graph_rmat": ([pp, i32, i32, u64, i32], i32),
@@ -144,6 +146,12 @@ def prof_get(name):
     ms = C.c_double()
     _ck(lib().gre_prof_get(name.encode(), C.byref(n), C.byref(ms)))
     return n.value, ms.value
+
+
+def copy_probe(nbytes, reps=10):
+    ms = C.c_double()
+    _ck(lib().gre_copy_probe(nbytes, reps, C.byref(ms)))
+    return ms.value
 
 
 def mem_in_use():
@@ -297,8 +305,15 @@ class PageRank:
         self.h = C.c_void_p()
         _ck(lib().gre_pr_create(C.byref(self.h), graph.h, dtype, k, 1 if weighted else 0))
 
+    SCHEDULES = {0: "auto", 1: "tile", 2: "tile_nt", 3: "compact", 4: "compact_nt"}
+
     def set_schedule(self, schedule, tile_items=0):
         _ck(lib().gre_pr_set_schedule(self.h, schedule, tile_items))
+
+    def schedule(self):
+        s, t = C.c_int(), C.c_int()
+        _ck(lib().gre_pr_get_schedule(self.h, C.byref(s), C.byref(t)))
+        return self.SCHEDULES.get(s.value, str(s.value)), t.value
 
     def set_edge_weights(self, w=None, seed=0):
         if w is not None:

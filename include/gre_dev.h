@@ -65,8 +65,12 @@ int gre_prof_enable(int on);
 int gre_prof_reset(void);
 /* name: "pull_tile", "push_expand", ...; returns launches and total ms */
 int gre_prof_get(const char* name, int64_t* launches, double* total_ms);
-/* Library version of the memory the handles hold on the device, in bytes. */
+/* Device memory the library's handles and scratch hold, in bytes. */
 int gre_mem_in_use(uint64_t* bytes);
+/* Measurement aid (no GRE counterpart): times a plain streaming copy of
+ * `bytes` bytes, reps times, on the library stream; the kernel that calibrates
+ * HBM byte counters and gives the achievable copy bandwidth. */
+int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy);
 
 /* ---- graph: device-resident CSR + CSC ------------------------------------
  * Replaces DistributedGraph<GraphInCSR>::ingress + GraphInCSR::build/rbuild
@@ -157,8 +161,15 @@ int gre_pr_adjoint(gre_pr* p, const uint32_t* targets, int64_t ntgt, double damp
  * row normalisation; CSR order, summed over the k vectors; d(loss)/d(p). */
 int gre_pr_gradients(gre_pr* p, double damping, float* gW, float* graw, void* gp);
 int gre_pr_get_adjoint(gre_pr* p, void* host_out);
-/* Selects the pull schedule: 0 = autotuned, 1 = tile-segmented, 2 = wave-per-row. */
+/* Pull schedules of the forward pass.  All are tile-segmented reductions with
+ * tile_items (4, 8 or 16) adjacency entries per thread; _NT streams the
+ * neighbour ids with a non-temporal hint; COMPACT gathers from a contrib array
+ * indexed by hub-sorted compact source ids (unweighted, single vector).
+ * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest. */
+enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,
+       GRE_SCHED_COMPACT_NT = 4 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
+int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
 
 /* ---- traversal kernels (plain single-source paths) ------------------------ */
 /* Direction-optimising BFS levels (int32, -1 unreached). edges_out: sum of the

@@ -15,12 +15,13 @@ def rel_l1(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
-@pytest.mark.parametrize("tile_items", [4, 8, 16])
-def test_pagerank_f64_matches_oracle(N, O, tile_items):
+@pytest.mark.parametrize("schedule,tile_items", [(1, 4), (1, 8), (1, 16), (2, 8), (3, 4), (3, 16), (4, 8)])
+def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     g = N.Graph.rmat(16, 16, 1, False)
     og = O.rmat_graph(16, 16, 1, False)
     pr = N.PageRank(g, N.F64)
-    pr.set_schedule(1, tile_items)
+    pr.set_schedule(schedule, tile_items)
+    assert pr.schedule() == (N.PageRank.SCHEDULES[schedule], tile_items)
     it, res = pr.run(0.85, 20)
     x = pr.ranks()
     ref = O.pagerank(og, 0.85, 20)
@@ -58,11 +59,12 @@ def test_pagerank_edge_cases(N, O):
     dst = np.concatenate([hub_dst, np.zeros(3000, np.uint32), np.array([11, 10], np.uint32)])
     g = N.Graph.from_edges(n, src, dst)
     og = O.graph_from_edges(n, src, dst)
-    for ti in (4, 8, 16):
-        pr = N.PageRank(g, N.F64)
-        pr.set_schedule(1, ti)
-        pr.run(0.85, 20)
-        assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
+    for sc in (1, 3):
+        for ti in (4, 8, 16):
+            pr = N.PageRank(g, N.F64)
+            pr.set_schedule(sc, ti)
+            pr.run(0.85, 20)
+            assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
 
 
 def ppr_problem(O, og, seed=5, nseeds=50):
