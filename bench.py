@@ -124,7 +124,7 @@ def main():
                 "fixup_avg_ms": round(ms_fix / max(n_fix, 1), 4)}
     log(f"pull_tile avg {avg_tile_s * 1e3:.3f} ms -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
     sched, tile_items = pr.schedule()
-    roofline["schedule"] = f"{sched}, {tile_items} entries/thread (autotuned)"
+    roofline["schedule"] = f"{sched}, {tile_items} entries/thread, hot sources {pr.hot_sources()} (autotuned)"
     # achievable streaming bandwidth on this device: a plain copy kernel
     copy_bytes = 4 << 30
     ms_copy = N.copy_probe(copy_bytes, 10)
@@ -133,19 +133,22 @@ def main():
     if not a.no_sweep:
         # every schedule the autotuner chooses from, timed on the same graph
         sweep = {}
-        for sc in (1, 2, 3, 4):
-            for ti in (4, 8, 16):
-                q = N.PageRank(g, N.F32)
-                q.set_schedule(sc, ti)
-                q.run(0.85, 2)
-                N.prof_reset()
-                N.prof_enable(True)
-                q.run(0.85, 10)
-                N.sync()
-                nt_, mt_ = N.prof_get("pull_tile")
-                N.prof_enable(False)
-                sweep[f"{N.PageRank.SCHEDULES[sc]}/{ti}"] = round(mt_ / max(nt_, 1), 3)
-                del q
+        cand = [(sc, ti, 0) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
+        cand += [(sc, 8, h) for sc in (3, 4) for h in (1 << 18, 1 << 20, 1 << 22)]
+        for sc, ti, hot in cand:
+            q = N.PageRank(g, N.F32)
+            q.set_schedule(sc, ti)
+            q.set_hot_sources(hot)
+            q.run(0.85, 2)
+            N.prof_reset()
+            N.prof_enable(True)
+            q.run(0.85, 10)
+            N.sync()
+            nt_, mt_ = N.prof_get("pull_tile")
+            N.prof_enable(False)
+            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/hot={hot}" if hot else "")
+            sweep[key] = round(mt_ / max(nt_, 1), 3)
+            del q
         roofline["schedule_sweep_ms"] = sweep
         log("schedule sweep (ms per pull_tile launch):", sweep)
 

@@ -40,8 +40,10 @@ struct ModeFwd {
   const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
+  uint32_t hot;           // compact ids >= hot are gathered with a non-temporal hint (0: off)
   __device__ TT gather(uint64_t e, uint32_t u) const {
     if (w) return (TT)w[e] * gin[u];
+    if (hot && u >= hot) return __builtin_nontemporal_load(gin + u);
     return gin[u];
   }
   __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const {
@@ -254,6 +256,7 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
   m.outdeg = P->g->outdeg.as<int32_t>();
   m.cid = use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr;
+  m.hot = use_compact(P) ? P->hot : 0u;
   m.damp = damp;
   m.invn = 1.0 / (double)P->g->nV;
   return m;
@@ -335,31 +338,42 @@ void autotune_forward(gre_pr* P, double damp) {
     scheds.push_back(GRE_SCHED_COMPACT_NT);
   }
   int best_s = GRE_SCHED_TILE, best_i = 8;
+  uint32_t best_h = 0;
   float best_ms = 1e30f;
   hipEvent_t a, b;
   GRE_HIP(hipEventCreate(&a));
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
   for (int sc : scheds) {
-    for (int ipt : ipts) {
-      apply_schedule(P, sc, ipt);
-      ensure_plan(P, true, ipt);
-      fwd_init<TT>(P, s);
-      double* scal = P->scal.as<double>();
-      fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
-      GRE_HIP(hipEventRecord(a, s));
-      fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
-      fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
-      GRE_HIP(hipEventRecord(b, s));
-      GRE_HIP(hipEventSynchronize(b));
-      float ms = 0.f;
-      GRE_HIP(hipEventElapsedTime(&ms, a, b));
-      if (ms < best_ms) { best_ms = ms; best_s = sc; best_i = ipt; }
+    const bool cmp = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT;
+    // compact schedules also try keeping only the hottest sources cacheable
+    std::vector<uint32_t> hots = {0u};
+    if (cmp)
+      for (uint32_t h : {1u << 18, 1u << 20, 1u << 22})
+        if (h < P->g->nV) hots.push_back(h);
+    for (uint32_t h : hots) {
+      for (int ipt : ipts) {
+        apply_schedule(P, sc, ipt);
+        P->hot = h;
+        ensure_plan(P, true, ipt);
+        fwd_init<TT>(P, s);
+        double* scal = P->scal.as<double>();
+        fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
+        GRE_HIP(hipEventRecord(a, s));
+        fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
+        fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
+        GRE_HIP(hipEventRecord(b, s));
+        GRE_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        GRE_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_ms) { best_ms = ms; best_s = sc; best_i = ipt; best_h = h; }
+      }
     }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   apply_schedule(P, best_s, best_i);
+  P->hot = best_h;
 }
 
 template <typename TT>
@@ -460,6 +474,29 @@ void alloc_state(gre_pr* P) {
   if (!P->weighted && P->k == 1) { P->contrib[0].alloc(ts * n); P->contrib[1].alloc(ts * n); }
 }
 
+void alloc_adjoint(gre_pr* P) {
+  if (P->c.p) return;
+  const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
+  P->c.alloc(bytes);
+  P->lam[0].alloc(bytes);
+  P->lam[1].alloc(bytes);
+  P->wl.alloc(bytes);
+}
+
+// Transposed power iteration from the seed vector already in P->c.
+void solve_adjoint(gre_pr* P, double damping, int iters, double tol, int* iters_done, double* residual) {
+  double r;
+  if (P->k > 1) {
+    r = batched_adjoint(P, damping, iters, tol, iters_done);
+  } else {
+    if (P->ipt == 0) P->ipt = 8;
+    r = P->dtype == GRE_F64 ? run_adjoint<double>(P, damping, iters, tol, iters_done)
+                            : run_adjoint<float>(P, damping, iters, tol, iters_done);
+  }
+  if (residual) *residual = r;
+  P->have_adj = true;
+}
+
 }  // namespace
 }  // namespace gre
 
@@ -518,6 +555,20 @@ int gre_pr_get_schedule(gre_pr* P, int* schedule, int* tile_items) {
     GRE_CHECK(P && schedule && tile_items, GRE_DEV_NULL, "null argument");
     *schedule = P->schedule;
     *tile_items = P->ipt;
+  });
+}
+
+int gre_pr_set_hot_sources(gre_pr* P, uint32_t hot) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    P->hot = hot;
+  });
+}
+
+int gre_pr_get_hot_sources(gre_pr* P, uint32_t* hot) {
+  return guard([&] {
+    GRE_CHECK(P && hot, GRE_DEV_NULL, "null argument");
+    *hot = P->hot;
   });
 }
 
@@ -652,12 +703,7 @@ int gre_pr_adjoint(gre_pr* P, const uint32_t* targets, int64_t ntgt, double damp
     hipStream_t s = stream();
     const size_t n = (size_t)P->g->nV * P->k;
     const size_t ts = tsize(P->dtype);
-    if (!P->c.p) {
-      P->c.alloc(ts * n);
-      P->lam[0].alloc(ts * n);
-      P->lam[1].alloc(ts * n);
-      P->wl.alloc(ts * n);
-    }
+    alloc_adjoint(P);
     GRE_HIP(hipMemsetAsync(P->c.p, 0, ts * n, s));
     if (ntgt > 0) {
       DevBuf dt;
@@ -676,16 +722,20 @@ int gre_pr_adjoint(gre_pr* P, const uint32_t* targets, int64_t ntgt, double damp
       GRE_HIP(hipGetLastError());
       GRE_HIP(hipStreamSynchronize(s));
     }
-    double r;
-    if (P->k > 1) {
-      r = batched_adjoint(P, damping, iters, tol, iters_done);
-    } else {
-      if (P->ipt == 0) P->ipt = 8;
-      r = P->dtype == GRE_F64 ? run_adjoint<double>(P, damping, iters, tol, iters_done)
-                              : run_adjoint<float>(P, damping, iters, tol, iters_done);
-    }
-    if (residual) *residual = r;
-    P->have_adj = true;
+    solve_adjoint(P, damping, iters, tol, iters_done, residual);
+  });
+}
+
+int gre_pr_adjoint_cotangent(gre_pr* P, const void* cot, double damping, int iters, double tol, int* iters_done,
+                             double* residual) {
+  return guard([&] {
+    GRE_CHECK(P && cot, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(P->have_fwd, GRE_DEV_NULL, "run the forward (gre_pr_run) first");
+    hipStream_t s = stream();
+    alloc_adjoint(P);
+    GRE_HIP(hipMemcpyAsync(P->c.p, cot, tsize(P->dtype) * (size_t)P->g->nV * P->k, hipMemcpyHostToDevice, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    solve_adjoint(P, damping, iters, tol, iters_done, residual);
   });
 }
 

@@ -20,6 +20,7 @@ struct gre_pr {
   int schedule = 0;       // 0 = autotune; else GRE_SCHED_* (gre_dev.h)
   bool nt = false;        // non-temporal neighbour-id stream
   bool compact = false;   // hub-sorted compact source ids (unweighted, k == 1)
+  uint32_t hot = 0;       // compact schedules: ids >= hot gathered non-temporally (0 = off)
   bool has_p = false;
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place

@@ -22,6 +22,36 @@ __global__ void k_reduce_pairs(const double* __restrict__ a, uint64_t na, const 
   if (threadIdx.x == 0) { out[0] = sh0[0]; out[1] = sh1[0]; }
 }
 
+__global__ __launch_bounds__(256) void k_reduce_pairs_part(const double* __restrict__ a, uint64_t na,
+                                                           const double* __restrict__ b, uint64_t nb,
+                                                           double* __restrict__ out) {
+  __shared__ double red[8];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < na; i += stride) {
+    const double2 v = reinterpret_cast<const double2*>(a)[i];
+    s0 += v.x;
+    s1 += v.y;
+  }
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nb; i += stride) {
+    const double2 v = reinterpret_cast<const double2*>(b)[i];
+    s0 += v.x;
+    s1 += v.y;
+  }
+  block_sum2(s0, s1, red);
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = s0; out[2 * blockIdx.x + 1] = s1; }
+}
+
+void reduce_pairs(const double* a, uint64_t na, const double* b, uint64_t nb, double* out, double* tmp,
+                  hipStream_t s) {
+  if (na + nb <= 16384) {
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, a, na, b, nb, out);
+    return;
+  }
+  hipLaunchKernelGGL(k_reduce_pairs_part, dim3(kRedBlocks), dim3(256), 0, s, a, na, b, nb, tmp);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)kRedBlocks, tmp, (uint64_t)0, out);
+}
+
 void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbytes) {
   const uint32_t te = (uint32_t)(kPullBlock * ipt);
   p.off = (csc ? g->coff : g->off).as<uint64_t>();
@@ -35,10 +65,12 @@ void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbyt
   part.alloc(2 * sizeof(double) * p.ntiles);
   const uint64_t nfb = (p.ntiles + kPullBlock - 1) / kPullBlock;
   part2.alloc(2 * sizeof(double) * nfb);
+  part3.alloc(2 * sizeof(double) * kRedBlocks);
   p.carryP = carryP.p;
   p.carryO = carryO.p;
   p.part = part.as<double>();
   p.part2 = part2.as<double>();
+  p.part3 = part3.as<double>();
 }
 
 // plain streaming copy (16-byte vectors): the calibration kernel for HBM byte

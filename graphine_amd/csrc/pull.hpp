@@ -151,6 +151,14 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_fixup(
 // Fixed-order reduction of n pairs into out[0], out[1] (single block).
 __global__ void k_reduce_pairs(const double* __restrict__ a, uint64_t na, const double* __restrict__ b,
                                uint64_t nb, double* __restrict__ out);
+// First level of the two-level form: block i writes its pair to out[2i..2i+1].
+constexpr int kRedBlocks = 256;
+__global__ void k_reduce_pairs_part(const double* __restrict__ a, uint64_t na, const double* __restrict__ b,
+                                    uint64_t nb, double* __restrict__ out);
+// Deterministic reduction of the pairs a[0..na) and b[0..nb) into out[0..1];
+// tmp holds 2*kRedBlocks doubles.
+void reduce_pairs(const double* a, uint64_t na, const double* b, uint64_t nb, double* out, double* tmp,
+                  hipStream_t s);
 
 // Host driver: one pull pass (tile + fix-up + reduce) over (off, idx).
 struct PullPlan {
@@ -165,6 +173,7 @@ struct PullPlan {
   void* carryO = nullptr;
   double* part = nullptr;   // 2*ntiles
   double* part2 = nullptr;  // 2*ceil(ntiles/256)
+  double* part3 = nullptr;  // 2*kRedBlocks
 };
 
 template <int IPT, class Mode>
@@ -185,8 +194,7 @@ void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream
     hipLaunchKernelGGL((k_pull_fixup<IPT, Mode>), dim3(nfb), dim3(kPullBlock), 0, s, p.off, p.tile_rb, p.E, p.V,
                        p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
   }
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, p.part, p.ntiles, p.part2, (uint64_t)nfb,
-                     scal_out);
+  reduce_pairs(p.part, p.ntiles, p.part2, (uint64_t)nfb, scal_out, p.part3, s);
 }
 
 template <class Mode>
@@ -202,7 +210,7 @@ void pull_pass(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t s
 
 // Allocates the plan's carry/partial buffers for a (graph, direction, ipt, T).
 struct PullBuffers {
-  DevBuf carryP, carryO, part, part2;
+  DevBuf carryP, carryO, part, part2, part3;
   void make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbytes);
 };
 

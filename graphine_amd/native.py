@@ -98,7 +98,10 @@ def _declare(L):
         "gre_pr_adjoint": ([vp, vp, i64, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_gradients": ([vp, dbl, vp, vp, vp], i32),
         "gre_pr_get_adjoint": ([vp, vp], i32),
+        "gre_pr_adjoint_cotangent": ([vp, vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_set_schedule": ([vp, i32, i32], i32),
+        "gre_pr_set_hot_sources": ([vp, u32], i32),
+        "gre_pr_get_hot_sources": ([vp, C.POINTER(u32)], i32),
         "gre_bfs": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_bfs_dev": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_msbfs": ([vp, vp, i32, vp, C.POINTER(u64)], i32),
@@ -315,6 +318,14 @@ class PageRank:
         _ck(lib().gre_pr_get_schedule(self.h, C.byref(s), C.byref(t)))
         return self.SCHEDULES.get(s.value, str(s.value)), t.value
 
+    def set_hot_sources(self, hot):
+        _ck(lib().gre_pr_set_hot_sources(self.h, int(hot)))
+
+    def hot_sources(self):
+        h = C.c_uint32()
+        _ck(lib().gre_pr_get_hot_sources(self.h, C.byref(h)))
+        return h.value
+
     def set_edge_weights(self, w=None, seed=0):
         if w is not None:
             w = np.ascontiguousarray(w, np.float32)
@@ -351,6 +362,15 @@ class PageRank:
         res = C.c_double()
         _ck(lib().gre_pr_adjoint(self.h, _p(t), len(t), damping, iters, tol, C.byref(it),
                                  C.byref(res)))
+        return it.value, res.value
+
+    def adjoint_cotangent(self, cot, damping=0.85, iters=1000, tol=0.0):
+        c = np.ascontiguousarray(cot, self.np_dtype)
+        assert c.size == self.g.nV * self.k
+        it = C.c_int()
+        res = C.c_double()
+        _ck(lib().gre_pr_adjoint_cotangent(self.h, _p(c), damping, iters, tol, C.byref(it),
+                                           C.byref(res)))
         return it.value, res.value
 
     def adjoint_vector(self):

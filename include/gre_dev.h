@@ -157,6 +157,10 @@ int gre_pr_get_ranks(gre_pr* p, void* host_out);
  * a pull over CSR; c = indicator of targets (ntgt ids, or dense nV x k in cvec). */
 int gre_pr_adjoint(gre_pr* p, const uint32_t* targets, int64_t ntgt, double damping, int iters,
                    double tol, int* iters_done, double* residual);
+/* The same adjoint seeded with an arbitrary cotangent c = d(loss)/d(ranks)
+ * (host, nV x k in the handle's dtype, the layout of gre_pr_get_ranks). */
+int gre_pr_adjoint_cotangent(gre_pr* p, const void* cot, double damping, int iters, double tol,
+                             int* iters_done, double* residual);
 /* d(loss)/d(weights): gW = d x_u lambda_v (normalised weights), graw via the
  * row normalisation; CSR order, summed over the k vectors; d(loss)/d(p). */
 int gre_pr_gradients(gre_pr* p, double damping, float* gW, float* graw, void* gp);
@@ -170,6 +174,10 @@ enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_
        GRE_SCHED_COMPACT_NT = 4 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
 int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
+/* COMPACT schedules: compact source ids >= hot are gathered with a
+ * non-temporal hint so the hottest sources stay cached (0 = off; tuned). */
+int gre_pr_set_hot_sources(gre_pr* p, uint32_t hot);
+int gre_pr_get_hot_sources(gre_pr* p, uint32_t* hot);
 
 /* ---- traversal kernels (plain single-source paths) ------------------------ */
 /* Direction-optimising BFS levels (int32, -1 unreached). edges_out: sum of the

@@ -30,6 +30,23 @@ def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     assert abs(x.sum() - 1.0) < 1e-12
 
 
+@pytest.mark.parametrize("schedule,hot", [(3, 1000), (4, 1 << 12), (3, 1 << 20)])
+def test_pagerank_hot_sources_same_result(N, O, schedule, hot):
+    # the cache hint on cold sources changes no arithmetic: bit-identical ranks
+    g = N.Graph.rmat(16, 16, 1, False)
+    ref = N.PageRank(g, N.F64)
+    ref.set_schedule(schedule, 8)
+    ref.run(0.85, 20)
+    pr = N.PageRank(g, N.F64)
+    pr.set_schedule(schedule, 8)
+    pr.set_hot_sources(hot)
+    assert pr.hot_sources() == hot
+    pr.run(0.85, 20)
+    assert np.array_equal(pr.ranks(), ref.ranks())
+    og = O.rmat_graph(16, 16, 1, False)
+    assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
+
+
 def test_pagerank_f32_within_1e6_of_f64(N, O):
     g = N.Graph.rmat(18, 16, 6, False)
     og = O.rmat_graph(18, 16, 6, False)
