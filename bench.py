@@ -3,8 +3,10 @@
 Headline (the `value`): FP32 PageRank, 20 iterations per step, on a directed
 R-MAT graph (scale 26, edge factor 16, seed 6) resident in HBM as CSR + CSC;
 GTEPS = stored edges x iterations / time.  The same JSON line carries:
-  roofline      the pull tile kernel's algorithmic bytes / its mean launch time
-                (HIP events around each launch, on the library stream);
+  roofline      one iteration's algorithmic bytes / the summed mean launch
+                times of the iteration's kernels (HIP events around each
+                launch, on the library stream), and the measured HBM traffic
+                of the kept schedule from its committed rocprofv3 summary;
   cpu_baseline  GRE's scatter/combine/apply superstep restated in C with 12
                 worker threads and 8192 striped locks, timed on this host on a
                 bounded sample (a few PageRank iterations of the same graph);
@@ -14,7 +16,7 @@ GTEPS = stored edges x iterations / time.  The same JSON line carries:
   ppr           32 personalised vectors in one batched launch sequence and the
                 adjoint on the same batch, vs the plain single-vector path.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26] [--schedule auto]
 """
 import argparse
 import json
@@ -48,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-sweep", action="store_true", help="skip the pull-schedule sweep")
+    ap.add_argument("--schedule", default="auto",
+                    help="pull schedule of the headline run: auto (autotuned) or NAME:TILE_ITEMS[:RANGES], "
+                         "e.g. compact_nt:4 or part:8:128")
     return ap.parse_args()
 
 
@@ -61,9 +66,10 @@ def sample_distinct(rng, pool, k):
     return pick if isinstance(pool, int) else np.asarray(pool)[pick]
 
 
-def pmc_traffic(scale):
-    """HBM bytes per pull-tile launch from the committed rocprofv3 --pmc summary, if present."""
-    f = os.path.join(ROOT, "profiles", f"pull_tile_pagerank_f32_s{scale}_pmc.txt")
+def pmc_traffic(scale, sched):
+    """HBM bytes per iteration from the committed rocprofv3 --pmc summary of this schedule, if present."""
+    # _nt variants stream the same bytes with a cache hint: one summary per family
+    f = os.path.join(ROOT, "profiles", f"pagerank_f32_{sched.replace('_nt', '')}_s{scale}_pmc.txt")
     if not os.path.exists(f):
         return None
     vals = {}
@@ -91,7 +97,13 @@ def main():
     g = N.Graph.rmat(S, 16, 6, False)
     log(f"directed R-MAT scale {S}: V={g.nV} E={g.nE} built in {time.time() - t0:.1f}s")
     pr = N.PageRank(g, N.F32)
-    pr.run(0.85, 2)                          # autotunes the tile size, warms up
+    if a.schedule != "auto":
+        parts = a.schedule.split(":")
+        code = {v: k for k, v in N.PageRank.SCHEDULES.items()}[parts[0]]
+        pr.set_schedule(code, int(parts[1]) if len(parts) > 1 else 8)
+        if len(parts) > 2:
+            pr.set_source_ranges(int(parts[2]))
+    pr.run(0.85, 2)                          # autotunes the schedule (unless fixed), warms up
     for _ in range(a.warmup):
         pr.run(0.85, iters)
     N.sync()
@@ -111,20 +123,32 @@ def main():
     N.sync()
     n_tile, ms_tile = N.prof_get("pull_tile")
     n_fix, ms_fix = N.prof_get("pull_fixup")
+    n_cmb, ms_cmb = N.prof_get("part_combine")
+    n_cfx, ms_cfx = N.prof_get("part_combine_fixup")
     N.prof_enable(False)
     V, E = g.nV, g.nE
     o, s_ = 8, 4
     bytes_iter = o * (V + 1) + (4 + s_) * E + (3 * s_ + 4) * V
+    # one PageRank iteration = the pull tile kernel, its fix-up and (partitioned
+    # schedules) the per-row combine of the range partial sums
     avg_tile_s = ms_tile / max(n_tile, 1) / 1e3
-    achieved = bytes_iter / avg_tile_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(S),
-                "kernel": "k_pull_tile (PageRank f32 iteration)",
-                "alg_bytes_per_launch": bytes_iter, "avg_launch_ms": round(avg_tile_s * 1e3, 4),
-                "fixup_avg_ms": round(ms_fix / max(n_fix, 1), 4)}
-    log(f"pull_tile avg {avg_tile_s * 1e3:.3f} ms -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
+    avg_fix_s = ms_fix / max(n_fix, 1) / 1e3
+    avg_cmb_s = (ms_cmb / max(n_cmb, 1) + ms_cfx / max(n_cfx, 1)) / 1e3 if n_cmb else 0.0
+    iter_s = avg_tile_s + avg_fix_s + avg_cmb_s
+    achieved = bytes_iter / iter_s / 1e9
     sched, tile_items = pr.schedule()
-    roofline["schedule"] = f"{sched}, {tile_items} entries/thread, hot sources {pr.hot_sources()} (autotuned)"
+    kern = "k_pull_tile + k_pull_fixup" + (" (segments, then per-row)" if n_cmb else "")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(S, sched),
+                "kernel": f"{kern} (one PageRank f32 iteration)",
+                "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4),
+                "pull_tile_avg_ms": round(avg_tile_s * 1e3, 4), "fixup_avg_ms": round(avg_fix_s * 1e3, 4),
+                "combine_avg_ms": round(avg_cmb_s * 1e3, 4)}
+    log(f"iteration kernels {iter_s * 1e3:.3f} ms (tile {avg_tile_s * 1e3:.3f}, fixup {avg_fix_s * 1e3:.3f}, "
+        f"combine {avg_cmb_s * 1e3:.3f}) -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
+    roofline["schedule"] = (f"{sched}, {tile_items} entries/thread"
+                            + (f", {pr.source_ranges()} source ranges" if sched.startswith("part") else "")
+                            + (" (autotuned)" if a.schedule == "auto" else " (fixed)"))
     # achievable streaming bandwidth on this device: a plain copy kernel
     copy_bytes = 4 << 30
     ms_copy = N.copy_probe(copy_bytes, 10)
@@ -133,24 +157,27 @@ def main():
     if not a.no_sweep:
         # every schedule the autotuner chooses from, timed on the same graph
         sweep = {}
-        cand = [(sc, ti, 0) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
-        cand += [(sc, 8, h) for sc in (3, 4) for h in (1 << 18, 1 << 20, 1 << 22)]
-        for sc, ti, hot in cand:
+        cand = [(sc, ti, 64) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
+        cand += [(sc, ti, r) for sc in (5, 6) for ti in (4, 8, 16) for r in (32, 64, 128)]
+        for sc, ti, r in cand:
             q = N.PageRank(g, N.F32)
             q.set_schedule(sc, ti)
-            q.set_hot_sources(hot)
+            q.set_source_ranges(r)
             q.run(0.85, 2)
             N.prof_reset()
             N.prof_enable(True)
             q.run(0.85, 10)
             N.sync()
-            nt_, mt_ = N.prof_get("pull_tile")
+            ms_ = 0.0
+            for kname in ("pull_tile", "pull_fixup", "part_combine", "part_combine_fixup"):
+                nt_, mt_ = N.prof_get(kname)
+                ms_ += mt_ / max(nt_, 1)
             N.prof_enable(False)
-            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/hot={hot}" if hot else "")
-            sweep[key] = round(mt_ / max(nt_, 1), 3)
+            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/R={r}" if sc >= 5 else "")
+            sweep[key] = round(ms_, 3)
             del q
         roofline["schedule_sweep_ms"] = sweep
-        log("schedule sweep (ms per pull_tile launch):", sweep)
+        log("schedule sweep (kernel ms per iteration):", sweep)
 
     # ---------------- CPU baseline (bounded sample, same graph) ---------------
     cpu = None

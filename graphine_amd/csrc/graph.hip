@@ -366,10 +366,14 @@ const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntile
   if (it != g->tiles.end()) return it->second.as<uint32_t>();
   DevBuf& b = g->tiles[key];
   b.alloc(sizeof(uint32_t) * (nt + 1));
-  hipLaunchKernelGGL(k_tiles, dim3(grid_cap(nt + 1)), dim3(kBlock), 0, stream(),
-                     (csc ? g->coff : g->off).as<uint64_t>(), g->nV, nt, te, b.as<uint32_t>());
-  GRE_HIP(hipGetLastError());
+  build_tile_rows((csc ? g->coff : g->off).as<uint64_t>(), g->nV, nt, te, b.as<uint32_t>(), stream());
   return b.as<uint32_t>();
+}
+
+void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_tiles, dim3(grid_cap(ntiles + 1)), dim3(kBlock), 0, s, off, nrows, ntiles, te, rb);
+  GRE_HIP(hipGetLastError());
 }
 
 }  // namespace gre

@@ -6,7 +6,37 @@
 #include "common.hpp"
 
 #include <map>
+#include <memory>
 #include <utility>
+
+namespace gre {
+// Source-range partition of the CSC for XCD-local gathers.  Compact source ids
+// are cut into R ranges of equal edge count; range r belongs to XCD class
+// r % 8.  Entries are regrouped by (class, range, row): a (row, range) run is a
+// segment, and the pull reduces segments instead of rows.  A second pull over
+// each row's list of segment ids sums them in a fixed order.
+struct XPart {
+  int R = 0;
+  uint64_t nseg = 0;
+  DevBuf idx;        // uint32 [E]   compact source ids, grouped by (class, range, row)
+  DevBuf seg_off;    // uint64 [nseg+1]
+  DevBuf pair_off;   // uint64 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
+  DevBuf pair_list;  // uint32 [nseg] segment ids, by row then class-major range
+  uint64_t class_start[9] = {0};  // first entry of each class; class_start[8] = E
+  struct Tiles {
+    DevBuf rb;       // uint32 [ntiles+1] tile row bounds over segments
+    DevBuf xmap;     // uint32 [9] first tile of each class; xmap[8] = ntiles
+    uint64_t ntiles = 0;
+    unsigned grid = 0;  // 8 * (largest class tile count)
+  };
+  std::map<uint32_t, Tiles> tiles;  // by tile edges
+  struct PairTiles {
+    DevBuf rb;       // uint32 [ntiles+1] tile row bounds over the pair lists
+    uint64_t ntiles = 0;
+  };
+  std::map<uint32_t, PairTiles> pair_tiles;  // by tile entries
+};
+}  // namespace gre
 
 struct gre_graph {
   uint32_t nV = 0;
@@ -25,6 +55,8 @@ struct gre_graph {
   // cidx_c[p] = cid[cidx[p]] (CSC in compact source ids)
   gre::DevBuf cid, cidx_c;
   uint32_t nsrc = 0;
+  // source-range partitions, by range count (built on demand, shared with handles)
+  std::map<int, std::shared_ptr<gre::XPart>> xpart;
 };
 
 namespace gre {
@@ -36,6 +68,15 @@ const uint32_t* graph_w_csc(gre_graph* g);
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles);
 // Builds g->cid / g->cidx_c / g->nsrc once (hub-sorted compact source ids).
 void graph_compact_sources(gre_graph* g);
+// tile_rb over an arbitrary offset array of nrows rows (see graph_tiles).
+void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
+                     hipStream_t s);
+// The R-range source partition of g (R in {8, 16, ..., 256}), built once and cached.
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R);
+// Its tiles of te entries (cached on the partition).
+const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s);
+// Tiles of te entries over the per-row pair lists (rows = vertices).
+const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hipStream_t s);
 
 // device helpers
 __device__ inline uint32_t row_of(const uint64_t* __restrict__ off, uint32_t nV, uint64_t e) {

@@ -11,6 +11,7 @@
 // order block partials, so every run gives the same bits.
 #include "pr.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 
@@ -40,10 +41,8 @@ struct ModeFwd {
   const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
-  uint32_t hot;           // compact ids >= hot are gathered with a non-temporal hint (0: off)
   __device__ TT gather(uint64_t e, uint32_t u) const {
     if (w) return (TT)w[e] * gin[u];
-    if (hot && u >= hot) return __builtin_nontemporal_load(gin + u);
     return gin[u];
   }
   __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const {
@@ -53,7 +52,7 @@ struct ModeFwd {
     const TT old = xin[r];
     xout[r] = y;
     const int32_t d = outdeg[r];
-    if (!w) {
+    if (!w && contrib_out) {
       if (cid) { if (d > 0) contrib_out[cid[r]] = y / (TT)d; }
       else contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
     }
@@ -88,6 +87,16 @@ struct ModeAdj {
     a0 += (double)(p ? p[u] : (TT)invn) * (double)y;
     a1 += fabs((double)y - (double)old);
   }
+};
+
+// partitioned pull, first pass: a segment's sum of contrib over its sources
+template <typename TT>
+struct ModeSeg {
+  using T = TT;
+  const TT* gin;  // contrib, compact source ids
+  TT* sum;        // per segment
+  __device__ TT gather(uint64_t, uint32_t u) const { return gin[u]; }
+  __device__ void finalize(uint32_t seg, TT s, double&, double&) const { sum[seg] = s; }
 };
 
 // row sums of the raw weights (pull over CSR, gather = weight)
@@ -225,8 +234,33 @@ void ensure_scal(gre_pr* P, int iters) {
 // its gathers read contrib[], which then holds one entry per vertex with
 // out-edges, hubs first (graph_compact_sources).
 bool use_compact(const gre_pr* P) { return P->compact && !P->weighted && P->k == 1; }
+bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
+// the source-range partition, its segment plan and the segment sums
+void ensure_part_plan(gre_pr* P, int ipt) {
+  if (!P->xp || P->xp->R != P->ranges) {
+    P->xp = graph_xpart(P->g, P->ranges);
+    P->plan_ipt_x = 0;
+  }
+  if (P->plan_ipt_x != ipt) {
+    P->buf_x.make(P->plan_x, *P->xp, ipt, tsize(P->dtype));
+    // the per-row pass always uses 8 entries per thread
+    P->buf_px.make_pairs(P->plan_px, *P->xp, P->g->nV, 8, tsize(P->dtype));
+    P->plan_px.tag = "part_combine";
+    P->plan_px.fix_tag = "part_combine_fixup";
+    P->plan_ipt_x = ipt;
+  }
+  P->plan_x.nt = P->nt;
+  P->plan_px.nt = false;
+  const size_t need = tsize(P->dtype) * (size_t)P->xp->nseg;
+  if (P->xsum.bytes < need) P->xsum.alloc(need);
+}
 
 void ensure_plan(gre_pr* P, bool csc, int ipt) {
+  if (csc && use_part(P)) {
+    graph_compact_sources(P->g);
+    ensure_part_plan(P, ipt);
+    return;
+  }
   int& have = csc ? P->plan_ipt_csc : P->plan_ipt_csr;
   PullPlan& plan = csc ? P->plan_csc : P->plan_csr;
   if (have != ipt) {
@@ -256,7 +290,6 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
   m.outdeg = P->g->outdeg.as<int32_t>();
   m.cid = use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr;
-  m.hot = use_compact(P) ? P->hot : 0u;
   m.damp = damp;
   m.invn = 1.0 / (double)P->g->nV;
   return m;
@@ -267,6 +300,15 @@ template <typename TT>
 void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, hipStream_t s) {
   ModeFwd<TT> m = fwd_mode<TT>(P, damp);
   m.scal_in = scal_in;
+  if (use_part(P)) {
+    // segment sums over XCD-local source ranges, then each row's segments
+    ModeSeg<TT> ms{m.gin, P->xsum.as<TT>()};
+    pull_pass(P->plan_x, ms, nullptr, s);
+    m.gin = P->xsum.as<TT>();
+    pull_pass(P->plan_px, m, scal_out, s);
+    P->ccur ^= 1;
+    return;
+  }
   // in-place x for the unweighted form: residual reads xin before xout writes
   pull_pass(P->plan_csc, m, scal_out, s);
   if (P->weighted) P->xcur ^= 1;
@@ -320,8 +362,9 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
 void apply_schedule(gre_pr* P, int sc, int ipt) {
   P->schedule = sc;
   P->ipt = ipt;
-  P->nt = sc == GRE_SCHED_TILE_NT || sc == GRE_SCHED_COMPACT_NT;
-  P->compact = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT;
+  P->nt = sc == GRE_SCHED_TILE_NT || sc == GRE_SCHED_COMPACT_NT || sc == GRE_SCHED_PART_NT;
+  P->part = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
+  P->compact = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT || P->part;
 }
 
 // Autotuner: times every (schedule, tile size) candidate on two forward
@@ -337,24 +380,26 @@ void autotune_forward(gre_pr* P, double damp) {
     scheds.push_back(GRE_SCHED_COMPACT);
     scheds.push_back(GRE_SCHED_COMPACT_NT);
   }
-  int best_s = GRE_SCHED_TILE, best_i = 8;
-  uint32_t best_h = 0;
+  const bool can_part = !P->weighted && P->k == 1 && P->g->nE > 0 && P->g->nE < 0xffffffffULL;
+  if (can_part) {
+    scheds.push_back(GRE_SCHED_PART);
+    scheds.push_back(GRE_SCHED_PART_NT);
+  }
+  int best_s = GRE_SCHED_TILE, best_i = 8, best_r = P->ranges;
   float best_ms = 1e30f;
   hipEvent_t a, b;
   GRE_HIP(hipEventCreate(&a));
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
   for (int sc : scheds) {
-    const bool cmp = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT;
-    // compact schedules also try keeping only the hottest sources cacheable
-    std::vector<uint32_t> hots = {0u};
-    if (cmp)
-      for (uint32_t h : {1u << 18, 1u << 20, 1u << 22})
-        if (h < P->g->nV) hots.push_back(h);
-    for (uint32_t h : hots) {
+    const bool prt = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
+    // partitioned schedules also try 32, 64 and 128 source ranges
+    std::vector<int> vars = {0};
+    if (prt) vars = {32, 64, 128};
+    for (int h : vars) {
       for (int ipt : ipts) {
         apply_schedule(P, sc, ipt);
-        P->hot = h;
+        if (prt) P->ranges = h;
         ensure_plan(P, true, ipt);
         fwd_init<TT>(P, s);
         double* scal = P->scal.as<double>();
@@ -366,14 +411,25 @@ void autotune_forward(gre_pr* P, double damp) {
         GRE_HIP(hipEventSynchronize(b));
         float ms = 0.f;
         GRE_HIP(hipEventElapsedTime(&ms, a, b));
-        if (ms < best_ms) { best_ms = ms; best_s = sc; best_i = ipt; best_h = h; }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_s = sc;
+          best_i = ipt;
+          if (prt) best_r = h;
+        }
       }
     }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   apply_schedule(P, best_s, best_i);
-  P->hot = best_h;
+  P->ranges = best_r;
+  if (!P->part) P->xp.reset();
+  // partitions of the candidates that lost, unless another handle holds them
+  for (auto it = P->g->xpart.begin(); it != P->g->xpart.end();) {
+    if (it->second.use_count() == 1) it = P->g->xpart.erase(it);
+    else ++it;
+  }
 }
 
 template <typename TT>
@@ -541,7 +597,7 @@ int gre_pr_destroy(gre_pr* P) {
 int gre_pr_set_schedule(gre_pr* P, int schedule, int tile_items) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_COMPACT_NT, GRE_DEV_NULL,
+    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_PART_NT, GRE_DEV_NULL,
               "schedule must be one of GRE_SCHED_*");
     GRE_CHECK(tile_items == 0 || tile_items == 4 || tile_items == 8 || tile_items == 16, GRE_DEV_NULL,
               "tile items per thread must be 4, 8 or 16");
@@ -558,17 +614,19 @@ int gre_pr_get_schedule(gre_pr* P, int* schedule, int* tile_items) {
   });
 }
 
-int gre_pr_set_hot_sources(gre_pr* P, uint32_t hot) {
+int gre_pr_set_source_ranges(gre_pr* P, int ranges) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    P->hot = hot;
+    GRE_CHECK(ranges == 8 || ranges == 16 || ranges == 32 || ranges == 64 || ranges == 128 || ranges == 256,
+              GRE_DEV_NULL, "source ranges must be 8, 16, 32, 64, 128 or 256");
+    P->ranges = ranges;
   });
 }
 
-int gre_pr_get_hot_sources(gre_pr* P, uint32_t* hot) {
+int gre_pr_get_source_ranges(gre_pr* P, int* ranges) {
   return guard([&] {
-    GRE_CHECK(P && hot, GRE_DEV_NULL, "null argument");
-    *hot = P->hot;
+    GRE_CHECK(P && ranges, GRE_DEV_NULL, "null argument");
+    *ranges = P->ranges;
   });
 }
 

@@ -20,7 +20,13 @@ struct gre_pr {
   int schedule = 0;       // 0 = autotune; else GRE_SCHED_* (gre_dev.h)
   bool nt = false;        // non-temporal neighbour-id stream
   bool compact = false;   // hub-sorted compact source ids (unweighted, k == 1)
-  uint32_t hot = 0;       // compact schedules: ids >= hot gathered non-temporally (0 = off)
+  bool part = false;      // source-range partitioned pull (implies compact)
+  int ranges = 64;        // source ranges of the partitioned pull
+  std::shared_ptr<gre::XPart> xp;  // the partition in use
+  gre::DevBuf xsum;       // per-segment partial sums (compute type)
+  gre::PullPlan plan_x, plan_px;   // segment pull; per-row pull over the segment sums
+  gre::PullBuffers buf_x, buf_px;
+  int plan_ipt_x = 0;
   bool has_p = false;
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place

@@ -60,6 +60,42 @@ void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbyt
   p.E = g->nE;
   p.V = g->nV;
   p.ipt = ipt;
+  p.xmap = nullptr;
+  p.grid = 0;
+  alloc_for(p, tbytes);
+}
+
+void PullBuffers::make(PullPlan& p, XPart& xp, int ipt, size_t tbytes) {
+  const uint32_t te = (uint32_t)(kPullBlock * ipt);
+  const XPart::Tiles& T = xpart_tiles(xp, te, stream());
+  p.off = xp.seg_off.as<uint64_t>();
+  p.idx = xp.idx.as<uint32_t>();
+  p.tile_rb = T.rb.as<uint32_t>();
+  p.ntiles = T.ntiles;
+  p.E = xp.class_start[8];
+  p.V = (uint32_t)xp.nseg;
+  p.ipt = ipt;
+  p.xmap = T.xmap.as<uint32_t>();
+  p.grid = T.grid;
+  alloc_for(p, tbytes);
+}
+
+void PullBuffers::make_pairs(PullPlan& p, XPart& xp, uint32_t nV, int ipt, size_t tbytes) {
+  const uint32_t te = (uint32_t)(kPullBlock * ipt);
+  const XPart::PairTiles& T = xpart_pair_tiles(xp, nV, te, stream());
+  p.off = xp.pair_off.as<uint64_t>();
+  p.idx = xp.pair_list.as<uint32_t>();
+  p.tile_rb = T.rb.as<uint32_t>();
+  p.ntiles = T.ntiles;
+  p.E = xp.nseg;
+  p.V = nV;
+  p.ipt = ipt;
+  p.xmap = nullptr;
+  p.grid = 0;
+  alloc_for(p, tbytes);
+}
+
+void PullBuffers::alloc_for(PullPlan& p, size_t tbytes) {
   carryP.alloc(tbytes * p.ntiles);
   carryO.alloc(tbytes * p.ntiles);
   part.alloc(2 * sizeof(double) * p.ntiles);

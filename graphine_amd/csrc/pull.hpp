@@ -36,24 +36,38 @@ __device__ inline void block_sum2(double& a, double& b, double* sh /*[2*4]*/) {
 //   using T = ...;
 //   __device__ T gather(uint64_t e, uint32_t nbr) const;
 //   __device__ void finalize(uint32_t row, T sum, double& acc0, double& acc1) const;
-template <int IPT, bool NT, class Mode>
-__global__ __launch_bounds__(kPullBlock) void k_pull_tile(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+
+// Fixed-order sum of two doubles over one 256-thread group (gid-th group of the
+// block); tid 0 of the group gets the totals.  red holds 8 doubles per group.
+// Every thread of the block must call it (it holds a block barrier).
+__device__ inline void group_sum2(double& a, double& b, double* red, int tid) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = tid >> 6, l = tid & 63;
+  if (l == 0) { red[w] = a; red[4 + w] = b; }
+  __syncthreads();
+  if (tid == 0) {
+    a = red[0] + red[1] + red[2] + red[3];
+    b = red[4] + red[5] + red[6] + red[7];
+  }
+}
+
+// One tile of TE = 256*IPT entries, processed by one 256-thread group (tid in
+// [0, 256)).  vals/longj/nlong/red are the group's LDS.  gat(e, u) gives the
+// value gathered for entry e with neighbour u.  valid = false: the group has no
+// tile this round but still meets every block barrier.
+template <int IPT, bool NT, class Mode, class Gather>
+__device__ __forceinline__ void pull_tile_body(
+    uint64_t t, bool valid, const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
-    typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
-    double* __restrict__ part, Mode m) {
+    typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
+    const Mode& m, const Gather& gat, typename Mode::T* vals, uint32_t* longj, int* nlong, double* red,
+    int tid) {
   using T = typename Mode::T;
   constexpr int TE = kPullBlock * IPT;
-  __shared__ T vals[TE];
-  __shared__ uint32_t longj[TE / kLongRow + 2];
-  __shared__ int nlong;
-  __shared__ double red[8];
-
-  const uint64_t t = blockIdx.x;
   const uint64_t e0 = t * (uint64_t)TE;
-  const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
+  const uint64_t e1 = valid ? (e0 + TE < E ? e0 + TE : E) : e0;
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
-  const int tid = threadIdx.x;
 
   // stage 1: neighbour ids (coalesced; NT = streamed with a non-temporal hint
   // so they do not push the gathered values out of the caches), then gathers
@@ -67,17 +81,21 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = j * kPullBlock + tid;
-    if (i < n) vals[i] = m.gather(e0 + i, nb[j]);
+    if (i < n) vals[i] = gat(e0 + i, nb[j]);
   }
-  if (tid == 0) nlong = 0;
+  if (tid == 0) *nlong = 0;
   __syncthreads();
 
   // stage 2: rows that start in the tile, plus the row continuing into it
-  const uint32_t rb = tile_rb[t];
-  const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
-  const bool has_rc = rb > 0 && n > 0 && off[rb] > e0;
-  const uint32_t nr = (re - rb) + (has_rc ? 1u : 0u);
   double acc0 = 0.0, acc1 = 0.0;
+  uint32_t rb = 0, nr = 0;
+  bool has_rc = false;
+  if (valid) {
+    rb = tile_rb[t];
+    const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
+    has_rc = rb > 0 && n > 0 && off[rb] > e0;
+    nr = (re - rb) + (has_rc ? 1u : 0u);
+  }
   for (uint32_t j = tid; j < nr; j += kPullBlock) {
     const uint32_t r = has_rc ? (j == 0 ? rb - 1 : rb + j - 1) : rb + j;
     const uint64_t ob = off[r], oe = off[r + 1];
@@ -85,7 +103,7 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
     const uint64_t se = oe < e1 ? oe : e1;
     const int len = se > sb ? (int)(se - sb) : 0;
     if (len > kLongRow) {
-      const int k = atomicAdd(&nlong, 1);
+      const int k = atomicAdd(nlong, 1);
       longj[k] = j;
       continue;
     }
@@ -100,7 +118,7 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
 
   // stage 3: long rows, one wave each
   const int w = tid >> 6, l = tid & 63;
-  const int nl = nlong;
+  const int nl = *nlong;
   for (int q = w; q < nl; q += kPullBlock / 64) {
     const uint32_t j = longj[q];
     const uint32_t r = has_rc ? (j == 0 ? rb - 1 : rb + j - 1) : rb + j;
@@ -118,8 +136,39 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
     }
   }
 
-  block_sum2(acc0, acc1, red);
-  if (tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+  group_sum2(acc0, acc1, red, tid);
+  if (valid && tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+}
+
+template <class Mode>
+struct ModeGather {
+  const Mode& m;
+  __device__ typename Mode::T operator()(uint64_t e, uint32_t u) const { return m.gather(e, u); }
+};
+
+template <int IPT, bool NT, class Mode>
+__global__ __launch_bounds__(kPullBlock) void k_pull_tile(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
+    typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
+    double* __restrict__ part, const uint32_t* __restrict__ xmap, Mode m) {
+  using T = typename Mode::T;
+  constexpr int TE = kPullBlock * IPT;
+  __shared__ T vals[TE];
+  __shared__ uint32_t longj[TE / kLongRow + 2];
+  __shared__ int nlong;
+  __shared__ double red[8];
+
+  // xmap: workgroup b runs tile xmap[c] + b / 8 of class c = b % 8, so every
+  // tile of a class runs on the same XCD (workgroups go to XCD b % 8)
+  uint64_t t = blockIdx.x;
+  if (xmap) {
+    const uint32_t c = blockIdx.x & 7u;
+    t = (uint64_t)xmap[c] + (blockIdx.x >> 3);
+    if (t >= xmap[c + 1]) return;
+  }
+  pull_tile_body<IPT, NT>(t, true, off, idx, tile_rb, E, V, ntiles, carryP, carryO, part, m, ModeGather<Mode>{m},
+                          vals, longj, &nlong, red, (int)threadIdx.x);
 }
 
 // Fix-up: tile t owns the row that starts in it and runs past its end.
@@ -174,27 +223,41 @@ struct PullPlan {
   double* part = nullptr;   // 2*ntiles
   double* part2 = nullptr;  // 2*ceil(ntiles/256)
   double* part3 = nullptr;  // 2*kRedBlocks
+  const uint32_t* xmap = nullptr;  // class tile starts (source-range partitions) or nullptr
+  unsigned grid = 0;               // workgroups of the tile kernel when xmap is set
+  const char* tag = "pull_tile";   // profiling names of the two kernels
+  const char* fix_tag = "pull_fixup";
 };
+
+// Fix-up of the rows that span tiles, then (scal_out != nullptr) the fixed-order
+// reduction of the tiles' (acc0, acc1) pairs into scal_out.
+template <int IPT, class Mode>
+void pull_finish(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t s) {
+  using T = typename Mode::T;
+  const unsigned nfb = (unsigned)((p.ntiles + kPullBlock - 1) / kPullBlock);
+  {
+    ProfScope ps(p.fix_tag);
+    hipLaunchKernelGGL((k_pull_fixup<IPT, Mode>), dim3(nfb), dim3(kPullBlock), 0, s, p.off, p.tile_rb, p.E, p.V,
+                       p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
+  }
+  // scal_out == nullptr: the mode's finalize accumulates nothing (segment sums)
+  if (scal_out) reduce_pairs(p.part, p.ntiles, p.part2, (uint64_t)nfb, scal_out, p.part3, s);
+}
 
 template <int IPT, class Mode>
 void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t s) {
   using T = typename Mode::T;
   {
-    ProfScope ps("pull_tile");
+    ProfScope ps(p.tag);
+    const unsigned grid = p.xmap ? p.grid : (unsigned)p.ntiles;
     if (p.nt)
-      hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
-                         p.idx, p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, m);
+      hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
+                         p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
     else
-      hipLaunchKernelGGL((k_pull_tile<IPT, false, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
-                         p.idx, p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, m);
+      hipLaunchKernelGGL((k_pull_tile<IPT, false, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
+                         p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
   }
-  const unsigned nfb = (unsigned)((p.ntiles + kPullBlock - 1) / kPullBlock);
-  {
-    ProfScope ps("pull_fixup");
-    hipLaunchKernelGGL((k_pull_fixup<IPT, Mode>), dim3(nfb), dim3(kPullBlock), 0, s, p.off, p.tile_rb, p.E, p.V,
-                       p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
-  }
-  reduce_pairs(p.part, p.ntiles, p.part2, (uint64_t)nfb, scal_out, p.part3, s);
+  pull_finish<IPT>(p, m, scal_out, s);
 }
 
 template <class Mode>
@@ -212,6 +275,11 @@ void pull_pass(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t s
 struct PullBuffers {
   DevBuf carryP, carryO, part, part2, part3;
   void make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbytes);
+  // the same over the segments of a source-range partition
+  void make(PullPlan& p, XPart& xp, int ipt, size_t tbytes);
+  // and over its per-row segment lists (rows = the nV vertices)
+  void make_pairs(PullPlan& p, XPart& xp, uint32_t nV, int ipt, size_t tbytes);
+  void alloc_for(PullPlan& p, size_t tbytes);
 };
 
 }  // namespace gre

@@ -1,0 +1,307 @@
+// Source-range partition of the CSC, for pulls whose gathers stay inside one
+// XCD's L2.
+//
+// MI355X has 8 XCDs, each with its own 4 MB L2, and workgroups are dispatched
+// to them round-robin by workgroup id.  A plain pull gathers from the whole
+// source array on every XCD, so each L2 holds the same hot sources and misses
+// on the rest.  Here the hub-sorted compact source ids are cut into R ranges of
+// equal edge count; range r is owned by XCD class r % 8, and the CSC entries
+// are regrouped by (class, range, row).  Each run of one row's entries from one
+// range is a segment; the pull reduces segments, and a second tile-segmented
+// pull over per-row lists of segment ids sums each row's segments in a fixed
+// order.
+#include "graph.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+
+namespace gre {
+namespace {
+
+constexpr unsigned kB = 256;
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kB - 1) / kB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+// degc[cid[v]] = out-degree of v, for the vertices with out-edges
+__global__ void k_deg_compact(const int32_t* __restrict__ outdeg, const uint32_t* __restrict__ cid, uint32_t nV,
+                              uint64_t* __restrict__ degc) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t d = outdeg[v];
+    if (d > 0) degc[cid[v]] = (uint64_t)d;
+  }
+}
+
+// bnd[r] = first compact id c in [0, nsrc] with ex[c] >= r*E/R (ex: exclusive
+// prefix of the compact out-degrees, ex[nsrc] = E); one thread per r in [0, R]
+__global__ void k_bounds(const uint64_t* __restrict__ ex, uint32_t nsrc, uint64_t E, int R,
+                         uint32_t* __restrict__ bnd) {
+  const int r = (int)threadIdx.x;
+  if (r > R) return;
+  const uint64_t target = (uint64_t)r * E / (uint64_t)R;
+  uint32_t lo = 0, hi = nsrc;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ex[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  bnd[r] = lo;
+}
+
+// Per CSC entry e: the sort key (class-major range position) and the value
+// (row << 32 | compact source).  The row is found inside the entry's
+// 2048-entry tile (rb: graph_tiles of the CSC).
+__global__ void k_xkeys(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ rb, uint32_t te,
+                        const uint32_t* __restrict__ cidx_c, uint64_t E, const uint32_t* __restrict__ bnd, int R,
+                        uint8_t* __restrict__ key, uint64_t* __restrict__ val) {
+  __shared__ uint32_t sb[257];
+  for (int i = (int)threadIdx.x; i <= R; i += (int)blockDim.x) sb[i] = bnd[i];
+  __syncthreads();
+  const int per = R / 8;
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = e / te;
+    // largest row r in [lo, hi) with coff[r] <= e
+    uint32_t lo = rb[t] > 0 ? rb[t] - 1 : 0, hi = rb[t + 1];
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (coff[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t s = cidx_c[e];
+    // range = (number of inner bounds <= s)
+    int a = 1, b = R;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (sb[mid] <= s) a = mid + 1;
+      else b = mid;
+    }
+    const int r = a - 1;
+    key[e] = (uint8_t)((r & 7) * per + (r >> 3));
+    val[e] = ((uint64_t)lo << 32) | s;
+  }
+}
+
+__global__ void k_xflags(const uint8_t* __restrict__ key, const uint64_t* __restrict__ val, uint64_t E,
+                         uint32_t* __restrict__ flag, uint32_t* __restrict__ idx) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = val[i];
+    const bool f = i == 0 || key[i] != key[i - 1] || (v >> 32) != (val[i - 1] >> 32);
+    flag[i] = f ? 1u : 0u;
+    idx[i] = (uint32_t)v;
+  }
+}
+
+__global__ void k_xsegs(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ segid,
+                        const uint64_t* __restrict__ val, uint64_t E, uint64_t* __restrict__ seg_off,
+                        uint32_t* __restrict__ seg_row) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (flag[i]) {
+      const uint32_t s = segid[i] - 1;
+      seg_off[s] = i;
+      seg_row[s] = (uint32_t)(val[i] >> 32);
+    }
+  }
+}
+
+// cs[c] = first entry whose key belongs to class >= c (keys sorted ascending)
+__global__ void k_xclass(const uint8_t* __restrict__ key, uint64_t E, int per, uint64_t* __restrict__ cs) {
+  const int c = threadIdx.x;
+  if (c > 8) return;
+  const unsigned target = (unsigned)(c * per);
+  uint64_t lo = 0, hi = E;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (key[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  cs[c] = lo;
+}
+
+__global__ void k_xpcount(const uint32_t* __restrict__ seg_row, uint64_t nseg, uint32_t* __restrict__ cnt) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nseg; s += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[seg_row[s]], 1u);
+}
+
+__global__ void k_iota32(uint32_t* __restrict__ o, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    o[i] = (uint32_t)i;
+}
+
+unsigned bits_of(uint64_t n) {
+  unsigned b = 1;
+  while (b < 32 && (1ULL << b) < n) ++b;
+  return b;
+}
+
+}  // namespace
+
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
+  GRE_CHECK(R == 8 || R == 16 || R == 32 || R == 64 || R == 128 || R == 256, GRE_DEV_NULL,
+            "source ranges must be 8, 16, 32, 64, 128 or 256");
+  auto it = g->xpart.find(R);
+  if (it != g->xpart.end()) return it->second;
+  const uint64_t E = g->nE;
+  const uint32_t nV = g->nV;
+  GRE_CHECK(E > 0 && E < 0xffffffffULL, GRE_DEV_NULL, "the partitioned schedule needs 0 < edges < 2^32");
+  graph_compact_sources(g);
+  hipStream_t s = stream();
+  const uint32_t nsrc = g->nsrc;
+  auto xp = std::make_shared<XPart>();
+  xp->R = R;
+
+  // range bounds from the compact out-degree prefix
+  DevBuf bnd;
+  bnd.alloc(4 * (size_t)(R + 1));
+  {
+    DevBuf degc, ex;
+    degc.alloc(8 * ((size_t)nsrc + 1));
+    ex.alloc(8 * ((size_t)nsrc + 1));
+    GRE_HIP(hipMemsetAsync(degc.p, 0, 8 * ((size_t)nsrc + 1), s));
+    hipLaunchKernelGGL(k_deg_compact, dim3(gcap(nV)), dim3(kB), 0, s, g->outdeg.as<int32_t>(), g->cid.as<uint32_t>(),
+                       nV, degc.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
+    scan_excl_u64(degc.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nsrc + 1, s);
+    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(320), 0, s, ex.as<uint64_t>(), nsrc, E, R, bnd.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+
+  // keys and values, stable sort by key (rows stay ascending inside a key)
+  DevBuf key_s, val_s;
+  key_s.alloc(E);
+  val_s.alloc(8 * E);
+  {
+    DevBuf key, val;
+    key.alloc(E);
+    val.alloc(8 * E);
+    uint64_t nt = 0;
+    const uint32_t te = 2048;
+    const uint32_t* rb = graph_tiles(g, true, te, &nt);
+    hipLaunchKernelGGL(k_xkeys, dim3(gcap(E)), dim3(kB), 0, s, g->coff.as<uint64_t>(), rb, te,
+                       g->cidx_c.as<uint32_t>(), E, bnd.as<uint32_t>(), R, key.as<uint8_t>(), val.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
+    size_t tb = 0;
+    const unsigned end_bit = bits_of((uint64_t)R);
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint8_t>(), key_s.as<uint8_t>(), val.as<uint64_t>(),
+                                      val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
+    void* tmp = scratch(0, tb);
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, key.as<uint8_t>(), key_s.as<uint8_t>(), val.as<uint64_t>(),
+                                      val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+
+  // class starts
+  {
+    DevBuf cs;
+    cs.alloc(8 * 9);
+    hipLaunchKernelGGL(k_xclass, dim3(1), dim3(64), 0, s, key_s.as<uint8_t>(), E, R / 8, cs.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipMemcpyAsync(xp->class_start, cs.p, 8 * 9, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+
+  // segments: a new one wherever the key or the row changes
+  DevBuf seg_row;
+  {
+    DevBuf flag, segid;
+    flag.alloc(4 * E);
+    segid.alloc(4 * E);
+    xp->idx.alloc(4 * E);
+    hipLaunchKernelGGL(k_xflags, dim3(gcap(E)), dim3(kB), 0, s, key_s.as<uint8_t>(), val_s.as<uint64_t>(), E,
+                       flag.as<uint32_t>(), xp->idx.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    size_t tb = 0;
+    GRE_HIP(rocprim::inclusive_scan(nullptr, tb, flag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)E,
+                                    rocprim::plus<uint32_t>(), s));
+    void* tmp = scratch(2, tb);
+    GRE_HIP(rocprim::inclusive_scan(tmp, tb, flag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)E,
+                                    rocprim::plus<uint32_t>(), s));
+    uint32_t ns = 0;
+    GRE_HIP(hipMemcpyAsync(&ns, segid.as<uint32_t>() + (E - 1), 4, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    xp->nseg = ns;
+    xp->seg_off.alloc(8 * ((size_t)ns + 1));
+    seg_row.alloc(4 * (size_t)ns);
+    hipLaunchKernelGGL(k_xsegs, dim3(gcap(E)), dim3(kB), 0, s, flag.as<uint32_t>(), segid.as<uint32_t>(),
+                       val_s.as<uint64_t>(), E, xp->seg_off.as<uint64_t>(), seg_row.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    const uint64_t eh = E;
+    GRE_HIP(hipMemcpyAsync(xp->seg_off.as<uint64_t>() + ns, &eh, 8, hipMemcpyHostToDevice, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  key_s.release();
+  val_s.release();
+
+  // per-row segment lists, by range (stable sort of segment ids by row)
+  const uint64_t nseg = xp->nseg;
+  {
+    DevBuf cnt, ids, rows_s;
+    cnt.alloc(4 * ((size_t)nV + 1));
+    GRE_HIP(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)nV + 1), s));
+    hipLaunchKernelGGL(k_xpcount, dim3(gcap(nseg)), dim3(kB), 0, s, seg_row.as<uint32_t>(), nseg, cnt.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    xp->pair_off.alloc(8 * ((size_t)nV + 1));
+    scan_excl_u32_to_u64(cnt.as<uint32_t>(), xp->pair_off.as<uint64_t>(), (size_t)nV + 1, s);
+    size_t tb = 0;
+    void* tmp = nullptr;
+    ids.alloc(4 * nseg);
+    rows_s.alloc(4 * nseg);
+    xp->pair_list.alloc(4 * nseg);
+    hipLaunchKernelGGL(k_iota32, dim3(gcap(nseg)), dim3(kB), 0, s, ids.as<uint32_t>(), nseg);
+    GRE_HIP(hipGetLastError());
+    tb = 0;
+    const unsigned vb = bits_of(nV);
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, seg_row.as<uint32_t>(), rows_s.as<uint32_t>(),
+                                      ids.as<uint32_t>(), xp->pair_list.as<uint32_t>(), (size_t)nseg, 0, vb, s));
+    tmp = scratch(0, tb);
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, seg_row.as<uint32_t>(), rows_s.as<uint32_t>(), ids.as<uint32_t>(),
+                                      xp->pair_list.as<uint32_t>(), (size_t)nseg, 0, vb, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  g->xpart[R] = xp;
+  return xp;
+}
+
+const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hipStream_t s) {
+  auto it = xp.pair_tiles.find(te);
+  if (it != xp.pair_tiles.end()) return it->second;
+  XPart::PairTiles& T = xp.pair_tiles[te];
+  uint64_t nt = (xp.nseg + te - 1) / te;
+  if (nt == 0) nt = 1;
+  T.ntiles = nt;
+  T.rb.alloc(4 * (nt + 1));
+  build_tile_rows(xp.pair_off.as<uint64_t>(), nV, nt, te, T.rb.as<uint32_t>(), s);
+  return T;
+}
+
+const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s) {
+  auto it = xp.tiles.find(te);
+  if (it != xp.tiles.end()) return it->second;
+  XPart::Tiles& T = xp.tiles[te];
+  const uint64_t E = xp.class_start[8];
+  uint64_t nt = (E + te - 1) / te;
+  if (nt == 0) nt = 1;
+  T.ntiles = nt;
+  T.rb.alloc(4 * (nt + 1));
+  build_tile_rows(xp.seg_off.as<uint64_t>(), (uint32_t)xp.nseg, nt, te, T.rb.as<uint32_t>(), s);
+  // class c runs tiles [xm[c], xm[c+1]) on the workgroups with id % 8 == c
+  uint32_t xm[9];
+  for (int c = 0; c < 8; ++c) xm[c] = (uint32_t)std::min<uint64_t>(xp.class_start[c] / te, nt);
+  xm[0] = 0;
+  xm[8] = (uint32_t)nt;
+  uint32_t most = 0;
+  for (int c = 0; c < 8; ++c) most = std::max(most, xm[c + 1] - xm[c]);
+  T.grid = 8u * std::max(most, 1u);
+  T.xmap.alloc(4 * 9);
+  GRE_HIP(hipMemcpyAsync(T.xmap.p, xm, 4 * 9, hipMemcpyHostToDevice, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  return T;
+}
+
+}  // namespace gre

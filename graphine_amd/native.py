@@ -100,8 +100,8 @@ def _declare(L):
         "gre_pr_get_adjoint": ([vp, vp], i32),
         "gre_pr_adjoint_cotangent": ([vp, vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_set_schedule": ([vp, i32, i32], i32),
-        "gre_pr_set_hot_sources": ([vp, u32], i32),
-        "gre_pr_get_hot_sources": ([vp, C.POINTER(u32)], i32),
+        "gre_pr_set_source_ranges": ([vp, i32], i32),
+        "gre_pr_get_source_ranges": ([vp, C.POINTER(i32)], i32),
         "gre_bfs": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_bfs_dev": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_msbfs": ([vp, vp, i32, vp, C.POINTER(u64)], i32),
@@ -308,7 +308,8 @@ class PageRank:
         self.h = C.c_void_p()
         _ck(lib().gre_pr_create(C.byref(self.h), graph.h, dtype, k, 1 if weighted else 0))
 
-    SCHEDULES = {0: "auto", 1: "tile", 2: "tile_nt", 3: "compact", 4: "compact_nt"}
+    SCHEDULES = {0: "auto", 1: "tile", 2: "tile_nt", 3: "compact", 4: "compact_nt", 5: "part",
+                 6: "part_nt"}
 
     def set_schedule(self, schedule, tile_items=0):
         _ck(lib().gre_pr_set_schedule(self.h, schedule, tile_items))
@@ -318,13 +319,13 @@ class PageRank:
         _ck(lib().gre_pr_get_schedule(self.h, C.byref(s), C.byref(t)))
         return self.SCHEDULES.get(s.value, str(s.value)), t.value
 
-    def set_hot_sources(self, hot):
-        _ck(lib().gre_pr_set_hot_sources(self.h, int(hot)))
+    def set_source_ranges(self, ranges):
+        _ck(lib().gre_pr_set_source_ranges(self.h, int(ranges)))
 
-    def hot_sources(self):
-        h = C.c_uint32()
-        _ck(lib().gre_pr_get_hot_sources(self.h, C.byref(h)))
-        return h.value
+    def source_ranges(self):
+        r = C.c_int()
+        _ck(lib().gre_pr_get_source_ranges(self.h, C.byref(r)))
+        return r.value
 
     def set_edge_weights(self, w=None, seed=0):
         if w is not None:

@@ -169,15 +169,18 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * tile_items (4, 8 or 16) adjacency entries per thread; _NT streams the
  * neighbour ids with a non-temporal hint; COMPACT gathers from a contrib array
  * indexed by hub-sorted compact source ids (unweighted, single vector).
+ * PART additionally cuts the compact sources into R ranges of equal edge
+ * count, each pinned to one XCD so its gathers stay in that XCD's L2, and sums
+ * a row's per-range partial sums in a second pull (unweighted, single vector,
+ * fewer than 2^32 edges; R set with gre_pr_set_source_ranges).
  * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest. */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,
-       GRE_SCHED_COMPACT_NT = 4 };
+       GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
 int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
-/* COMPACT schedules: compact source ids >= hot are gathered with a
- * non-temporal hint so the hottest sources stay cached (0 = off; tuned). */
-int gre_pr_set_hot_sources(gre_pr* p, uint32_t hot);
-int gre_pr_get_hot_sources(gre_pr* p, uint32_t* hot);
+/* PART schedules: number of source ranges (8, 16, 32, 64, 128 or 256; default 64; tuned). */
+int gre_pr_set_source_ranges(gre_pr* p, int ranges);
+int gre_pr_get_source_ranges(gre_pr* p, int* ranges);
 
 /* ---- traversal kernels (plain single-source paths) ------------------------ */
 /* Direction-optimising BFS levels (int32, -1 unreached). edges_out: sum of the

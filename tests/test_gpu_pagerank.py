@@ -15,7 +15,8 @@ def rel_l1(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
-@pytest.mark.parametrize("schedule,tile_items", [(1, 4), (1, 8), (1, 16), (2, 8), (3, 4), (3, 16), (4, 8)])
+@pytest.mark.parametrize("schedule,tile_items", [(1, 4), (1, 8), (1, 16), (2, 8), (3, 4), (3, 16), (4, 8),
+                                                  (5, 4), (5, 8), (5, 16), (6, 8)])
 def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     g = N.Graph.rmat(16, 16, 1, False)
     og = O.rmat_graph(16, 16, 1, False)
@@ -30,21 +31,28 @@ def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     assert abs(x.sum() - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("schedule,hot", [(3, 1000), (4, 1 << 12), (3, 1 << 20)])
-def test_pagerank_hot_sources_same_result(N, O, schedule, hot):
-    # the cache hint on cold sources changes no arithmetic: bit-identical ranks
-    g = N.Graph.rmat(16, 16, 1, False)
-    ref = N.PageRank(g, N.F64)
-    ref.set_schedule(schedule, 8)
-    ref.run(0.85, 20)
-    pr = N.PageRank(g, N.F64)
-    pr.set_schedule(schedule, 8)
-    pr.set_hot_sources(hot)
-    assert pr.hot_sources() == hot
-    pr.run(0.85, 20)
-    assert np.array_equal(pr.ranks(), ref.ranks())
-    og = O.rmat_graph(16, 16, 1, False)
-    assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
+@pytest.mark.parametrize("ranges", [8, 16, 32, 64, 128, 256])
+def test_pagerank_partitioned_ranges(N, O, ranges):
+    # source-range partitions: every range count gives the oracle's ranks, and
+    # two handles with the same partition give the same bits
+    g = N.Graph.rmat(17, 16, 4, False)
+    og = O.rmat_graph(17, 16, 4, False)
+    ref = O.pagerank(og, 0.85, 20)
+    xs = []
+    for _ in range(2):
+        pr = N.PageRank(g, N.F64)
+        pr.set_schedule(5, 8)
+        pr.set_source_ranges(ranges)
+        assert pr.source_ranges() == ranges
+        pr.run(0.85, 20)
+        xs.append(pr.ranks())
+    assert rel_l1(xs[0], ref) < 1e-12
+    assert np.array_equal(xs[0], xs[1])
+    pr32 = N.PageRank(g, N.F32)
+    pr32.set_schedule(6, 16)
+    pr32.set_source_ranges(ranges)
+    pr32.run(0.85, 20)
+    assert rel_l1(pr32.ranks(), ref) < 1e-6
 
 
 def test_pagerank_f32_within_1e6_of_f64(N, O):
@@ -76,7 +84,7 @@ def test_pagerank_edge_cases(N, O):
     dst = np.concatenate([hub_dst, np.zeros(3000, np.uint32), np.array([11, 10], np.uint32)])
     g = N.Graph.from_edges(n, src, dst)
     og = O.graph_from_edges(n, src, dst)
-    for sc in (1, 3):
+    for sc in (1, 3, 5):
         for ti in (4, 8, 16):
             pr = N.PageRank(g, N.F64)
             pr.set_schedule(sc, ti)
