@@ -51,8 +51,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-sweep", action="store_true", help="skip the pull-schedule sweep")
     ap.add_argument("--schedule", default="auto",
-                    help="pull schedule of the headline run: auto (autotuned) or NAME:TILE_ITEMS[:RANGES], "
-                         "e.g. compact_nt:4 or part:8:128")
+                    help="pull schedule of the headline run: auto (autotuned) or NAME:TILE_ITEMS[:RANGES[:SHARED]], "
+                         "e.g. compact_nt:4 or part:8:128:262144")
     return ap.parse_args()
 
 
@@ -102,7 +102,7 @@ def main():
         code = {v: k for k, v in N.PageRank.SCHEDULES.items()}[parts[0]]
         pr.set_schedule(code, int(parts[1]) if len(parts) > 1 else 8)
         if len(parts) > 2:
-            pr.set_source_ranges(int(parts[2]))
+            pr.set_partition(int(parts[2]), int(parts[3]) if len(parts) > 3 else 0)
     pr.run(0.85, 2)                          # autotunes the schedule (unless fixed), warms up
     for _ in range(a.warmup):
         pr.run(0.85, iters)
@@ -147,7 +147,8 @@ def main():
     log(f"iteration kernels {iter_s * 1e3:.3f} ms (tile {avg_tile_s * 1e3:.3f}, fixup {avg_fix_s * 1e3:.3f}, "
         f"combine {avg_cmb_s * 1e3:.3f}) -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
     roofline["schedule"] = (f"{sched}, {tile_items} entries/thread"
-                            + (f", {pr.source_ranges()} source ranges" if sched.startswith("part") else "")
+                            + (", {} tail source ranges, {} shared sources".format(*pr.partition())
+                               if sched.startswith("part") else "")
                             + (" (autotuned)" if a.schedule == "auto" else " (fixed)"))
     # achievable streaming bandwidth on this device: a plain copy kernel
     copy_bytes = 4 << 30
@@ -157,12 +158,13 @@ def main():
     if not a.no_sweep:
         # every schedule the autotuner chooses from, timed on the same graph
         sweep = {}
-        cand = [(sc, ti, 64) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
-        cand += [(sc, ti, r) for sc in (5, 6) for ti in (4, 8, 16) for r in (32, 64, 128)]
-        for sc, ti, r in cand:
+        cand = [(sc, ti, 64, 0) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
+        cand += [(sc, ti, r, h) for sc in (5, 6) for ti in (4, 8)
+                 for r, h in ((128, 0), (64, 1 << 18), (128, 1 << 18), (64, 1 << 20), (32, 1 << 20))]
+        for sc, ti, r, h in cand:
             q = N.PageRank(g, N.F32)
             q.set_schedule(sc, ti)
-            q.set_source_ranges(r)
+            q.set_partition(r, h)
             q.run(0.85, 2)
             N.prof_reset()
             N.prof_enable(True)
@@ -173,7 +175,7 @@ def main():
                 nt_, mt_ = N.prof_get(kname)
                 ms_ += mt_ / max(nt_, 1)
             N.prof_enable(False)
-            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/R={r}" if sc >= 5 else "")
+            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/R={r}/H={h}" if sc >= 5 else "")
             sweep[key] = round(ms_, 3)
             del q
         roofline["schedule_sweep_ms"] = sweep

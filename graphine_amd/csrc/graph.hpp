@@ -17,6 +17,7 @@ namespace gre {
 // each row's list of segment ids sums them in a fixed order.
 struct XPart {
   int R = 0;
+  uint32_t H = 0;    // sources [0, H) form one range shared by all XCDs (cut by rows)
   uint64_t nseg = 0;
   DevBuf idx;        // uint32 [E]   compact source ids, grouped by (class, range, row)
   DevBuf seg_off;    // uint64 [nseg+1]
@@ -55,8 +56,8 @@ struct gre_graph {
   // cidx_c[p] = cid[cidx[p]] (CSC in compact source ids)
   gre::DevBuf cid, cidx_c;
   uint32_t nsrc = 0;
-  // source-range partitions, by range count (built on demand, shared with handles)
-  std::map<int, std::shared_ptr<gre::XPart>> xpart;
+  // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with handles)
+  std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
 };
 
 namespace gre {
@@ -71,8 +72,9 @@ void graph_compact_sources(gre_graph* g);
 // tile_rb over an arbitrary offset array of nrows rows (see graph_tiles).
 void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
                      hipStream_t s);
-// The R-range source partition of g (R in {8, 16, ..., 256}), built once and cached.
-std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R);
+// The source partition of g: the H hottest compact sources as one shared range
+// and the rest in R tail ranges (R in {8, 16, ..., 256}); built once and cached.
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H);
 // Its tiles of te entries (cached on the partition).
 const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s);
 // Tiles of te entries over the per-row pair lists (rows = vertices).

@@ -237,8 +237,8 @@ bool use_compact(const gre_pr* P) { return P->compact && !P->weighted && P->k ==
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
-  if (!P->xp || P->xp->R != P->ranges) {
-    P->xp = graph_xpart(P->g, P->ranges);
+  if (!P->xp || P->xp->R != P->ranges || P->xp->H != std::min(P->shared, P->g->nsrc)) {
+    P->xp = graph_xpart(P->g, P->ranges, P->shared);
     P->plan_ipt_x = 0;
   }
   if (P->plan_ipt_x != ipt) {
@@ -386,6 +386,7 @@ void autotune_forward(gre_pr* P, double damp) {
     scheds.push_back(GRE_SCHED_PART_NT);
   }
   int best_s = GRE_SCHED_TILE, best_i = 8, best_r = P->ranges;
+  uint32_t best_h = P->shared;
   float best_ms = 1e30f;
   hipEvent_t a, b;
   GRE_HIP(hipEventCreate(&a));
@@ -393,13 +394,14 @@ void autotune_forward(gre_pr* P, double damp) {
   ensure_scal(P, 4);
   for (int sc : scheds) {
     const bool prt = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
-    // partitioned schedules also try 32, 64 and 128 source ranges
-    std::vector<int> vars = {0};
-    if (prt) vars = {32, 64, 128};
-    for (int h : vars) {
+    // partitioned schedules also try (tail ranges, shared sources) pairs
+    std::vector<std::pair<int, uint32_t>> vars = {{P->ranges, P->shared}};
+    if (prt) vars = {{128, 0u}, {64, 1u << 18}, {128, 1u << 18}, {64, 1u << 20}, {32, 1u << 20}};
+    for (const auto& h : vars) {
       for (int ipt : ipts) {
+        if (prt && ipt == 16) continue;
         apply_schedule(P, sc, ipt);
-        if (prt) P->ranges = h;
+        if (prt) { P->ranges = h.first; P->shared = h.second; }
         ensure_plan(P, true, ipt);
         fwd_init<TT>(P, s);
         double* scal = P->scal.as<double>();
@@ -415,7 +417,7 @@ void autotune_forward(gre_pr* P, double damp) {
           best_ms = ms;
           best_s = sc;
           best_i = ipt;
-          if (prt) best_r = h;
+          if (prt) { best_r = h.first; best_h = h.second; }
         }
       }
     }
@@ -424,6 +426,7 @@ void autotune_forward(gre_pr* P, double damp) {
   (void)hipEventDestroy(b);
   apply_schedule(P, best_s, best_i);
   P->ranges = best_r;
+  P->shared = best_h;
   if (!P->part) P->xp.reset();
   // partitions of the candidates that lost, unless another handle holds them
   for (auto it = P->g->xpart.begin(); it != P->g->xpart.end();) {
@@ -614,19 +617,21 @@ int gre_pr_get_schedule(gre_pr* P, int* schedule, int* tile_items) {
   });
 }
 
-int gre_pr_set_source_ranges(gre_pr* P, int ranges) {
+int gre_pr_set_partition(gre_pr* P, int ranges, uint32_t shared_sources) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
     GRE_CHECK(ranges == 8 || ranges == 16 || ranges == 32 || ranges == 64 || ranges == 128 || ranges == 256,
               GRE_DEV_NULL, "source ranges must be 8, 16, 32, 64, 128 or 256");
     P->ranges = ranges;
+    P->shared = shared_sources;
   });
 }
 
-int gre_pr_get_source_ranges(gre_pr* P, int* ranges) {
+int gre_pr_get_partition(gre_pr* P, int* ranges, uint32_t* shared_sources) {
   return guard([&] {
-    GRE_CHECK(P && ranges, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(P && ranges && shared_sources, GRE_DEV_NULL, "null argument");
     *ranges = P->ranges;
+    *shared_sources = P->shared;
   });
 }
 

@@ -21,7 +21,8 @@ struct gre_pr {
   bool nt = false;        // non-temporal neighbour-id stream
   bool compact = false;   // hub-sorted compact source ids (unweighted, k == 1)
   bool part = false;      // source-range partitioned pull (implies compact)
-  int ranges = 64;        // source ranges of the partitioned pull
+  int ranges = 64;        // tail source ranges of the partitioned pull
+  uint32_t shared = 1u << 18;  // hottest sources processed as one range by all XCDs
   std::shared_ptr<gre::XPart> xp;  // the partition in use
   gre::DevBuf xsum;       // per-segment partial sums (compute type)
   gre::PullPlan plan_x, plan_px;   // segment pull; per-row pull over the segment sums

@@ -4,12 +4,14 @@
 // MI355X has 8 XCDs, each with its own 4 MB L2, and workgroups are dispatched
 // to them round-robin by workgroup id.  A plain pull gathers from the whole
 // source array on every XCD, so each L2 holds the same hot sources and misses
-// on the rest.  Here the hub-sorted compact source ids are cut into R ranges of
-// equal edge count; range r is owned by XCD class r % 8, and the CSC entries
-// are regrouped by (class, range, row).  Each run of one row's entries from one
-// range is a segment; the pull reduces segments, and a second tile-segmented
-// pull over per-row lists of segment ids sums each row's segments in a fixed
-// order.
+// on the rest.  Here the hub-sorted compact source ids are cut into ranges:
+// the H hottest sources form one shared range, small enough to sit in every
+// L2, whose entries are cut by rows into 8 chunks (chunk c on XCD c); the
+// other sources form R tail ranges of equal edge count, range r owned by XCD
+// class r % 8.  The CSC entries are regrouped by (class, range, row).  Each run
+// of one row's entries from one range is a segment; the pull reduces segments,
+// and a second tile-segmented pull over per-row lists of segment ids sums each
+// row's segments in a fixed order.
 #include "graph.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -38,14 +40,16 @@ __global__ void k_deg_compact(const int32_t* __restrict__ outdeg, const uint32_t
   }
 }
 
-// bnd[r] = first compact id c in [0, nsrc] with ex[c] >= r*E/R (ex: exclusive
-// prefix of the compact out-degrees, ex[nsrc] = E); one thread per r in [0, R]
-__global__ void k_bounds(const uint64_t* __restrict__ ex, uint32_t nsrc, uint64_t E, int R,
+// Tail range bounds: bnd[r] = first compact id c in [H, nsrc] with
+// ex[c] >= ex[H] + r * (E - ex[H]) / R (ex: exclusive prefix of the compact
+// out-degrees, ex[nsrc] = E); one thread per r in [0, R]
+__global__ void k_bounds(const uint64_t* __restrict__ ex, uint32_t H, uint32_t nsrc, uint64_t E, int R,
                          uint32_t* __restrict__ bnd) {
   const int r = (int)threadIdx.x;
   if (r > R) return;
-  const uint64_t target = (uint64_t)r * E / (uint64_t)R;
-  uint32_t lo = 0, hi = nsrc;
+  const uint64_t base = ex[H];
+  const uint64_t target = base + (uint64_t)r * (E - base) / (uint64_t)R;
+  uint32_t lo = H, hi = nsrc;
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
     if (ex[mid] < target) lo = mid + 1;
@@ -54,16 +58,46 @@ __global__ void k_bounds(const uint64_t* __restrict__ ex, uint32_t nsrc, uint64_
   bnd[r] = lo;
 }
 
-// Per CSC entry e: the sort key (class-major range position) and the value
-// (row << 32 | compact source).  The row is found inside the entry's
-// 2048-entry tile (rb: graph_tiles of the CSC).
+// cnt[v] = entries of row v whose compact source is < H
+__global__ void k_hub_rowcnt(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx_c, uint32_t nV,
+                             uint32_t H, uint64_t* __restrict__ cnt) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c = 0;
+    for (uint64_t e = coff[v]; e < coff[v + 1]; ++e) c += cidx_c[e] < H ? 1u : 0u;
+    cnt[v] = c;
+  }
+}
+
+// rb[c] = first row v in [0, nV] with ex[v] >= c * ex[nV] / 8, c in [0, 8]
+__global__ void k_row_bounds(const uint64_t* __restrict__ ex, uint32_t nV, uint32_t* __restrict__ rb) {
+  const int c = (int)threadIdx.x;
+  if (c > 8) return;
+  const uint64_t target = (uint64_t)c * ex[nV] / 8;
+  uint32_t lo = 0, hi = nV;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ex[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  rb[c] = lo;
+}
+
+// Per CSC entry e: the sort key and the value (row << 32 | compact source).
+// Sources < H form the shared range, cut by rows into 8 chunks of equal entry
+// count (chunk c runs on XCD c); the other sources fall into R tail ranges,
+// range r on XCD r % 8.  key = class * (1 + R/8) + (0 for the shared chunk,
+// 1 + r/8 for tail range r).  The row is found inside the entry's 2048-entry
+// tile (rb: graph_tiles of the CSC).
 __global__ void k_xkeys(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ rb, uint32_t te,
                         const uint32_t* __restrict__ cidx_c, uint64_t E, const uint32_t* __restrict__ bnd, int R,
-                        uint8_t* __restrict__ key, uint64_t* __restrict__ val) {
+                        uint32_t H, const uint32_t* __restrict__ hub_rows, uint16_t* __restrict__ key,
+                        uint64_t* __restrict__ val) {
   __shared__ uint32_t sb[257];
+  __shared__ uint32_t hr[9];
   for (int i = (int)threadIdx.x; i <= R; i += (int)blockDim.x) sb[i] = bnd[i];
+  if (threadIdx.x < 9) hr[threadIdx.x] = hub_rows[threadIdx.x];
   __syncthreads();
-  const int per = R / 8;
+  const int per = 1 + R / 8;
   for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t t = e / te;
     // largest row r in [lo, hi) with coff[r] <= e
@@ -74,20 +108,28 @@ __global__ void k_xkeys(const uint64_t* __restrict__ coff, const uint32_t* __res
       else hi = mid;
     }
     const uint32_t s = cidx_c[e];
-    // range = (number of inner bounds <= s)
-    int a = 1, b = R;
-    while (a < b) {
-      const int mid = (a + b) >> 1;
-      if (sb[mid] <= s) a = mid + 1;
-      else b = mid;
+    int k;
+    if (s < H) {
+      int c = 0;
+      for (int i = 1; i < 8; ++i) c += (hr[i] <= lo) ? 1 : 0;
+      k = c * per;
+    } else {
+      // tail range = (number of inner bounds <= s)
+      int a = 1, b = R;
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (sb[mid] <= s) a = mid + 1;
+        else b = mid;
+      }
+      const int r = a - 1;
+      k = (r & 7) * per + 1 + (r >> 3);
     }
-    const int r = a - 1;
-    key[e] = (uint8_t)((r & 7) * per + (r >> 3));
+    key[e] = (uint16_t)k;
     val[e] = ((uint64_t)lo << 32) | s;
   }
 }
 
-__global__ void k_xflags(const uint8_t* __restrict__ key, const uint64_t* __restrict__ val, uint64_t E,
+__global__ void k_xflags(const uint16_t* __restrict__ key, const uint64_t* __restrict__ val, uint64_t E,
                          uint32_t* __restrict__ flag, uint32_t* __restrict__ idx) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t v = val[i];
@@ -110,7 +152,7 @@ __global__ void k_xsegs(const uint32_t* __restrict__ flag, const uint32_t* __res
 }
 
 // cs[c] = first entry whose key belongs to class >= c (keys sorted ascending)
-__global__ void k_xclass(const uint8_t* __restrict__ key, uint64_t E, int per, uint64_t* __restrict__ cs) {
+__global__ void k_xclass(const uint16_t* __restrict__ key, uint64_t E, int per, uint64_t* __restrict__ cs) {
   const int c = threadIdx.x;
   if (c > 8) return;
   const unsigned target = (unsigned)(c * per);
@@ -141,23 +183,26 @@ unsigned bits_of(uint64_t n) {
 
 }  // namespace
 
-std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
   GRE_CHECK(R == 8 || R == 16 || R == 32 || R == 64 || R == 128 || R == 256, GRE_DEV_NULL,
             "source ranges must be 8, 16, 32, 64, 128 or 256");
-  auto it = g->xpart.find(R);
+  graph_compact_sources(g);
+  const uint32_t nsrc = g->nsrc;
+  H = std::min(H, nsrc);
+  auto it = g->xpart.find(std::make_pair(R, H));
   if (it != g->xpart.end()) return it->second;
   const uint64_t E = g->nE;
   const uint32_t nV = g->nV;
   GRE_CHECK(E > 0 && E < 0xffffffffULL, GRE_DEV_NULL, "the partitioned schedule needs 0 < edges < 2^32");
-  graph_compact_sources(g);
   hipStream_t s = stream();
-  const uint32_t nsrc = g->nsrc;
   auto xp = std::make_shared<XPart>();
   xp->R = R;
+  xp->H = H;
 
-  // range bounds from the compact out-degree prefix
-  DevBuf bnd;
+  // tail range bounds from the compact out-degree prefix
+  DevBuf bnd, hub_rows;
   bnd.alloc(4 * (size_t)(R + 1));
+  hub_rows.alloc(4 * 9);
   {
     DevBuf degc, ex;
     degc.alloc(8 * ((size_t)nsrc + 1));
@@ -167,31 +212,46 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
                        nV, degc.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     scan_excl_u64(degc.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nsrc + 1, s);
-    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(320), 0, s, ex.as<uint64_t>(), nsrc, E, R, bnd.as<uint32_t>());
+    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(320), 0, s, ex.as<uint64_t>(), H, nsrc, E, R, bnd.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+  }
+  // row chunks of the shared range, equal shared-entry counts
+  {
+    DevBuf cnt, ex;
+    cnt.alloc(8 * ((size_t)nV + 1));
+    ex.alloc(8 * ((size_t)nV + 1));
+    GRE_HIP(hipMemsetAsync(cnt.p, 0, 8 * ((size_t)nV + 1), s));
+    hipLaunchKernelGGL(k_hub_rowcnt, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
+                       g->cidx_c.as<uint32_t>(), nV, H, cnt.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
+    scan_excl_u64(cnt.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nV + 1, s);
+    hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, s, ex.as<uint64_t>(), nV, hub_rows.as<uint32_t>());
     GRE_HIP(hipGetLastError());
     GRE_HIP(hipStreamSynchronize(s));
   }
 
   // keys and values, stable sort by key (rows stay ascending inside a key)
+  const int per = 1 + R / 8;
   DevBuf key_s, val_s;
-  key_s.alloc(E);
+  key_s.alloc(2 * E);
   val_s.alloc(8 * E);
   {
     DevBuf key, val;
-    key.alloc(E);
+    key.alloc(2 * E);
     val.alloc(8 * E);
     uint64_t nt = 0;
     const uint32_t te = 2048;
     const uint32_t* rb = graph_tiles(g, true, te, &nt);
     hipLaunchKernelGGL(k_xkeys, dim3(gcap(E)), dim3(kB), 0, s, g->coff.as<uint64_t>(), rb, te,
-                       g->cidx_c.as<uint32_t>(), E, bnd.as<uint32_t>(), R, key.as<uint8_t>(), val.as<uint64_t>());
+                       g->cidx_c.as<uint32_t>(), E, bnd.as<uint32_t>(), R, H, hub_rows.as<uint32_t>(),
+                       key.as<uint16_t>(), val.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     size_t tb = 0;
-    const unsigned end_bit = bits_of((uint64_t)R);
-    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint8_t>(), key_s.as<uint8_t>(), val.as<uint64_t>(),
+    const unsigned end_bit = bits_of((uint64_t)(8 * per));
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), val.as<uint64_t>(),
                                       val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
     void* tmp = scratch(0, tb);
-    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, key.as<uint8_t>(), key_s.as<uint8_t>(), val.as<uint64_t>(),
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), val.as<uint64_t>(),
                                       val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
     GRE_HIP(hipStreamSynchronize(s));
   }
@@ -200,7 +260,7 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
   {
     DevBuf cs;
     cs.alloc(8 * 9);
-    hipLaunchKernelGGL(k_xclass, dim3(1), dim3(64), 0, s, key_s.as<uint8_t>(), E, R / 8, cs.as<uint64_t>());
+    hipLaunchKernelGGL(k_xclass, dim3(1), dim3(64), 0, s, key_s.as<uint16_t>(), E, per, cs.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     GRE_HIP(hipMemcpyAsync(xp->class_start, cs.p, 8 * 9, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
@@ -213,7 +273,7 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
     flag.alloc(4 * E);
     segid.alloc(4 * E);
     xp->idx.alloc(4 * E);
-    hipLaunchKernelGGL(k_xflags, dim3(gcap(E)), dim3(kB), 0, s, key_s.as<uint8_t>(), val_s.as<uint64_t>(), E,
+    hipLaunchKernelGGL(k_xflags, dim3(gcap(E)), dim3(kB), 0, s, key_s.as<uint16_t>(), val_s.as<uint64_t>(), E,
                        flag.as<uint32_t>(), xp->idx.as<uint32_t>());
     GRE_HIP(hipGetLastError());
     size_t tb = 0;
@@ -264,7 +324,7 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R) {
                                       xp->pair_list.as<uint32_t>(), (size_t)nseg, 0, vb, s));
     GRE_HIP(hipStreamSynchronize(s));
   }
-  g->xpart[R] = xp;
+  g->xpart[std::make_pair(R, H)] = xp;
   return xp;
 }
 

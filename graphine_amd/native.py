@@ -100,8 +100,8 @@ def _declare(L):
         "gre_pr_get_adjoint": ([vp, vp], i32),
         "gre_pr_adjoint_cotangent": ([vp, vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_set_schedule": ([vp, i32, i32], i32),
-        "gre_pr_set_source_ranges": ([vp, i32], i32),
-        "gre_pr_get_source_ranges": ([vp, C.POINTER(i32)], i32),
+        "gre_pr_set_partition": ([vp, i32, u32], i32),
+        "gre_pr_get_partition": ([vp, C.POINTER(i32), C.POINTER(u32)], i32),
         "gre_bfs": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_bfs_dev": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_msbfs": ([vp, vp, i32, vp, C.POINTER(u64)], i32),
@@ -319,13 +319,13 @@ class PageRank:
         _ck(lib().gre_pr_get_schedule(self.h, C.byref(s), C.byref(t)))
         return self.SCHEDULES.get(s.value, str(s.value)), t.value
 
-    def set_source_ranges(self, ranges):
-        _ck(lib().gre_pr_set_source_ranges(self.h, int(ranges)))
+    def set_partition(self, ranges, shared_sources=0):
+        _ck(lib().gre_pr_set_partition(self.h, int(ranges), int(shared_sources)))
 
-    def source_ranges(self):
-        r = C.c_int()
-        _ck(lib().gre_pr_get_source_ranges(self.h, C.byref(r)))
-        return r.value
+    def partition(self):
+        r, h = C.c_int(), C.c_uint32()
+        _ck(lib().gre_pr_get_partition(self.h, C.byref(r), C.byref(h)))
+        return r.value, h.value
 
     def set_edge_weights(self, w=None, seed=0):
         if w is not None:

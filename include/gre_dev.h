@@ -169,18 +169,21 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * tile_items (4, 8 or 16) adjacency entries per thread; _NT streams the
  * neighbour ids with a non-temporal hint; COMPACT gathers from a contrib array
  * indexed by hub-sorted compact source ids (unweighted, single vector).
- * PART additionally cuts the compact sources into R ranges of equal edge
- * count, each pinned to one XCD so its gathers stay in that XCD's L2, and sums
- * a row's per-range partial sums in a second pull (unweighted, single vector,
- * fewer than 2^32 edges; R set with gre_pr_set_source_ranges).
+ * PART additionally cuts the compact sources into ranges, each pinned to one
+ * XCD so its gathers stay in that XCD's L2, and sums a row's per-range partial
+ * sums in a second pull (unweighted, single vector, fewer than 2^32 edges;
+ * ranges set with gre_pr_set_partition).
  * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest. */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,
        GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
 int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
-/* PART schedules: number of source ranges (8, 16, 32, 64, 128 or 256; default 64; tuned). */
-int gre_pr_set_source_ranges(gre_pr* p, int ranges);
-int gre_pr_get_source_ranges(gre_pr* p, int* ranges);
+/* PART schedules: the shared_sources hottest compact sources (0 = none; fewer
+ * if the graph has fewer sources) form one range pulled by all XCDs, the rest
+ * are cut into `ranges` tail ranges (8, 16, 32, 64, 128 or 256) pinned to
+ * XCDs.  Defaults 64 and 2^18; tuned. */
+int gre_pr_set_partition(gre_pr* p, int ranges, uint32_t shared_sources);
+int gre_pr_get_partition(gre_pr* p, int* ranges, uint32_t* shared_sources);
 
 /* ---- traversal kernels (plain single-source paths) ------------------------ */
 /* Direction-optimising BFS levels (int32, -1 unreached). edges_out: sum of the

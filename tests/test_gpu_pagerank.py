@@ -31,10 +31,11 @@ def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     assert abs(x.sum() - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("ranges", [8, 16, 32, 64, 128, 256])
-def test_pagerank_partitioned_ranges(N, O, ranges):
-    # source-range partitions: every range count gives the oracle's ranks, and
-    # two handles with the same partition give the same bits
+@pytest.mark.parametrize("ranges,shared", [(8, 0), (16, 0), (32, 0), (64, 0), (128, 0), (256, 0),
+                                           (64, 1000), (128, 1 << 12), (8, 1 << 30)])
+def test_pagerank_partitioned_ranges(N, O, ranges, shared):
+    # source partitions (shared hub range + tail ranges): every layout gives the
+    # oracle's ranks, and two handles with the same partition give the same bits
     g = N.Graph.rmat(17, 16, 4, False)
     og = O.rmat_graph(17, 16, 4, False)
     ref = O.pagerank(og, 0.85, 20)
@@ -42,15 +43,15 @@ def test_pagerank_partitioned_ranges(N, O, ranges):
     for _ in range(2):
         pr = N.PageRank(g, N.F64)
         pr.set_schedule(5, 8)
-        pr.set_source_ranges(ranges)
-        assert pr.source_ranges() == ranges
+        pr.set_partition(ranges, shared)
+        assert pr.partition() == (ranges, shared)
         pr.run(0.85, 20)
         xs.append(pr.ranks())
     assert rel_l1(xs[0], ref) < 1e-12
     assert np.array_equal(xs[0], xs[1])
     pr32 = N.PageRank(g, N.F32)
     pr32.set_schedule(6, 16)
-    pr32.set_source_ranges(ranges)
+    pr32.set_partition(ranges, shared)
     pr32.run(0.85, 20)
     assert rel_l1(pr32.ranks(), ref) < 1e-6
 
