@@ -4,7 +4,9 @@
 //              an edge-balanced frontier, bottom-up pull over CSC with early
 //              exit, switched by frontier/unvisited edge counts;
 //   gre_msbfs  up to 64 BFS sources at once, one 64-bit mask per vertex,
-//              pull over CSC;
+//              direction-optimising like gre_bfs: top-down push of the
+//              frontier masks with 64-bit atomic ORs, bottom-up pull over CSC
+//              with early exit once a vertex holds every source bit;
 //   gre_bc     Brandes BC, FP64 path counts and dependencies, 64 sources per
 //              chunk as the 64 lanes of a wave, pull over CSC (forward) and
 //              CSR (backward), deterministic sums.
@@ -270,39 +272,146 @@ __global__ void k_ms_seed(const uint32_t* __restrict__ src, int n, unsigned long
   }
 }
 
-__global__ __launch_bounds__(kB) void k_ms_level(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
-                                                 uint32_t nV, unsigned long long full,
-                                                 const unsigned long long* __restrict__ fr,
-                                                 unsigned long long* __restrict__ nfr,
-                                                 unsigned long long* __restrict__ vis, int32_t* __restrict__ lev,
-                                                 int32_t L, unsigned int* __restrict__ any) {
-  bool changed = false;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    const unsigned long long seen = vis[v];
-    unsigned long long m = 0;
-    if (seen != full) {
-      const uint64_t b = coff[v], e = coff[v + 1];
-      for (uint64_t p = b; p < e; ++p) {
-        m |= fr[cidx[p]];
-        if ((m | seen) == full) break;
-      }
-      m &= ~seen;
+// Records level L+1 for every source bit in m at vertex v.
+__device__ inline void ms_record(int32_t* __restrict__ lev, uint32_t nV, uint32_t v, unsigned long long m,
+                                 int32_t L) {
+  while (m) {
+    const int i = __ffsll(m) - 1;
+    lev[(uint64_t)i * nV + v] = L + 1;
+    m &= m - 1;
+  }
+}
+
+// Appends v to the queue once per wave (ballot + one atomic).
+__device__ inline void wave_append(bool take, uint32_t v, uint32_t* __restrict__ q, uint32_t* __restrict__ qn) {
+  const unsigned long long bal = __ballot(take);
+  if (!bal) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll(bal) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(qn, (uint32_t)__popcll(bal));
+  base = __shfl(base, leader, 64);
+  if (take) q[base + (uint32_t)__popcll(bal & ((1ULL << lane) - 1ULL))] = v;
+}
+
+// top-down: the frontier's out-edges, edge-balanced; a vertex that gains new
+// source bits is queued by the first edge that reaches it (nxt was zero)
+__global__ __launch_bounds__(kB) void k_ms_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                              const uint32_t* __restrict__ q, const uint64_t* __restrict__ qpos,
+                                              uint32_t qn, uint64_t T, const unsigned long long* __restrict__ fr,
+                                              const unsigned long long* __restrict__ vis,
+                                              unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nq,
+                                              uint32_t* __restrict__ ncount) {
+  __shared__ uint32_t range[2];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
+  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+  if (threadIdx.x < 2) {
+    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+    uint32_t lo = 0, hi = qn;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= key) lo = mid; else hi = mid;
     }
+    range[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint32_t i0 = range[0], i1 = range[1];
+  // a fixed number of rounds for every lane keeps the wave-level append uniform
+  for (int round = 0; round < kEB / kB; ++round) {
+    const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0;
+    if (k < k1) {
+      uint32_t lo = i0, hi = i1 + 1;
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (qpos[mid] <= k) lo = mid; else hi = mid;
+      }
+      const uint32_t u = q[lo];
+      v = idx[off[u] + (k - qpos[lo])];
+      const unsigned long long bits = fr[u] & ~vis[v];
+      if (bits) take = atomicOr(nxt + v, bits) == 0ULL;
+    }
+    wave_append(take, v, nq, ncount);
+  }
+}
+
+// top-down, second pass over the queued vertices: publish the new bits
+__global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, unsigned long long full,
+                            unsigned long long* __restrict__ nxt, unsigned long long* __restrict__ nfr,
+                            unsigned long long* __restrict__ vis, const uint64_t* __restrict__ off,
+                            int32_t* __restrict__ lev, uint32_t nV, int32_t L, unsigned long long* __restrict__ sums) {
+  uint64_t deg = 0, fulldeg = 0;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nn; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = nq[j];
+    const unsigned long long seen = vis[v];
+    const unsigned long long m = nxt[v] & ~seen;
+    nxt[v] = 0ULL;
     nfr[v] = m;
-    if (m) {
-      vis[v] = seen | m;
-      changed = true;
-      if (lev) {
-        unsigned long long t = m;
-        while (t) {
-          const int i = __ffsll(t) - 1;
-          lev[(uint64_t)i * nV + v] = L + 1;
-          t &= t - 1;
+    vis[v] = seen | m;
+    const uint64_t d = off[v + 1] - off[v];
+    deg += d;
+    if ((seen | m) == full) fulldeg += d;
+    if (lev) ms_record(lev, nV, v, m, L);
+  }
+  deg = wave_sum(deg);
+  fulldeg = wave_sum(fulldeg);
+  if ((threadIdx.x & 63) == 0) {
+    if (deg) atomicAdd(sums, (unsigned long long)deg);
+    if (fulldeg) atomicAdd(sums + 1, (unsigned long long)fulldeg);
+  }
+}
+
+// bottom-up: every vertex still missing source bits ORs its in-neighbours'
+// frontier masks, stopping once it has them all
+__global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
+                                              const uint64_t* __restrict__ off, uint32_t nV, unsigned long long full,
+                                              const unsigned long long* __restrict__ fr,
+                                              unsigned long long* __restrict__ nfr,
+                                              unsigned long long* __restrict__ vis, int32_t* __restrict__ lev,
+                                              int32_t L, uint32_t* __restrict__ nq, uint32_t* __restrict__ ncount,
+                                              unsigned long long* __restrict__ sums) {
+  uint64_t deg = 0, fulldeg = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // every lane runs the same number of rounds so the wave-level append is uniform
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nV; base += stride) {
+    const uint64_t v = base + threadIdx.x;
+    bool take = false;
+    if (v < nV) {
+      const unsigned long long seen = vis[v];
+      if (seen != full) {
+        unsigned long long m = 0;
+        const uint64_t b = coff[v], e = coff[v + 1];
+        for (uint64_t p = b; p < e; ++p) {
+          m |= fr[cidx[p]];
+          if ((m | seen) == full) break;
+        }
+        m &= ~seen;
+        if (m) {
+          nfr[v] = m;
+          vis[v] = seen | m;
+          take = true;
+          const uint64_t d = off[v + 1] - off[v];
+          deg += d;
+          if ((seen | m) == full) fulldeg += d;
+          if (lev) ms_record(lev, nV, (uint32_t)v, m, L);
         }
       }
     }
+    wave_append(take, (uint32_t)v, nq, ncount);
   }
-  if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
+  deg = wave_sum(deg);
+  fulldeg = wave_sum(fulldeg);
+  if ((threadIdx.x & 63) == 0) {
+    if (deg) atomicAdd(sums, (unsigned long long)deg);
+    if (fulldeg) atomicAdd(sums + 1, (unsigned long long)fulldeg);
+  }
+}
+
+// clears the frontier masks of the queued vertices
+__global__ void k_ms_clear(const uint32_t* __restrict__ q, uint32_t n, unsigned long long* __restrict__ fr) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+    fr[q[j]] = 0ULL;
 }
 
 __global__ void k_ms_edges(const unsigned long long* __restrict__ vis, const uint64_t* __restrict__ off, uint32_t nV,
@@ -478,12 +587,19 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
-    DevBuf vis, fr[2], lev, dsrc, flag;
+    const uint64_t* off = g->off.as<uint64_t>();
+    DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, lev, dsrc, flag, sums;
     vis.alloc(8 * (size_t)nV);
     fr[0].alloc(8 * (size_t)nV);
     fr[1].alloc(8 * (size_t)nV);
+    nxt.alloc(8 * (size_t)nV);
+    q[0].alloc(4 * (size_t)nV);
+    q[1].alloc(4 * (size_t)nV);
+    qdeg.alloc(8 * ((size_t)nV + 1));
+    qpos.alloc(8 * ((size_t)nV + 1));
     dsrc.alloc(4 * (size_t)nsrc);
     flag.alloc(16);
+    sums.alloc(16);
     if (level_out) {
       lev.alloc(4 * (size_t)nV * nsrc);
       hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
@@ -491,27 +607,75 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     }
     GRE_HIP(hipMemsetAsync(vis.p, 0, vis.bytes, s));
     GRE_HIP(hipMemsetAsync(fr[0].p, 0, fr[0].bytes, s));
+    GRE_HIP(hipMemsetAsync(fr[1].p, 0, fr[1].bytes, s));
+    GRE_HIP(hipMemsetAsync(nxt.p, 0, nxt.bytes, s));
     GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(1), 0, s, dsrc.as<uint32_t>(), nsrc, (unsigned long long*)vis.p,
                        (unsigned long long*)fr[0].p, level_out ? lev.as<int32_t>() : nullptr, nV);
+    GRE_HIP(hipGetLastError());
+    // the first frontier: the distinct sources
+    std::vector<uint32_t> uq(srcs, srcs + nsrc);
+    std::sort(uq.begin(), uq.end());
+    uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+    GRE_HIP(hipMemcpyAsync(q[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
+    GRE_HIP(hipStreamSynchronize(s));
     const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+    uint32_t qn = (uint32_t)uq.size(), prev_n = 0;
+    uint64_t mf = 0;                // edges out of the frontier (known after the first scan)
+    uint64_t mu = g->nE;            // edges of vertices still missing source bits
+    bool topdown = true;
+    const double alpha = 14.0, beta = 24.0;
     int cur = 0;
-    for (int32_t L = 0;; ++L) {
+    for (int32_t L = 0; qn > 0; ++L) {
+      GRE_CHECK(L <= (int32_t)nV, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+      const int nx = cur ^ 1;
+      if (topdown && L > 0 && (double)mf > (double)mu / alpha && qn > prev_n) topdown = false;
+      else if (!topdown && (double)qn < (double)nV / beta && qn < prev_n) topdown = true;
       GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-      {
-        ProfScope ps("msbfs_level");
-        hipLaunchKernelGGL(k_ms_level, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
-                           nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[cur ^ 1].p,
+      GRE_HIP(hipMemsetAsync(sums.p, 0, 16, s));
+      uint32_t nn = 0;
+      if (topdown) {
+        hipLaunchKernelGGL(k_qdeg, dim3(gcap((uint64_t)qn + 1)), dim3(kB), 0, s, q[cur].as<uint32_t>(), qn, off,
+                           qdeg.as<uint64_t>());
+        scan_excl_u64(qdeg.as<uint64_t>(), qpos.as<uint64_t>(), (size_t)qn + 1, s);
+        uint64_t T = 0;
+        GRE_HIP(hipMemcpyAsync(&T, qpos.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        if (T > 0) {
+          ProfScope ps("msbfs_topdown");
+          hipLaunchKernelGGL(k_ms_td, dim3((unsigned)((T + kEB - 1) / kEB)), dim3(kB), 0, s, off,
+                             g->idx.as<uint32_t>(), q[cur].as<uint32_t>(), qpos.as<uint64_t>(), qn, T,
+                             (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                             (unsigned long long*)nxt.p, q[nx].as<uint32_t>(), flag.as<uint32_t>());
+        }
+        GRE_HIP(hipGetLastError());
+        GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        if (nn > 0)
+          hipLaunchKernelGGL(k_ms_td_fin, dim3(gcap(nn)), dim3(kB), 0, s, q[nx].as<uint32_t>(), nn, full,
+                             (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
+                             off, level_out ? lev.as<int32_t>() : nullptr, nV, L, (unsigned long long*)sums.p);
+      } else {
+        ProfScope ps("msbfs_bottomup");
+        hipLaunchKernelGGL(k_ms_bu, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
+                           off, nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
                            (unsigned long long*)vis.p, level_out ? lev.as<int32_t>() : nullptr, L,
-                           flag.as<unsigned int>());
+                           q[nx].as<uint32_t>(), flag.as<uint32_t>(), (unsigned long long*)sums.p);
+        GRE_HIP(hipGetLastError());
+        GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
       }
+      // the old frontier's masks go back to zero
+      hipLaunchKernelGGL(k_ms_clear, dim3(gcap(qn)), dim3(kB), 0, s, q[cur].as<uint32_t>(), qn,
+                         (unsigned long long*)fr[cur].p);
       GRE_HIP(hipGetLastError());
-      uint32_t any = 0;
-      GRE_HIP(hipMemcpyAsync(&any, flag.p, 4, hipMemcpyDeviceToHost, s));
+      uint64_t h[2];
+      GRE_HIP(hipMemcpyAsync(h, sums.p, 16, hipMemcpyDeviceToHost, s));
       GRE_HIP(hipStreamSynchronize(s));
-      cur ^= 1;
-      if (!any) break;
-      GRE_CHECK(L < (int32_t)nV + 1, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+      prev_n = qn;
+      qn = nn;
+      mf = h[0];
+      mu = mu > h[1] ? mu - h[1] : 0;
+      cur = nx;
     }
     if (edges_out) {
       GRE_HIP(hipMemsetAsync(flag.p, 0, 8, s));
