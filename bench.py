@@ -14,7 +14,10 @@ GTEPS = stored edges x iterations / time.  The same JSON line carries:
                 plain direction-optimising BFS per source vs one 64-bit-mask
                 multi-source launch (Graph500 GTEPS);
   ppr           32 personalised vectors in one batched launch sequence and the
-                adjoint on the same batch, vs the plain single-vector path.
+                adjoint on the same batch, vs the plain single-vector path;
+  bc            256-source FP64 Brandes BC on symmetrised scale 20 (seed 4),
+                batched vs one source per call;
+  graph_replay  20 PageRank iterations on scale 16, eager vs hipGraph replay.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26] [--schedule auto]
 """
@@ -77,6 +80,57 @@ def pmc_traffic(scale, sched):
         if "=" in line and line.strip().startswith("traffic_bytes_per_launch"):
             vals["t"] = float(line.split("=")[1])
     return vals.get("t")
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_graph_replay(N):
+    """Launch-bound case: 20 FP32 PageRank iterations on R-MAT scale 16, eager vs a replayed hipGraph."""
+    g = N.Graph.rmat(16, 16, 1, False)
+    out = {"graph": "R-MAT scale 16 directed, seed 1", "iterations": 20}
+    for on in (False, True):
+        pr = N.PageRank(g, N.F32)
+        pr.set_schedule(3, 8)
+        pr.set_graphs(on)
+        for _ in range(3):
+            pr.run(0.85, 20)
+        N.sync()
+        t = time.perf_counter()
+        for _ in range(50):
+            pr.run(0.85, 20)
+        N.sync()
+        out["graph_ms" if on else "eager_ms"] = round(1e3 * (time.perf_counter() - t) / 50, 4)
+    out["speedup"] = round(out["eager_ms"] / out["graph_ms"], 2)
+    return out
+
+
+def bench_bc(N, rng):
+    """256-source FP64 Brandes BC on symmetrised R-MAT scale 20 (seed 4), batched vs one source per call."""
+    g = N.Graph.rmat(20, 16, 4, True)
+    srcs = sample_distinct(rng, g.nV, 256).astype(np.uint32)
+    N.bc(g, srcs[:64])
+    N.sync()
+    t = time.perf_counter()
+    N.bc(g, srcs)
+    N.sync()
+    tb = time.perf_counter() - t
+    t = time.perf_counter()
+    for s_ in srcs[:8]:
+        N.bc(g, s_[None])
+    N.sync()
+    tp = (time.perf_counter() - t) / 8
+    return {"graph": "R-MAT scale 20 symmetrised, seed 4", "sources": 256, "edges": g.nE,
+            "batched_ms": round(1e3 * tb, 2), "batched_ms_per_source": round(1e3 * tb / 256, 3),
+            "plain_ms_per_source": round(1e3 * tp, 3), "batched_vs_plain": round(tp * 256 / tb, 2),
+            "teps_batched": round(256 * g.nE / tb / 1e9, 2)}
 
 
 def main():
@@ -193,8 +247,16 @@ def main():
         cg = E * a.cpu_iters / secs / 1e9
         cpu = {"value": round(cg, 4), "unit": "GTEPS", "cores": nthreads, "kind": "port",
                "sample": f"{a.cpu_iters} PageRank iterations of the same scale-{S} graph, GRE-style push "
-                         f"superstep (12 workers, 8192 striped locks), {secs:.1f}s"}
+                         f"superstep (12 workers + 8192 striped locks, as GRE runs one process), {secs:.1f}s",
+               "cpu_model": cpu_model(), "host_threads": os.cpu_count()}
         log(f"cpu baseline: {cg:.3f} GTEPS in {secs:.1f}s")
+        # the same superstep on every core this process may use (at most 16 here)
+        allc = min(len(os.sched_getaffinity(0)), 16)
+        if allc != nthreads:
+            _, secs2 = oracle.gre_pagerank_mt(og, nthreads=allc, iters=1)
+            cpu["all_cores"] = {"value": round(E / secs2 / 1e9, 4), "cores": allc,
+                                "sample": f"1 iteration, {secs2:.1f}s"}
+            log(f"cpu baseline, {allc} threads: {E / secs2 / 1e9:.3f} GTEPS")
         del d
 
     extra = {}
@@ -276,6 +338,7 @@ def main():
         extra["bfs"] = {"sources": len(srcs), "graph": f"R-MAT scale {S} symmetrised, seed 6",
                         "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
                         "gteps_max": round(max(rates), 3)}
+        N.msbfs(gs, srcs[:4], want_levels=False)   # first call sizes the working arrays
         N.sync()
         t1 = time.perf_counter()
         _, ems = N.msbfs(gs, srcs[:64], want_levels=False)
@@ -285,6 +348,11 @@ def main():
                           "ms": round(1e3 * t_ms, 2),
                           "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
         log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
+        del gs
+        extra["bc"] = bench_bc(N, rng)
+        log("bc:", extra["bc"])
+        extra["graph_replay"] = bench_graph_replay(N)
+        log("graph replay:", extra["graph_replay"])
 
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,

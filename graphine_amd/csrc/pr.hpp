@@ -2,6 +2,9 @@
 #pragma once
 #include "pull.hpp"
 
+#include <cstring>
+#include <vector>
+
 namespace gre {
 constexpr int kMaxBatch = 64;
 struct BatchState {
@@ -48,6 +51,16 @@ struct gre_pr {
   gre::PullBuffers buf_csc, buf_csr;
   int plan_ipt_csc = 0, plan_ipt_csr = 0;
   gre::BatchState bstate;
+  // fixed-iteration forward runs replayed from a captured hipGraph
+  bool graphs = true;
+  hipGraph_t fwd_graph = nullptr;
+  hipGraphExec_t fwd_exec = nullptr;
+  std::vector<uint64_t> fwd_key;   // what the capture baked in (sizes, pointers, parameters)
+  int fwd_post_xcur = 0, fwd_post_ccur = 0;
+  ~gre_pr() {
+    if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
+    if (fwd_graph) (void)hipGraphDestroy(fwd_graph);
+  }
 };
 
 namespace gre {

@@ -263,6 +263,30 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
 // ---------------------------------------------------------------------------
 // multi-source BFS: 64 sources as bits of one mask per vertex
 // ---------------------------------------------------------------------------
+// Working arrays, grown to the largest graph seen and kept for the process
+// (like the single-source ones): a call allocates nothing of size V.
+struct MsBufs {
+  DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums;
+  uint32_t nV = 0;
+};
+MsBufs& ms_bufs(uint32_t nV) {
+  static MsBufs& b = *new MsBufs();  // lives until the process ends
+  if (b.nV < nV || !b.vis.p) {
+    b.vis.alloc(8 * (size_t)nV);
+    b.fr[0].alloc(8 * (size_t)nV);
+    b.fr[1].alloc(8 * (size_t)nV);
+    b.nxt.alloc(8 * (size_t)nV);
+    b.q[0].alloc(4 * (size_t)nV);
+    b.q[1].alloc(4 * (size_t)nV);
+    b.qdeg.alloc(8 * ((size_t)nV + 1));
+    b.qpos.alloc(8 * ((size_t)nV + 1));
+    b.flag.alloc(16);
+    b.sums.alloc(16);
+    b.nV = nV;
+  }
+  return b;
+}
+
 __global__ void k_ms_seed(const uint32_t* __restrict__ src, int n, unsigned long long* vis, unsigned long long* fr,
                           int32_t* lev, uint32_t nV) {
   for (int i = 0; i < n; ++i) {
@@ -588,27 +612,21 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
     const uint64_t* off = g->off.as<uint64_t>();
-    DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, lev, dsrc, flag, sums;
-    vis.alloc(8 * (size_t)nV);
-    fr[0].alloc(8 * (size_t)nV);
-    fr[1].alloc(8 * (size_t)nV);
-    nxt.alloc(8 * (size_t)nV);
-    q[0].alloc(4 * (size_t)nV);
-    q[1].alloc(4 * (size_t)nV);
-    qdeg.alloc(8 * ((size_t)nV + 1));
-    qpos.alloc(8 * ((size_t)nV + 1));
+    MsBufs& mb = ms_bufs(nV);
+    DevBuf* fr = mb.fr;
+    DevBuf* q = mb.q;
+    DevBuf &vis = mb.vis, &nxt = mb.nxt, &qdeg = mb.qdeg, &qpos = mb.qpos, &flag = mb.flag, &sums = mb.sums;
+    DevBuf lev, dsrc;
     dsrc.alloc(4 * (size_t)nsrc);
-    flag.alloc(16);
-    sums.alloc(16);
     if (level_out) {
       lev.alloc(4 * (size_t)nV * nsrc);
       hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
                          (uint64_t)nV * nsrc, -1);
     }
-    GRE_HIP(hipMemsetAsync(vis.p, 0, vis.bytes, s));
-    GRE_HIP(hipMemsetAsync(fr[0].p, 0, fr[0].bytes, s));
-    GRE_HIP(hipMemsetAsync(fr[1].p, 0, fr[1].bytes, s));
-    GRE_HIP(hipMemsetAsync(nxt.p, 0, nxt.bytes, s));
+    GRE_HIP(hipMemsetAsync(vis.p, 0, 8 * (size_t)nV, s));
+    GRE_HIP(hipMemsetAsync(fr[0].p, 0, 8 * (size_t)nV, s));
+    GRE_HIP(hipMemsetAsync(fr[1].p, 0, 8 * (size_t)nV, s));
+    GRE_HIP(hipMemsetAsync(nxt.p, 0, 8 * (size_t)nV, s));
     GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(1), 0, s, dsrc.as<uint32_t>(), nsrc, (unsigned long long*)vis.p,
                        (unsigned long long*)fr[0].p, level_out ? lev.as<int32_t>() : nullptr, nV);

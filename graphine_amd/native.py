@@ -100,6 +100,7 @@ def _declare(L):
         "gre_pr_get_adjoint": ([vp, vp], i32),
         "gre_pr_adjoint_cotangent": ([vp, vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_set_schedule": ([vp, i32, i32], i32),
+        "gre_pr_set_graphs": ([vp, i32], i32),
         "gre_pr_set_partition": ([vp, i32, u32], i32),
         "gre_pr_get_partition": ([vp, C.POINTER(i32), C.POINTER(u32)], i32),
         "gre_bfs": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
@@ -318,6 +319,9 @@ class PageRank:
         s, t = C.c_int(), C.c_int()
         _ck(lib().gre_pr_get_schedule(self.h, C.byref(s), C.byref(t)))
         return self.SCHEDULES.get(s.value, str(s.value)), t.value
+
+    def set_graphs(self, on):
+        _ck(lib().gre_pr_set_graphs(self.h, 1 if on else 0))
 
     def set_partition(self, ranges, shared_sources=0):
         _ck(lib().gre_pr_set_partition(self.h, int(ranges), int(shared_sources)))

@@ -152,6 +152,10 @@ int gre_pr_set_personalization(gre_pr* p, const void* pvec);
  * is below tol (tol <= 0: fixed iters).  Returns iterations and residual. */
 int gre_pr_run(gre_pr* p, double damping, int iters, double tol, int* iters_done,
                double* residual);
+/* Single-vector runs with tol <= 0 (a fixed iteration count) are captured
+ * once into a hipGraph and replayed while the handle's schedule, buffers and
+ * parameters are unchanged (on = 1, the default); on = 0 launches eagerly. */
+int gre_pr_set_graphs(gre_pr* p, int on);
 int gre_pr_get_ranks(gre_pr* p, void* host_out);
 /* Adjoint of the converged forward: lambda = c + d (W lambda + g (p.lambda)),
  * a pull over CSR; c = indicator of targets (ntgt ids, or dense nV x k in cvec). */

@@ -56,6 +56,37 @@ def test_pagerank_partitioned_ranges(N, O, ranges, shared):
     assert rel_l1(pr32.ranks(), ref) < 1e-6
 
 
+@pytest.mark.parametrize("dtype,schedule,ti", [("f32", 3, 8), ("f32", 5, 4), ("f64", 1, 8), ("f64", 6, 8)])
+def test_pagerank_graph_replay_matches_eager(N, O, dtype, schedule, ti):
+    # fixed-iteration runs replay a captured hipGraph: every replay, with eager
+    # work in between and after parameter changes, gives the eager bits
+    g = N.Graph.rmat(15, 16, 7, False)
+    dt = N.F64 if dtype == "f64" else N.F32
+    ref = {}
+    eager = N.PageRank(g, dt)
+    eager.set_schedule(schedule, ti)
+    eager.set_graphs(False)
+    for it in (20, 7):
+        eager.run(0.85, it)
+        ref[it] = eager.ranks()
+    eager.run(0.8, 20)
+    ref["d"] = eager.ranks()
+    pr = N.PageRank(g, dt)
+    pr.set_schedule(schedule, ti)
+    other = N.PageRank(g, dt)
+    for rep in range(3):
+        it, res = pr.run(0.85, 20)
+        assert it == 20
+        assert np.array_equal(pr.ranks(), ref[20]), rep
+        other.run(0.85, 3, 1e-30)          # eager launches between replays
+    pr.run(0.85, 7)
+    assert np.array_equal(pr.ranks(), ref[7])
+    pr.run(0.8, 20)
+    assert np.array_equal(pr.ranks(), ref["d"])
+    pr.run(0.85, 20)
+    assert np.array_equal(pr.ranks(), ref[20])
+
+
 def test_pagerank_f32_within_1e6_of_f64(N, O):
     g = N.Graph.rmat(18, 16, 6, False)
     og = O.rmat_graph(18, 16, 6, False)
