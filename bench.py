@@ -70,16 +70,18 @@ def sample_distinct(rng, pool, k):
 
 
 def pmc_traffic(scale, sched):
-    """HBM bytes per iteration from the committed rocprofv3 --pmc summary of this schedule, if present."""
-    # _nt variants stream the same bytes with a cache hint: one summary per family
+    """HBM bytes per iteration and the profiled configuration, from the committed rocprofv3 --pmc
+    summary of this schedule family (plain or _nt), if present."""
     f = os.path.join(ROOT, "profiles", f"pagerank_f32_{sched.replace('_nt', '')}_s{scale}_pmc.txt")
     if not os.path.exists(f):
-        return None
+        return None, None
     vals = {}
     for line in open(f):
-        if "=" in line and line.strip().startswith("traffic_bytes_per_launch"):
-            vals["t"] = float(line.split("=")[1])
-    return vals.get("t")
+        if "=" in line and not line.startswith("#"):
+            k, v = (x.strip() for x in line.split("=", 1))
+            vals[k] = v
+    return (float(vals["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in vals else None,
+            vals.get("schedule"))
 
 
 def cpu_model():
@@ -192,8 +194,9 @@ def main():
     achieved = bytes_iter / iter_s / 1e9
     sched, tile_items = pr.schedule()
     kern = "k_pull_tile + k_pull_fixup" + (" (segments, then per-row)" if n_cmb else "")
+    traffic, traffic_sched = pmc_traffic(S, sched)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(S, sched),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_schedule": traffic_sched,
                 "kernel": f"{kern} (one PageRank f32 iteration)",
                 "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4),
                 "pull_tile_avg_ms": round(avg_tile_s * 1e3, 4), "fixup_avg_ms": round(avg_fix_s * 1e3, 4),

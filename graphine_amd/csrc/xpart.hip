@@ -58,13 +58,36 @@ __global__ void k_bounds(const uint64_t* __restrict__ ex, uint32_t H, uint32_t n
   bnd[r] = lo;
 }
 
-// cnt[v] = entries of row v whose compact source is < H
-__global__ void k_hub_rowcnt(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx_c, uint32_t nV,
-                             uint32_t H, uint64_t* __restrict__ cnt) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t c = 0;
-    for (uint64_t e = coff[v]; e < coff[v + 1]; ++e) c += cidx_c[e] < H ? 1u : 0u;
-    cnt[v] = c;
+// cnt[v] += entries of row v whose compact source is < H: one thread per 8
+// consecutive CSC entries (rows found inside the entries' 2048-entry tile, rb
+// from graph_tiles), one atomic per run of a row
+__global__ void k_hub_rowcnt(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ rb, uint32_t te,
+                             const uint32_t* __restrict__ cidx_c, uint64_t E, uint32_t H,
+                             unsigned long long* __restrict__ cnt) {
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c * 8 < E; c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e0 = c * 8, e1 = e0 + 8 < E ? e0 + 8 : E;
+    const uint64_t t = e0 / te;
+    uint32_t lo = rb[t] > 0 ? rb[t] - 1 : 0, hi = rb[t + 1];
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (coff[mid] <= e0) lo = mid;
+      else hi = mid;
+    }
+    uint32_t r = lo;
+    uint64_t rend = coff[r + 1];
+    unsigned long long run = 0;
+    for (uint64_t e = e0; e < e1; ++e) {
+      if (rend <= e) {
+        if (run) atomicAdd(cnt + r, run);
+        run = 0;
+        do {
+          ++r;
+          rend = coff[r + 1];
+        } while (rend <= e);
+      }
+      run += cidx_c[e] < H ? 1ull : 0ull;
+    }
+    if (run) atomicAdd(cnt + r, run);
   }
 }
 
@@ -221,8 +244,10 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     cnt.alloc(8 * ((size_t)nV + 1));
     ex.alloc(8 * ((size_t)nV + 1));
     GRE_HIP(hipMemsetAsync(cnt.p, 0, 8 * ((size_t)nV + 1), s));
-    hipLaunchKernelGGL(k_hub_rowcnt, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
-                       g->cidx_c.as<uint32_t>(), nV, H, cnt.as<uint64_t>());
+    uint64_t nt = 0;
+    const uint32_t* rb = graph_tiles(g, true, 2048, &nt);
+    hipLaunchKernelGGL(k_hub_rowcnt, dim3(gcap((E + 7) / 8)), dim3(kB), 0, s, g->coff.as<uint64_t>(), rb, 2048u,
+                       g->cidx_c.as<uint32_t>(), E, H, cnt.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
     scan_excl_u64(cnt.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nV + 1, s);
     hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, s, ex.as<uint64_t>(), nV, hub_rows.as<uint32_t>());
