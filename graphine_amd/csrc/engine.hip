@@ -21,6 +21,7 @@
 // rebuilds the bitmap words with wave ballots; rebuild = block scan + write.
 #include "graph.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <type_traits>
 
@@ -199,6 +200,250 @@ template <> struct Prog<GRE_PROG_COUNTIN> {             // CountIn.hpp:18-106
   }
 };
 
+// ---- pull form of the compute phase ---------------------------------------
+// For a dense frontier every destination v reads the messages its active
+// neighbours would send it and folds them with the program's combine algebra
+// in registers, then merges the result into its combine data once; a true
+// merge marks v in nextSet, as a true combine() does in the push form.
+//   T      accumulator;  id() its identity;  op(a, b) the fold;
+//   msg    the message of active neighbour u (w = the edge's weight);
+//   fin    merges the folded value into C[v]; any = at least one message;
+//   skip   v cannot change this superstep (no need to read its row);
+//   kFirst the first active neighbour in row order decides (first writer).
+struct PullSumF64 {                  // PageRank, PageRank_delta, BC Backward: locked +=
+  using T = double;
+  static constexpr bool kFirst = false;
+  __device__ static T id() { return 0.0; }
+  __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const double*)c.S)[u]; }
+  __device__ static T op(T a, T b) { return a + b; }
+  __device__ static bool skip(const EngCtx&, uint32_t) { return false; }
+  __device__ static bool fin(const EngCtx& c, uint32_t v, T acc, bool any) {
+    if (!any) return false;
+    ((double*)c.C)[v] += acc;
+    return true;
+  }
+};
+struct PullSumI32 {                  // BC Forward, CountIn: int += (wraps like GRE's int)
+  using T = int32_t;
+  static constexpr bool kFirst = false;
+  __device__ static T id() { return 0; }
+  __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const int32_t*)c.S)[u]; }
+  __device__ static T op(T a, T b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+  __device__ static bool skip(const EngCtx&, uint32_t) { return false; }
+  __device__ static bool fin(const EngCtx& c, uint32_t v, T acc, bool any) {
+    if (!any) return false;
+    int32_t* cm = (int32_t*)c.C;
+    cm[v] = op(cm[v], acc);
+    return true;
+  }
+};
+struct PullMinU64 {                  // ConnectedComponents: min label
+  using T = unsigned long long;
+  static constexpr bool kFirst = false;
+  __device__ static T id() { return ~0ULL; }
+  __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
+  __device__ static T op(T a, T b) { return a < b ? a : b; }
+  __device__ static bool skip(const EngCtx&, uint32_t) { return false; }
+  __device__ static bool fin(const EngCtx& c, uint32_t v, T acc, bool) {
+    unsigned long long* cm = (unsigned long long*)c.C;
+    if (!(acc < cm[v])) return false;
+    cm[v] = acc;
+    return true;
+  }
+};
+struct PullFirstU64 {                // BFS: the first message into an unset combine slot wins
+  using T = unsigned long long;
+  static constexpr bool kFirst = true;
+  __device__ static T id() { return ~0ULL; }
+  __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
+  __device__ static T op(T a, T b) { return a != ~0ULL ? a : b; }
+  __device__ static bool skip(const EngCtx& c, uint32_t v) { return ((const unsigned long long*)c.C)[v] != ~0ULL; }
+  __device__ static bool fin(const EngCtx& c, uint32_t v, T acc, bool any) {
+    unsigned long long* cm = (unsigned long long*)c.C;
+    if (!any || cm[v] != ~0ULL) return false;
+    cm[v] = acc;
+    return true;
+  }
+};
+struct PullMinU32W {                 // SSSP: min of lastStates[u] + edgeWgt
+  using T = uint32_t;
+  static constexpr bool kFirst = false;
+  __device__ static T id() { return 0xffffffffu; }
+  __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t w) { return ((const uint32_t*)c.S)[u] + w; }
+  __device__ static T op(T a, T b) { return a < b ? a : b; }
+  __device__ static bool skip(const EngCtx&, uint32_t) { return false; }
+  __device__ static bool fin(const EngCtx& c, uint32_t v, T acc, bool any) {
+    uint32_t* st = (uint32_t*)c.C;
+    if (!any || !(acc < st[v])) return false;
+    st[v] = acc;
+    return true;
+  }
+};
+template <int P> struct Pull;
+template <> struct Pull<GRE_PROG_PAGERANK> : PullSumF64 {};
+template <> struct Pull<GRE_PROG_PAGERANK_DELTA> : PullSumF64 {};
+template <> struct Pull<GRE_PROG_BC_BACKWARD> : PullSumF64 {};
+template <> struct Pull<GRE_PROG_BC_FORWARD> : PullSumI32 {};
+template <> struct Pull<GRE_PROG_COUNTIN> : PullSumI32 {};
+template <> struct Pull<GRE_PROG_CC> : PullMinU64 {};
+template <> struct Pull<GRE_PROG_BFS> : PullFirstU64 {};
+template <> struct Pull<GRE_PROG_SSSP> : PullMinU32W {};
+
+// v's pull row: for Out, its in-edges (CSC row, an active u's out-edge u -> v);
+// for In, its out-edges (CSR row, an active u's in-edge v -> u); InOut: CSC then CSR
+struct PullAdj {
+  const uint64_t* off; const uint32_t* idx;     // CSR
+  const uint64_t* coff; const uint32_t* cidx;   // CSC
+  const uint32_t* w_csr; const uint32_t* w_csc; // SSSP weights in CSR / CSC order, or nullptr
+  int dir;
+};
+
+__device__ inline uint64_t pull_deg(const PullAdj& a, uint32_t v) {
+  uint64_t d = 0;
+  if (a.dir != GRE_DIR_IN) d += a.coff[v + 1] - a.coff[v];
+  if (a.dir != GRE_DIR_OUT) d += a.off[v + 1] - a.off[v];
+  return d;
+}
+
+__device__ inline void pull_entry(const PullAdj& a, uint32_t v, uint64_t j, uint32_t& u, uint32_t& w) {
+  const uint64_t din = a.dir != GRE_DIR_IN ? a.coff[v + 1] - a.coff[v] : 0;
+  if (j < din) {
+    const uint64_t p = a.coff[v] + j;
+    u = a.cidx[p];
+    w = a.w_csc ? a.w_csc[p] : 0u;
+  } else {
+    const uint64_t p = a.off[v] + (j - din);
+    u = a.idx[p];
+    w = a.w_csr ? a.w_csr[p] : 0u;
+  }
+}
+
+constexpr uint64_t kPullShort = 32;      // rows up to this many entries: one thread
+constexpr uint64_t kPullLong = 65536;    // rows above this: one 1024-thread block; between: one wave
+
+__device__ inline bool bit_test(const uint32_t* b, uint32_t v) { return (b[v >> 5] >> (v & 31)) & 1u; }
+
+__device__ inline void mark_next(uint32_t* next, uint32_t v) {
+  const uint32_t bit = 1u << (v & 31);
+  if (!(next[v >> 5] & bit)) atomicOr(next + (v >> 5), bit);
+}
+
+template <int P>
+__global__ __launch_bounds__(kB) void k_pull_short(EngCtx c, PullAdj a, uint32_t nV,
+                                                   const uint32_t* __restrict__ curr, uint32_t* __restrict__ next) {
+  using R = Pull<P>;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = pull_deg(a, (uint32_t)v);
+    if (d > kPullShort || d == 0 || R::skip(c, (uint32_t)v)) continue;
+    typename R::T acc = R::id();
+    bool any = false;
+    for (uint64_t j = 0; j < d; ++j) {
+      uint32_t u, w;
+      pull_entry(a, (uint32_t)v, j, u, w);
+      if (bit_test(curr, u)) {
+        acc = R::op(acc, R::msg(c, u, w));
+        any = true;
+        if (R::kFirst) break;
+      }
+    }
+    if (R::fin(c, (uint32_t)v, acc, any)) mark_next(next, (uint32_t)v);
+  }
+}
+
+// one wave (or one block of W waves) per listed row; the fold runs over lanes
+// in a fixed order, and for first-writer programs the smallest row position
+// with an active neighbour decides
+template <typename R>
+__device__ inline void wave_fold(typename R::T& acc, uint64_t& first) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const typename R::T oa = __shfl_xor(acc, o, 64);
+    const uint64_t of = __shfl_xor(first, o, 64);
+    if (R::kFirst) {
+      if (of < first) { first = of; acc = oa; }
+    } else {
+      acc = R::op(acc, oa);
+      first = of < first ? of : first;
+    }
+  }
+}
+
+// W == 1: blocks of kB threads, one wave per row; W > 1: blocks of 64*W threads, one block per row
+template <int P, int W>
+__global__ __launch_bounds__(W == 1 ? kB : 64 * W) void k_pull_rows(EngCtx c, PullAdj a, const uint32_t* __restrict__ rows,
+                                                      uint32_t nrows, const uint32_t* __restrict__ curr,
+                                                      uint32_t* __restrict__ next) {
+  using R = Pull<P>;
+  using T = typename R::T;
+  __shared__ T sacc[W];
+  __shared__ uint64_t sfirst[W];
+  const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+  const uint64_t per = W == 1 ? (uint64_t)(blockDim.x >> 6) : 1;
+  const uint64_t r0 = W == 1 ? (uint64_t)blockIdx.x * per + wv : blockIdx.x;
+  const uint64_t rstep = W == 1 ? (uint64_t)gridDim.x * per : gridDim.x;
+  for (uint64_t r = r0; r < nrows; r += rstep) {
+    const uint32_t v = rows[r];
+    const bool skip = R::skip(c, v);
+    const uint64_t d = skip ? 0 : pull_deg(a, v);
+    T acc = R::id();
+    uint64_t first = ~0ULL;
+    const uint64_t t0 = W == 1 ? (uint64_t)lane : threadIdx.x;
+    const uint64_t ts = W == 1 ? 64 : 64 * W;
+    for (uint64_t base = 0; base < d; base += ts) {
+      const uint64_t j = base + t0;
+      if (j < d && (!R::kFirst || first == ~0ULL)) {
+        uint32_t u, w;
+        pull_entry(a, v, j, u, w);
+        if (bit_test(curr, u)) {
+          if (R::kFirst) { if (first == ~0ULL) { first = j; acc = R::msg(c, u, w); } }
+          else { acc = R::op(acc, R::msg(c, u, w)); first = j < first ? j : first; }
+        }
+      }
+      if (R::kFirst) {
+        // stop at the first chunk that found one (every lane sees the same chunks)
+        bool found = __any(first != ~0ULL);
+        if (W > 1) {
+          __syncthreads();
+          if (lane == 0) sfirst[wv] = found ? 1ULL : 0ULL;
+          __syncthreads();
+          found = false;
+          for (int q = 0; q < W; ++q) found = found || sfirst[q] != 0ULL;
+          __syncthreads();
+        }
+        if (found) break;
+      }
+    }
+    wave_fold<R>(acc, first);
+    if (W > 1) {
+      if (lane == 0) { sacc[wv] = acc; sfirst[wv] = first; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int q = 1; q < W; ++q) {
+          if (R::kFirst) {
+            if (sfirst[q] < first) { first = sfirst[q]; acc = sacc[q]; }
+          } else {
+            acc = R::op(acc, sacc[q]);
+            first = sfirst[q] < first ? sfirst[q] : first;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if ((W == 1 ? lane : (int)threadIdx.x) == 0 && !skip) {
+      if (R::fin(c, v, acc, first != ~0ULL)) mark_next(next, v);
+    }
+  }
+}
+
+// rows of pull degree in (kPullShort, kPullLong] and above kPullLong
+__global__ void k_pull_classify(PullAdj a, uint32_t nV, uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
+                                uint32_t* __restrict__ lng, uint32_t* __restrict__ nlng) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = pull_deg(a, (uint32_t)v);
+    if (d > kPullLong) lng[atomicAdd(nlng, 1u)] = (uint32_t)v;
+    else if (d > kPullShort) mid[atomicAdd(nmid, 1u)] = (uint32_t)v;
+  }
+}
+
 // ---- frontier plumbing ------------------------------------------------------
 struct Adj {
   const uint64_t* off; const uint32_t* idx;     // CSR
@@ -213,8 +458,6 @@ __device__ inline uint64_t degree(const Adj& a, uint32_t v) {
   if (a.dir != GRE_DIR_OUT) d += a.coff[v + 1] - a.coff[v];
   return d;
 }
-
-__device__ inline bool bit_test(const uint32_t* b, uint32_t v) { return (b[v >> 5] >> (v & 31)) & 1u; }
 
 template <int P>
 __global__ void k_init_all(EngCtx c, uint32_t nV, uint32_t nvpad, uint32_t* __restrict__ curr) {
@@ -445,6 +688,11 @@ struct gre_engine {
   uint32_t qn = 0;
   uint64_t qedges = 0;
   double times[5] = {0, 0, 0, 0, 0};
+  int mode = 0;                // GRE_ENGINE_AUTO / _PUSH / _PULL
+  int pull_dir = -1;           // direction the row classes below were built for
+  gre::DevBuf pull_mid, pull_long, pull_cnt;
+  uint32_t n_mid = 0, n_long = 0;
+  int64_t pull_steps = 0;      // supersteps of the last run done in the pull form
 };
 
 namespace gre {
@@ -472,6 +720,41 @@ Adj make_adj(gre_engine* e) {
   a.ceid = (e->prog == GRE_PROG_SSSP && e->dir != GRE_DIR_OUT) ? graph_ceid(g) : nullptr;
   a.dir = e->dir;
   return a;
+}
+
+PullAdj make_pull_adj(gre_engine* e) {
+  PullAdj a;
+  gre_graph* g = e->g;
+  a.off = g->off.as<uint64_t>();
+  a.idx = g->idx.as<uint32_t>();
+  a.coff = g->coff.as<uint64_t>();
+  a.cidx = g->cidx.as<uint32_t>();
+  const bool w = e->prog == GRE_PROG_SSSP;
+  a.w_csr = (w && e->dir != GRE_DIR_OUT) ? g->w_u32.as<uint32_t>() : nullptr;
+  a.w_csc = (w && e->dir != GRE_DIR_IN) ? graph_w_csc(g) : nullptr;
+  a.dir = e->dir;
+  return a;
+}
+
+// rows too long for one thread, listed once per direction
+void ensure_pull_rows(gre_engine* e, const PullAdj& a, hipStream_t s) {
+  if (e->pull_dir == e->dir) return;
+  const uint32_t nV = e->g->nV;
+  if (!e->pull_mid.p) {
+    e->pull_mid.alloc(4 * (size_t)nV);
+    e->pull_long.alloc(4 * (size_t)nV);
+    e->pull_cnt.alloc(8);
+  }
+  GRE_HIP(hipMemsetAsync(e->pull_cnt.p, 0, 8, s));
+  hipLaunchKernelGGL(k_pull_classify, dim3(gcap(nV)), dim3(kB), 0, s, a, nV, e->pull_mid.as<uint32_t>(),
+                     e->pull_cnt.as<uint32_t>(), e->pull_long.as<uint32_t>(), e->pull_cnt.as<uint32_t>() + 1);
+  GRE_HIP(hipGetLastError());
+  uint32_t h[2];
+  GRE_HIP(hipMemcpyAsync(h, e->pull_cnt.p, 8, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  e->n_mid = h[0];
+  e->n_long = h[1];
+  e->pull_dir = e->dir;
 }
 
 template <typename F>
@@ -577,6 +860,7 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
                      e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
   rebuild_queue(e, a, s);
   e->superstep = 0;
+  e->pull_steps = 0;
   e->begun = true;
   for (double& t : e->times) t = 0.0;
   e->times[0] = now() - t0;
@@ -590,7 +874,32 @@ int64_t engine_step(gre_engine* e) {
   const double t0 = now();
   // nextSet.clear() (Engine.hpp:322)
   hipLaunchKernelGGL(k_zero_u32, dim3(gcap(e->nwords)), dim3(kB), 0, s, e->next.as<uint32_t>(), (uint64_t)e->nwords);
-  if (e->qedges > 0) {
+  // push while the frontier's edges are few, pull once they are a large share
+  // of every destination's rows (the pull reads all rows, without atomics)
+  const uint64_t epull = (e->dir == GRE_DIR_INOUT ? 2 : 1) * e->g->nE;
+  const bool pull = e->qedges > 0 && (e->mode == GRE_ENGINE_PULL ||
+                                      (e->mode == GRE_ENGINE_AUTO && e->qedges * 8 > epull));
+  if (pull) {
+    const PullAdj pa = make_pull_adj(e);
+    ensure_pull_rows(e, pa, s);
+    ProfScope ps("pull_compute");
+    const uint32_t nV = e->g->nV;
+    dispatch(e->prog, [&](auto P) {
+      constexpr int PV = decltype(P)::value;
+      hipLaunchKernelGGL(k_pull_short<PV>, dim3(gcap(nV)), dim3(kB), 0, s, c, pa, nV, e->curr.as<uint32_t>(),
+                         e->next.as<uint32_t>());
+      if (e->n_mid)
+        hipLaunchKernelGGL((k_pull_rows<PV, 1>), dim3(std::min<uint32_t>((e->n_mid + 3) / 4, 1u << 20)), dim3(kB),
+                           0, s, c, pa, e->pull_mid.as<uint32_t>(), e->n_mid, e->curr.as<uint32_t>(),
+                           e->next.as<uint32_t>());
+      if (e->n_long)
+        hipLaunchKernelGGL((k_pull_rows<PV, 16>), dim3(std::min<uint32_t>(e->n_long, 1u << 20)), dim3(1024), 0, s, c,
+                           pa, e->pull_long.as<uint32_t>(), e->n_long, e->curr.as<uint32_t>(),
+                           e->next.as<uint32_t>());
+    });
+    GRE_HIP(hipGetLastError());
+    ++e->pull_steps;
+  } else if (e->qedges > 0) {
     const uint64_t nblk = (e->qedges + kEB - 1) / kEB;
     GRE_CHECK(nblk < 0x7fffffffULL, GRE_DEV_FAILURE, "frontier too large for one launch");
     ProfScope ps("push_expand");
@@ -766,6 +1075,22 @@ int gre_engine_phase_times(const gre_engine* e, double* t5) {
   return guard([&] {
     GRE_CHECK(e && t5, GRE_DEV_NULL, "null argument");
     for (int i = 0; i < 5; ++i) t5[i] = e->times[i];
+  });
+}
+
+int gre_engine_set_mode(gre_engine* e, int mode) {
+  return guard([&] {
+    GRE_CHECK(e, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(mode == GRE_ENGINE_AUTO || mode == GRE_ENGINE_PUSH || mode == GRE_ENGINE_PULL, GRE_DEV_NULL,
+              "mode must be GRE_ENGINE_AUTO, _PUSH or _PULL");
+    e->mode = mode;
+  });
+}
+
+int gre_engine_pull_steps(const gre_engine* e, int64_t* steps) {
+  return guard([&] {
+    GRE_CHECK(e && steps, GRE_DEV_NULL, "null argument");
+    *steps = e->pull_steps;
   });
 }
 

@@ -88,6 +88,8 @@ def _declare(L):
         "gre_engine_begin": ([vp, i32, vp, i64], i32),
         "gre_engine_step": ([vp, C.POINTER(i64)], i32),
         "gre_engine_phase_times": ([vp, vp], i32),
+        "gre_engine_set_mode": ([vp, i32], i32),
+        "gre_engine_pull_steps": ([vp, C.POINTER(i64)], i32),
         "gre_pr_create": ([pp, vp, i32, i32, i32], i32),
         "gre_pr_destroy": ([vp], i32),
         "gre_pr_set_edge_weights": ([vp, vp, u64], i32),
@@ -280,6 +282,16 @@ class Engine:
         # to halt recorded ss + 1 (the last one is 0)
         n = ss.value if ss.value >= self.max_ss else ss.value + 1
         return ss.value, counts[:min(n, max_counts)].copy()
+
+    AUTO, PUSH, PULL = 0, 1, 2
+
+    def set_mode(self, mode):
+        _ck(lib().gre_engine_set_mode(self.h, mode))
+
+    def pull_steps(self):
+        v = C.c_int64()
+        _ck(lib().gre_engine_pull_steps(self.h, C.byref(v)))
+        return v.value
 
     def phase_times(self):
         t = np.zeros(5, np.float64)

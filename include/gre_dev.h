@@ -131,6 +131,15 @@ int gre_engine_step(gre_engine* e, int64_t* active);
 /* Phase times of the last run in seconds: init, sync-scatter, compute,
  * sync-combiner, apply (Engine.hpp:1292-1306; the two sync phases are 0). */
 int gre_engine_phase_times(const gre_engine* e, double* t5);
+/* Compute-phase form: AUTO (default) pushes along the frontier's edges while
+ * they are under 1/8 of the direction's edges and otherwise pulls (each
+ * destination folds its active neighbours' messages with the program's
+ * combine, no atomics); PUSH and PULL force one form.  GRE has only the push
+ * form (Engine.hpp:592-625); both give GRE's results. */
+enum { GRE_ENGINE_AUTO = 0, GRE_ENGINE_PUSH = 1, GRE_ENGINE_PULL = 2 };
+int gre_engine_set_mode(gre_engine* e, int mode);
+/* Supersteps of the last run that used the pull form. */
+int gre_engine_pull_steps(const gre_engine* e, int64_t* steps);
 
 /* ---- PageRank / personalised PageRank (pull over CSC) --------------------
  * No GRE counterpart: GRE's shipped PageRank never refreshes its scatter data

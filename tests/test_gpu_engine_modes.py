@@ -1,0 +1,124 @@
+"""GPU parity of the engine's compute-phase forms: every shipped vertex program
+run with the push form only, the pull form only and the automatic switch gives
+the oracle's GRE-engine results (integers bit-exact, including the
+per-superstep active counts; FP64 sums to 1e-12 relative), on R-MAT graphs and
+on a graph with a hub row long enough for the block-per-row pull path.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = [("push", 1), ("pull", 2), ("auto", 0)]
+
+
+def run_mode(N, O, g, og, prog, direction, mode, srcs=None, max_ss=0, aux=None, level=1):
+    e = N.Engine(g, prog)
+    e.set_mode(mode)
+    if max_ss:
+        e.config_max_supersteps(max_ss)
+    if prog == N.BC_BACKWARD:
+        e.upload(N.ARR_AUX, aux)
+    e.set_level(level)
+    ss, counts = e.run(direction, srcs)
+    ost = O.default_state(prog, og.nV)
+    oss, ocounts, _ = O.gre_run(prog, direction, og, ost, aux=aux, level=level, srcs=srcs, max_ss=max_ss)
+    if mode == 2 and ss > 0:
+        # every superstep with a non-empty frontier ran in the pull form (a run
+        # that voted to halt ran ss + 1 of them)
+        assert e.pull_steps() in (ss, ss + 1)
+    if mode == 1:
+        assert e.pull_steps() == 0
+    return e, ss, counts, oss, ocounts, ost
+
+
+def hub_graph(N, O):
+    # a random graph plus 70,000 edges into vertex 3 and out of vertex 5, symmetrised
+    rng = np.random.default_rng(11)
+    n = 90000
+    src = rng.integers(0, n, 300000)
+    dst = rng.integers(0, n, 300000)
+    hub_in = np.arange(10, 70010)
+    src = np.concatenate([src, hub_in, np.full(70000, 5)])
+    dst = np.concatenate([dst, np.full(70000, 3), hub_in])
+    keep = src != dst
+    s2 = np.concatenate([src[keep], dst[keep]])
+    d2 = np.concatenate([dst[keep], src[keep]])
+    key = np.unique(s2.astype(np.int64) * n + d2)
+    s3 = (key // n).astype(np.uint32)
+    d3 = (key % n).astype(np.uint32)
+    return N.Graph.from_edges(n, s3, d3), O.graph_from_edges(n, s3, d3)
+
+
+@pytest.mark.parametrize("mode_name,mode", MODES)
+def test_modes_bfs_cc_countin(N, O, mode_name, mode):
+    g, og = N.Graph.rmat(14, 16, 3, True), O.rmat_graph(14, 16, 3, True)
+    src = int(np.argmax(og.outdeg))
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.BFS, N.DIR_IN, mode, srcs=[src])
+    assert ss == oss and np.array_equal(counts, ocounts)
+    assert np.array_equal(e.download(N.ARR_VERTEX), ost[2])
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.CC, N.DIR_INOUT, mode)
+    assert ss == oss and np.array_equal(counts, ocounts)
+    assert np.array_equal(e.download(N.ARR_VERTEX), ost[2])
+    gd, ogd = N.Graph.rmat(13, 16, 4, False), O.rmat_graph(13, 16, 4, False)
+    for d in (N.DIR_OUT, N.DIR_IN):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, gd, ogd, N.COUNTIN, d, mode, max_ss=1)
+        assert np.array_equal(e.download(N.ARR_VERTEX), ost[2])
+
+
+@pytest.mark.parametrize("mode_name,mode", MODES)
+def test_modes_sssp(N, O, mode_name, mode):
+    g, og = N.Graph.rmat(12, 16, 2, True), O.rmat_graph(12, 16, 2, True)
+    g.set_weights_u32(None, seed=3)
+    w = O.sssp_weights(og, seed=3)
+    src = int(np.argmax(og.outdeg))
+    for d in (N.DIR_OUT, N.DIR_IN):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.SSSP, d, mode, srcs=[src], aux=w)
+        assert ss == oss and np.array_equal(counts, ocounts)
+        assert np.array_equal(e.download(N.ARR_VERTEX), ost[1])
+
+
+@pytest.mark.parametrize("mode_name,mode", MODES)
+def test_modes_pagerank(N, O, mode_name, mode):
+    g, og = N.Graph.rmat(13, 16, 1, False), O.rmat_graph(13, 16, 1, False)
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=10)
+    assert ss == oss == 10 and np.array_equal(counts, ocounts)
+    assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode_name,mode", MODES)
+def test_modes_bc_pipeline(N, O, mode_name, mode):
+    g, og = N.Graph.rmat(12, 16, 4, True), O.rmat_graph(12, 16, 4, True)
+    src = int(np.argmax(og.outdeg))
+    f, ss, counts, oss, ocounts, fst = run_mode(N, O, g, og, N.BC_FORWARD, N.DIR_OUT, mode, srcs=[src])
+    assert ss == oss and np.array_equal(counts, ocounts)
+    vtx = f.download(N.ARR_VERTEX)
+    assert np.array_equal(vtx, fst[2])
+    b = N.Engine(g, N.BC_BACKWARD)
+    b.set_mode(mode)
+    b.upload(N.ARR_AUX, vtx)
+    bst = O.default_state(O.BC_BACKWARD, og.nV)
+    for i in range(f.level() - 2, -1, -1):
+        b.set_level(i)
+        b.run(N.DIR_IN, None)
+        O.gre_run(O.BC_BACKWARD, O.DIR_IN, og, bst, aux=fst[2], level=i)
+    assert np.allclose(b.download(N.ARR_VERTEX), bst[2], rtol=1e-12, atol=1e-12)
+
+
+def test_modes_hub_rows(N, O):
+    # rows of 70,000 entries take the block-per-row pull path
+    g, og = hub_graph(N, O)
+    assert og.indeg.max() > 65536
+    g.set_weights_u32(None, seed=5)
+    w = O.sssp_weights(og, seed=5)
+    for mode in (1, 2):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.CC, N.DIR_INOUT, mode)
+        assert ss == oss and np.array_equal(counts, ocounts)
+        assert np.array_equal(e.download(N.ARR_VERTEX), ost[2])
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.BFS, N.DIR_IN, mode, srcs=[7])
+        assert ss == oss and np.array_equal(counts, ocounts)
+        assert np.array_equal(e.download(N.ARR_VERTEX), ost[2])
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.SSSP, N.DIR_OUT, mode, srcs=[3], aux=w)
+        assert ss == oss and np.array_equal(e.download(N.ARR_VERTEX), ost[1])
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=5)
+        assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
