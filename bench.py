@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--bfs-sources", type=int, default=16)
+    ap.add_argument("--bfs-sources", type=int, default=64)
     ap.add_argument("--ppr-k", type=int, default=32)
     ap.add_argument("--no-extra", action="store_true", help="headline PageRank only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -111,6 +111,48 @@ def bench_graph_replay(N):
         N.sync()
         out["graph_ms" if on else "eager_ms"] = round(1e3 * (time.perf_counter() - t) / 50, 4)
     out["speedup"] = round(out["eager_ms"] / out["graph_ms"], 2)
+    return out
+
+
+def bench_engine_pr(N, g):
+    """GRE's PageRank vertex program through the superstep engine (10 supersteps), push vs the automatic switch."""
+    out = {"program": "GRE PageRank (FP64 scatter data), 10 supersteps", "graph": "the headline graph"}
+    for name, mode in (("push", N.Engine.PUSH), ("auto", N.Engine.AUTO)):
+        e = N.Engine(g, N.PAGERANK)
+        e.set_mode(mode)
+        e.config_max_supersteps(10)
+        e.run(N.DIR_OUT)
+        N.sync()
+        t = time.perf_counter()
+        e.run(N.DIR_OUT)
+        N.sync()
+        dt = time.perf_counter() - t
+        out[name + "_ms"] = round(1e3 * dt, 2)
+        out[name + "_gteps"] = round(10 * g.nE / dt / 1e9, 3)
+        out[name + "_pull_steps"] = e.pull_steps()
+        e.close()
+    out["auto_vs_push"] = round(out["push_ms"] / out["auto_ms"], 2)
+    return out
+
+
+def bench_engine_bfs(N, gs, src):
+    """GRE's BFS vertex program through the engine from one source, push vs the automatic switch."""
+    out = {"program": "GRE BFS", "source": int(src)}
+    for name, mode in (("push", N.Engine.PUSH), ("auto", N.Engine.AUTO)):
+        for rep in range(2):   # the first run builds the cached row classes
+            e = N.Engine(gs, N.BFS)
+            e.set_mode(mode)
+            N.sync()
+            t = time.perf_counter()
+            ss, _ = e.run(N.DIR_IN, srcs=[src])
+            N.sync()
+            dt = time.perf_counter() - t
+            pulls = e.pull_steps()
+            e.close()
+        out[name + "_ms"] = round(1e3 * dt, 2)
+        out[name + "_pull_steps"] = pulls
+    out["supersteps"] = int(ss)
+    out["auto_vs_push"] = round(out["push_ms"] / out["auto_ms"], 2)
     return out
 
 
@@ -321,7 +363,10 @@ def main():
             "batched_gteps_equiv": round(E * k * iters / t_bf / 1e9, 2),
         }
         log("ppr:", extra["ppr"])
-        del g, pr
+        del pr
+        extra["engine_pagerank"] = bench_engine_pr(N, g)
+        log("engine pagerank:", extra["engine_pagerank"])
+        del g
         # ---------------- BFS on the symmetrised graph ----------------------
         t0 = time.time()
         gs = N.Graph.rmat(S, 16, 6, True)
@@ -351,6 +396,8 @@ def main():
                           "ms": round(1e3 * t_ms, 2),
                           "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
         log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
+        extra["engine_bfs"] = bench_engine_bfs(N, gs, int(srcs[0]))
+        log("engine bfs:", extra["engine_bfs"])
         del gs
         extra["bc"] = bench_bc(N, rng)
         log("bc:", extra["bc"])

@@ -84,6 +84,20 @@ struct ProfScope {
 };
 bool prof_on();
 
+// Appends v to the queue q (length counter qn) for every lane with take set,
+// with one atomic per wave (ballot + prefix popcount); order within a wave is
+// by lane, across waves by atomic order.
+__device__ inline void wave_append(bool take, uint32_t v, uint32_t* __restrict__ q, uint32_t* __restrict__ qn) {
+  const unsigned long long bal = __ballot(take);
+  if (!bal) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll(bal) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(qn, (uint32_t)__popcll(bal));
+  base = __shfl(base, leader, 64);
+  if (take) q[base + (uint32_t)__popcll(bal & ((1ULL << lane) - 1ULL))] = v;
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) {
   uint64_t g = (n + block - 1) / block;
   if (g == 0) g = 1;

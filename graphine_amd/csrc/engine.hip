@@ -20,6 +20,7 @@
 // spin-locks; apply + halt + count are one thread-per-vertex kernel that
 // rebuilds the bitmap words with wave ballots; rebuild = block scan + write.
 #include "graph.hpp"
+#include "pull.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -209,10 +210,16 @@ template <> struct Prog<GRE_PROG_COUNTIN> {             // CountIn.hpp:18-106
 //   msg    the message of active neighbour u (w = the edge's weight);
 //   fin    merges the folded value into C[v]; any = at least one message;
 //   skip   v cannot change this superstep (no need to read its row);
-//   kFirst the first active neighbour in row order decides (first writer).
+//   kFirst the first active neighbour in row order decides (first writer);
+//   kStale an inactive neighbour's message was already merged into C[v] by the
+//          superstep after its last change, so folding it again cannot change
+//          fin's result -- true for min-label CC when every vertex began
+//          active (labels only fall, C[v] <= every label ever sent to v), and
+//          the pull then skips the frontier test.
 struct PullSumF64 {                  // PageRank, PageRank_delta, BC Backward: locked +=
   using T = double;
   static constexpr bool kFirst = false;
+  static constexpr bool kStale = false;
   __device__ static T id() { return 0.0; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const double*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a + b; }
@@ -226,6 +233,7 @@ struct PullSumF64 {                  // PageRank, PageRank_delta, BC Backward: l
 struct PullSumI32 {                  // BC Forward, CountIn: int += (wraps like GRE's int)
   using T = int32_t;
   static constexpr bool kFirst = false;
+  static constexpr bool kStale = false;
   __device__ static T id() { return 0; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const int32_t*)c.S)[u]; }
   __device__ static T op(T a, T b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
@@ -240,6 +248,7 @@ struct PullSumI32 {                  // BC Forward, CountIn: int += (wraps like 
 struct PullMinU64 {                  // ConnectedComponents: min label
   using T = unsigned long long;
   static constexpr bool kFirst = false;
+  static constexpr bool kStale = true;
   __device__ static T id() { return ~0ULL; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a < b ? a : b; }
@@ -254,6 +263,7 @@ struct PullMinU64 {                  // ConnectedComponents: min label
 struct PullFirstU64 {                // BFS: the first message into an unset combine slot wins
   using T = unsigned long long;
   static constexpr bool kFirst = true;
+  static constexpr bool kStale = false;
   __device__ static T id() { return ~0ULL; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a != ~0ULL ? a : b; }
@@ -268,6 +278,7 @@ struct PullFirstU64 {                // BFS: the first message into an unset com
 struct PullMinU32W {                 // SSSP: min of lastStates[u] + edgeWgt
   using T = uint32_t;
   static constexpr bool kFirst = false;
+  static constexpr bool kStale = false;
   __device__ static T id() { return 0xffffffffu; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t w) { return ((const uint32_t*)c.S)[u] + w; }
   __device__ static T op(T a, T b) { return a < b ? a : b; }
@@ -328,24 +339,39 @@ __device__ inline void mark_next(uint32_t* next, uint32_t v) {
   if (!(next[v >> 5] & bit)) atomicOr(next + (v >> 5), bit);
 }
 
+// folds the active entries [b, e) of one adjacency array into acc; false once a
+// first-writer program has its message
+template <typename R>
+__device__ inline bool fold_run(const EngCtx& c, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ w,
+                                uint64_t b, uint64_t e, const uint32_t* __restrict__ curr, bool all,
+                                typename R::T& acc, bool& any) {
+  for (uint64_t p = b; p < e; ++p) {
+    const uint32_t u = idx[p];
+    if (all || bit_test(curr, u)) {
+      acc = R::op(acc, R::msg(c, u, w ? w[p] : 0u));
+      any = true;
+      if (R::kFirst) return false;
+    }
+  }
+  return true;
+}
+
 template <int P>
 __global__ __launch_bounds__(kB) void k_pull_short(EngCtx c, PullAdj a, uint32_t nV,
-                                                   const uint32_t* __restrict__ curr, uint32_t* __restrict__ next) {
+                                                   const uint32_t* __restrict__ curr, bool all,
+                                                   uint32_t* __restrict__ next) {
   using R = Pull<P>;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t d = pull_deg(a, (uint32_t)v);
+    uint64_t b0 = 0, e0 = 0, b1 = 0, e1 = 0;
+    if (a.dir != GRE_DIR_IN) { b0 = a.coff[v]; e0 = a.coff[v + 1]; }
+    if (a.dir != GRE_DIR_OUT) { b1 = a.off[v]; e1 = a.off[v + 1]; }
+    const uint64_t d = (e0 - b0) + (e1 - b1);
     if (d > kPullShort || d == 0 || R::skip(c, (uint32_t)v)) continue;
     typename R::T acc = R::id();
     bool any = false;
-    for (uint64_t j = 0; j < d; ++j) {
-      uint32_t u, w;
-      pull_entry(a, (uint32_t)v, j, u, w);
-      if (bit_test(curr, u)) {
-        acc = R::op(acc, R::msg(c, u, w));
-        any = true;
-        if (R::kFirst) break;
-      }
-    }
+    // CSC part first, then CSR (the row order of pull_entry)
+    if (fold_run<R>(c, a.cidx, a.w_csc, b0, e0, curr, all, acc, any))
+      fold_run<R>(c, a.idx, a.w_csr, b1, e1, curr, all, acc, any);
     if (R::fin(c, (uint32_t)v, acc, any)) mark_next(next, (uint32_t)v);
   }
 }
@@ -371,7 +397,7 @@ __device__ inline void wave_fold(typename R::T& acc, uint64_t& first) {
 template <int P, int W>
 __global__ __launch_bounds__(W == 1 ? kB : 64 * W) void k_pull_rows(EngCtx c, PullAdj a, const uint32_t* __restrict__ rows,
                                                       uint32_t nrows, const uint32_t* __restrict__ curr,
-                                                      uint32_t* __restrict__ next) {
+                                                      bool all, uint32_t* __restrict__ next) {
   using R = Pull<P>;
   using T = typename R::T;
   __shared__ T sacc[W];
@@ -393,7 +419,7 @@ __global__ __launch_bounds__(W == 1 ? kB : 64 * W) void k_pull_rows(EngCtx c, Pu
       if (j < d && (!R::kFirst || first == ~0ULL)) {
         uint32_t u, w;
         pull_entry(a, v, j, u, w);
-        if (bit_test(curr, u)) {
+        if (all || bit_test(curr, u)) {
           if (R::kFirst) { if (first == ~0ULL) { first = j; acc = R::msg(c, u, w); } }
           else { acc = R::op(acc, R::msg(c, u, w)); first = j < first ? j : first; }
         }
@@ -437,10 +463,38 @@ __global__ __launch_bounds__(W == 1 ? kB : 64 * W) void k_pull_rows(EngCtx c, Pu
 // rows of pull degree in (kPullShort, kPullLong] and above kPullLong
 __global__ void k_pull_classify(PullAdj a, uint32_t nV, uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
                                 uint32_t* __restrict__ lng, uint32_t* __restrict__ nlng) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t d = pull_deg(a, (uint32_t)v);
-    if (d > kPullLong) lng[atomicAdd(nlng, 1u)] = (uint32_t)v;
-    else if (d > kPullShort) mid[atomicAdd(nmid, 1u)] = (uint32_t)v;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t nvpad = (nV + 63) & ~63ULL;   // whole waves stay in the loop for the ballots
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nvpad; v += stride) {
+    const uint64_t d = v < nV ? pull_deg(a, (uint32_t)v) : 0;
+    wave_append(d > kPullLong, (uint32_t)v, lng, nlng);
+    wave_append(d > kPullShort && d <= kPullLong, (uint32_t)v, mid, nmid);
+  }
+}
+
+// Complete frontier (every vertex with edges in the push direction is active)
+// and an FP64 sum combine: the pull is the tile-segmented reduction over the
+// pull adjacency (one array: CSC for Out, CSR for In), summed in a fixed order;
+// every vertex with a non-empty pull row receives a message.
+struct EngSumMode {
+  using T = double;
+  const double* S;
+  double* C;
+  __device__ double gather(uint64_t, uint32_t u) const { return S[u]; }
+  __device__ void finalize(uint32_t r, double s, double&, double&) const { C[r] += s; }
+};
+
+// nextSet = vertices with a non-empty row in off (whole words, as k_init_all)
+__global__ void k_mark_rows(const uint64_t* __restrict__ off, uint32_t nV, uint32_t nvpad,
+                            uint32_t* __restrict__ next) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nvpad;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const bool on = v < nV && off[v + 1] > off[v];
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+      next[v >> 5] = (uint32_t)m;
+      next[(v >> 5) + 1] = (uint32_t)(m >> 32);
+    }
   }
 }
 
@@ -689,10 +743,11 @@ struct gre_engine {
   uint64_t qedges = 0;
   double times[5] = {0, 0, 0, 0, 0};
   int mode = 0;                // GRE_ENGINE_AUTO / _PUSH / _PULL
-  int pull_dir = -1;           // direction the row classes below were built for
-  gre::DevBuf pull_mid, pull_long, pull_cnt;
-  uint32_t n_mid = 0, n_long = 0;
   int64_t pull_steps = 0;      // supersteps of the last run done in the pull form
+  bool all_began = false;      // the run began with every vertex active (no sources)
+  int sum_dir = -1;            // direction of the tile plan below
+  gre::PullPlan sum_plan;      // tile-segmented pull of complete-frontier FP64 sums
+  gre::PullBuffers sum_buf;
 };
 
 namespace gre {
@@ -736,25 +791,25 @@ PullAdj make_pull_adj(gre_engine* e) {
   return a;
 }
 
-// rows too long for one thread, listed once per direction
-void ensure_pull_rows(gre_engine* e, const PullAdj& a, hipStream_t s) {
-  if (e->pull_dir == e->dir) return;
+// rows too long for one thread, listed once per graph and direction
+const PullRows& ensure_pull_rows(gre_engine* e, const PullAdj& a, hipStream_t s) {
+  PullRows& r = e->g->pull_rows[e->dir];
+  if (r.mid.p) return r;
   const uint32_t nV = e->g->nV;
-  if (!e->pull_mid.p) {
-    e->pull_mid.alloc(4 * (size_t)nV);
-    e->pull_long.alloc(4 * (size_t)nV);
-    e->pull_cnt.alloc(8);
-  }
-  GRE_HIP(hipMemsetAsync(e->pull_cnt.p, 0, 8, s));
-  hipLaunchKernelGGL(k_pull_classify, dim3(gcap(nV)), dim3(kB), 0, s, a, nV, e->pull_mid.as<uint32_t>(),
-                     e->pull_cnt.as<uint32_t>(), e->pull_long.as<uint32_t>(), e->pull_cnt.as<uint32_t>() + 1);
+  DevBuf cnt;
+  cnt.alloc(8);
+  r.mid.alloc(4 * (size_t)nV);
+  r.lng.alloc(4 * (size_t)nV);
+  GRE_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
+  hipLaunchKernelGGL(k_pull_classify, dim3(gcap(nV)), dim3(kB), 0, s, a, nV, r.mid.as<uint32_t>(),
+                     cnt.as<uint32_t>(), r.lng.as<uint32_t>(), cnt.as<uint32_t>() + 1);
   GRE_HIP(hipGetLastError());
   uint32_t h[2];
-  GRE_HIP(hipMemcpyAsync(h, e->pull_cnt.p, 8, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipMemcpyAsync(h, cnt.p, 8, hipMemcpyDeviceToHost, s));
   GRE_HIP(hipStreamSynchronize(s));
-  e->n_mid = h[0];
-  e->n_long = h[1];
-  e->pull_dir = e->dir;
+  r.n_mid = h[0];
+  r.n_long = h[1];
+  return r;
 }
 
 template <typename F>
@@ -861,6 +916,7 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
   rebuild_queue(e, a, s);
   e->superstep = 0;
   e->pull_steps = 0;
+  e->all_began = srcs == nullptr;
   e->begun = true;
   for (double& t : e->times) t = 0.0;
   e->times[0] = now() - t0;
@@ -874,27 +930,50 @@ int64_t engine_step(gre_engine* e) {
   const double t0 = now();
   // nextSet.clear() (Engine.hpp:322)
   hipLaunchKernelGGL(k_zero_u32, dim3(gcap(e->nwords)), dim3(kB), 0, s, e->next.as<uint32_t>(), (uint64_t)e->nwords);
-  // push while the frontier's edges are few, pull once they are a large share
-  // of every destination's rows (the pull reads all rows, without atomics)
+  // push while the frontier's edges are few, pull once they are an eighth of
+  // the pull rows -- for sum combines (the push pays an atomic per edge, the
+  // pull none) and first-writer ones (the pull stops at a row's first active
+  // neighbour and skips reached vertices).  Min combines stay in the push
+  // form: it skips the atomic wherever the label cannot fall, while the pull
+  // reads every neighbour's label; on R-MAT scale 26 the pull step is slower
+  // at every frontier size, the complete one included.
   const uint64_t epull = (e->dir == GRE_DIR_INOUT ? 2 : 1) * e->g->nE;
+  const bool min_combine = e->prog == GRE_PROG_CC || e->prog == GRE_PROG_SSSP;
   const bool pull = e->qedges > 0 && (e->mode == GRE_ENGINE_PULL ||
-                                      (e->mode == GRE_ENGINE_AUTO && e->qedges * 8 > epull));
-  if (pull) {
+                                      (e->mode == GRE_ENGINE_AUTO && !min_combine && e->qedges * 8 > epull));
+  const bool fsum = e->prog == GRE_PROG_PAGERANK || e->prog == GRE_PROG_PAGERANK_DELTA ||
+                    e->prog == GRE_PROG_BC_BACKWARD;
+  if (pull && fsum && e->dir != GRE_DIR_INOUT && e->qedges == epull) {
+    if (e->sum_dir != e->dir) {
+      e->sum_buf.make(e->sum_plan, e->g, e->dir == GRE_DIR_OUT, 8, sizeof(double));
+      e->sum_plan.tag = "pull_compute";
+      e->sum_plan.fix_tag = "pull_fixup";
+      e->sum_dir = e->dir;
+    }
+    const uint32_t nvpad = e->nwords * 32;
+    hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, e->sum_plan.off, e->g->nV, nvpad,
+                       e->next.as<uint32_t>());
+    pull_pass(e->sum_plan, EngSumMode{(const double*)c.S, (double*)c.C}, nullptr, s);
+    ++e->pull_steps;
+  } else if (pull) {
     const PullAdj pa = make_pull_adj(e);
-    ensure_pull_rows(e, pa, s);
+    const PullRows& rows = ensure_pull_rows(e, pa, s);
     ProfScope ps("pull_compute");
     const uint32_t nV = e->g->nV;
     dispatch(e->prog, [&](auto P) {
       constexpr int PV = decltype(P)::value;
-      hipLaunchKernelGGL(k_pull_short<PV>, dim3(gcap(nV)), dim3(kB), 0, s, c, pa, nV, e->curr.as<uint32_t>(),
+      // every neighbour's message is folded (no frontier test) when every
+      // neighbour is active, or when the stale ones cannot change the result
+      const bool all = e->qedges == epull || (Pull<PV>::kStale && e->all_began && e->qedges * 2 > epull);
+      hipLaunchKernelGGL(k_pull_short<PV>, dim3(gcap(nV)), dim3(kB), 0, s, c, pa, nV, e->curr.as<uint32_t>(), all,
                          e->next.as<uint32_t>());
-      if (e->n_mid)
-        hipLaunchKernelGGL((k_pull_rows<PV, 1>), dim3(std::min<uint32_t>((e->n_mid + 3) / 4, 1u << 20)), dim3(kB),
-                           0, s, c, pa, e->pull_mid.as<uint32_t>(), e->n_mid, e->curr.as<uint32_t>(),
+      if (rows.n_mid)
+        hipLaunchKernelGGL((k_pull_rows<PV, 1>), dim3(std::min<uint32_t>((rows.n_mid + 3) / 4, 1u << 20)),
+                           dim3(kB), 0, s, c, pa, rows.mid.as<uint32_t>(), rows.n_mid, e->curr.as<uint32_t>(), all,
                            e->next.as<uint32_t>());
-      if (e->n_long)
-        hipLaunchKernelGGL((k_pull_rows<PV, 16>), dim3(std::min<uint32_t>(e->n_long, 1u << 20)), dim3(1024), 0, s, c,
-                           pa, e->pull_long.as<uint32_t>(), e->n_long, e->curr.as<uint32_t>(),
+      if (rows.n_long)
+        hipLaunchKernelGGL((k_pull_rows<PV, 16>), dim3(std::min<uint32_t>(rows.n_long, 1u << 20)), dim3(1024), 0, s,
+                           c, pa, rows.lng.as<uint32_t>(), rows.n_long, e->curr.as<uint32_t>(), all,
                            e->next.as<uint32_t>());
     });
     GRE_HIP(hipGetLastError());

@@ -37,6 +37,12 @@ struct XPart {
   };
   std::map<uint32_t, PairTiles> pair_tiles;  // by tile entries
 };
+// Vertices whose engine pull rows (in-edges, out-edges or both) are too long
+// for one thread: mid = one wave per row, lng = one block per row.
+struct PullRows {
+  DevBuf mid, lng;
+  uint32_t n_mid = 0, n_long = 0;
+};
 }  // namespace gre
 
 struct gre_graph {
@@ -58,6 +64,8 @@ struct gre_graph {
   uint32_t nsrc = 0;
   // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with handles)
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
+  // engine pull row classes by direction (built on demand)
+  std::map<int, gre::PullRows> pull_rows;
 };
 
 namespace gre {

@@ -306,18 +306,6 @@ __device__ inline void ms_record(int32_t* __restrict__ lev, uint32_t nV, uint32_
   }
 }
 
-// Appends v to the queue once per wave (ballot + one atomic).
-__device__ inline void wave_append(bool take, uint32_t v, uint32_t* __restrict__ q, uint32_t* __restrict__ qn) {
-  const unsigned long long bal = __ballot(take);
-  if (!bal) return;
-  const int lane = (int)(threadIdx.x & 63);
-  const int leader = __ffsll(bal) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(qn, (uint32_t)__popcll(bal));
-  base = __shfl(base, leader, 64);
-  if (take) q[base + (uint32_t)__popcll(bal & ((1ULL << lane) - 1ULL))] = v;
-}
-
 // top-down: the frontier's out-edges, edge-balanced; a vertex that gains new
 // source bits is queued by the first edge that reaches it (nxt was zero)
 __global__ __launch_bounds__(kB) void k_ms_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,

@@ -134,8 +134,9 @@ int gre_engine_phase_times(const gre_engine* e, double* t5);
 /* Compute-phase form: AUTO (default) pushes along the frontier's edges while
  * they are under 1/8 of the direction's edges and otherwise pulls (each
  * destination folds its active neighbours' messages with the program's
- * combine, no atomics); PUSH and PULL force one form.  GRE has only the push
- * form (Engine.hpp:592-625); both give GRE's results. */
+ * combine, no atomics); CC and SSSP (min combines) always push under AUTO.
+ * PUSH and PULL force one form.  GRE has only the push form
+ * (Engine.hpp:592-625); both give GRE's results. */
 enum { GRE_ENGINE_AUTO = 0, GRE_ENGINE_PUSH = 1, GRE_ENGINE_PULL = 2 };
 int gre_engine_set_mode(gre_engine* e, int mode);
 /* Supersteps of the last run that used the pull form. */
