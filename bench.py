@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--bfs-sources", type=int, default=64)
+    ap.add_argument("--bfs-sources", type=int, default=16)
     ap.add_argument("--ppr-k", type=int, default=32)
     ap.add_argument("--no-extra", action="store_true", help="headline PageRank only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -254,6 +254,15 @@ def main():
     ms_copy = N.copy_probe(copy_bytes, 10)
     roofline["copy_probe_gbs"] = round(2 * copy_bytes / (ms_copy / 1e3) / 1e9, 1)
     log(f"copy probe: {roofline['copy_probe_gbs']} GB/s")
+    # achievable rate of scattered 4-byte reads, by where the table fits
+    ng = 1 << 28
+    roofline["gather_probe_g_per_s"] = {
+        name: round(ng / (N.gather_probe(n_tab, ng, 5) / 1e3) / 1e9, 1)
+        for name, n_tab in (("l1_16KB", 1 << 12), ("l2_4MB", 1 << 20), ("infinity_cache_64MB", 1 << 24),
+                            ("beyond_256MB", 1 << 28))}
+    roofline["iteration_gathers_g_per_s"] = round(E / iter_s / 1e9, 1)
+    log("gather probe (G gathers/s):", roofline["gather_probe_g_per_s"],
+        "iteration:", roofline["iteration_gathers_g_per_s"])
     if not a.no_sweep:
         # every schedule the autotuner chooses from, timed on the same graph
         sweep = {}
@@ -373,17 +382,19 @@ def main():
         log(f"symmetrised R-MAT scale {S}: V={gs.nV} E={gs.nE} built in {time.time() - t0:.1f}s")
         lev, _, _ = N.bfs(gs, int(np.argmax(gs.outdeg())))
         reached = np.nonzero(lev >= 0)[0]
-        srcs = sample_distinct(rng, reached, a.bfs_sources).astype(np.uint32)
+        # the plain BFS runs from the first bfs_sources of 64 sources; the
+        # 64-bit-mask BFS runs from all 64 in one launch
+        srcs = sample_distinct(rng, reached, max(64, a.bfs_sources)).astype(np.uint32)
         N.bfs(gs, int(srcs[0]), want_levels=False)
         rates = []
-        for s in srcs:
+        for s in srcs[:a.bfs_sources]:
             N.sync()
             t1 = time.perf_counter()
             _, er, _ = N.bfs(gs, int(s), want_levels=False)
             N.sync()
             rates.append((er / 2) / (time.perf_counter() - t1) / 1e9)
         hm = len(rates) / sum(1.0 / r for r in rates)
-        extra["bfs"] = {"sources": len(srcs), "graph": f"R-MAT scale {S} symmetrised, seed 6",
+        extra["bfs"] = {"sources": len(rates), "graph": f"R-MAT scale {S} symmetrised, seed 6",
                         "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
                         "gteps_max": round(max(rates), 3)}
         N.msbfs(gs, srcs[:4], want_levels=False)   # first call sizes the working arrays

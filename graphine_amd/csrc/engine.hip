@@ -23,6 +23,7 @@
 #include "pull.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <chrono>
 #include <type_traits>
 
@@ -484,6 +485,34 @@ struct EngSumMode {
   __device__ void finalize(uint32_t r, double s, double&, double&) const { C[r] += s; }
 };
 
+// The same over the XCD source partition of the CSC (unweighted PageRank's
+// partitioned pull): the scatter data is permuted to hub-sorted compact source
+// order, each (row, source range) segment is summed, then each row adds its
+// segments' sums, all in a fixed order.
+struct EngSegMode {
+  using T = double;
+  const double* gin;   // scatter data in compact source order
+  double* sum;         // per segment
+  __device__ double gather(uint64_t, uint32_t u) const { return gin[u]; }
+  __device__ void finalize(uint32_t seg, double s, double&, double&) const { sum[seg] = s; }
+};
+struct EngRowMode {
+  using T = double;
+  const double* seg;   // segment sums
+  double* C;
+  __device__ double gather(uint64_t, uint32_t sg) const { return seg[sg]; }
+  __device__ void finalize(uint32_t r, double s, double&, double&) const { C[r] += s; }
+};
+
+// cid[v] is v's compact id if v has out-edges, else 0xffffffff (no entry)
+__global__ void k_to_compact(const double* __restrict__ S, const uint32_t* __restrict__ cid, uint32_t nV,
+                             double* __restrict__ Sc) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = cid[v];
+    if (k < nV) Sc[k] = S[v];
+  }
+}
+
 // nextSet = vertices with a non-empty row in off (whole words, as k_init_all)
 __global__ void k_mark_rows(const uint64_t* __restrict__ off, uint32_t nV, uint32_t nvpad,
                             uint32_t* __restrict__ next) {
@@ -748,6 +777,11 @@ struct gre_engine {
   int sum_dir = -1;            // direction of the tile plan below
   gre::PullPlan sum_plan;      // tile-segmented pull of complete-frontier FP64 sums
   gre::PullBuffers sum_buf;
+  // the same over the XCD source partition (Out, large graphs)
+  std::shared_ptr<gre::XPart> xp;
+  gre::PullPlan seg_plan, row_plan;
+  gre::PullBuffers seg_buf, row_buf;
+  gre::DevBuf Sc, xsum;
 };
 
 namespace gre {
@@ -922,6 +956,14 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
   e->times[0] = now() - t0;
 }
 
+// Source partition of the complete-frontier pull: 64 tail ranges and the 2^18
+// hottest sources shared (2 MB of FP64 scatter data, resident in every L2);
+// used from 2^22 edges (below, the one-time partition build outweighs it) up to
+// the partition's 32-bit entry limit.
+constexpr int kEngRanges = 64;
+constexpr uint32_t kEngShared = 1u << 18;
+bool engine_part_ok(const gre_graph* g) { return g->nE >= (1ull << 22) && g->nE < 0xffffffffULL; }
+
 int64_t engine_step(gre_engine* e) {
   GRE_CHECK(e->begun, GRE_DEV_NULL, "call gre_engine_begin first");
   hipStream_t s = stream();
@@ -943,7 +985,30 @@ int64_t engine_step(gre_engine* e) {
                                       (e->mode == GRE_ENGINE_AUTO && !min_combine && e->qedges * 8 > epull));
   const bool fsum = e->prog == GRE_PROG_PAGERANK || e->prog == GRE_PROG_PAGERANK_DELTA ||
                     e->prog == GRE_PROG_BC_BACKWARD;
-  if (pull && fsum && e->dir != GRE_DIR_INOUT && e->qedges == epull) {
+  if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
+    gre_graph* g = e->g;
+    if (!e->xp) {
+      graph_compact_sources(g);
+      e->xp = graph_xpart(g, kEngRanges, kEngShared);
+      e->seg_buf.make(e->seg_plan, *e->xp, 4, sizeof(double));
+      e->seg_plan.nt = true;
+      e->seg_plan.tag = "pull_compute";
+      e->seg_plan.fix_tag = "pull_fixup";
+      e->row_buf.make_pairs(e->row_plan, *e->xp, g->nV, 8, sizeof(double));
+      e->row_plan.tag = "pull_combine";
+      e->row_plan.fix_tag = "pull_combine_fixup";
+      e->Sc.alloc(sizeof(double) * (size_t)g->nV);
+      e->xsum.alloc(sizeof(double) * (size_t)e->xp->nseg);
+    }
+    const uint32_t nvpad = e->nwords * 32;
+    hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->nV, nvpad,
+                       e->next.as<uint32_t>());
+    hipLaunchKernelGGL(k_to_compact, dim3(gcap(g->nV)), dim3(kB), 0, s, (const double*)c.S, g->cid.as<uint32_t>(),
+                       g->nV, e->Sc.as<double>());
+    pull_pass(e->seg_plan, EngSegMode{e->Sc.as<double>(), e->xsum.as<double>()}, nullptr, s);
+    pull_pass(e->row_plan, EngRowMode{e->xsum.as<double>(), (double*)c.C}, nullptr, s);
+    ++e->pull_steps;
+  } else if (pull && fsum && e->dir != GRE_DIR_INOUT && e->qedges == epull) {
     if (e->sum_dir != e->dir) {
       e->sum_buf.make(e->sum_plan, e->g, e->dir == GRE_DIR_OUT, 8, sizeof(double));
       e->sum_plan.tag = "pull_compute";

@@ -116,7 +116,72 @@ __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ in
     out[i] = in[i];
 }
 
+// Gather probe: ids drawn uniformly from [0, n_tab) by a hash, then one
+// 4-byte gather each from a table of n_tab floats (ids streamed, 8 per thread)
+__global__ __launch_bounds__(256) void k_gather_ids(uint32_t* __restrict__ ids, uint64_t n, uint64_t n_tab) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = (i + 1) * GRE_K0;
+    z = (z ^ (z >> 30)) * GRE_K1;
+    z = (z ^ (z >> 27)) * GRE_K2;
+    ids[i] = (uint32_t)((z ^ (z >> 31)) % n_tab);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict__ ids, const float* __restrict__ tab,
+                                                      uint64_t n, float* __restrict__ out) {
+  const uint64_t base = (uint64_t)blockIdx.x * 256 * 8;
+  uint32_t u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint64_t i = base + j * 256 + threadIdx.x;
+    u[j] = i < n ? __builtin_nontemporal_load(ids + i) : 0u;
+  }
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint64_t i = base + j * 256 + threadIdx.x;
+    if (i < n) acc += tab[u[j]];
+  }
+  out[(uint64_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
 }  // namespace gre
+
+extern "C" int gre_gather_probe(uint64_t table_entries, uint64_t gathers, int reps, double* ms_per_pass) {
+  using namespace gre;
+  return guard([&] {
+    GRE_CHECK(ms_per_pass && reps > 0 && table_entries > 0 && table_entries <= 0xffffffffULL && gathers >= 2048,
+              GRE_DEV_NULL, "bad argument");
+    hipStream_t s = stream();
+    const uint64_t nblk = (gathers + 2047) / 2048;
+    GRE_CHECK(nblk <= 0x7fffffffULL, GRE_DEV_NULL, "too many gathers");
+    DevBuf ids, tab, out;
+    ids.alloc(4 * gathers);
+    tab.alloc(4 * table_entries);
+    out.alloc(4 * 256 * nblk);
+    GRE_HIP(hipMemsetAsync(tab.p, 0, 4 * table_entries, s));
+    hipLaunchKernelGGL(k_gather_ids, dim3(grid_for(gathers, 256) < 65536u ? grid_for(gathers, 256) : 65536u),
+                       dim3(256), 0, s, ids.as<uint32_t>(), gathers, table_entries);
+    hipLaunchKernelGGL(k_gather_probe, dim3((unsigned)nblk), dim3(256), 0, s, ids.as<uint32_t>(), tab.as<float>(),
+                       gathers, out.as<float>());
+    hipEvent_t e0, e1;
+    GRE_HIP(hipEventCreate(&e0));
+    GRE_HIP(hipEventCreate(&e1));
+    GRE_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+      ProfScope ps("gather_probe");
+      hipLaunchKernelGGL(k_gather_probe, dim3((unsigned)nblk), dim3(256), 0, s, ids.as<uint32_t>(), tab.as<float>(),
+                         gathers, out.as<float>());
+    }
+    GRE_HIP(hipEventRecord(e1, s));
+    GRE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GRE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_pass = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
 
 extern "C" int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy) {
   using namespace gre;

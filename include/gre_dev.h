@@ -71,6 +71,11 @@ int gre_mem_in_use(uint64_t* bytes);
  * `bytes` bytes, reps times, on the library stream; the kernel that calibrates
  * HBM byte counters and gives the achievable copy bandwidth. */
 int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy);
+/* Measurement aid (no GRE counterpart): times `gathers` uniform random 4-byte
+ * gathers from a table of table_entries floats (ids streamed from HBM), reps
+ * times: the ceiling of scattered reads served from L1, L2, the Infinity
+ * Cache or HBM, depending on the table size. */
+int gre_gather_probe(uint64_t table_entries, uint64_t gathers, int reps, double* ms_per_pass);
 
 /* ---- graph: device-resident CSR + CSC ------------------------------------
  * Replaces DistributedGraph<GraphInCSR>::ingress + GraphInCSR::build/rbuild

@@ -122,3 +122,15 @@ def test_modes_hub_rows(N, O):
         assert ss == oss and np.array_equal(e.download(N.ARR_VERTEX), ost[1])
         e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=5)
         assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
+
+
+def test_modes_pagerank_partitioned(N, O):
+    # from 2^22 edges the complete-frontier pull runs over the XCD source
+    # partition (segment sums, then each row's segments)
+    g, og = N.Graph.rmat(19, 16, 1, False), O.rmat_graph(19, 16, 1, False)
+    assert len(og.idx) >= 1 << 22
+    for mode in (2, 0):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=10)
+        assert ss == oss == 10 and np.array_equal(counts, ocounts)
+        assert e.pull_steps() == 10
+        assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
