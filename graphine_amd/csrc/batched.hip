@@ -2,14 +2,21 @@
 // adjoint and the batched edge gradient.
 //
 // Vectors are stored vertex-major (nV x k), so one neighbour gather fetches
-// k contiguous values.  A group of G = pow2ceil(k) lanes walks one tile of
-// kTileG adjacency entries: lanes load G neighbour ids / weights at a time
-// (coalesced), issue G independent k-wide gathers, then fold them into a
-// per-lane running sum, flushing at row boundaries.  Rows crossing a tile
-// boundary leave partial vectors that a fix-up pass adds in tile order, so
-// results are deterministic.  Per-vector FP64 reductions (dangling mass,
-// p.lambda, L1 residual) are fixed-order block partials.
+// k contiguous values.  A pass is three kernels:
+//   walk   a group of G = pow2ceil(k) lanes walks one tile of kTileG adjacency
+//          entries: lanes load G neighbour ids at a time (coalesced), issue G
+//          independent k-wide gathers, then fold them into a per-lane running
+//          sum and store each row's raw sum when the row ends.  Row ends come
+//          from a window of G offsets held one per lane (one coalesced load
+//          per G rows), so nothing in the fold waits on memory;
+//   fix-up rows crossing a tile boundary add their partial vectors in tile
+//          order, so results are deterministic;
+//   apply  one coalesced pass over the nV x k sums applies the update and
+//          forms the per-vector FP64 reductions (dangling mass, p.lambda, L1
+//          residual) as fixed-order block partials.
 #include "pr.hpp"
+
+#include <algorithm>
 
 namespace gre {
 namespace {
@@ -31,10 +38,11 @@ int pow2ceil(int k) {
 }
 
 template <typename TT>
-struct BFwd {  // forward: gather x (CSC), finalise ranks
+struct BFwd {  // forward: gather x (CSC), finalise ranks (raw sums go to xout)
   const TT* xin; TT* xout; const TT* w; const TT* p; const int32_t* outdeg;
   const double* scal;  // [2k]: dang_j at scal[2j]
   double damp; int k;
+  __device__ TT* sums() const { return xout; }
   __device__ TT gather(uint64_t e, uint32_t u, int j) const { return (TT)w[e] * xin[(uint64_t)u * k + j]; }
   __device__ void finalize(uint32_t r, int j, TT s, double& a0, double& a1) const {
     const uint64_t o = (uint64_t)r * k + j;
@@ -48,10 +56,11 @@ struct BFwd {  // forward: gather x (CSC), finalise ranks
 };
 
 template <typename TT>
-struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda
+struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda (raw sums go to wl)
   const TT* lin; TT* lout; const TT* w; const TT* c; const TT* p; const int32_t* outdeg; TT* wl;
   const double* scal;  // [2k]: p.lambda_j at scal[2j]
   double damp; int k;
+  __device__ TT* sums() const { return wl; }
   __device__ TT gather(uint64_t e, uint32_t v, int j) const { return (TT)w[e] * lin[(uint64_t)v * k + j]; }
   __device__ void finalize(uint32_t u, int j, TT s, double& a0, double& a1) const {
     const uint64_t o = (uint64_t)u * k + j;
@@ -69,80 +78,75 @@ template <int G, class M, typename TT>
 __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                                 const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
                                                 uint64_t ntiles, TT* __restrict__ carryP, TT* __restrict__ carryO,
-                                                double* __restrict__ part, uint64_t nblk, M m) {
+                                                M m) {
   constexpr int NG = kBlk / G;
-  __shared__ double red0[NG][G], red1[NG][G];
   const int gl = threadIdx.x % G, gi = threadIdx.x / G;
   const int k = m.k;
   const bool act = gl < k;
   const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
-  double a0 = 0.0, a1 = 0.0;
-  if (t < ntiles) {
-    const uint64_t e0 = t * (uint64_t)kTileG;
-    const uint64_t e1 = e0 + kTileG < E ? e0 + kTileG : E;
-    const uint32_t rb = tile_rb[t];
-    const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
-    uint32_t r;
-    if (rb > 0 && e1 > e0 && off[rb] > e0) r = rb - 1; else r = rb;
-    uint64_t rbeg = r < V ? off[r] : E;
-    uint64_t rend = r < V ? off[r + 1] : E;
-    TT acc = (TT)0;
-    for (uint64_t cb = e0; cb < e1; cb += G) {
-      const uint64_t my = cb + gl;
-      const uint32_t myu = my < e1 ? idx[my] : 0u;
-      TT g[G];
-#pragma unroll
-      for (int q = 0; q < G; ++q) {
-        const uint32_t u = __shfl(myu, q, G);
-        g[q] = (act && cb + q < e1) ? m.gather(cb + q, u, gl) : (TT)0;
-      }
-#pragma unroll
-      for (int q = 0; q < G; ++q) {
-        const uint64_t e = cb + q;
-        if (e >= e1) break;
-        while (e >= rend) {  // row r ends before e: flush it
-          if (rbeg < e0) { if (act) carryP[t * k + gl] = acc; }
-          else if (act) m.finalize(r, gl, acc, a0, a1);
-          acc = (TT)0;
-          ++r;
-          rbeg = off[r];
-          rend = off[r + 1];
-        }
-        acc += g[q];
-      }
+  if (t >= ntiles || V == 0) return;
+  TT* __restrict__ sums = m.sums();
+  const uint64_t e0 = t * (uint64_t)kTileG;
+  const uint64_t e1 = e0 + kTileG < E ? e0 + kTileG : E;
+  const uint32_t rb = tile_rb[t];
+  const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
+  uint32_t r = (rb > 0 && e1 > e0 && off[rb] > e0) ? rb - 1 : rb;
+  // row ends of rows wr .. wr+G-1, one per lane (E past the last row)
+  uint32_t wr = r;
+  uint64_t endw = (uint64_t)wr + 1 + gl <= V ? off[(uint64_t)wr + 1 + gl] : E;
+  uint64_t rbeg = r < V ? off[r] : E;
+  uint64_t rend = __shfl(endw, 0, G);
+  TT acc = (TT)0;
+  // stores row r's sum (or a carry) and moves to the next row
+  auto flush_next = [&]() {
+    if (act) {
+      if (rbeg < e0) carryP[t * k + gl] = acc;
+      else if (rend > e1) carryO[t * k + gl] = acc;
+      else sums[(uint64_t)r * k + gl] = acc;
     }
-    // rows still open at the tile end, plus empty rows that start in the tile
-    while (r < re) {
-      if (rbeg < e0) { if (act) carryP[t * k + gl] = acc; }
-      else if (rend > e1) { if (act) carryO[t * k + gl] = acc; }
-      else if (act) m.finalize(r, gl, acc, a0, a1);
-      acc = (TT)0;
-      ++r;
-      if (r < V) { rbeg = off[r]; rend = off[r + 1]; }
+    acc = (TT)0;
+    rbeg = rend;
+    ++r;
+    if (r - wr >= (uint32_t)G) {
+      wr = r;
+      endw = (uint64_t)wr + 1 + gl <= V ? off[(uint64_t)wr + 1 + gl] : E;
+    }
+    rend = __shfl(endw, (int)(r - wr), G);
+  };
+  for (uint64_t cb = e0; cb < e1; cb += G) {
+    // the group's next G entries: ids one per lane (coalesced), then G
+    // weighted gathers of k contiguous values (a weight is one broadcast
+    // load, served by L1 for the G entries of a chunk)
+    const uint64_t my = cb + gl;
+    const uint32_t myu = my < e1 ? idx[my] : 0u;
+    TT g[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const uint32_t u = __shfl(myu, q, G);
+      g[q] = (act && cb + q < e1) ? m.gather(cb + q, u, gl) : (TT)0;
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const uint64_t e = cb + q;
+      if (e >= e1) break;
+      while (e >= rend) flush_next();  // rows ending before e, empty rows included
+      acc += g[q];
     }
   }
-  red0[gi][gl] = a0;
-  red1[gi][gl] = a1;
-  __syncthreads();
-  if (threadIdx.x < (unsigned)k) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int q = 0; q < NG; ++q) { s0 += red0[q][threadIdx.x]; s1 += red1[q][threadIdx.x]; }
-    part[(uint64_t)(2 * threadIdx.x) * nblk + blockIdx.x] = s0;
-    part[(uint64_t)(2 * threadIdx.x + 1) * nblk + blockIdx.x] = s1;
-  }
+  // rows still open at the tile end, plus empty rows that start in the tile
+  while (r < re) flush_next();
 }
 
+// rows that start in tile t and run past its end: carryO[t] + carryP of the
+// tiles they continue into, in tile order
 template <int G, class M, typename TT>
 __global__ __launch_bounds__(kBlk) void k_bfix(const uint64_t* __restrict__ off, const uint32_t* __restrict__ tile_rb,
                                                uint64_t E, uint32_t V, uint64_t ntiles,
-                                               const TT* __restrict__ carryP, const TT* __restrict__ carryO,
-                                               double* __restrict__ part, uint64_t nblk, M m) {
+                                               const TT* __restrict__ carryP, const TT* __restrict__ carryO, M m) {
   constexpr int NG = kBlk / G;
-  __shared__ double red0[NG][G], red1[NG][G];
   const int gl = threadIdx.x % G, gi = threadIdx.x / G;
   const int k = m.k;
   const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
-  double a0 = 0.0, a1 = 0.0;
   if (t + 1 < ntiles && gl < k) {
     const uint32_t rb = tile_rb[t], re = tile_rb[t + 1];
     const uint64_t e1 = (t + 1) * (uint64_t)kTileG;
@@ -151,8 +155,24 @@ __global__ __launch_bounds__(kBlk) void k_bfix(const uint64_t* __restrict__ off,
       const uint64_t end = off[r + 1];
       TT s = carryO[t * k + gl];
       for (uint64_t q = t + 1; q * kTileG < end; ++q) s += carryP[q * k + gl];
-      m.finalize(r, gl, s, a0, a1);
+      m.sums()[(uint64_t)r * k + gl] = s;
     }
+  }
+}
+
+// apply: every row's update from its raw sum; per-vector block partials of
+// (a0, a1) in part[2j][nblk], part[2j+1][nblk]
+template <int G, class M, typename TT>
+__global__ __launch_bounds__(kBlk) void k_bapply(uint32_t V, double* __restrict__ part, uint64_t nblk, M m) {
+  constexpr int NG = kBlk / G;
+  __shared__ double red0[NG][G], red1[NG][G];
+  const int gl = threadIdx.x % G, gi = threadIdx.x / G;
+  const int k = m.k;
+  double a0 = 0.0, a1 = 0.0;
+  if (gl < k) {
+    const TT* __restrict__ sums = m.sums();
+    for (uint64_t r = (uint64_t)blockIdx.x * NG + gi; r < V; r += (uint64_t)gridDim.x * NG)
+      m.finalize((uint32_t)r, gl, sums[r * k + gl], a0, a1);
   }
   red0[gi][gl] = a0;
   red1[gi][gl] = a1;
@@ -286,7 +306,7 @@ size_t tsz(const gre_pr* P) { return P->dtype == GRE_F64 ? 8 : 4; }
 struct BPlan {
   const uint64_t* off; const uint32_t* idx; const uint32_t* trb;
   uint64_t E, ntiles; uint32_t V;
-  unsigned nblk, nfix;
+  unsigned nblk, nfix, napply;   // walk and fix-up blocks (one tile per group), apply blocks
 };
 
 BPlan make_plan(gre_pr* P, bool csc) {
@@ -301,6 +321,8 @@ BPlan make_plan(gre_pr* P, bool csc) {
   const int NG = kBlk / G;
   b.nblk = (unsigned)((b.ntiles + NG - 1) / NG);
   b.nfix = b.nblk;
+  const uint64_t nrowblk = ((uint64_t)b.V + NG - 1) / NG;
+  b.napply = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nrowblk, 16384));
   BatchState& bs = P->bstate;
   const size_t vec = tsz(P) * (size_t)P->k;
   if (bs.ntiles < b.ntiles) {
@@ -308,9 +330,8 @@ BPlan make_plan(gre_pr* P, bool csc) {
     bs.carryO.alloc(vec * b.ntiles);
     bs.ntiles = b.ntiles;
   }
-  const size_t need = sizeof(double) * 2 * (size_t)P->k * b.nblk;
+  const size_t need = sizeof(double) * 2 * (size_t)P->k * b.napply;
   if (bs.part.bytes < need) bs.part.alloc(need);
-  if (bs.part2.bytes < need) bs.part2.alloc(need);
   return b;
 }
 
@@ -327,15 +348,20 @@ void bpass(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t 
   {
     ProfScope ps("batch_walk");
     hipLaunchKernelGGL((k_bwalk<G, M, TT>), dim3(b.nblk), dim3(kBlk), 0, s, b.off, b.idx, b.trb, b.E, b.V, b.ntiles,
-                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), bs.part.as<double>(), (uint64_t)b.nblk, m);
+                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
   }
   {
     ProfScope ps("batch_fixup");
     hipLaunchKernelGGL((k_bfix<G, M, TT>), dim3(b.nfix), dim3(kBlk), 0, s, b.off, b.trb, b.E, b.V, b.ntiles,
-                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), bs.part2.as<double>(), (uint64_t)b.nfix, m);
+                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
   }
-  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * P->k), dim3(256), 0, s, bs.part.as<double>(), (uint64_t)b.nblk,
-                     bs.part2.as<double>(), (uint64_t)b.nfix, scal_out);
+  {
+    ProfScope ps("batch_apply");
+    hipLaunchKernelGGL((k_bapply<G, M, TT>), dim3(b.napply), dim3(kBlk), 0, s, b.V, bs.part.as<double>(),
+                       (uint64_t)b.napply, m);
+  }
+  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * P->k), dim3(256), 0, s, bs.part.as<double>(), (uint64_t)b.napply,
+                     bs.part.as<double>(), (uint64_t)0, scal_out);
   GRE_HIP(hipGetLastError());
 }
 

@@ -8,7 +8,7 @@
 namespace gre {
 constexpr int kMaxBatch = 64;
 struct BatchState {
-  DevBuf carryP, carryO, part, part2, scal;
+  DevBuf carryP, carryO, part, scal;
   uint64_t ntiles = 0;
   int scal_cap = 0;
 };
