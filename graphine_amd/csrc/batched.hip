@@ -119,11 +119,15 @@ __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off
     // load, served by L1 for the G entries of a chunk)
     const uint64_t my = cb + gl;
     const uint32_t myu = my < e1 ? idx[my] : 0u;
+    // unconditional gathers (past the tile's end: entry e1 - 1, neighbour 0;
+    // idle lanes: component 0), so all G are in flight before the first use
     TT g[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const uint32_t u = __shfl(myu, q, G);
-      g[q] = (act && cb + q < e1) ? m.gather(cb + q, u, gl) : (TT)0;
+      const bool ok = cb + q < e1;
+      const TT v = m.gather(ok ? cb + q : e1 - 1, u, act ? gl : 0);
+      g[q] = (act && ok) ? v : (TT)0;
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {

@@ -41,10 +41,6 @@ struct ModeFwd {
   const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
-  __device__ TT gather(uint64_t e, uint32_t u) const {
-    if (w) return (TT)w[e] * gin[u];
-    return gin[u];
-  }
   __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const {
     const double dang = scal_in[0];
     const TT pv = p ? p[r] : (TT)invn;
@@ -61,6 +57,22 @@ struct ModeFwd {
   }
 };
 
+// The weighting is a compile-time property of the launched mode: a run-time
+// test inside gather() would put each weighted gather in its own branch, where
+// the multiply waits for the loads before the next gather is issued.
+template <typename TT, bool W>
+struct ModeFwdG : ModeFwd<TT> {
+  __device__ TT gather(uint64_t e, uint32_t u) const {
+    if (W) return (TT)this->w[e] * this->gin[u];
+    return this->gin[u];
+  }
+};
+template <typename TT>
+void fwd_pull(const PullPlan& p, const ModeFwd<TT>& m, double* scal_out, hipStream_t s) {
+  if (m.w) pull_pass(p, ModeFwdG<TT, true>{m}, scal_out, s);
+  else pull_pass(p, ModeFwdG<TT, false>{m}, scal_out, s);
+}
+
 template <typename TT>
 struct ModeAdj {
   using T = TT;
@@ -73,10 +85,6 @@ struct ModeAdj {
   TT* wl;                 // (W lambda)_u
   const double* scal_in;  // [0] = p.lambda of lin
   double damp, invn;
-  __device__ TT gather(uint64_t e, uint32_t v) const {
-    if (w) return (TT)w[e] * lin[v];
-    return lin[v];
-  }
   __device__ void finalize(uint32_t u, TT s, double& a0, double& a1) const {
     const int32_t d = outdeg[u];
     if (!w && d > 0) s = s / (TT)d;
@@ -88,6 +96,19 @@ struct ModeAdj {
     a1 += fabs((double)y - (double)old);
   }
 };
+
+template <typename TT, bool W>
+struct ModeAdjG : ModeAdj<TT> {
+  __device__ TT gather(uint64_t e, uint32_t v) const {
+    if (W) return (TT)this->w[e] * this->lin[v];
+    return this->lin[v];
+  }
+};
+template <typename TT>
+void adj_pull(const PullPlan& p, const ModeAdj<TT>& m, double* scal_out, hipStream_t s) {
+  if (m.w) pull_pass(p, ModeAdjG<TT, true>{m}, scal_out, s);
+  else pull_pass(p, ModeAdjG<TT, false>{m}, scal_out, s);
+}
 
 // partitioned pull, first pass: a segment's sum of contrib over its sources
 template <typename TT>
@@ -305,12 +326,12 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
     ModeSeg<TT> ms{m.gin, P->xsum.as<TT>()};
     pull_pass(P->plan_x, ms, nullptr, s);
     m.gin = P->xsum.as<TT>();
-    pull_pass(P->plan_px, m, scal_out, s);
+    fwd_pull(P->plan_px, m, scal_out, s);
     P->ccur ^= 1;
     return;
   }
   // in-place x for the unweighted form: residual reads xin before xout writes
-  pull_pass(P->plan_csc, m, scal_out, s);
+  fwd_pull(P->plan_csc, m, scal_out, s);
   if (P->weighted) P->xcur ^= 1;
   else P->ccur ^= 1;
 }
@@ -519,7 +540,7 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal_in = sc + 2 * it;
     m.damp = damp;
     m.invn = 1.0 / (double)nV;
-    pull_pass(P->plan_csr, m, sc + 2 * (it + 1), s);
+    adj_pull(P->plan_csr, m, sc + 2 * (it + 1), s);
     P->lcur ^= 1;
     if (tol > 0.0) {
       double h[2];

@@ -136,12 +136,13 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict
     const uint64_t i = base + j * 256 + threadIdx.x;
     u[j] = i < n ? __builtin_nontemporal_load(ids + i) : 0u;
   }
+  // unconditional gathers (u = 0 past the end), all in flight before the sum
+  float v[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = tab[u[j]];
   float acc = 0.f;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const uint64_t i = base + j * 256 + threadIdx.x;
-    if (i < n) acc += tab[u[j]];
-  }
+  for (int j = 0; j < 8; ++j) acc += base + j * 256 + threadIdx.x < n ? v[j] : 0.f;
   out[(uint64_t)blockIdx.x * 256 + threadIdx.x] = acc;
 }
 

@@ -78,10 +78,19 @@ __device__ __forceinline__ void pull_tile_body(
     if (NT) nb[j] = i < n ? __builtin_nontemporal_load(idx + e0 + i) : 0u;
     else nb[j] = i < n ? idx[e0 + i] : 0u;
   }
+  // every gather is issued before any is used: a lane past the tile's end
+  // reads entry e0 with neighbour 0 (nb = 0 there), a valid element, and
+  // stores it to a slot of vals that no row reads (i < TE always).  Guarded
+  // gathers or stores would make each load wait for the one before it.
+  if (n > 0) {
+    typename Mode::T gv[IPT];
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const int i = j * kPullBlock + tid;
-    if (i < n) vals[i] = gat(e0 + i, nb[j]);
+    for (int j = 0; j < IPT; ++j) {
+      const int i = j * kPullBlock + tid;
+      gv[j] = gat(e0 + (i < n ? (uint64_t)i : 0ull), nb[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) vals[j * kPullBlock + tid] = gv[j];
   }
   if (tid == 0) *nlong = 0;
   __syncthreads();

@@ -453,15 +453,38 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
     const unsigned long long seen = vis[v];
     if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
     double acc = 0.0;
-    unsigned long long got = 0;
+    unsigned long long gotl = 0;
     const uint64_t b = coff[v], e = coff[v + 1];
-    for (uint64_t p = b; p < e; ++p) {
-      const uint32_t w = cidx[p];
-      const unsigned long long f = fr[w] & ~seen;
-      if (!f) continue;
-      got |= f;
-      if ((f >> lane) & 1ULL) acc += sigma[(uint64_t)w * 64 + lane];
+    for (uint64_t base = b; base < e; base += 64) {
+      // 64 in-neighbours and their frontier masks at once
+      const uint64_t p = base + lane;
+      const uint32_t w = p < e ? cidx[p] : 0u;
+      const unsigned long long f = p < e ? (fr[w] & ~seen) : 0ULL;
+      gotl |= f;
+      unsigned long long act = __ballot(f != 0ULL);
+      // sigma rows of the contributing neighbours, 8 gathers in flight, added
+      // in row order
+      while (act) {
+        uint32_t wj[8];
+        unsigned long long fj[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = act ? __ffsll(act) - 1 : 0;
+          fj[t] = act ? __shfl(f, j, 64) : 0ULL;
+          wj[t] = __shfl(w, j, 64);
+          act &= act - 1ULL;
+        }
+        double sv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sv[t] = sigma[(uint64_t)wj[t] * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if ((fj[t] >> lane) & 1ULL) acc += sv[t];
+      }
     }
+    // got = OR over the lanes of gotl
+    unsigned long long got = gotl;
+    for (int o = 32; o > 0; o >>= 1) got |= __shfl_xor(got, o, 64);
     if (lane == 0) nfr[v] = got;
     if (got) {
       if ((got >> lane) & 1ULL) {
@@ -475,11 +498,12 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
   if (__any(changed) && lane == 0) atomicOr(any, 1u);
 }
 
-// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of (1+delta_w[j])/sigma_w[j]
+// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of (1+delta_w[j])/sigma_w[j];
+// atl[v] / nxt[v] = lanes of v at level L / L+1
 __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                                uint32_t nV, const double* __restrict__ sigma,
-                                               double* __restrict__ delta, const uint8_t* __restrict__ lev,
-                                               const unsigned long long* __restrict__ atl, int32_t L) {
+                                               double* __restrict__ delta, const unsigned long long* __restrict__ atl,
+                                               const unsigned long long* __restrict__ nxt) {
   const int lane = threadIdx.x & 63;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -489,10 +513,33 @@ __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off,
     const bool mine = (a >> lane) & 1ULL;
     double acc = 0.0;
     const uint64_t b = off[v], e = off[v + 1];
-    for (uint64_t p = b; p < e; ++p) {
-      const uint32_t w = idx[p];
-      const uint64_t o = (uint64_t)w * 64 + lane;
-      if (mine && lev[o] == (uint8_t)(L + 1)) acc += (1.0 + delta[o]) / sigma[o];
+    for (uint64_t base = b; base < e; base += 64) {
+      // 64 out-neighbours and the lanes they hold at L+1, restricted to v's lanes at L
+      const uint64_t p = base + lane;
+      const uint32_t w = p < e ? idx[p] : 0u;
+      const unsigned long long m = p < e ? (nxt[w] & a) : 0ULL;
+      unsigned long long act = __ballot(m != 0ULL);
+      while (act) {
+        uint32_t wj[8];
+        unsigned long long mj[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = act ? __ffsll(act) - 1 : 0;
+          mj[t] = act ? __shfl(m, j, 64) : 0ULL;
+          wj[t] = __shfl(w, j, 64);
+          act &= act - 1ULL;
+        }
+        double dv[8], sv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const uint64_t o = (uint64_t)wj[t] * 64 + lane;
+          dv[t] = delta[o];
+          sv[t] = sigma[o];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if ((mj[t] >> lane) & 1ULL) acc += (1.0 + dv[t]) / sv[t];
+      }
     }
     if (mine) delta[(uint64_t)v * 64 + lane] = sigma[(uint64_t)v * 64 + lane] * acc;
   }
@@ -703,14 +750,15 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
     const uint64_t n64 = (uint64_t)nV * 64;
-    DevBuf sigma, delta, lev, vis, fr[2], msk, bc, dsrc, flag;
+    DevBuf sigma, delta, lev, vis, fr[2], msk[2], bc, dsrc, flag;
     sigma.alloc(8 * n64);
     delta.alloc(8 * n64);
     lev.alloc(n64);
     vis.alloc(8 * (size_t)nV);
     fr[0].alloc(8 * (size_t)nV);
     fr[1].alloc(8 * (size_t)nV);
-    msk.alloc(8 * (size_t)nV);
+    msk[0].alloc(8 * (size_t)nV);
+    msk[1].alloc(8 * (size_t)nV);
     bc.alloc(8 * (size_t)nV);
     dsrc.alloc(4 * (size_t)nsrc);
     flag.alloc(8);
@@ -744,13 +792,20 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
         cur ^= 1;
         if (!any) break;
       }
-      // deepest level reached is L; dependencies from L-1 down to 0
+      // deepest level reached is L; dependencies from L-1 down to 0, with the
+      // lane masks of level l (mc) and l + 1 (mn, the previous level's mc)
+      int mc = 0;
+      if (L > 0)
+        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), L,
+                           (unsigned long long*)msk[1].p);
       for (int32_t l = L - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l, (unsigned long long*)msk.p);
+        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l,
+                           (unsigned long long*)msk[mc].p);
         ProfScope ps("bc_backward");
         hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), nV,
-                           sigma.as<double>(), delta.as<double>(), lev.as<uint8_t>(), (const unsigned long long*)msk.p,
-                           l);
+                           sigma.as<double>(), delta.as<double>(), (const unsigned long long*)msk[mc].p,
+                           (const unsigned long long*)msk[mc ^ 1].p);
+        mc ^= 1;
       }
       hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, nV, nc, cs, delta.as<double>(), bc.as<double>());
       GRE_HIP(hipGetLastError());
