@@ -113,12 +113,14 @@ __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off
     }
     rend = __shfl(endw, (int)(r - wr), G);
   };
+  // ids of the chunk after the current one are loaded before the current
+  // chunk is folded
+  uint32_t myn = e0 + gl < e1 ? idx[e0 + gl] : 0u;
   for (uint64_t cb = e0; cb < e1; cb += G) {
     // the group's next G entries: ids one per lane (coalesced), then G
     // weighted gathers of k contiguous values (a weight is one broadcast
     // load, served by L1 for the G entries of a chunk)
-    const uint64_t my = cb + gl;
-    const uint32_t myu = my < e1 ? idx[my] : 0u;
+    const uint32_t myu = myn;
     // unconditional gathers (past the tile's end: entry e1 - 1, neighbour 0;
     // idle lanes: component 0), so all G are in flight before the first use
     TT g[G];
@@ -128,6 +130,10 @@ __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off
       const bool ok = cb + q < e1;
       const TT v = m.gather(ok ? cb + q : e1 - 1, u, act ? gl : 0);
       g[q] = (act && ok) ? v : (TT)0;
+    }
+    {
+      const uint64_t nx = cb + G + gl;
+      myn = nx < e1 ? idx[nx] : 0u;
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
@@ -242,7 +248,10 @@ __global__ void k_btargets(const uint32_t* __restrict__ tg, int64_t nt, int k, T
     c[(uint64_t)tg[i / k] * k + (i % k)] = (TT)1;
 }
 
-// batched edge gradient: one group per tile of CSR entries
+// batched edge gradient: one group per tile of CSR entries.  Per chunk of G
+// entries the lambda rows of all G destinations are gathered before the first
+// is used; rows without edges are skipped through a window of row ends held
+// one per lane, and x, wl and the row sum are loaded once per row with edges.
 template <int G, typename TT>
 __global__ __launch_bounds__(kBlk) void k_bgrad(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                                 const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
@@ -253,6 +262,7 @@ __global__ __launch_bounds__(kBlk) void k_bgrad(const uint64_t* __restrict__ off
   constexpr int NG = kBlk / G;
   const int gl = threadIdx.x % G, gi = threadIdx.x / G;
   const bool act = gl < k;
+  const int jj = act ? gl : 0;
   const uint64_t t = (uint64_t)blockIdx.x * NG + gi;
   if (t >= ntiles) return;
   const uint64_t e0 = t * (uint64_t)kTileG;
@@ -260,31 +270,46 @@ __global__ __launch_bounds__(kBlk) void k_bgrad(const uint64_t* __restrict__ off
   if (e1 <= e0) return;
   const uint32_t rb = tile_rb[t];
   uint32_t r = (rb > 0 && off[rb] > e0) ? rb - 1 : rb;
-  uint64_t rend = off[r + 1];
-  while (rend <= e0) { ++r; rend = off[r + 1]; }
-  double xu = act ? (double)x[(uint64_t)r * k + gl] : 0.0;
-  double xwl = act ? xu * (double)wl[(uint64_t)r * k + gl] : 0.0;
-  for (int o = G / 2; o > 0; o >>= 1) xwl += __shfl_xor(xwl, o, G);
+  // row ends of rows wr .. wr+G-1, one per lane
+  uint32_t wr = r;
+  uint64_t endw = (uint64_t)wr + 1 + gl <= V ? off[(uint64_t)wr + 1 + gl] : E;
+  uint64_t rend = __shfl(endw, 0, G);
+  double xu = 0.0, xwl = 0.0, rs = 0.0;
+  // moves r to the row holding entry e and loads that row's x, x.wl and row sum
+  auto seek = [&](uint64_t e) {
+    while (e >= rend) {
+      ++r;
+      if (r - wr >= (uint32_t)G) {
+        wr = r;
+        endw = (uint64_t)wr + 1 + gl <= V ? off[(uint64_t)wr + 1 + gl] : E;
+      }
+      rend = __shfl(endw, (int)(r - wr), G);
+    }
+    const uint64_t o = (uint64_t)r * k + jj;
+    const double xv = (double)x[o], wv = (double)wl[o];
+    rs = rowsum[r];
+    xu = act ? xv : 0.0;
+    double p = act ? xv * wv : 0.0;
+    for (int sh = G / 2; sh > 0; sh >>= 1) p += __shfl_xor(p, sh, G);
+    xwl = p;
+  };
+  seek(e0);
   for (uint64_t cb = e0; cb < e1; cb += G) {
     const uint64_t my = cb + gl;
     const uint32_t myv = my < e1 ? idx[my] : 0u;
+    TT lv[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) lv[q] = lam[(uint64_t)__shfl(myv, q, G) * k + jj];
     float outW = 0.f, outR = 0.f;
+#pragma unroll
     for (int q = 0; q < G; ++q) {
       const uint64_t e = cb + q;
       if (e >= e1) break;
-      while (e >= rend) {
-        ++r;
-        rend = off[r + 1];
-        xu = act ? (double)x[(uint64_t)r * k + gl] : 0.0;
-        xwl = act ? xu * (double)wl[(uint64_t)r * k + gl] : 0.0;
-        for (int o = G / 2; o > 0; o >>= 1) xwl += __shfl_xor(xwl, o, G);
-      }
-      const uint32_t v = __shfl(myv, q, G);
-      double d = act ? xu * (double)lam[(uint64_t)v * k + gl] : 0.0;
-      for (int o = G / 2; o > 0; o >>= 1) d += __shfl_xor(d, o, G);
+      if (e >= rend) seek(e);
+      double d = act ? xu * (double)lv[q] : 0.0;
+      for (int sh = G / 2; sh > 0; sh >>= 1) d += __shfl_xor(d, sh, G);
       if (gl == q) {
         outW = (float)(damp * d);
-        const double rs = rowsum[r];
         outR = rs > 0.0 ? (float)(damp * (d - xwl) / rs) : 0.f;
       }
     }
