@@ -101,15 +101,24 @@ __global__ __launch_bounds__(kB) void k_bu(const uint64_t* __restrict__ coff, co
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
     if (level[v] >= 0) continue;
     const uint64_t b = coff[v], e = coff[v + 1];
-    for (uint64_t p = b; p < e; ++p) {
-      const uint32_t w = cidx[p];
-      if ((fbits[w >> 5] >> (w & 31)) & 1u) {
-        level[v] = L + 1;
-        atomicOr(nbits + (v >> 5), 1u << (v & 31));
-        ++cnt;
-        dsum += off[v + 1] - off[v];
-        break;
-      }
+    // 8 in-neighbours per step, their frontier bits read together (past the
+    // row's end: the first neighbour again, which changes nothing)
+    bool found = false;
+    for (uint64_t p = b; p < e && !found; p += 8) {
+      uint32_t w[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) w[j] = cidx[p + j < e ? p + j : b];
+      uint32_t fb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) fb[j] = fbits[w[j] >> 5];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) found = found || ((fb[j] >> (w[j] & 31)) & 1u);
+    }
+    if (found) {
+      level[v] = L + 1;
+      atomicOr(nbits + (v >> 5), 1u << (v & 31));
+      ++cnt;
+      dsum += off[v + 1] - off[v];
     }
   }
   cnt = wave_sum(cnt);
@@ -394,8 +403,18 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
       if (seen != full) {
         unsigned long long m = 0;
         const uint64_t b = coff[v], e = coff[v + 1];
-        for (uint64_t p = b; p < e; ++p) {
-          m |= fr[cidx[p]];
+        // 8 in-neighbours per step, their masks gathered together (past the
+        // row's end: the first neighbour again); once (m | seen) == full the new
+        // bits m & ~seen are all the missing ones, whichever neighbours were ORed
+        for (uint64_t p = b; p < e; p += 8) {
+          uint32_t w[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) w[j] = cidx[p + j < e ? p + j : b];
+          unsigned long long f[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) f[j] = fr[w[j]];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) m |= f[j];
           if ((m | seen) == full) break;
         }
         m &= ~seen;
