@@ -17,6 +17,8 @@
 #include "pr.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <initializer_list>
 
 namespace gre {
 namespace {
@@ -145,6 +147,42 @@ __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off
   }
   // rows still open at the tile end, plus empty rows that start in the tile
   while (r < re) flush_next();
+}
+
+// Group-per-row kernel (GRE_SCHED_ROWS): a group of G lanes takes whole
+// destination rows (grid-stride), G entries at a time with the next chunk's
+// ids prefetched; no tile splits, so no carries.  Rows are summed in row order.
+template <int G, class M, typename TT>
+__global__ __launch_bounds__(kBlk) void k_brows(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                                uint32_t V, M m) {
+  constexpr int NG = kBlk / G;
+  const int gl = threadIdx.x % G, gi = threadIdx.x / G;
+  const int k = m.k;
+  const bool act = gl < k;
+  TT* __restrict__ sums = m.sums();
+  for (uint64_t r = (uint64_t)blockIdx.x * NG + gi; r < V; r += (uint64_t)gridDim.x * NG) {
+    const uint64_t b = off[r], e = off[r + 1];
+    TT acc = (TT)0;
+    uint32_t myn = b + gl < e ? idx[b + gl] : 0u;
+    for (uint64_t cb = b; cb < e; cb += G) {
+      const uint32_t myu = myn;
+      TT g[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const uint32_t u = __shfl(myu, q, G);
+        const bool ok = cb + q < e;
+        const TT v = m.gather(ok ? cb + q : e - 1, u, act ? gl : 0);
+        g[q] = (act && ok) ? v : (TT)0;
+      }
+      {
+        const uint64_t nx = cb + G + gl;
+        myn = nx < e ? idx[nx] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) acc += g[q];
+    }
+    if (act) sums[r * k + gl] = acc;
+  }
 }
 
 // rows that start in tile t and run past its end: carryO[t] + carryP of the
@@ -372,17 +410,22 @@ void ensure_bscal(gre_pr* P, int iters) {
 }
 
 template <int G, class M, typename TT>
-void bpass(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s) {
+void bpass(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s, int sched) {
   BatchState& bs = P->bstate;
-  {
+  if (sched == GRE_SCHED_ROWS) {
     ProfScope ps("batch_walk");
-    hipLaunchKernelGGL((k_bwalk<G, M, TT>), dim3(b.nblk), dim3(kBlk), 0, s, b.off, b.idx, b.trb, b.E, b.V, b.ntiles,
-                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
-  }
-  {
-    ProfScope ps("batch_fixup");
-    hipLaunchKernelGGL((k_bfix<G, M, TT>), dim3(b.nfix), dim3(kBlk), 0, s, b.off, b.trb, b.E, b.V, b.ntiles,
-                       bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
+    hipLaunchKernelGGL((k_brows<G, M, TT>), dim3(b.napply), dim3(kBlk), 0, s, b.off, b.idx, b.V, m);
+  } else {
+    {
+      ProfScope ps("batch_walk");
+      hipLaunchKernelGGL((k_bwalk<G, M, TT>), dim3(b.nblk), dim3(kBlk), 0, s, b.off, b.idx, b.trb, b.E, b.V,
+                         b.ntiles, bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
+    }
+    {
+      ProfScope ps("batch_fixup");
+      hipLaunchKernelGGL((k_bfix<G, M, TT>), dim3(b.nfix), dim3(kBlk), 0, s, b.off, b.trb, b.E, b.V, b.ntiles,
+                         bs.carryP.as<TT>(), bs.carryO.as<TT>(), m);
+    }
   }
   {
     ProfScope ps("batch_apply");
@@ -395,16 +438,62 @@ void bpass(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t 
 }
 
 template <class M, typename TT>
-void bpass_g(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s) {
+void bpass_g(const BPlan& b, gre_pr* P, const M& m, double* scal_out, hipStream_t s, int sched) {
   switch (pow2ceil(P->k)) {
-    case 1: bpass<1, M, TT>(b, P, m, scal_out, s); break;
-    case 2: bpass<2, M, TT>(b, P, m, scal_out, s); break;
-    case 4: bpass<4, M, TT>(b, P, m, scal_out, s); break;
-    case 8: bpass<8, M, TT>(b, P, m, scal_out, s); break;
-    case 16: bpass<16, M, TT>(b, P, m, scal_out, s); break;
-    case 32: bpass<32, M, TT>(b, P, m, scal_out, s); break;
-    default: bpass<64, M, TT>(b, P, m, scal_out, s); break;
+    case 1: bpass<1, M, TT>(b, P, m, scal_out, s, sched); break;
+    case 2: bpass<2, M, TT>(b, P, m, scal_out, s, sched); break;
+    case 4: bpass<4, M, TT>(b, P, m, scal_out, s, sched); break;
+    case 8: bpass<8, M, TT>(b, P, m, scal_out, s, sched); break;
+    case 16: bpass<16, M, TT>(b, P, m, scal_out, s, sched); break;
+    case 32: bpass<32, M, TT>(b, P, m, scal_out, s, sched); break;
+    default: bpass<64, M, TT>(b, P, m, scal_out, s, sched); break;
   }
+}
+
+// The batched schedule of direction d: the handle's fixed schedule, or (auto)
+// the faster of the tile walk and the group-per-row kernel, timed on two
+// passes after a warm-up one at first use.  pass(sched, it) runs one pass.
+template <typename F>
+int batch_sched(gre_pr* P, int d, hipStream_t s, F&& pass) {
+  BatchState& bs = P->bstate;
+  if (P->schedule == GRE_SCHED_ROWS) return GRE_SCHED_ROWS;
+  if (P->schedule != GRE_SCHED_AUTO) return GRE_SCHED_TILE;
+  if (bs.sched[d]) return bs.sched[d];
+  hipEvent_t a, b;
+  GRE_HIP(hipEventCreate(&a));
+  GRE_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  int pick = GRE_SCHED_TILE;
+  for (int sc : {GRE_SCHED_TILE, GRE_SCHED_ROWS}) {
+    pass(sc, 0);
+    GRE_HIP(hipEventRecord(a, s));
+    pass(sc, 1);
+    pass(sc, 2);
+    GRE_HIP(hipEventRecord(b, s));
+    GRE_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GRE_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best) { best = ms; pick = sc; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  bs.sched[d] = pick;
+  return pick;
+}
+
+// Everything a captured batched run bakes into its launches.
+std::vector<uint64_t> batch_key(const gre_pr* P, const BPlan& b, int sched, double damp, int iters,
+                                std::initializer_list<const void*> ptrs) {
+  uint64_t dbits = 0;
+  std::memcpy(&dbits, &damp, sizeof(dbits));
+  std::vector<uint64_t> k = {dbits, (uint64_t)iters, (uint64_t)P->dtype, (uint64_t)P->k, (uint64_t)sched,
+                             b.ntiles, (uint64_t)b.nblk, (uint64_t)b.napply};
+  for (const void* p : ptrs) k.push_back((uint64_t)(uintptr_t)p);
+  const BatchState& bs = P->bstate;
+  for (const void* p : {(const void*)b.off, (const void*)b.idx, (const void*)b.trb, (const void*)bs.carryP.p,
+                        (const void*)bs.carryO.p, (const void*)bs.part.p, (const void*)bs.scal.p})
+    k.push_back((uint64_t)(uintptr_t)p);
+  return k;
 }
 
 // max over vectors of the L1 residual (scal layout: [a0_0, a1_0, a0_1, a1_1, ...])
@@ -421,20 +510,20 @@ template <typename TT>
 double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
   hipStream_t s = stream();
   const BPlan b = make_plan(P, true);
-  ensure_bscal(P, iters);
+  ensure_bscal(P, std::max(iters, 3));  // 3: the schedule timing passes
   BatchState& bs = P->bstate;
   const int k = P->k;
   double* sc = bs.scal.as<double>();
   const unsigned gb = gcap((uint64_t)P->g->nV * 64);
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
-  P->xcur = 0;
-  hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->p.as<TT>(), P->p.as<TT>(),
-                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), 0, tmp, (uint64_t)gb);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
-  GRE_HIP(hipGetLastError());
-  int it;
-  double res = 0.0;
-  for (it = 0; it < iters; ++it) {
+  auto init = [&]() {
+    P->xcur = 0;
+    hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->p.as<TT>(), P->p.as<TT>(),
+                       P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), 0, tmp, (uint64_t)gb);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+    GRE_HIP(hipGetLastError());
+  };
+  auto pass = [&](int sched, int it) {
     BFwd<TT> m;
     m.xin = P->x[P->xcur].as<TT>();
     m.xout = P->x[P->xcur ^ 1].as<TT>();
@@ -444,8 +533,38 @@ double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal = sc + 2 * k * it;  // a0_j at index 2j
     m.damp = damp;
     m.k = k;
-    bpass_g<BFwd<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s);
+    bpass_g<BFwd<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s, sched);
     P->xcur ^= 1;
+  };
+  int sched = GRE_SCHED_TILE;
+  if (P->schedule == GRE_SCHED_AUTO && !bs.sched[0]) init();  // the timing passes start from the initial state
+  sched = batch_sched(P, 0, s, pass);
+  if (tol <= 0.0 && iters > 0 && P->graphs && !prof_on()) {
+    // fixed iteration count: one captured graph per direction, replayed while
+    // nothing it depends on has changed
+    const std::vector<uint64_t> key =
+        batch_key(P, b, sched, damp, iters, {P->x[0].p, P->x[1].p, P->p.p, P->wcsc.p, tmp});
+    if (!bs.exec[0] || key != bs.key[0]) {
+      bs.drop(0);
+      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      init();
+      for (int it = 0; it < iters; ++it) pass(sched, it);
+      GRE_HIP(hipStreamEndCapture(s, &bs.graph[0]));
+      GRE_HIP(hipGraphInstantiate(&bs.exec[0], bs.graph[0], nullptr, nullptr, 0));
+      bs.key[0] = key;
+      bs.post_cur[0] = P->xcur;
+    }
+    GRE_HIP(hipGraphLaunch(bs.exec[0], s));
+    P->xcur = bs.post_cur[0];
+    const double res = max_resid(P, sc + 2 * k * iters, s);
+    if (done) *done = iters;
+    return res;
+  }
+  init();
+  int it;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    pass(sched, it);
     if (tol > 0.0) {
       res = max_resid(P, sc + 2 * k * (it + 1), s);
       if (res < tol) { ++it; break; }
@@ -461,20 +580,20 @@ template <typename TT>
 double badjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
   hipStream_t s = stream();
   const BPlan b = make_plan(P, false);
-  ensure_bscal(P, iters);
+  ensure_bscal(P, std::max(iters, 3));  // 3: the schedule timing passes
   BatchState& bs = P->bstate;
   const int k = P->k;
   double* sc = bs.scal.as<double>();
   const unsigned gb = gcap((uint64_t)P->g->nV * 64);
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * k * (size_t)gb));
-  P->lcur = 0;
-  hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->c.as<TT>(), P->p.as<TT>(),
-                     P->g->outdeg.as<int32_t>(), P->lam[0].as<TT>(), 1, tmp, (uint64_t)gb);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
-  GRE_HIP(hipGetLastError());
-  int it;
-  double res = 0.0;
-  for (it = 0; it < iters; ++it) {
+  auto init = [&]() {
+    P->lcur = 0;
+    hipLaunchKernelGGL(k_binit<TT>, dim3(gb), dim3(kBlk), 0, s, P->g->nV, k, P->c.as<TT>(), P->p.as<TT>(),
+                       P->g->outdeg.as<int32_t>(), P->lam[0].as<TT>(), 1, tmp, (uint64_t)gb);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * k), dim3(256), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+    GRE_HIP(hipGetLastError());
+  };
+  auto pass = [&](int sched, int it) {
     BAdj<TT> m;
     m.lin = P->lam[P->lcur].as<TT>();
     m.lout = P->lam[P->lcur ^ 1].as<TT>();
@@ -486,8 +605,36 @@ double badjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal = sc + 2 * k * it;
     m.damp = damp;
     m.k = k;
-    bpass_g<BAdj<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s);
+    bpass_g<BAdj<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s, sched);
     P->lcur ^= 1;
+  };
+  int sched = GRE_SCHED_TILE;
+  if (P->schedule == GRE_SCHED_AUTO && !bs.sched[1]) init();  // the timing passes start from the initial state
+  sched = batch_sched(P, 1, s, pass);
+  if (tol <= 0.0 && iters > 0 && P->graphs && !prof_on()) {
+    const std::vector<uint64_t> key = batch_key(
+        P, b, sched, damp, iters, {P->lam[0].p, P->lam[1].p, P->c.p, P->p.p, P->wcsr.p, P->wl.p, tmp});
+    if (!bs.exec[1] || key != bs.key[1]) {
+      bs.drop(1);
+      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      init();
+      for (int it = 0; it < iters; ++it) pass(sched, it);
+      GRE_HIP(hipStreamEndCapture(s, &bs.graph[1]));
+      GRE_HIP(hipGraphInstantiate(&bs.exec[1], bs.graph[1], nullptr, nullptr, 0));
+      bs.key[1] = key;
+      bs.post_cur[1] = P->lcur;
+    }
+    GRE_HIP(hipGraphLaunch(bs.exec[1], s));
+    P->lcur = bs.post_cur[1];
+    const double res = max_resid(P, sc + 2 * k * iters, s);
+    if (done) *done = iters;
+    return res;
+  }
+  init();
+  int it;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    pass(sched, it);
     if (tol > 0.0) {
       res = max_resid(P, sc + 2 * k * (it + 1), s);
       if (res < tol) { ++it; break; }

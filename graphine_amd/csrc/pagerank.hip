@@ -684,8 +684,11 @@ int gre_pr_set_graphs(gre_pr* P, int on) {
 int gre_pr_set_schedule(gre_pr* P, int schedule, int tile_items) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_PART_NT, GRE_DEV_NULL,
+    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_ROWS, GRE_DEV_NULL,
               "schedule must be one of GRE_SCHED_*");
+    GRE_CHECK(P->k > 1 ? (schedule == GRE_SCHED_AUTO || schedule == GRE_SCHED_TILE || schedule == GRE_SCHED_ROWS)
+                       : schedule != GRE_SCHED_ROWS,
+              GRE_DEV_NULL, "batched handles take GRE_SCHED_AUTO, _TILE or _ROWS; GRE_SCHED_ROWS needs k > 1");
     GRE_CHECK(tile_items == 0 || tile_items == 4 || tile_items == 8 || tile_items == 16, GRE_DEV_NULL,
               "tile items per thread must be 4, 8 or 16");
     if (schedule == GRE_SCHED_AUTO) apply_schedule(P, GRE_SCHED_AUTO, 0);
@@ -698,6 +701,8 @@ int gre_pr_get_schedule(gre_pr* P, int* schedule, int* tile_items) {
     GRE_CHECK(P && schedule && tile_items, GRE_DEV_NULL, "null argument");
     *schedule = P->schedule;
     *tile_items = P->ipt;
+    // batched handles under AUTO: the schedule the forward timing picked
+    if (P->k > 1 && P->schedule == GRE_SCHED_AUTO && P->bstate.sched[0]) *schedule = P->bstate.sched[0];
   });
 }
 

@@ -11,6 +11,23 @@ struct BatchState {
   DevBuf carryP, carryO, part, scal;
   uint64_t ntiles = 0;
   int scal_cap = 0;
+  // schedule of each direction (0 forward over CSC, 1 adjoint over CSR):
+  // 0 = not chosen yet, GRE_SCHED_TILE = tile walk, GRE_SCHED_ROWS = group per row
+  int sched[2] = {0, 0};
+  // fixed-iteration runs replayed from captured hipGraphs, per direction
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<uint64_t> key[2];
+  int post_cur[2] = {0, 0};
+  void drop(int d) {
+    if (exec[d]) { (void)hipGraphExecDestroy(exec[d]); exec[d] = nullptr; }
+    if (graph[d]) { (void)hipGraphDestroy(graph[d]); graph[d] = nullptr; }
+    key[d].clear();
+  }
+  BatchState() = default;
+  BatchState(const BatchState&) = delete;
+  BatchState& operator=(const BatchState&) = delete;
+  ~BatchState() { drop(0); drop(1); }
 };
 }  // namespace gre
 

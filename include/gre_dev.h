@@ -192,9 +192,13 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * XCD so its gathers stay in that XCD's L2, and sums a row's per-range partial
  * sums in a second pull (unweighted, single vector, fewer than 2^32 edges;
  * ranges set with gre_pr_set_partition).
- * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest. */
+ * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
+ * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
+ * ROWS (one group of lanes per destination row, wavefront per vertex).
+ * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest
+ * (for batched handles, per direction: forward and adjoint). */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,
-       GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6 };
+       GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6, GRE_SCHED_ROWS = 7 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
 int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
 /* PART schedules: the shared_sources hottest compact sources (0 = none; fewer

@@ -221,8 +221,9 @@ def test_gradient_central_differences_f64(N, O):
         assert abs(fd - graw[e]) <= 2e-4 * max(abs(fd), 1e-9) + 1e-9
 
 
-@pytest.mark.parametrize("k,dtype", [(4, "f64"), (32, "f32")])
-def test_batched_ppr_equals_single_vectors(N, O, k, dtype):
+@pytest.mark.parametrize("k,dtype,sched", [(4, "f64", 0), (32, "f32", 0), (4, "f64", 7), (32, "f32", 7),
+                                           (3, "f64", 1), (17, "f32", 7)])
+def test_batched_ppr_equals_single_vectors(N, O, k, dtype, sched):
     scale = 13
     g = N.Graph.rmat(scale, 16, 6, False)
     og = O.rmat_graph(scale, 16, 6, False)
@@ -233,9 +234,14 @@ def test_batched_ppr_equals_single_vectors(N, O, k, dtype):
     for j in range(k):
         P[rng.choice(og.nV, 100, replace=False), j] = 1.0 / 100
     bp = N.PageRank(g, dt, k, weighted=True)
+    bp.set_schedule(sched)
     bp.set_edge_weights(None, seed=6)
     bp.set_personalization(P)
     bp.run(0.85, 30)
+    if sched:
+        assert bp.schedule()[0] == N.PageRank.SCHEDULES[sched]
+    else:
+        assert bp.schedule()[0] in ("tile", "rows")
     X = bp.ranks()
     tg = rng.choice(og.nV, 128, replace=False).astype(np.uint32)
     bp.adjoint(tg, 0.85, 30)
@@ -258,3 +264,34 @@ def test_batched_ppr_equals_single_vectors(N, O, k, dtype):
         assert rel_l1(gp[:, j], c.astype(np.float64)) < tol
     assert rel_l1(gW, gW_sum) < 1e-5
     assert rel_l1(graw, graw_sum) < 1e-4
+
+
+@pytest.mark.parametrize("sched", [1, 7])
+def test_batched_graph_replay_matches_eager(N, O, sched):
+    # fixed-iteration batched forward and adjoint runs replay captured graphs:
+    # every replay gives the eager bits, also after a damping change
+    g = N.Graph.rmat(14, 16, 3, False)
+    k = 8
+    rng = np.random.default_rng(3)
+    P = np.zeros((g.nV, k), np.float32)
+    for j in range(k):
+        P[rng.choice(g.nV, 50, replace=False), j] = 1.0 / 50
+    tg = rng.choice(g.nV, 64, replace=False).astype(np.uint32)
+    out = {}
+    for graphs in (False, True):
+        bp = N.PageRank(g, N.F32, k, weighted=True)
+        bp.set_schedule(sched)
+        bp.set_graphs(graphs)
+        bp.set_edge_weights(None, seed=2)
+        bp.set_personalization(P)
+        res = []
+        for d in (0.85, 0.85, 0.8, 0.85):
+            bp.run(d, 15)
+            x = bp.ranks()
+            bp.adjoint(tg, d, 15)
+            res.append((x, bp.adjoint_vector()))
+        out[graphs] = res
+    for (xe, le), (xg, lg) in zip(out[False], out[True]):
+        assert np.array_equal(xe, xg)
+        assert np.array_equal(le, lg)
+    assert np.array_equal(out[True][0][0], out[True][1][0])
