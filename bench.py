@@ -427,6 +427,7 @@ def main():
         "config": {"workload": f"PageRank FP32, {iters} iterations per step, directed R-MAT scale {S}, "
                                f"edge factor 16, seed 6, CSR+CSC int32 ids in HBM",
                    "vertices": V, "edges": E, "iterations_per_step": iters, "parallelism": "single GPU"},
+        "iterations_per_s": round(iters * a.steps / dt, 1),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -517,11 +517,13 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
   if (__any(changed) && lane == 0) atomicOr(any, 1u);
 }
 
-// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of (1+delta_w[j])/sigma_w[j];
+// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of q_w[j], where
+// q_w[j] = (1+delta_w[j])/sigma_w[j] was stored when w's level was done; v's own q follows.
 // atl[v] / nxt[v] = lanes of v at level L / L+1
 __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                                uint32_t nV, const double* __restrict__ sigma,
-                                               double* __restrict__ delta, const unsigned long long* __restrict__ atl,
+                                               double* __restrict__ delta, double* __restrict__ q,
+                                               const unsigned long long* __restrict__ atl,
                                                const unsigned long long* __restrict__ nxt) {
   const int lane = threadIdx.x & 63;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
@@ -548,19 +550,34 @@ __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off,
           wj[t] = __shfl(w, j, 64);
           act &= act - 1ULL;
         }
-        double dv[8], sv[8];
+        double qv[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const uint64_t o = (uint64_t)wj[t] * 64 + lane;
-          dv[t] = delta[o];
-          sv[t] = sigma[o];
-        }
+        for (int t = 0; t < 8; ++t) qv[t] = q[(uint64_t)wj[t] * 64 + lane];
 #pragma unroll
         for (int t = 0; t < 8; ++t)
-          if ((mj[t] >> lane) & 1ULL) acc += (1.0 + dv[t]) / sv[t];
+          if ((mj[t] >> lane) & 1ULL) acc += qv[t];
       }
     }
-    if (mine) delta[(uint64_t)v * 64 + lane] = sigma[(uint64_t)v * 64 + lane] * acc;
+    if (mine) {
+      const uint64_t o = (uint64_t)v * 64 + lane;
+      const double sv = sigma[o];
+      const double d = sv * acc;
+      delta[o] = d;
+      q[o] = (1.0 + d) / sv;
+    }
+  }
+}
+
+// q of the deepest level's vertices (their delta is 0): q_v[j] = 1/sigma_v[j]
+__global__ void k_bc_q_last(uint32_t nV, const double* __restrict__ sigma, const double* __restrict__ delta,
+                            double* __restrict__ q, const unsigned long long* __restrict__ atl) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t v = wid; v < nV; v += nw) {
+    if (!((atl[v] >> lane) & 1ULL)) continue;
+    const uint64_t o = v * 64 + lane;
+    q[o] = (1.0 + delta[o]) / sigma[o];
   }
 }
 
@@ -769,9 +786,10 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
     const uint64_t n64 = (uint64_t)nV * 64;
-    DevBuf sigma, delta, lev, vis, fr[2], msk[2], bc, dsrc, flag;
+    DevBuf sigma, delta, qv, lev, vis, fr[2], msk[2], bc, dsrc, flag;
     sigma.alloc(8 * n64);
     delta.alloc(8 * n64);
+    qv.alloc(8 * n64);
     lev.alloc(n64);
     vis.alloc(8 * (size_t)nV);
     fr[0].alloc(8 * (size_t)nV);
@@ -814,16 +832,19 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
       // deepest level reached is L; dependencies from L-1 down to 0, with the
       // lane masks of level l (mc) and l + 1 (mn, the previous level's mc)
       int mc = 0;
-      if (L > 0)
+      if (L > 0) {
         hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), L,
                            (unsigned long long*)msk[1].p);
+        hipLaunchKernelGGL(k_bc_q_last, dim3(gw), dim3(kB), 0, s, nV, sigma.as<double>(), delta.as<double>(),
+                           qv.as<double>(), (const unsigned long long*)msk[1].p);
+      }
       for (int32_t l = L - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l,
                            (unsigned long long*)msk[mc].p);
         ProfScope ps("bc_backward");
         hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), nV,
-                           sigma.as<double>(), delta.as<double>(), (const unsigned long long*)msk[mc].p,
-                           (const unsigned long long*)msk[mc ^ 1].p);
+                           sigma.as<double>(), delta.as<double>(), qv.as<double>(),
+                           (const unsigned long long*)msk[mc].p, (const unsigned long long*)msk[mc ^ 1].p);
         mc ^= 1;
       }
       hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, nV, nc, cs, delta.as<double>(), bc.as<double>());
