@@ -581,6 +581,96 @@ __global__ void k_bc_q_last(uint32_t nV, const double* __restrict__ sigma, const
   }
 }
 
+// forward, top-down level L -> L+1 over the frontier's out-edges (edge-balanced
+// like k_ms_td): v gains the lanes nw = fr[u] & ~vis[v], and sigma_v[j] +=
+// sigma_u[j] for each of them -- path counts are integer-valued doubles below
+// 2^53, so the atomic sums are exact in any order; the first edge that reaches
+// v queues it.  vis stays the pre-level mask until k_bc_td_fin.
+__global__ __launch_bounds__(kB) void k_bc_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                                              const uint32_t* __restrict__ q, const uint64_t* __restrict__ qpos,
+                                              uint32_t qn, uint64_t T, const unsigned long long* __restrict__ fr,
+                                              const unsigned long long* __restrict__ vis, double* __restrict__ sigma,
+                                              unsigned long long* __restrict__ nfr, uint32_t* __restrict__ nq,
+                                              uint32_t* __restrict__ ncount) {
+  __shared__ uint32_t range[2];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
+  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+  if (threadIdx.x < 2) {
+    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+    uint32_t lo = 0, hi = qn;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (qpos[mid] <= key) lo = mid; else hi = mid;
+    }
+    range[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint32_t i0 = range[0], i1 = range[1];
+  for (int round = 0; round < kEB / kB; ++round) {
+    const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0;
+    if (k < k1) {
+      uint32_t lo = i0, hi = i1 + 1;
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (qpos[mid] <= k) lo = mid; else hi = mid;
+      }
+      const uint32_t u = q[lo];
+      v = idx[off[u] + (k - qpos[lo])];
+      const unsigned long long nw = fr[u] & ~vis[v];
+      if (nw) {
+        take = atomicOr(nfr + v, nw) == 0ULL;
+        unsigned long long m = nw;
+        while (m) {
+          const int j = __ffsll(m) - 1;
+          atomicAdd(sigma + (uint64_t)v * 64 + j, sigma[(uint64_t)u * 64 + j]);
+          m &= m - 1ULL;
+        }
+      }
+    }
+    wave_append(take, v, nq, ncount);
+  }
+}
+
+// after a top-down level: the queued vertices take their new lanes at level L+1;
+// sums[0] += their out-degrees
+__global__ void k_bc_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, const unsigned long long* __restrict__ nfr,
+                            unsigned long long* __restrict__ vis, uint8_t* __restrict__ lev, int32_t L,
+                            const uint64_t* __restrict__ off, unsigned long long* __restrict__ sums) {
+  uint64_t deg = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = nq[i];
+    unsigned long long m = nfr[v];
+    vis[v] |= m;
+    while (m) {
+      const int j = __ffsll(m) - 1;
+      lev[(uint64_t)v * 64 + j] = (uint8_t)(L + 1);
+      m &= m - 1ULL;
+    }
+    deg += off[v + 1] - off[v];
+  }
+  deg = wave_sum(deg);
+  if ((threadIdx.x & 63) == 0 && deg) atomicAdd(sums, (unsigned long long)deg);
+}
+
+// after a bottom-up level: the new frontier (nonzero masks) as a queue, with
+// sums[0] += its out-degrees
+__global__ __launch_bounds__(kB) void k_bc_queue(uint32_t nV, const unsigned long long* __restrict__ nfr,
+                                                 const uint64_t* __restrict__ off, uint32_t* __restrict__ nq,
+                                                 uint32_t* __restrict__ ncount, unsigned long long* __restrict__ sums) {
+  uint64_t deg = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nV; base += stride) {
+    const uint64_t v = base + threadIdx.x;
+    const bool take = v < nV && nfr[v] != 0ULL;
+    if (take) deg += off[v + 1] - off[v];
+    wave_append(take, (uint32_t)v, nq, ncount);
+  }
+  deg = wave_sum(deg);
+  if ((threadIdx.x & 63) == 0 && deg) atomicAdd(sums, (unsigned long long)deg);
+}
+
 // mask of lanes whose level is L
 __global__ void k_bc_mask(uint32_t nV, const uint8_t* __restrict__ lev, int32_t L, unsigned long long* __restrict__ m) {
   const int lane = threadIdx.x & 63;
@@ -786,7 +876,12 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
     const uint64_t n64 = (uint64_t)nV * 64;
-    DevBuf sigma, delta, qv, lev, vis, fr[2], msk[2], bc, dsrc, flag;
+    DevBuf sigma, delta, qv, lev, vis, fr[2], msk[2], bc, dsrc, flag, bq[2], bqdeg, bqpos, bsum;
+    bq[0].alloc(4 * (size_t)nV + 4 * 64);
+    bq[1].alloc(4 * (size_t)nV + 4 * 64);
+    bqdeg.alloc(8 * ((size_t)nV + 1));
+    bqpos.alloc(8 * ((size_t)nV + 1));
+    bsum.alloc(8);
     sigma.alloc(8 * n64);
     delta.alloc(8 * n64);
     qv.alloc(8 * n64);
@@ -810,24 +905,70 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
       hipLaunchKernelGGL(k_bc_seed, dim3(1), dim3(1), 0, s, cs, nc, sigma.as<double>(), lev.as<uint8_t>(),
                          (unsigned long long*)vis.p, (unsigned long long*)fr[0].p);
       GRE_HIP(hipGetLastError());
+      // forward levels: top-down over the frontier's out-edges while they are
+      // few (under E / 14), else the bottom-up pull over every vertex
+      std::vector<uint32_t> uq;
+      for (int i = 0; i < nc; ++i) uq.push_back(srcs[c0 + i]);
+      std::sort(uq.begin(), uq.end());
+      uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+      GRE_HIP(hipMemcpyAsync(bq[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
+      uint32_t qn = (uint32_t)uq.size();
+      uint64_t mf = 0;  // out-edges of the frontier (measured on the device)
+      bool first = true;
       int cur = 0;
       int32_t L = 0;
       for (;; ++L) {
         GRE_CHECK(L < 250, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
+        const int nx = cur ^ 1;
+        uint64_t T = 0;
+        if (first || mf * 14 < g->nE) {
+          hipLaunchKernelGGL(k_qdeg, dim3(gcap((uint64_t)qn + 1)), dim3(kB), 0, s, bq[cur].as<uint32_t>(), qn,
+                             g->off.as<uint64_t>(), bqdeg.as<uint64_t>());
+          scan_excl_u64(bqdeg.as<uint64_t>(), bqpos.as<uint64_t>(), (size_t)qn + 1, s);
+          GRE_HIP(hipMemcpyAsync(&T, bqpos.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
+          GRE_HIP(hipStreamSynchronize(s));
+          mf = T;
+        }
+        first = false;
         GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-        {
+        GRE_HIP(hipMemsetAsync(bsum.p, 0, 8, s));
+        uint32_t nn = 0;
+        if (mf * 14 < g->nE) {
+          ProfScope ps("bc_forward");
+          GRE_HIP(hipMemsetAsync(fr[nx].p, 0, 8 * (size_t)nV, s));
+          if (T > 0)
+            hipLaunchKernelGGL(k_bc_td, dim3((unsigned)((T + kEB - 1) / kEB)), dim3(kB), 0, s, g->off.as<uint64_t>(),
+                               g->idx.as<uint32_t>(), bq[cur].as<uint32_t>(), bqpos.as<uint64_t>(), qn, T,
+                               (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                               sigma.as<double>(), (unsigned long long*)fr[nx].p, bq[nx].as<uint32_t>(),
+                               flag.as<uint32_t>());
+          GRE_HIP(hipGetLastError());
+          GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
+          GRE_HIP(hipStreamSynchronize(s));
+          if (nn > 0)
+            hipLaunchKernelGGL(k_bc_td_fin, dim3(gcap(nn)), dim3(kB), 0, s, bq[nx].as<uint32_t>(), nn,
+                               (const unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, lev.as<uint8_t>(), L,
+                               g->off.as<uint64_t>(), (unsigned long long*)bsum.p);
+        } else {
           ProfScope ps("bc_forward");
           hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), nV, nc,
-                             (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[cur ^ 1].p,
+                             (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
                              (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
-                             flag.as<unsigned int>());
+                             flag.as<unsigned int>() + 1);
+          hipLaunchKernelGGL(k_bc_queue, dim3(gcap(nV)), dim3(kB), 0, s, nV, (const unsigned long long*)fr[nx].p,
+                             g->off.as<uint64_t>(), bq[nx].as<uint32_t>(), flag.as<uint32_t>(),
+                             (unsigned long long*)bsum.p);
+          GRE_HIP(hipGetLastError());
+          GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
         }
         GRE_HIP(hipGetLastError());
-        uint32_t any = 0;
-        GRE_HIP(hipMemcpyAsync(&any, flag.p, 4, hipMemcpyDeviceToHost, s));
+        unsigned long long ms = 0;
+        GRE_HIP(hipMemcpyAsync(&ms, bsum.p, 8, hipMemcpyDeviceToHost, s));
         GRE_HIP(hipStreamSynchronize(s));
-        cur ^= 1;
-        if (!any) break;
+        mf = ms;
+        qn = nn;
+        cur = nx;
+        if (!nn) break;
       }
       // deepest level reached is L; dependencies from L-1 down to 0, with the
       // lane masks of level l (mc) and l + 1 (mn, the previous level's mc)
