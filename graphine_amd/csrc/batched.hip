@@ -342,13 +342,14 @@ __global__ __launch_bounds__(kBlk) void k_bgrad(const uint64_t* __restrict__ off
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const uint64_t e = cb + q;
-      if (e >= e1) break;
-      if (e >= rend) seek(e);
-      double d = act ? xu * (double)lv[q] : 0.0;
-      for (int sh = G / 2; sh > 0; sh >>= 1) d += __shfl_xor(d, sh, G);
-      if (gl == q) {
-        outW = (float)(damp * d);
-        outR = rs > 0.0 ? (float)(damp * (d - xwl) / rs) : 0.f;
+      if (e < e1) {
+        if (e >= rend) seek(e);
+        double d = act ? xu * (double)lv[q] : 0.0;
+        for (int sh = G / 2; sh > 0; sh >>= 1) d += __shfl_xor(d, sh, G);
+        if (gl == q) {
+          outW = (float)(damp * d);
+          outR = rs > 0.0 ? (float)(damp * (d - xwl) / rs) : 0.f;
+        }
       }
     }
     if (my < e1) {
