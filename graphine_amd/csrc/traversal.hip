@@ -206,11 +206,11 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
   bool topdown = true;
   int32_t L = 0;
   bool queue_valid = true;            // q[cur] holds the frontier
-  const double alpha = 14.0, beta = 24.0;
+  const double alpha = 40.0, beta = 24.0;
   uint32_t prev_n = 0;
   while (qn > 0) {
     const int nxt = cur ^ 1;
-    if (topdown && (double)mf > (double)mu / alpha && qn > prev_n) topdown = false;
+    if (topdown && (double)mf > (double)mu / alpha) topdown = false;
     else if (!topdown && (double)qn < (double)nV / beta && qn < prev_n) topdown = true;
     GRE_HIP(hipMemsetAsync(b.bits[nxt].p, 0, 4 * (size_t)nwords, s));
     GRE_HIP(hipMemsetAsync(b.cnt.p, 0, 8, s));
@@ -808,7 +808,7 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     for (int32_t L = 0; qn > 0; ++L) {
       GRE_CHECK(L <= (int32_t)nV, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
       const int nx = cur ^ 1;
-      if (topdown && L > 0 && (double)mf > (double)mu / alpha && qn > prev_n) topdown = false;
+      if (topdown && L > 0 && (double)mf > (double)mu / alpha) topdown = false;
       else if (!topdown && (double)qn < (double)nV / beta && qn < prev_n) topdown = true;
       GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
       GRE_HIP(hipMemsetAsync(sums.p, 0, 16, s));
