@@ -161,7 +161,7 @@ __global__ void k_reached_edges(const int32_t* __restrict__ level, const uint64_
 }
 
 struct BfsBufs {
-  DevBuf q[2], qdeg, qpos, bits[2], cnt, deg;
+  DevBuf q[2], qdeg, qpos, bits[2], cnt, deg, tail, lev;
   uint32_t nV = 0;
 };
 BfsBufs& bfs_bufs(uint32_t nV) {
@@ -176,6 +176,8 @@ BfsBufs& bfs_bufs(uint32_t nV) {
     b.bits[1].alloc(4 * nw);
     b.cnt.alloc(64);
     b.deg.alloc(64);
+    b.tail.alloc(64);
+    b.lev.alloc(4 * (size_t)nV);   // levels when the caller keeps none on the device
     b.nV = nV;
   }
   return b;
@@ -255,8 +257,7 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
     cur = nxt;
     ++L;
   }
-  DevBuf tmp;
-  tmp.alloc(16);
+  DevBuf& tmp = b.tail;
   GRE_HIP(hipMemsetAsync(tmp.p, 0, 8, s));
   hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(1), 0, s, (int32_t*)((char*)tmp.p + 8), (uint64_t)1, -1);
   hipLaunchKernelGGL(k_reached_edges, dim3(gcap(nV)), dim3(kB), 0, s, level, off, nV, (unsigned long long*)tmp.p,
@@ -275,7 +276,7 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
 // Working arrays, grown to the largest graph seen and kept for the process
 // (like the single-source ones): a call allocates nothing of size V.
 struct MsBufs {
-  DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums;
+  DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums, src;
   uint32_t nV = 0;
 };
 MsBufs& ms_bufs(uint32_t nV) {
@@ -291,6 +292,7 @@ MsBufs& ms_bufs(uint32_t nV) {
     b.qpos.alloc(8 * ((size_t)nV + 1));
     b.flag.alloc(16);
     b.sums.alloc(16);
+    b.src.alloc(4 * 64);
     b.nV = nV;
   }
   return b;
@@ -741,12 +743,7 @@ int gre_bfs_dev(gre_graph* g, uint32_t src, int32_t* level_dev, uint64_t* edges_
   return guard([&] {
     GRE_CHECK(g, GRE_DEV_NULL, "null graph");
     GRE_CHECK(src < g->nV, GRE_DEV_NULL, "source id >= nV");
-    static DevBuf& own = *new DevBuf();  // lives until the process ends
-    int32_t* lev = level_dev;
-    if (!lev) {
-      if (own.bytes < 4 * (size_t)g->nV) own.alloc(4 * (size_t)g->nV);
-      lev = own.as<int32_t>();
-    }
+    int32_t* lev = level_dev ? level_dev : bfs_bufs(g->nV).lev.as<int32_t>();
     bfs_run(g, src, lev, edges_out, depth_out);
   });
 }
@@ -755,8 +752,7 @@ int gre_bfs(gre_graph* g, uint32_t src, int32_t* level_out, uint64_t* edges_out,
   return guard([&] {
     GRE_CHECK(g, GRE_DEV_NULL, "null graph");
     GRE_CHECK(src < g->nV, GRE_DEV_NULL, "source id >= nV");
-    DevBuf lev;
-    lev.alloc(4 * (size_t)g->nV);
+    DevBuf& lev = bfs_bufs(g->nV).lev;
     bfs_run(g, src, lev.as<int32_t>(), edges_out, depth_out);
     if (level_out) {
       GRE_HIP(hipMemcpyAsync(level_out, lev.p, 4 * (size_t)g->nV, hipMemcpyDeviceToHost, stream()));
@@ -777,8 +773,8 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     DevBuf* fr = mb.fr;
     DevBuf* q = mb.q;
     DevBuf &vis = mb.vis, &nxt = mb.nxt, &qdeg = mb.qdeg, &qpos = mb.qpos, &flag = mb.flag, &sums = mb.sums;
-    DevBuf lev, dsrc;
-    dsrc.alloc(4 * (size_t)nsrc);
+    DevBuf lev;
+    DevBuf& dsrc = mb.src;
     if (level_out) {
       lev.alloc(4 * (size_t)nV * nsrc);
       hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
