@@ -15,6 +15,8 @@ namespace gre {
 // r % 8.  Entries are regrouped by (class, range, row): a (row, range) run is a
 // segment, and the pull reduces segments instead of rows.  A second pull over
 // each row's list of segment ids sums them in a fixed order.
+// most (class, range) keys of a partition: 8 classes * (1 + 256/8)
+constexpr int kMaxKeys = 264;
 struct XPart {
   int R = 0;
   uint32_t H = 0;    // sources [0, H) form one range shared by all XCDs (cut by rows)
@@ -24,6 +26,8 @@ struct XPart {
   DevBuf pair_off;   // uint64 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
   DevBuf pair_list;  // uint32 [nseg] segment ids, by row then class-major range
   uint64_t class_start[9] = {0};  // first entry of each class; class_start[8] = E
+  int K = 0;         // (class, range) keys: 8 * (1 + R/8)
+  DevBuf kseg;       // uint32 [K+1] first segment of each key; kseg[K] = nseg
   struct Tiles {
     DevBuf rb;       // uint32 [ntiles+1] tile row bounds over segments
     DevBuf xmap;     // uint32 [9] first tile of each class; xmap[8] = ntiles
@@ -31,8 +35,15 @@ struct XPart {
     unsigned grid = 0;  // 8 * (largest class tile count)
   };
   std::map<uint32_t, Tiles> tiles;  // by tile edges
+  // Tiles of the per-row pass.  A tile's segments are, per key, one run of
+  // consecutive segment ids (segments of a key are ordered by row), so the
+  // tile stages its runs into LDS with coalesced loads and reads each pair
+  // entry's sum from there.
   struct PairTiles {
     DevBuf rb;       // uint32 [ntiles+1] tile row bounds over the pair lists
+    DevBuf runA;     // uint32 [(ntiles+1)*K] first segment of key k at or after tile t
+    DevBuf runO;     // uint16 [ntiles*(K+1)] LDS offset of key k's run in tile t; [K] = tile entries
+    DevBuf slot;     // uint16 [nseg] LDS slot of each pair entry's segment sum
     uint64_t ntiles = 0;
   };
   std::map<uint32_t, PairTiles> pair_tiles;  // by tile entries

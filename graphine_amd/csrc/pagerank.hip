@@ -41,20 +41,33 @@ struct ModeFwd {
   const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
-  __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const {
+  // what row r's update reads besides its sum, loaded ahead by staged pulls
+  struct Pre {
+    TT pv, old;
+    int32_t d;
+    uint32_t c;
+  };
+  __device__ Pre pre(uint32_t r) const {
+    Pre q;
+    q.pv = p ? p[r] : (TT)invn;
+    q.old = xin[r];
+    q.d = outdeg[r];
+    q.c = (!w && contrib_out && cid) ? cid[r] : 0u;
+    return q;
+  }
+  __device__ void finalize_pre(uint32_t r, TT s, const Pre& q, double& a0, double& a1) const {
     const double dang = scal_in[0];
-    const TT pv = p ? p[r] : (TT)invn;
-    const TT y = (TT)damp * (s + (TT)dang * pv) + (TT)(1.0 - damp) * pv;
-    const TT old = xin[r];
+    const TT y = (TT)damp * (s + (TT)dang * q.pv) + (TT)(1.0 - damp) * q.pv;
     xout[r] = y;
-    const int32_t d = outdeg[r];
+    const int32_t d = q.d;
     if (!w && contrib_out) {
-      if (cid) { if (d > 0) contrib_out[cid[r]] = y / (TT)d; }
+      if (cid) { if (d > 0) contrib_out[q.c] = y / (TT)d; }
       else contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
     }
     if (d == 0) a0 += (double)y;
-    a1 += fabs((double)y - (double)old);
+    a1 += fabs((double)y - (double)q.old);
   }
+  __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const { finalize_pre(r, s, pre(r), a0, a1); }
 };
 
 // The weighting is a compile-time property of the launched mode: a run-time
@@ -370,6 +383,7 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters) {
     k.push_back(pl->ntiles);
     k.push_back((uint64_t)pl->ipt);
     k.push_back(ptr(pl->xmap));
+    k.push_back(ptr(pl->slot));
     k.push_back(pl->grid);
   }
   if (P->xp) {

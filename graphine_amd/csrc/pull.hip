@@ -61,6 +61,7 @@ void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbyt
   p.V = g->nV;
   p.ipt = ipt;
   p.xmap = nullptr;
+  p.slot = nullptr;
   p.grid = 0;
   alloc_for(p, tbytes);
 }
@@ -76,6 +77,7 @@ void PullBuffers::make(PullPlan& p, XPart& xp, int ipt, size_t tbytes) {
   p.V = (uint32_t)xp.nseg;
   p.ipt = ipt;
   p.xmap = T.xmap.as<uint32_t>();
+  p.slot = nullptr;
   p.grid = T.grid;
   alloc_for(p, tbytes);
 }
@@ -85,6 +87,10 @@ void PullBuffers::make_pairs(PullPlan& p, XPart& xp, uint32_t nV, int ipt, size_
   const XPart::PairTiles& T = xpart_pair_tiles(xp, nV, te, stream());
   p.off = xp.pair_off.as<uint64_t>();
   p.idx = xp.pair_list.as<uint32_t>();
+  p.slot = T.slot.as<uint16_t>();
+  p.runA = T.runA.as<uint32_t>();
+  p.runO = T.runO.as<uint16_t>();
+  p.K = xp.K;
   p.tile_rb = T.rb.as<uint32_t>();
   p.ntiles = T.ntiles;
   p.E = xp.nseg;

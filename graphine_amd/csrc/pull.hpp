@@ -14,6 +14,9 @@
 #pragma once
 #include "graph.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace gre {
 
 constexpr int kPullBlock = 256;
@@ -52,13 +55,20 @@ __device__ inline void group_sum2(double& a, double& b, double* red, int tid) {
   }
 }
 
+template <int IPT, class Mode>
+__device__ __forceinline__ void pull_tile_rows(
+    uint64_t t, bool valid, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ tile_rb, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
+    typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const typename Mode::T* vals,
+    uint32_t* longj, int* nlong, double* red, int tid);
+
 // One tile of TE = 256*IPT entries, processed by one 256-thread group (tid in
 // [0, 256)).  vals/longj/nlong/red are the group's LDS.  gat(e, u) gives the
 // value gathered for entry e with neighbour u.  valid = false: the group has no
 // tile this round but still meets every block barrier.
-template <int IPT, bool NT, class Mode, class Gather>
+template <int IPT, bool NT, class Mode, class Gather, class Id>
 __device__ __forceinline__ void pull_tile_body(
-    uint64_t t, bool valid, const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+    uint64_t t, bool valid, const uint64_t* __restrict__ off, const Id* __restrict__ idx,
     const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
     typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
     const Mode& m, const Gather& gat, typename Mode::T* vals, uint32_t* longj, int* nlong, double* red,
@@ -75,8 +85,8 @@ __device__ __forceinline__ void pull_tile_body(
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = j * kPullBlock + tid;
-    if (NT) nb[j] = i < n ? __builtin_nontemporal_load(idx + e0 + i) : 0u;
-    else nb[j] = i < n ? idx[e0 + i] : 0u;
+    if (NT) nb[j] = i < n ? (uint32_t)__builtin_nontemporal_load(idx + e0 + i) : 0u;
+    else nb[j] = i < n ? (uint32_t)idx[e0 + i] : 0u;
   }
   // every gather is issued before any is used: a lane past the tile's end
   // reads entry e0 with neighbour 0 (nb = 0 there), a valid element, and
@@ -94,7 +104,19 @@ __device__ __forceinline__ void pull_tile_body(
   }
   if (tid == 0) *nlong = 0;
   __syncthreads();
+  pull_tile_rows<IPT>(t, valid, e0, e1, n, off, tile_rb, V, ntiles, carryP, carryO, part, m, vals, longj, nlong, red,
+                      tid);
+}
 
+// Stages 2 and 3 of a tile whose values vals[0..n) (entries e0..e1) are in
+// LDS and *nlong = 0, both visible to the whole group.
+template <int IPT, class Mode>
+__device__ __forceinline__ void pull_tile_rows(
+    uint64_t t, bool valid, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ tile_rb, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
+    typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const typename Mode::T* vals,
+    uint32_t* longj, int* nlong, double* red, int tid) {
+  using T = typename Mode::T;
   // stage 2: rows that start in the tile, plus the row continuing into it
   double acc0 = 0.0, acc1 = 0.0;
   uint32_t rb = 0, nr = 0;
@@ -180,6 +202,168 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
                           vals, longj, &nlong, red, (int)threadIdx.x);
 }
 
+// The per-row pass over a partition's segment sums (rows = vertices, entries =
+// each row's segment ids).  A tile's segments form one run of consecutive ids
+// per (class, range) key; the tile first copies those runs from the mode's
+// gather array into LDS (coalesced: entry i of the stage belongs to the run k
+// with runO[k] <= i < runO[k+1], at segment runA[k] + i - runO[k]), then pulls
+// with each entry's value read from its LDS slot.  The entries, and so the
+// order of every row's sum, are those of the plain tile pull over the segment
+// ids.  The loads of the tile are issued in two rounds: (slots, run table, row
+// bounds), then (staged sums, row offsets, each row's own data through the
+// mode's pre(r)), so a tile waits for two memory latencies, not one per step.
+// Thread j takes row rb - 1 + j: row rb - 1 only when it continues into the
+// tile.  Modes provide pre(r) and finalize_pre(r, s, pre, acc0, acc1).
+template <class M, class = void>
+struct HasPre : std::false_type {};
+template <class M>
+struct HasPre<M, std::void_t<decltype(std::declval<const M&>().pre(0u))>> : std::true_type {};
+
+constexpr int kStagedRows = 4;  // rows per thread whose data is loaded with the stage
+
+template <int IPT, class Mode>
+__global__ __launch_bounds__(kPullBlock) void k_pull_staged(
+    const uint64_t* __restrict__ off, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ tile_rb,
+    uint64_t E, uint32_t V, uint64_t ntiles, const uint32_t* __restrict__ runA, const uint16_t* __restrict__ runO,
+    int K, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
+    Mode m) {
+  using T = typename Mode::T;
+  using Pre = decltype(m.pre(0u));
+  constexpr int TE = kPullBlock * IPT;
+  constexpr int Q = kStagedRows;
+  static_assert((TE & (TE - 1)) == 0, "tile entries must be a power of two");
+  __shared__ T stage[TE];
+  __shared__ T vals[TE];
+  __shared__ uint32_t sA[kMaxKeys];
+  __shared__ uint32_t sO[2 * kMaxKeys + 1];
+  __shared__ uint32_t longj[TE / kLongRow + 2];
+  __shared__ int nlong;
+  __shared__ double red[8];
+  const uint64_t t = blockIdx.x;
+  const int tid = (int)threadIdx.x;
+  const uint64_t e0 = t * (uint64_t)TE;
+  const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
+  const int n = e1 > e0 ? (int)(e1 - e0) : 0;
+
+  // round 1: slots (the slot array is padded to whole tiles), run table, row bounds
+  const uint32_t rb = tile_rb[t];
+  const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
+  uint32_t sl[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) sl[j] = slot[e0 + j * kPullBlock + tid];
+  // sO[k] for k in (K, 2K] is a sentinel above every entry index, so the
+  // search below needs no bound test
+  for (int k = tid; k <= 2 * K; k += kPullBlock) {
+    sO[k] = k <= K ? (uint32_t)runO[t * (uint64_t)(K + 1) + k] : 0xffffffffu;
+    if (k < K) sA[k] = runA[t * (uint64_t)K + k];
+  }
+  if (tid == 0) nlong = 0;
+  __syncthreads();
+
+  // round 2: staged sums, then each of the thread's first Q rows' offsets and data
+  T sv[IPT];
+  if (n > 0) {
+    // run of stage entry i: the largest k with sO[k] <= i, for all IPT entries
+    // at once (lanes past n take entry 0)
+    uint32_t ii[IPT];
+    uint32_t lo[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = j * kPullBlock + tid;
+      ii[j] = i < n ? (uint32_t)i : 0u;
+      lo[j] = 0;
+    }
+    int b = 1;
+    while (2 * b < K) b *= 2;
+    for (; b > 0; b >>= 1) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t c = lo[j] + (uint32_t)b;
+        lo[j] = sO[c] <= ii[j] ? c : lo[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) sv[j] = m.gather(0, sA[lo[j]] + (ii[j] - sO[lo[j]]));
+  }
+  uint64_t rob[Q], roe[Q];
+  Pre rpre[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t r = (int64_t)rb - 1 + tid + q * kPullBlock;
+    const uint32_t rs = (r >= 0 && r < (int64_t)re) ? (uint32_t)r : 0u;
+    rob[q] = off[rs];
+    roe[q] = off[rs + 1];
+    rpre[q] = m.pre(rs);
+  }
+  if (n > 0) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) stage[j * kPullBlock + tid] = sv[j];
+  }
+  __syncthreads();
+  if (n > 0) {
+    T gv[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) gv[j] = stage[sl[j] & (uint32_t)(TE - 1)];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) vals[j * kPullBlock + tid] = gv[j];
+  }
+  __syncthreads();
+
+  // rows rb - 1 .. re - 1; a row longer than kLongRow in-tile entries goes to a wave
+  const uint32_t nrows = re - rb + 1;
+  // row rb - 1 is thread 0's first row: roe[0] = off[rb] there
+  const bool has_rc = rb > 0 && n > 0 && roe[0] > e0;
+  double acc0 = 0.0, acc1 = 0.0;
+  auto row = [&](uint32_t j, uint64_t ob, uint64_t oe, const Pre& pr) {
+    const uint32_t r = rb - 1 + j;
+    const uint64_t sb = ob > e0 ? ob : e0;
+    const uint64_t se = oe < e1 ? oe : e1;
+    const int len = se > sb ? (int)(se - sb) : 0;
+    if (len > kLongRow) {
+      longj[atomicAdd(&nlong, 1)] = j;
+      return;
+    }
+    T s = (T)0;
+    const int base = (int)(sb - e0);
+    for (int q = 0; q < len; ++q) s += vals[base + q];
+    if (ob < e0) carryP[t] = s;
+    else if (oe > e1) carryO[t] = s;
+    else m.finalize_pre(r, s, pr, acc0, acc1);
+  };
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const uint32_t j = (uint32_t)(tid + q * kPullBlock);
+    if (j < nrows && (j > 0 || has_rc)) row(j, rob[q], roe[q], rpre[q]);
+  }
+  for (uint32_t j = (uint32_t)(tid + Q * kPullBlock); j < nrows; j += kPullBlock) {
+    const uint32_t r = rb - 1 + j;
+    row(j, off[r], off[r + 1], m.pre(r));
+  }
+  __syncthreads();
+
+  // long rows, one wave each
+  const int w = tid >> 6, l = tid & 63;
+  const int nl = nlong;
+  for (int q = w; q < nl; q += kPullBlock / 64) {
+    const uint32_t r = rb - 1 + longj[q];
+    const uint64_t ob = off[r], oe = off[r + 1];
+    const uint64_t sb = ob > e0 ? ob : e0;
+    const uint64_t se = oe < e1 ? oe : e1;
+    const int base = (int)(sb - e0), len = (int)(se - sb);
+    T s = (T)0;
+    for (int i = l; i < len; i += 64) s += vals[base + i];
+    s = wave_sum(s);
+    if (l == 0) {
+      if (ob < e0) carryP[t] = s;
+      else if (oe > e1) carryO[t] = s;
+      else m.finalize(r, s, acc0, acc1);
+    }
+  }
+
+  group_sum2(acc0, acc1, red, tid);
+  if (tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+}
+
 // Fix-up: tile t owns the row that starts in it and runs past its end.
 template <int IPT, class Mode>
 __global__ __launch_bounds__(kPullBlock) void k_pull_fixup(
@@ -233,6 +417,11 @@ struct PullPlan {
   double* part2 = nullptr;  // 2*ceil(ntiles/256)
   double* part3 = nullptr;  // 2*kRedBlocks
   const uint32_t* xmap = nullptr;  // class tile starts (source-range partitions) or nullptr
+  // per-row pass over segment sums staged in LDS (k_pull_staged), or nullptr
+  const uint16_t* slot = nullptr;
+  const uint32_t* runA = nullptr;
+  const uint16_t* runO = nullptr;
+  int K = 0;
   unsigned grid = 0;               // workgroups of the tile kernel when xmap is set
   const char* tag = "pull_tile";   // profiling names of the two kernels
   const char* fix_tag = "pull_fixup";
@@ -259,7 +448,14 @@ void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream
   {
     ProfScope ps(p.tag);
     const unsigned grid = p.xmap ? p.grid : (unsigned)p.ntiles;
-    if (p.nt)
+    if (p.slot) {
+      if constexpr (HasPre<Mode>::value)
+        hipLaunchKernelGGL((k_pull_staged<IPT, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
+                           p.slot, p.tile_rb, p.E, p.V, p.ntiles, p.runA, p.runO, p.K, (T*)p.carryP, (T*)p.carryO,
+                           p.part, m);
+      else
+        throw Error(GRE_DEV_NULL, "this pull has no staged per-row form");
+    } else if (p.nt)
       hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
                          p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
     else

@@ -198,6 +198,85 @@ __global__ void k_iota32(uint32_t* __restrict__ o, uint64_t n) {
     o[i] = (uint32_t)i;
 }
 
+// kseg[k] = first segment whose key is >= k, k in [0, K] (keys ascending; a
+// key change always starts a segment)
+__global__ void k_xkeyseg(const uint16_t* __restrict__ key, const uint32_t* __restrict__ segid, uint64_t E, int K,
+                          uint32_t nseg, uint32_t* __restrict__ kseg) {
+  for (int k = (int)threadIdx.x; k <= K; k += (int)blockDim.x) {
+    uint64_t lo = 0, hi = E;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if ((int)key[mid] < k) lo = mid + 1;
+      else hi = mid;
+    }
+    kseg[k] = lo < E ? segid[lo] - 1 : nseg;
+  }
+}
+
+// key of segment s: the largest k with kseg[k] <= s (kseg[0] = 0 <= s < kseg[K])
+__device__ inline int seg_key(const uint32_t* kseg, int K, uint32_t s) {
+  int lo = 0, hi = K;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (kseg[mid] <= s) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// pos[s] = position of segment s in the pair lists
+__global__ void k_xpos(const uint32_t* __restrict__ pair_list, uint64_t nseg, uint32_t* __restrict__ pos) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nseg; j += (uint64_t)gridDim.x * blockDim.x)
+    pos[pair_list[j]] = (uint32_t)j;
+}
+
+// runA[t*K + k] = first segment s of key k with pos[s] >= t*te, or the key's
+// end; t in [0, ntiles].  Inside a key, pos grows with s (rows ascending).
+__global__ void k_xruna(const uint32_t* __restrict__ kseg, int K, const uint32_t* __restrict__ pos, uint64_t ntiles,
+                        uint32_t te, uint32_t* __restrict__ runA) {
+  const uint64_t cells = (ntiles + 1) * (uint64_t)K;
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < cells; c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = c / (uint64_t)K;
+    const int k = (int)(c % (uint64_t)K);
+    const uint64_t target = t * te;
+    uint32_t lo = kseg[k], hi = kseg[k + 1];
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if ((uint64_t)pos[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    runA[c] = lo;
+  }
+}
+
+// runO[t*(K+1) + k] = entries of tile t in keys below k (the LDS offset of key
+// k's run); runO[t*(K+1) + K] = the tile's entry count
+__global__ void k_xruno(const uint32_t* __restrict__ runA, int K, uint64_t ntiles, uint16_t* __restrict__ runO) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntiles; t += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t acc = 0;
+    for (int k = 0; k < K; ++k) {
+      runO[t * (K + 1) + k] = (uint16_t)acc;
+      acc += runA[(t + 1) * K + k] - runA[t * K + k];
+    }
+    runO[t * (K + 1) + K] = (uint16_t)acc;
+  }
+}
+
+// slot[j] = LDS slot of pair entry j's segment inside its tile's staged runs
+__global__ void k_xslot(const uint32_t* __restrict__ pair_list, uint64_t nseg, uint32_t te,
+                        const uint32_t* __restrict__ kseg, int K, const uint32_t* __restrict__ runA,
+                        const uint16_t* __restrict__ runO, uint16_t* __restrict__ slot) {
+  __shared__ uint32_t ks[kMaxKeys + 1];
+  for (int k = (int)threadIdx.x; k <= K; k += (int)blockDim.x) ks[k] = kseg[k];
+  __syncthreads();
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nseg; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = pair_list[j];
+    const uint64_t t = j / te;
+    const int k = seg_key(ks, K, s);
+    slot[j] = (uint16_t)(runO[t * (K + 1) + k] + (s - runA[t * K + k]));
+  }
+}
+
 unsigned bits_of(uint64_t n) {
   unsigned b = 1;
   while (b < 32 && (1ULL << b) < n) ++b;
@@ -318,6 +397,11 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     GRE_HIP(hipGetLastError());
     const uint64_t eh = E;
     GRE_HIP(hipMemcpyAsync(xp->seg_off.as<uint64_t>() + ns, &eh, 8, hipMemcpyHostToDevice, s));
+    xp->K = 8 * per;
+    xp->kseg.alloc(4 * (size_t)(xp->K + 1));
+    hipLaunchKernelGGL(k_xkeyseg, dim3(1), dim3(kB), 0, s, key_s.as<uint16_t>(), segid.as<uint32_t>(), E, xp->K, ns,
+                       xp->kseg.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
     GRE_HIP(hipStreamSynchronize(s));
   }
   key_s.release();
@@ -362,6 +446,31 @@ const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hi
   T.ntiles = nt;
   T.rb.alloc(4 * (nt + 1));
   build_tile_rows(xp.pair_off.as<uint64_t>(), nV, nt, te, T.rb.as<uint32_t>(), s);
+  // staged runs: LDS offsets are 16-bit, and a tile's keys fit its LDS table
+  GRE_CHECK(te <= 65535 && xp.K > 0 && xp.K <= kMaxKeys, GRE_DEV_NULL, "pair tiles too large to stage");
+  const int K = xp.K;
+  const uint64_t ns = xp.nseg;
+  T.runA.alloc(4 * (nt + 1) * (uint64_t)K);
+  T.runO.alloc(2 * nt * (uint64_t)(K + 1));
+  // padded to whole tiles, so the row pass loads every slot of a tile unguarded
+  T.slot.alloc(2 * nt * (uint64_t)te);
+  GRE_HIP(hipMemsetAsync(T.slot.p, 0, 2 * nt * (uint64_t)te, s));
+  {
+    DevBuf pos;
+    pos.alloc(4 * std::max<uint64_t>(ns, 1));
+    hipLaunchKernelGGL(k_xpos, dim3(gcap(ns)), dim3(kB), 0, s, xp.pair_list.as<uint32_t>(), ns, pos.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_xruna, dim3(gcap((nt + 1) * K)), dim3(kB), 0, s, xp.kseg.as<uint32_t>(), K,
+                       pos.as<uint32_t>(), nt, te, T.runA.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_xruno, dim3(gcap(nt)), dim3(kB), 0, s, T.runA.as<uint32_t>(), K, nt, T.runO.as<uint16_t>());
+    GRE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_xslot, dim3(gcap(ns)), dim3(kB), 0, s, xp.pair_list.as<uint32_t>(), ns, te,
+                       xp.kseg.as<uint32_t>(), K, T.runA.as<uint32_t>(), T.runO.as<uint16_t>(),
+                       T.slot.as<uint16_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipStreamSynchronize(s));
+  }
   return T;
 }
 
