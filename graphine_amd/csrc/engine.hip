@@ -482,7 +482,9 @@ struct EngSumMode {
   const double* S;
   double* C;
   __device__ double gather(uint64_t, uint32_t u) const { return S[u]; }
-  __device__ void finalize(uint32_t r, double s, double&, double&) const { C[r] += s; }
+  __device__ double pre(uint32_t r) const { return C[r]; }
+  __device__ void finalize_pre(uint32_t r, double s, double c, double&, double&) const { C[r] = c + s; }
+  __device__ void finalize(uint32_t r, double s, double& a0, double& a1) const { finalize_pre(r, s, pre(r), a0, a1); }
 };
 
 // The same over the XCD source partition of the CSC (unweighted PageRank's

@@ -41,9 +41,14 @@ struct XPart {
   // entry's sum from there.
   struct PairTiles {
     DevBuf rb;       // uint32 [ntiles+1] tile row bounds over the pair lists
-    DevBuf runA;     // uint32 [(ntiles+1)*K] first segment of key k at or after tile t
-    DevBuf runO;     // uint16 [ntiles*(K+1)] LDS offset of key k's run in tile t; [K] = tile entries
-    DevBuf slot;     // uint16 [nseg] LDS slot of each pair entry's segment sum
+    // A tile's LDS stage holds its runs in key order.  words[t*W + w] (W =
+    // tile entries / 32) = (bit b set if a non-empty run starts at stage entry
+    // 32w + b, number of non-empty runs starting before entry 32w); base[t*K
+    // + r] = first segment of the tile's r-th non-empty run minus its stage
+    // offset, so stage entry i holds segment base[rank(i)] + i.
+    DevBuf words;    // uint2 [ntiles*W]
+    DevBuf base;     // uint32 [ntiles*K]
+    DevBuf slot;     // uint16 [ntiles*te] LDS slot of each pair entry's segment sum (zero-padded)
     uint64_t ntiles = 0;
   };
   std::map<uint32_t, PairTiles> pair_tiles;  // by tile entries

@@ -98,16 +98,29 @@ struct ModeAdj {
   TT* wl;                 // (W lambda)_u
   const double* scal_in;  // [0] = p.lambda of lin
   double damp, invn;
-  __device__ void finalize(uint32_t u, TT s, double& a0, double& a1) const {
-    const int32_t d = outdeg[u];
+  // what row u's update reads besides its sum, loaded ahead by the tile pull
+  struct Pre {
+    TT cu, old, pv;
+    int32_t d;
+  };
+  __device__ Pre pre(uint32_t u) const {
+    Pre q;
+    q.d = outdeg[u];
+    q.cu = c[u];
+    q.old = lin[u];
+    q.pv = p ? p[u] : (TT)invn;
+    return q;
+  }
+  __device__ void finalize_pre(uint32_t u, TT s, const Pre& q, double& a0, double& a1) const {
+    const int32_t d = q.d;
     if (!w && d > 0) s = s / (TT)d;
     wl[u] = s;
-    const TT y = c[u] + (TT)damp * (s + (d == 0 ? (TT)scal_in[0] : (TT)0));
-    const TT old = lin[u];
+    const TT y = q.cu + (TT)damp * (s + (d == 0 ? (TT)scal_in[0] : (TT)0));
     lout[u] = y;
-    a0 += (double)(p ? p[u] : (TT)invn) * (double)y;
-    a1 += fabs((double)y - (double)old);
+    a0 += (double)q.pv * (double)y;
+    a1 += fabs((double)y - (double)q.old);
   }
+  __device__ void finalize(uint32_t u, TT s, double& a0, double& a1) const { finalize_pre(u, s, pre(u), a0, a1); }
 };
 
 template <typename TT, bool W>
@@ -487,7 +500,7 @@ void autotune_forward(gre_pr* P, double damp) {
     const bool prt = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
     // partitioned schedules also try (tail ranges, shared sources) pairs
     std::vector<std::pair<int, uint32_t>> vars = {{P->ranges, P->shared}};
-    if (prt) vars = {{128, 0u}, {64, 1u << 18}, {128, 1u << 18}, {64, 1u << 20}, {32, 1u << 20}};
+    if (prt) vars = {{128, 0u}, {256, 0u}, {128, 1u << 17}, {256, 1u << 16}, {64, 1u << 18}};
     for (const auto& h : vars) {
       for (int ipt : ipts) {
         if (prt && ipt == 16) continue;

@@ -88,8 +88,8 @@ void PullBuffers::make_pairs(PullPlan& p, XPart& xp, uint32_t nV, int ipt, size_
   p.off = xp.pair_off.as<uint64_t>();
   p.idx = xp.pair_list.as<uint32_t>();
   p.slot = T.slot.as<uint16_t>();
-  p.runA = T.runA.as<uint32_t>();
-  p.runO = T.runO.as<uint16_t>();
+  p.words = T.words.as<uint2>();
+  p.base = T.base.as<uint32_t>();
   p.K = xp.K;
   p.tile_rb = T.rb.as<uint32_t>();
   p.ntiles = T.ntiles;

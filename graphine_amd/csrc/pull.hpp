@@ -39,6 +39,25 @@ __device__ inline void block_sum2(double& a, double& b, double* sh /*[2*4]*/) {
 //   using T = ...;
 //   __device__ T gather(uint64_t e, uint32_t nbr) const;
 //   __device__ void finalize(uint32_t row, T sum, double& acc0, double& acc1) const;
+// and optionally, for the row's own data to be loaded ahead with the tile:
+//   __device__ Pre pre(uint32_t row) const;
+//   __device__ void finalize_pre(uint32_t row, T sum, const Pre&, double& acc0, double& acc1) const;
+template <class M, class = void>
+struct HasPre : std::false_type {};
+template <class M>
+struct HasPre<M, std::void_t<decltype(std::declval<const M&>().pre(0u))>> : std::true_type {};
+struct NoPre {};
+template <class Mode>
+__device__ __forceinline__ auto row_pre(const Mode& m, uint32_t r) {
+  if constexpr (HasPre<Mode>::value) return m.pre(r);
+  else return NoPre{};
+}
+template <class Mode, class P>
+__device__ __forceinline__ void row_fin(const Mode& m, uint32_t r, typename Mode::T s, const P& q, double& a0,
+                                        double& a1) {
+  if constexpr (HasPre<Mode>::value) m.finalize_pre(r, s, q, a0, a1);
+  else m.finalize(r, s, a0, a1);
+}
 
 // Fixed-order sum of two doubles over one 256-thread group (gid-th group of the
 // block); tid 0 of the group gets the totals.  red holds 8 doubles per group.
@@ -55,29 +74,108 @@ __device__ inline void group_sum2(double& a, double& b, double* red, int tid) {
   }
 }
 
-template <int IPT, class Mode>
-__device__ __forceinline__ void pull_tile_rows(
-    uint64_t t, bool valid, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ tile_rb, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
-    typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const typename Mode::T* vals,
-    uint32_t* longj, int* nlong, double* red, int tid);
+// Rows of a tile: thread j takes row rb - 1 + j, j in [0, re - rb + 1); row
+// rb - 1 only when it continues into the tile.  The first Q rows of each
+// thread have their offsets (and the mode's pre data) loaded together with the
+// tile's gathers, so the tile waits for one memory latency less per step; the
+// rest are loaded where they are walked.  Q trades that against registers.
+template <class Mode, int Q>
+struct TileRows {
+  using Pre = decltype(row_pre(std::declval<const Mode&>(), 0u));
+  uint64_t ob[Q], oe[Q];
+  Pre pre[Q];
+  // issued without waiting: lanes past the rows load row 0
+  __device__ __forceinline__ void load(const uint64_t* __restrict__ off, uint32_t rb, uint32_t re, const Mode& m,
+                                       int tid) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int64_t r = (int64_t)rb - 1 + tid + q * kPullBlock;
+      const uint32_t rs = (r >= 0 && r < (int64_t)re) ? (uint32_t)r : 0u;
+      ob[q] = off[rs];
+      oe[q] = off[rs + 1];
+      pre[q] = row_pre(m, rs);
+    }
+  }
+};
+
+// Stages 2 and 3 of tile t (entries e0..e1, n of them) whose values vals[0..n)
+// are in LDS and *nlong = 0, both visible to the whole group; rows as above.
+template <int IPT, class Mode, int Q>
+__device__ __forceinline__ void tile_rows(
+    uint64_t t, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off, uint32_t rb, uint32_t re,
+    const TileRows<Mode, Q>& tr, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
+    double* __restrict__ part, const Mode& m, const typename Mode::T* vals, uint32_t* longj, int* nlong, double* red,
+    int tid) {
+  using T = typename Mode::T;
+  const uint32_t nrows = re - rb + 1;
+  // row rb - 1 is thread 0's first row: tr.oe[0] = off[rb] there
+  const bool has_rc = rb > 0 && n > 0 && tr.oe[0] > e0;
+  double acc0 = 0.0, acc1 = 0.0;
+  auto row = [&](uint32_t j, uint64_t ob, uint64_t oe, const typename TileRows<Mode, Q>::Pre& q) {
+    const uint32_t r = rb - 1 + j;
+    const uint64_t sb = ob > e0 ? ob : e0;
+    const uint64_t se = oe < e1 ? oe : e1;
+    const int len = se > sb ? (int)(se - sb) : 0;
+    if (len > kLongRow) {
+      longj[atomicAdd(nlong, 1)] = j;
+      return;
+    }
+    T s = (T)0;
+    const int base = (int)(sb - e0);
+    for (int i = 0; i < len; ++i) s += vals[base + i];
+    if (ob < e0) carryP[t] = s;
+    else if (oe > e1) carryO[t] = s;
+    else row_fin(m, r, s, q, acc0, acc1);
+  };
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const uint32_t j = (uint32_t)(tid + q * kPullBlock);
+    if (j < nrows && (j > 0 || has_rc)) row(j, tr.ob[q], tr.oe[q], tr.pre[q]);
+  }
+  for (uint32_t j = (uint32_t)(tid + Q * kPullBlock); j < nrows; j += kPullBlock) {
+    const uint32_t r = rb - 1 + j;
+    row(j, off[r], off[r + 1], row_pre(m, r));
+  }
+  __syncthreads();
+
+  // stage 3: long rows, one wave each
+  const int w = tid >> 6, l = tid & 63;
+  const int nl = *nlong;
+  for (int q = w; q < nl; q += kPullBlock / 64) {
+    const uint32_t r = rb - 1 + longj[q];
+    const uint64_t ob = off[r], oe = off[r + 1];
+    const uint64_t sb = ob > e0 ? ob : e0;
+    const uint64_t se = oe < e1 ? oe : e1;
+    const int base = (int)(sb - e0), len = (int)(se - sb);
+    T s = (T)0;
+    for (int i = l; i < len; i += 64) s += vals[base + i];
+    s = wave_sum(s);
+    if (l == 0) {
+      if (ob < e0) carryP[t] = s;
+      else if (oe > e1) carryO[t] = s;
+      else m.finalize(r, s, acc0, acc1);
+    }
+  }
+
+  group_sum2(acc0, acc1, red, tid);
+  if (tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+}
 
 // One tile of TE = 256*IPT entries, processed by one 256-thread group (tid in
 // [0, 256)).  vals/longj/nlong/red are the group's LDS.  gat(e, u) gives the
-// value gathered for entry e with neighbour u.  valid = false: the group has no
-// tile this round but still meets every block barrier.
+// value gathered for entry e with neighbour u.
 template <int IPT, bool NT, class Mode, class Gather, class Id>
 __device__ __forceinline__ void pull_tile_body(
-    uint64_t t, bool valid, const uint64_t* __restrict__ off, const Id* __restrict__ idx,
-    const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
-    typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
-    const Mode& m, const Gather& gat, typename Mode::T* vals, uint32_t* longj, int* nlong, double* red,
-    int tid) {
-  using T = typename Mode::T;
+    uint64_t t, const uint64_t* __restrict__ off, const Id* __restrict__ idx, const uint32_t* __restrict__ tile_rb,
+    uint64_t E, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
+    typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const Gather& gat,
+    typename Mode::T* vals, uint32_t* longj, int* nlong, double* red, int tid) {
   constexpr int TE = kPullBlock * IPT;
   const uint64_t e0 = t * (uint64_t)TE;
-  const uint64_t e1 = valid ? (e0 + TE < E ? e0 + TE : E) : e0;
+  const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
+  const uint32_t rb = tile_rb[t];
+  const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
 
   // stage 1: neighbour ids (coalesced; NT = streamed with a non-temporal hint
   // so they do not push the gathered values out of the caches), then gathers
@@ -88,6 +186,14 @@ __device__ __forceinline__ void pull_tile_body(
     if (NT) nb[j] = i < n ? (uint32_t)__builtin_nontemporal_load(idx + e0 + i) : 0u;
     else nb[j] = i < n ? (uint32_t)idx[e0 + i] : 0u;
   }
+  // the rows' offsets are loaded in the same round as the gathers (issued
+  // first: a load still in flight on only one side of the branch below would
+  // make the code after it wait for every load)
+  // rows loaded ahead per thread: 256*Q covers the ~TE/3.3 rows of a segment
+  // plan's tile on R-MAT graphs; modes with pre data keep fewer in registers
+  constexpr int Q = HasPre<Mode>::value ? (IPT >= 8 ? 2 : 1) : (IPT >= 8 ? 4 : 2);
+  TileRows<Mode, Q> tr;
+  tr.load(off, rb, re, m, tid);
   // every gather is issued before any is used: a lane past the tile's end
   // reads entry e0 with neighbour 0 (nb = 0 there), a valid element, and
   // stores it to a slot of vals that no row reads (i < TE always).  Guarded
@@ -104,71 +210,7 @@ __device__ __forceinline__ void pull_tile_body(
   }
   if (tid == 0) *nlong = 0;
   __syncthreads();
-  pull_tile_rows<IPT>(t, valid, e0, e1, n, off, tile_rb, V, ntiles, carryP, carryO, part, m, vals, longj, nlong, red,
-                      tid);
-}
-
-// Stages 2 and 3 of a tile whose values vals[0..n) (entries e0..e1) are in
-// LDS and *nlong = 0, both visible to the whole group.
-template <int IPT, class Mode>
-__device__ __forceinline__ void pull_tile_rows(
-    uint64_t t, bool valid, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ tile_rb, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
-    typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const typename Mode::T* vals,
-    uint32_t* longj, int* nlong, double* red, int tid) {
-  using T = typename Mode::T;
-  // stage 2: rows that start in the tile, plus the row continuing into it
-  double acc0 = 0.0, acc1 = 0.0;
-  uint32_t rb = 0, nr = 0;
-  bool has_rc = false;
-  if (valid) {
-    rb = tile_rb[t];
-    const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
-    has_rc = rb > 0 && n > 0 && off[rb] > e0;
-    nr = (re - rb) + (has_rc ? 1u : 0u);
-  }
-  for (uint32_t j = tid; j < nr; j += kPullBlock) {
-    const uint32_t r = has_rc ? (j == 0 ? rb - 1 : rb + j - 1) : rb + j;
-    const uint64_t ob = off[r], oe = off[r + 1];
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
-    const int len = se > sb ? (int)(se - sb) : 0;
-    if (len > kLongRow) {
-      const int k = atomicAdd(nlong, 1);
-      longj[k] = j;
-      continue;
-    }
-    T s = (T)0;
-    const int base = (int)(sb - e0);
-    for (int q = 0; q < len; ++q) s += vals[base + q];
-    if (ob < e0) carryP[t] = s;
-    else if (oe > e1) carryO[t] = s;
-    else m.finalize(r, s, acc0, acc1);
-  }
-  __syncthreads();
-
-  // stage 3: long rows, one wave each
-  const int w = tid >> 6, l = tid & 63;
-  const int nl = *nlong;
-  for (int q = w; q < nl; q += kPullBlock / 64) {
-    const uint32_t j = longj[q];
-    const uint32_t r = has_rc ? (j == 0 ? rb - 1 : rb + j - 1) : rb + j;
-    const uint64_t ob = off[r], oe = off[r + 1];
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
-    const int base = (int)(sb - e0), len = (int)(se - sb);
-    T s = (T)0;
-    for (int i = l; i < len; i += 64) s += vals[base + i];
-    s = wave_sum(s);
-    if (l == 0) {
-      if (ob < e0) carryP[t] = s;
-      else if (oe > e1) carryO[t] = s;
-      else m.finalize(r, s, acc0, acc1);
-    }
-  }
-
-  group_sum2(acc0, acc1, red, tid);
-  if (valid && tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+  tile_rows<IPT>(t, e0, e1, n, off, rb, re, tr, carryP, carryO, part, m, vals, longj, nlong, red, tid);
 }
 
 template <class Mode>
@@ -198,44 +240,35 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
     t = (uint64_t)xmap[c] + (blockIdx.x >> 3);
     if (t >= xmap[c + 1]) return;
   }
-  pull_tile_body<IPT, NT>(t, true, off, idx, tile_rb, E, V, ntiles, carryP, carryO, part, m, ModeGather<Mode>{m},
-                          vals, longj, &nlong, red, (int)threadIdx.x);
+  pull_tile_body<IPT, NT>(t, off, idx, tile_rb, E, V, ntiles, carryP, carryO, part, m, ModeGather<Mode>{m}, vals,
+                          longj, &nlong, red, (int)threadIdx.x);
 }
 
 // The per-row pass over a partition's segment sums (rows = vertices, entries =
 // each row's segment ids).  A tile's segments form one run of consecutive ids
 // per (class, range) key; the tile first copies those runs from the mode's
-// gather array into LDS (coalesced: entry i of the stage belongs to the run k
-// with runO[k] <= i < runO[k+1], at segment runA[k] + i - runO[k]), then pulls
-// with each entry's value read from its LDS slot.  The entries, and so the
-// order of every row's sum, are those of the plain tile pull over the segment
-// ids.  The loads of the tile are issued in two rounds: (slots, run table, row
-// bounds), then (staged sums, row offsets, each row's own data through the
-// mode's pre(r)), so a tile waits for two memory latencies, not one per step.
-// Thread j takes row rb - 1 + j: row rb - 1 only when it continues into the
-// tile.  Modes provide pre(r) and finalize_pre(r, s, pre, acc0, acc1).
-template <class M, class = void>
-struct HasPre : std::false_type {};
-template <class M>
-struct HasPre<M, std::void_t<decltype(std::declval<const M&>().pre(0u))>> : std::true_type {};
-
-constexpr int kStagedRows = 4;  // rows per thread whose data is loaded with the stage
-
+// gather array into LDS in key order (coalesced: stage entry i belongs to the
+// run whose start is the last set bit of the tile's run-start bitmap at or
+// before i, and holds segment base[rank] + i), then pulls with each entry's
+// value read from its LDS slot.  The entries, and so the order of every row's
+// sum, are those of the plain tile pull over the segment ids.  The loads of
+// the tile are issued in two rounds: (slots, run bitmap and bases, row
+// bounds), then (staged sums, row offsets and data), so a tile waits for two
+// memory latencies.
 template <int IPT, class Mode>
 __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
     const uint64_t* __restrict__ off, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ tile_rb,
-    uint64_t E, uint32_t V, uint64_t ntiles, const uint32_t* __restrict__ runA, const uint16_t* __restrict__ runO,
+    uint64_t E, uint32_t V, uint64_t ntiles, const uint2* __restrict__ words, const uint32_t* __restrict__ base,
     int K, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
     Mode m) {
   using T = typename Mode::T;
-  using Pre = decltype(m.pre(0u));
   constexpr int TE = kPullBlock * IPT;
-  constexpr int Q = kStagedRows;
+  constexpr int W = TE / 32;
   static_assert((TE & (TE - 1)) == 0, "tile entries must be a power of two");
   __shared__ T stage[TE];
   __shared__ T vals[TE];
-  __shared__ uint32_t sA[kMaxKeys];
-  __shared__ uint32_t sO[2 * kMaxKeys + 1];
+  __shared__ uint2 sW[W];
+  __shared__ uint32_t sD[kMaxKeys];
   __shared__ uint32_t longj[TE / kLongRow + 2];
   __shared__ int nlong;
   __shared__ double red[8];
@@ -245,56 +278,34 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
   const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
 
-  // round 1: slots (the slot array is padded to whole tiles), run table, row bounds
+  // round 1: slots (the slot array is padded to whole tiles), run bitmap and
+  // bases, row bounds
   const uint32_t rb = tile_rb[t];
   const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
   uint32_t sl[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) sl[j] = slot[e0 + j * kPullBlock + tid];
-  // sO[k] for k in (K, 2K] is a sentinel above every entry index, so the
-  // search below needs no bound test
-  for (int k = tid; k <= 2 * K; k += kPullBlock) {
-    sO[k] = k <= K ? (uint32_t)runO[t * (uint64_t)(K + 1) + k] : 0xffffffffu;
-    if (k < K) sA[k] = runA[t * (uint64_t)K + k];
-  }
+  for (int w = tid; w < W; w += kPullBlock) sW[w] = words[t * (uint64_t)W + w];
+  for (int k = tid; k < K; k += kPullBlock) sD[k] = base[t * (uint64_t)K + k];
   if (tid == 0) nlong = 0;
   __syncthreads();
 
-  // round 2: staged sums, then each of the thread's first Q rows' offsets and data
+  // round 2: the staged sums, then the rows' offsets and data
   T sv[IPT];
   if (n > 0) {
-    // run of stage entry i: the largest k with sO[k] <= i, for all IPT entries
-    // at once (lanes past n take entry 0)
-    uint32_t ii[IPT];
-    uint32_t lo[IPT];
+    // run rank of stage entry i: set bits at or before i, minus one (entry 0
+    // always starts a run; lanes past n take entry 0)
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const int i = j * kPullBlock + tid;
-      ii[j] = i < n ? (uint32_t)i : 0u;
-      lo[j] = 0;
+      const uint32_t ii = i < n ? (uint32_t)i : 0u;
+      const uint2 wd = sW[ii >> 5];
+      const uint32_t r = wd.y + (uint32_t)__popc(wd.x & ((2u << (ii & 31)) - 1u)) - 1u;
+      sv[j] = m.gather(0, sD[r] + ii);
     }
-    int b = 1;
-    while (2 * b < K) b *= 2;
-    for (; b > 0; b >>= 1) {
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t c = lo[j] + (uint32_t)b;
-        lo[j] = sO[c] <= ii[j] ? c : lo[j];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) sv[j] = m.gather(0, sA[lo[j]] + (ii[j] - sO[lo[j]]));
   }
-  uint64_t rob[Q], roe[Q];
-  Pre rpre[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int64_t r = (int64_t)rb - 1 + tid + q * kPullBlock;
-    const uint32_t rs = (r >= 0 && r < (int64_t)re) ? (uint32_t)r : 0u;
-    rob[q] = off[rs];
-    roe[q] = off[rs + 1];
-    rpre[q] = m.pre(rs);
-  }
+  TileRows<Mode, 4> tr;
+  tr.load(off, rb, re, m, tid);
   if (n > 0) {
 #pragma unroll
     for (int j = 0; j < IPT; ++j) stage[j * kPullBlock + tid] = sv[j];
@@ -308,60 +319,7 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
     for (int j = 0; j < IPT; ++j) vals[j * kPullBlock + tid] = gv[j];
   }
   __syncthreads();
-
-  // rows rb - 1 .. re - 1; a row longer than kLongRow in-tile entries goes to a wave
-  const uint32_t nrows = re - rb + 1;
-  // row rb - 1 is thread 0's first row: roe[0] = off[rb] there
-  const bool has_rc = rb > 0 && n > 0 && roe[0] > e0;
-  double acc0 = 0.0, acc1 = 0.0;
-  auto row = [&](uint32_t j, uint64_t ob, uint64_t oe, const Pre& pr) {
-    const uint32_t r = rb - 1 + j;
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
-    const int len = se > sb ? (int)(se - sb) : 0;
-    if (len > kLongRow) {
-      longj[atomicAdd(&nlong, 1)] = j;
-      return;
-    }
-    T s = (T)0;
-    const int base = (int)(sb - e0);
-    for (int q = 0; q < len; ++q) s += vals[base + q];
-    if (ob < e0) carryP[t] = s;
-    else if (oe > e1) carryO[t] = s;
-    else m.finalize_pre(r, s, pr, acc0, acc1);
-  };
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const uint32_t j = (uint32_t)(tid + q * kPullBlock);
-    if (j < nrows && (j > 0 || has_rc)) row(j, rob[q], roe[q], rpre[q]);
-  }
-  for (uint32_t j = (uint32_t)(tid + Q * kPullBlock); j < nrows; j += kPullBlock) {
-    const uint32_t r = rb - 1 + j;
-    row(j, off[r], off[r + 1], m.pre(r));
-  }
-  __syncthreads();
-
-  // long rows, one wave each
-  const int w = tid >> 6, l = tid & 63;
-  const int nl = nlong;
-  for (int q = w; q < nl; q += kPullBlock / 64) {
-    const uint32_t r = rb - 1 + longj[q];
-    const uint64_t ob = off[r], oe = off[r + 1];
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
-    const int base = (int)(sb - e0), len = (int)(se - sb);
-    T s = (T)0;
-    for (int i = l; i < len; i += 64) s += vals[base + i];
-    s = wave_sum(s);
-    if (l == 0) {
-      if (ob < e0) carryP[t] = s;
-      else if (oe > e1) carryO[t] = s;
-      else m.finalize(r, s, acc0, acc1);
-    }
-  }
-
-  group_sum2(acc0, acc1, red, tid);
-  if (tid == 0) { part[2 * t] = acc0; part[2 * t + 1] = acc1; }
+  tile_rows<IPT>(t, e0, e1, n, off, rb, re, tr, carryP, carryO, part, m, vals, longj, &nlong, red, tid);
 }
 
 // Fix-up: tile t owns the row that starts in it and runs past its end.
@@ -419,8 +377,8 @@ struct PullPlan {
   const uint32_t* xmap = nullptr;  // class tile starts (source-range partitions) or nullptr
   // per-row pass over segment sums staged in LDS (k_pull_staged), or nullptr
   const uint16_t* slot = nullptr;
-  const uint32_t* runA = nullptr;
-  const uint16_t* runO = nullptr;
+  const uint2* words = nullptr;
+  const uint32_t* base = nullptr;
   int K = 0;
   unsigned grid = 0;               // workgroups of the tile kernel when xmap is set
   const char* tag = "pull_tile";   // profiling names of the two kernels
@@ -448,14 +406,10 @@ void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream
   {
     ProfScope ps(p.tag);
     const unsigned grid = p.xmap ? p.grid : (unsigned)p.ntiles;
-    if (p.slot) {
-      if constexpr (HasPre<Mode>::value)
-        hipLaunchKernelGGL((k_pull_staged<IPT, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off,
-                           p.slot, p.tile_rb, p.E, p.V, p.ntiles, p.runA, p.runO, p.K, (T*)p.carryP, (T*)p.carryO,
-                           p.part, m);
-      else
-        throw Error(GRE_DEV_NULL, "this pull has no staged per-row form");
-    } else if (p.nt)
+    if (p.slot)
+      hipLaunchKernelGGL((k_pull_staged<IPT, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off, p.slot,
+                         p.tile_rb, p.E, p.V, p.ntiles, p.words, p.base, p.K, (T*)p.carryP, (T*)p.carryO, p.part, m);
+    else if (p.nt)
       hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
                          p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
     else

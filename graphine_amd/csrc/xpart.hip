@@ -277,6 +277,29 @@ __global__ void k_xslot(const uint32_t* __restrict__ pair_list, uint64_t nseg, u
   }
 }
 
+// Per tile: the run-start bitmap with its running counts, and each non-empty
+// run's segment base (first segment minus stage offset), runs in key order
+__global__ void k_xstage(const uint32_t* __restrict__ runA, const uint16_t* __restrict__ runO, int K,
+                         uint64_t ntiles, uint32_t W, uint2* __restrict__ words, uint32_t* __restrict__ base) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntiles; t += (uint64_t)gridDim.x * blockDim.x) {
+    uint2* wt = words + t * W;
+    uint32_t m = 0;
+    for (int k = 0; k < K; ++k) {
+      const uint32_t o = runO[t * (K + 1) + k], o1 = runO[t * (K + 1) + k + 1];
+      if (o1 > o) {
+        wt[o >> 5].x |= 1u << (o & 31);
+        base[t * K + m] = runA[t * K + k] - o;
+        ++m;
+      }
+    }
+    uint32_t acc = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      wt[w].y = acc;
+      acc += __popc(wt[w].x);
+    }
+  }
+}
+
 unsigned bits_of(uint64_t n) {
   unsigned b = 1;
   while (b < 32 && (1ULL << b) < n) ++b;
@@ -448,10 +471,17 @@ const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hi
   build_tile_rows(xp.pair_off.as<uint64_t>(), nV, nt, te, T.rb.as<uint32_t>(), s);
   // staged runs: LDS offsets are 16-bit, and a tile's keys fit its LDS table
   GRE_CHECK(te <= 65535 && xp.K > 0 && xp.K <= kMaxKeys, GRE_DEV_NULL, "pair tiles too large to stage");
+  GRE_CHECK(te % 32 == 0, GRE_DEV_NULL, "pair tile entries must be a multiple of 32");
   const int K = xp.K;
   const uint64_t ns = xp.nseg;
-  T.runA.alloc(4 * (nt + 1) * (uint64_t)K);
-  T.runO.alloc(2 * nt * (uint64_t)(K + 1));
+  const uint32_t W = te / 32;
+  DevBuf runA, runO;
+  runA.alloc(4 * (nt + 1) * (uint64_t)K);
+  runO.alloc(2 * nt * (uint64_t)(K + 1));
+  T.words.alloc(8 * nt * (uint64_t)W);
+  T.base.alloc(4 * nt * (uint64_t)K);
+  GRE_HIP(hipMemsetAsync(T.words.p, 0, 8 * nt * (uint64_t)W, s));
+  GRE_HIP(hipMemsetAsync(T.base.p, 0, 4 * nt * (uint64_t)K, s));
   // padded to whole tiles, so the row pass loads every slot of a tile unguarded
   T.slot.alloc(2 * nt * (uint64_t)te);
   GRE_HIP(hipMemsetAsync(T.slot.p, 0, 2 * nt * (uint64_t)te, s));
@@ -461,13 +491,16 @@ const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hi
     hipLaunchKernelGGL(k_xpos, dim3(gcap(ns)), dim3(kB), 0, s, xp.pair_list.as<uint32_t>(), ns, pos.as<uint32_t>());
     GRE_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_xruna, dim3(gcap((nt + 1) * K)), dim3(kB), 0, s, xp.kseg.as<uint32_t>(), K,
-                       pos.as<uint32_t>(), nt, te, T.runA.as<uint32_t>());
+                       pos.as<uint32_t>(), nt, te, runA.as<uint32_t>());
     GRE_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_xruno, dim3(gcap(nt)), dim3(kB), 0, s, T.runA.as<uint32_t>(), K, nt, T.runO.as<uint16_t>());
+    hipLaunchKernelGGL(k_xruno, dim3(gcap(nt)), dim3(kB), 0, s, runA.as<uint32_t>(), K, nt, runO.as<uint16_t>());
     GRE_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_xslot, dim3(gcap(ns)), dim3(kB), 0, s, xp.pair_list.as<uint32_t>(), ns, te,
-                       xp.kseg.as<uint32_t>(), K, T.runA.as<uint32_t>(), T.runO.as<uint16_t>(),
+                       xp.kseg.as<uint32_t>(), K, runA.as<uint32_t>(), runO.as<uint16_t>(),
                        T.slot.as<uint16_t>());
+    GRE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_xstage, dim3(gcap(nt)), dim3(kB), 0, s, runA.as<uint32_t>(), runO.as<uint16_t>(), K, nt, W,
+                       T.words.as<uint2>(), T.base.as<uint32_t>());
     GRE_HIP(hipGetLastError());
     GRE_HIP(hipStreamSynchronize(s));
   }
