@@ -235,7 +235,8 @@ def main():
     iter_s = avg_tile_s + avg_fix_s + avg_cmb_s
     achieved = bytes_iter / iter_s / 1e9
     sched, tile_items = pr.schedule()
-    kern = "k_pull_tile + k_pull_fixup" + (" (segments, then per-row)" if n_cmb else "")
+    kern = ("k_pull_tile + k_pull_fixup (segments), then k_pull_staged + k_pull_fixup (per row)" if n_cmb
+            else "k_pull_tile + k_pull_fixup")
     traffic, traffic_sched = pmc_traffic(S, sched)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_schedule": traffic_sched,
