@@ -173,7 +173,8 @@ __global__ void k_gather_w(const uint32_t* __restrict__ w, const uint64_t* __res
 }
 
 // tile_rb[t] = number of rows r < nV with off[r] < t*te
-__global__ void k_tiles(const uint64_t* __restrict__ off, uint32_t nV, uint64_t ntiles, uint32_t te,
+template <class Off>
+__global__ void k_tiles(const Off* __restrict__ off, uint32_t nV, uint64_t ntiles, uint32_t te,
                         uint32_t* __restrict__ rb) {
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= ntiles;
        t += (uint64_t)gridDim.x * blockDim.x) {
@@ -372,7 +373,13 @@ const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntile
 
 void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_tiles, dim3(grid_cap(ntiles + 1)), dim3(kBlock), 0, s, off, nrows, ntiles, te, rb);
+  hipLaunchKernelGGL(k_tiles<uint64_t>, dim3(grid_cap(ntiles + 1)), dim3(kBlock), 0, s, off, nrows, ntiles, te, rb);
+  GRE_HIP(hipGetLastError());
+}
+
+void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_tiles<uint32_t>, dim3(grid_cap(ntiles + 1)), dim3(kBlock), 0, s, off, nrows, ntiles, te, rb);
   GRE_HIP(hipGetLastError());
 }
 

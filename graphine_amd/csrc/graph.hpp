@@ -22,8 +22,8 @@ struct XPart {
   uint32_t H = 0;    // sources [0, H) form one range shared by all XCDs (cut by rows)
   uint64_t nseg = 0;
   DevBuf idx;        // uint32 [E]   compact source ids, grouped by (class, range, row)
-  DevBuf seg_off;    // uint64 [nseg+1]
-  DevBuf pair_off;   // uint64 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
+  DevBuf seg_off;    // uint32 [nseg+1] (E < 2^32)
+  DevBuf pair_off;   // uint32 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
   DevBuf pair_list;  // uint32 [nseg] segment ids, by row then class-major range
   uint64_t class_start[9] = {0};  // first entry of each class; class_start[8] = E
   int K = 0;         // (class, range) keys: 8 * (1 + R/8)
@@ -95,6 +95,8 @@ const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntile
 void graph_compact_sources(gre_graph* g);
 // tile_rb over an arbitrary offset array of nrows rows (see graph_tiles).
 void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
+                     hipStream_t s);
+void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
                      hipStream_t s);
 // The source partition of g: the H hottest compact sources as one shared range
 // and the rest in R tail ranges (R in {8, 16, ..., 256}); built once and cached.

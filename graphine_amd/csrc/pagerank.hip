@@ -389,6 +389,7 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters) {
                              ptr(P->contrib[0].p), ptr(P->contrib[1].p), ptr(P->wcsc.p), ptr(P->xsum.p)};
   for (const PullPlan* pl : {&P->plan_csc, &P->plan_x, &P->plan_px}) {
     k.push_back(ptr(pl->off));
+    k.push_back(ptr(pl->off32));
     k.push_back(ptr(pl->idx));
     k.push_back(ptr(pl->tile_rb));
     k.push_back(ptr(pl->carryP));

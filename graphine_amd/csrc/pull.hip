@@ -55,6 +55,7 @@ void reduce_pairs(const double* a, uint64_t na, const double* b, uint64_t nb, do
 void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbytes) {
   const uint32_t te = (uint32_t)(kPullBlock * ipt);
   p.off = (csc ? g->coff : g->off).as<uint64_t>();
+  p.off32 = nullptr;
   p.idx = (csc ? g->cidx : g->idx).as<uint32_t>();
   p.tile_rb = graph_tiles(g, csc, te, &p.ntiles);
   p.E = g->nE;
@@ -69,7 +70,8 @@ void PullBuffers::make(PullPlan& p, gre_graph* g, bool csc, int ipt, size_t tbyt
 void PullBuffers::make(PullPlan& p, XPart& xp, int ipt, size_t tbytes) {
   const uint32_t te = (uint32_t)(kPullBlock * ipt);
   const XPart::Tiles& T = xpart_tiles(xp, te, stream());
-  p.off = xp.seg_off.as<uint64_t>();
+  p.off = nullptr;
+  p.off32 = xp.seg_off.as<uint32_t>();
   p.idx = xp.idx.as<uint32_t>();
   p.tile_rb = T.rb.as<uint32_t>();
   p.ntiles = T.ntiles;
@@ -85,7 +87,8 @@ void PullBuffers::make(PullPlan& p, XPart& xp, int ipt, size_t tbytes) {
 void PullBuffers::make_pairs(PullPlan& p, XPart& xp, uint32_t nV, int ipt, size_t tbytes) {
   const uint32_t te = (uint32_t)(kPullBlock * ipt);
   const XPart::PairTiles& T = xpart_pair_tiles(xp, nV, te, stream());
-  p.off = xp.pair_off.as<uint64_t>();
+  p.off = nullptr;
+  p.off32 = xp.pair_off.as<uint32_t>();
   p.idx = xp.pair_list.as<uint32_t>();
   p.slot = T.slot.as<uint16_t>();
   p.words = T.words.as<uint2>();

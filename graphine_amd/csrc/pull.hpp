@@ -79,13 +79,13 @@ __device__ inline void group_sum2(double& a, double& b, double* red, int tid) {
 // thread have their offsets (and the mode's pre data) loaded together with the
 // tile's gathers, so the tile waits for one memory latency less per step; the
 // rest are loaded where they are walked.  Q trades that against registers.
-template <class Mode, int Q>
+template <class Mode, int Q, class Off>
 struct TileRows {
   using Pre = decltype(row_pre(std::declval<const Mode&>(), 0u));
-  uint64_t ob[Q], oe[Q];
+  Off ob[Q], oe[Q];
   Pre pre[Q];
   // issued without waiting: lanes past the rows load row 0
-  __device__ __forceinline__ void load(const uint64_t* __restrict__ off, uint32_t rb, uint32_t re, const Mode& m,
+  __device__ __forceinline__ void load(const Off* __restrict__ off, uint32_t rb, uint32_t re, const Mode& m,
                                        int tid) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -100,10 +100,10 @@ struct TileRows {
 
 // Stages 2 and 3 of tile t (entries e0..e1, n of them) whose values vals[0..n)
 // are in LDS and *nlong = 0, both visible to the whole group; rows as above.
-template <int IPT, class Mode, int Q>
+template <int IPT, class Mode, int Q, class Off>
 __device__ __forceinline__ void tile_rows(
-    uint64_t t, uint64_t e0, uint64_t e1, int n, const uint64_t* __restrict__ off, uint32_t rb, uint32_t re,
-    const TileRows<Mode, Q>& tr, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
+    uint64_t t, Off e0, Off e1, int n, const Off* __restrict__ off, uint32_t rb, uint32_t re,
+    const TileRows<Mode, Q, Off>& tr, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
     double* __restrict__ part, const Mode& m, const typename Mode::T* vals, uint32_t* longj, int* nlong, double* red,
     int tid) {
   using T = typename Mode::T;
@@ -111,10 +111,10 @@ __device__ __forceinline__ void tile_rows(
   // row rb - 1 is thread 0's first row: tr.oe[0] = off[rb] there
   const bool has_rc = rb > 0 && n > 0 && tr.oe[0] > e0;
   double acc0 = 0.0, acc1 = 0.0;
-  auto row = [&](uint32_t j, uint64_t ob, uint64_t oe, const typename TileRows<Mode, Q>::Pre& q) {
+  auto row = [&](uint32_t j, Off ob, Off oe, const typename TileRows<Mode, Q, Off>::Pre& q) {
     const uint32_t r = rb - 1 + j;
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
+    const Off sb = ob > e0 ? ob : e0;
+    const Off se = oe < e1 ? oe : e1;
     const int len = se > sb ? (int)(se - sb) : 0;
     if (len > kLongRow) {
       longj[atomicAdd(nlong, 1)] = j;
@@ -143,9 +143,9 @@ __device__ __forceinline__ void tile_rows(
   const int nl = *nlong;
   for (int q = w; q < nl; q += kPullBlock / 64) {
     const uint32_t r = rb - 1 + longj[q];
-    const uint64_t ob = off[r], oe = off[r + 1];
-    const uint64_t sb = ob > e0 ? ob : e0;
-    const uint64_t se = oe < e1 ? oe : e1;
+    const Off ob = off[r], oe = off[r + 1];
+    const Off sb = ob > e0 ? ob : e0;
+    const Off se = oe < e1 ? oe : e1;
     const int base = (int)(sb - e0), len = (int)(se - sb);
     T s = (T)0;
     for (int i = l; i < len; i += 64) s += vals[base + i];
@@ -163,16 +163,18 @@ __device__ __forceinline__ void tile_rows(
 
 // One tile of TE = 256*IPT entries, processed by one 256-thread group (tid in
 // [0, 256)).  vals/longj/nlong/red are the group's LDS.  gat(e, u) gives the
-// value gathered for entry e with neighbour u.
-template <int IPT, bool NT, class Mode, class Gather, class Id>
+// value gathered for entry e with neighbour u.  Off is the row offset type:
+// uint32 where every offset fits (fewer bytes and 32-bit row arithmetic).
+template <int IPT, bool NT, class Mode, class Gather, class Id, class Off>
 __device__ __forceinline__ void pull_tile_body(
-    uint64_t t, const uint64_t* __restrict__ off, const Id* __restrict__ idx, const uint32_t* __restrict__ tile_rb,
+    uint64_t t, const Off* __restrict__ off, const Id* __restrict__ idx, const uint32_t* __restrict__ tile_rb,
     uint64_t E, uint32_t V, uint64_t ntiles, typename Mode::T* __restrict__ carryP,
     typename Mode::T* __restrict__ carryO, double* __restrict__ part, const Mode& m, const Gather& gat,
     typename Mode::T* vals, uint32_t* longj, int* nlong, double* red, int tid) {
   constexpr int TE = kPullBlock * IPT;
-  const uint64_t e0 = t * (uint64_t)TE;
-  const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
+  const uint64_t e0l = t * (uint64_t)TE;
+  const Off e0 = (Off)e0l;
+  const Off e1 = (Off)(e0l + TE < E ? e0l + TE : E);
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
   const uint32_t rb = tile_rb[t];
   const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
@@ -192,7 +194,7 @@ __device__ __forceinline__ void pull_tile_body(
   // rows loaded ahead per thread: 256*Q covers the ~TE/3.3 rows of a segment
   // plan's tile on R-MAT graphs; modes with pre data keep fewer in registers
   constexpr int Q = HasPre<Mode>::value ? (IPT >= 8 ? 2 : 1) : (IPT >= 8 ? 4 : 2);
-  TileRows<Mode, Q> tr;
+  TileRows<Mode, Q, Off> tr;
   tr.load(off, rb, re, m, tid);
   // every gather is issued before any is used: a lane past the tile's end
   // reads entry e0 with neighbour 0 (nb = 0 there), a valid element, and
@@ -203,7 +205,7 @@ __device__ __forceinline__ void pull_tile_body(
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const int i = j * kPullBlock + tid;
-      gv[j] = gat(e0 + (i < n ? (uint64_t)i : 0ull), nb[j]);
+      gv[j] = gat(e0l + (i < n ? (uint64_t)i : 0ull), nb[j]);
     }
 #pragma unroll
     for (int j = 0; j < IPT; ++j) vals[j * kPullBlock + tid] = gv[j];
@@ -219,9 +221,9 @@ struct ModeGather {
   __device__ typename Mode::T operator()(uint64_t e, uint32_t u) const { return m.gather(e, u); }
 };
 
-template <int IPT, bool NT, class Mode>
+template <int IPT, bool NT, class Mode, class Off>
 __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+    const Off* __restrict__ off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V, uint64_t ntiles,
     typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO,
     double* __restrict__ part, const uint32_t* __restrict__ xmap, Mode m) {
@@ -255,9 +257,9 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_tile(
 // the tile are issued in two rounds: (slots, run bitmap and bases, row
 // bounds), then (staged sums, row offsets and data), so a tile waits for two
 // memory latencies.
-template <int IPT, class Mode>
+template <int IPT, class Mode, class Off>
 __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
-    const uint64_t* __restrict__ off, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ tile_rb,
+    const Off* __restrict__ off, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ tile_rb,
     uint64_t E, uint32_t V, uint64_t ntiles, const uint2* __restrict__ words, const uint32_t* __restrict__ base,
     int K, typename Mode::T* __restrict__ carryP, typename Mode::T* __restrict__ carryO, double* __restrict__ part,
     Mode m) {
@@ -274,8 +276,9 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
   __shared__ double red[8];
   const uint64_t t = blockIdx.x;
   const int tid = (int)threadIdx.x;
-  const uint64_t e0 = t * (uint64_t)TE;
-  const uint64_t e1 = e0 + TE < E ? e0 + TE : E;
+  const uint64_t e0l = t * (uint64_t)TE;
+  const Off e0 = (Off)e0l;
+  const Off e1 = (Off)(e0l + TE < E ? e0l + TE : E);
   const int n = e1 > e0 ? (int)(e1 - e0) : 0;
 
   // round 1: slots (the slot array is padded to whole tiles), run bitmap and
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
   const uint32_t re = (t + 1 == ntiles) ? V : tile_rb[t + 1];
   uint32_t sl[IPT];
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) sl[j] = slot[e0 + j * kPullBlock + tid];
+  for (int j = 0; j < IPT; ++j) sl[j] = slot[e0l + j * kPullBlock + tid];
   for (int w = tid; w < W; w += kPullBlock) sW[w] = words[t * (uint64_t)W + w];
   for (int k = tid; k < K; k += kPullBlock) sD[k] = base[t * (uint64_t)K + k];
   if (tid == 0) nlong = 0;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
       sv[j] = m.gather(0, sD[r] + ii);
     }
   }
-  TileRows<Mode, 4> tr;
+  TileRows<Mode, 4, Off> tr;
   tr.load(off, rb, re, m, tid);
   if (n > 0) {
 #pragma unroll
@@ -323,9 +326,9 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
 }
 
 // Fix-up: tile t owns the row that starts in it and runs past its end.
-template <int IPT, class Mode>
+template <int IPT, class Mode, class Off>
 __global__ __launch_bounds__(kPullBlock) void k_pull_fixup(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
+    const Off* __restrict__ off, const uint32_t* __restrict__ tile_rb, uint64_t E, uint32_t V,
     uint64_t ntiles, const typename Mode::T* __restrict__ carryP,
     const typename Mode::T* __restrict__ carryO, double* __restrict__ part2, Mode m) {
   using T = typename Mode::T;
@@ -363,6 +366,7 @@ void reduce_pairs(const double* a, uint64_t na, const double* b, uint64_t nb, do
 // Host driver: one pull pass (tile + fix-up + reduce) over (off, idx).
 struct PullPlan {
   const uint64_t* off = nullptr;
+  const uint32_t* off32 = nullptr;  // used instead of off when set (every offset < 2^32)
   const uint32_t* idx = nullptr;
   const uint32_t* tile_rb = nullptr;
   uint64_t E = 0, ntiles = 0;
@@ -393,28 +397,43 @@ void pull_finish(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t
   const unsigned nfb = (unsigned)((p.ntiles + kPullBlock - 1) / kPullBlock);
   {
     ProfScope ps(p.fix_tag);
-    hipLaunchKernelGGL((k_pull_fixup<IPT, Mode>), dim3(nfb), dim3(kPullBlock), 0, s, p.off, p.tile_rb, p.E, p.V,
-                       p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
+    if (p.off32)
+      hipLaunchKernelGGL((k_pull_fixup<IPT, Mode, uint32_t>), dim3(nfb), dim3(kPullBlock), 0, s, p.off32, p.tile_rb,
+                         p.E, p.V, p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
+    else
+      hipLaunchKernelGGL((k_pull_fixup<IPT, Mode, uint64_t>), dim3(nfb), dim3(kPullBlock), 0, s, p.off, p.tile_rb,
+                         p.E, p.V, p.ntiles, (const T*)p.carryP, (const T*)p.carryO, p.part2, m);
   }
   // scal_out == nullptr: the mode's finalize accumulates nothing (segment sums)
   if (scal_out) reduce_pairs(p.part, p.ntiles, p.part2, (uint64_t)nfb, scal_out, p.part3, s);
 }
 
+template <int IPT, class Mode, class Off>
+void pull_launch(const PullPlan& p, const Off* off, const Mode& m, hipStream_t s) {
+  using T = typename Mode::T;
+  const unsigned grid = p.xmap ? p.grid : (unsigned)p.ntiles;
+  if (p.slot) {
+    if constexpr (IPT == 8)
+      hipLaunchKernelGGL((k_pull_staged<IPT, Mode, Off>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, off,
+                         p.slot, p.tile_rb, p.E, p.V, p.ntiles, p.words, p.base, p.K, (T*)p.carryP, (T*)p.carryO,
+                         p.part, m);
+    else
+      throw Error(GRE_DEV_NULL, "the staged per-row pass runs with 8 entries per thread");
+  } else if (p.nt) {
+    hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode, Off>), dim3(grid), dim3(kPullBlock), 0, s, off, p.idx,
+                       p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
+  } else {
+    hipLaunchKernelGGL((k_pull_tile<IPT, false, Mode, Off>), dim3(grid), dim3(kPullBlock), 0, s, off, p.idx,
+                       p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
+  }
+}
+
 template <int IPT, class Mode>
 void pull_pass_ipt(const PullPlan& p, const Mode& m, double* scal_out, hipStream_t s) {
-  using T = typename Mode::T;
   {
     ProfScope ps(p.tag);
-    const unsigned grid = p.xmap ? p.grid : (unsigned)p.ntiles;
-    if (p.slot)
-      hipLaunchKernelGGL((k_pull_staged<IPT, Mode>), dim3((unsigned)p.ntiles), dim3(kPullBlock), 0, s, p.off, p.slot,
-                         p.tile_rb, p.E, p.V, p.ntiles, p.words, p.base, p.K, (T*)p.carryP, (T*)p.carryO, p.part, m);
-    else if (p.nt)
-      hipLaunchKernelGGL((k_pull_tile<IPT, true, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
-                         p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
-    else
-      hipLaunchKernelGGL((k_pull_tile<IPT, false, Mode>), dim3(grid), dim3(kPullBlock), 0, s, p.off, p.idx,
-                         p.tile_rb, p.E, p.V, p.ntiles, (T*)p.carryP, (T*)p.carryO, p.part, p.xmap, m);
+    if (p.off32) pull_launch<IPT>(p, p.off32, m, s);
+    else pull_launch<IPT>(p, p.off, m, s);
   }
   pull_finish<IPT>(p, m, scal_out, s);
 }

@@ -163,12 +163,12 @@ __global__ void k_xflags(const uint16_t* __restrict__ key, const uint64_t* __res
 }
 
 __global__ void k_xsegs(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ segid,
-                        const uint64_t* __restrict__ val, uint64_t E, uint64_t* __restrict__ seg_off,
+                        const uint64_t* __restrict__ val, uint64_t E, uint32_t* __restrict__ seg_off,
                         uint32_t* __restrict__ seg_row) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
     if (flag[i]) {
       const uint32_t s = segid[i] - 1;
-      seg_off[s] = i;
+      seg_off[s] = (uint32_t)i;
       seg_row[s] = (uint32_t)(val[i] >> 32);
     }
   }
@@ -413,13 +413,13 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     GRE_HIP(hipMemcpyAsync(&ns, segid.as<uint32_t>() + (E - 1), 4, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
     xp->nseg = ns;
-    xp->seg_off.alloc(8 * ((size_t)ns + 1));
+    xp->seg_off.alloc(4 * ((size_t)ns + 1));
     seg_row.alloc(4 * (size_t)ns);
     hipLaunchKernelGGL(k_xsegs, dim3(gcap(E)), dim3(kB), 0, s, flag.as<uint32_t>(), segid.as<uint32_t>(),
-                       val_s.as<uint64_t>(), E, xp->seg_off.as<uint64_t>(), seg_row.as<uint32_t>());
+                       val_s.as<uint64_t>(), E, xp->seg_off.as<uint32_t>(), seg_row.as<uint32_t>());
     GRE_HIP(hipGetLastError());
-    const uint64_t eh = E;
-    GRE_HIP(hipMemcpyAsync(xp->seg_off.as<uint64_t>() + ns, &eh, 8, hipMemcpyHostToDevice, s));
+    const uint32_t eh = (uint32_t)E;
+    GRE_HIP(hipMemcpyAsync(xp->seg_off.as<uint32_t>() + ns, &eh, 4, hipMemcpyHostToDevice, s));
     xp->K = 8 * per;
     xp->kseg.alloc(4 * (size_t)(xp->K + 1));
     hipLaunchKernelGGL(k_xkeyseg, dim3(1), dim3(kB), 0, s, key_s.as<uint16_t>(), segid.as<uint32_t>(), E, xp->K, ns,
@@ -438,10 +438,13 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     GRE_HIP(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)nV + 1), s));
     hipLaunchKernelGGL(k_xpcount, dim3(gcap(nseg)), dim3(kB), 0, s, seg_row.as<uint32_t>(), nseg, cnt.as<uint32_t>());
     GRE_HIP(hipGetLastError());
-    xp->pair_off.alloc(8 * ((size_t)nV + 1));
-    scan_excl_u32_to_u64(cnt.as<uint32_t>(), xp->pair_off.as<uint64_t>(), (size_t)nV + 1, s);
+    xp->pair_off.alloc(4 * ((size_t)nV + 1));
     size_t tb = 0;
-    void* tmp = nullptr;
+    GRE_HIP(rocprim::exclusive_scan(nullptr, tb, cnt.as<uint32_t>(), xp->pair_off.as<uint32_t>(), 0u,
+                                    (size_t)nV + 1, rocprim::plus<uint32_t>(), s));
+    void* tmp = scratch(2, tb);
+    GRE_HIP(rocprim::exclusive_scan(tmp, tb, cnt.as<uint32_t>(), xp->pair_off.as<uint32_t>(), 0u, (size_t)nV + 1,
+                                    rocprim::plus<uint32_t>(), s));
     ids.alloc(4 * nseg);
     rows_s.alloc(4 * nseg);
     xp->pair_list.alloc(4 * nseg);
@@ -468,7 +471,7 @@ const XPart::PairTiles& xpart_pair_tiles(XPart& xp, uint32_t nV, uint32_t te, hi
   if (nt == 0) nt = 1;
   T.ntiles = nt;
   T.rb.alloc(4 * (nt + 1));
-  build_tile_rows(xp.pair_off.as<uint64_t>(), nV, nt, te, T.rb.as<uint32_t>(), s);
+  build_tile_rows(xp.pair_off.as<uint32_t>(), nV, nt, te, T.rb.as<uint32_t>(), s);
   // staged runs: LDS offsets are 16-bit, and a tile's keys fit its LDS table
   GRE_CHECK(te <= 65535 && xp.K > 0 && xp.K <= kMaxKeys, GRE_DEV_NULL, "pair tiles too large to stage");
   GRE_CHECK(te % 32 == 0, GRE_DEV_NULL, "pair tile entries must be a multiple of 32");
@@ -516,7 +519,7 @@ const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s) {
   if (nt == 0) nt = 1;
   T.ntiles = nt;
   T.rb.alloc(4 * (nt + 1));
-  build_tile_rows(xp.seg_off.as<uint64_t>(), (uint32_t)xp.nseg, nt, te, T.rb.as<uint32_t>(), s);
+  build_tile_rows(xp.seg_off.as<uint32_t>(), (uint32_t)xp.nseg, nt, te, T.rb.as<uint32_t>(), s);
   // class c runs tiles [xm[c], xm[c+1]) on the workgroups with id % 8 == c
   uint32_t xm[9];
   for (int c = 0; c < 8; ++c) xm[c] = (uint32_t)std::min<uint64_t>(xp.class_start[c] / te, nt);
