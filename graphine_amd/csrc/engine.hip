@@ -994,7 +994,7 @@ int64_t engine_step(gre_engine* e) {
     if (!e->xp) {
       graph_compact_sources(g);
       e->xp = graph_xpart(g, kEngRanges, kEngShared);
-      e->seg_buf.make(e->seg_plan, *e->xp, 4, sizeof(double));
+      e->seg_buf.make(e->seg_plan, *e->xp, 8, sizeof(double));
       e->seg_plan.nt = true;
       e->seg_plan.tag = "pull_compute";
       e->seg_plan.fix_tag = "pull_fixup";
