@@ -86,8 +86,7 @@ __global__ __launch_bounds__(kB) void k_td(const uint64_t* __restrict__ off, con
       mydeg += off[v + 1] - off[v];
     }
   }
-  mydeg = wave_sum(mydeg);
-  if ((threadIdx.x & 63) == 0 && mydeg) atomicAdd(ndeg, (unsigned long long)mydeg);
+  block_add(static_cast<unsigned long long>(mydeg), ndeg);
 }
 
 // bottom-up: every unvisited vertex looks for a parent in the frontier bitmap
@@ -121,12 +120,7 @@ __global__ __launch_bounds__(kB) void k_bu(const uint64_t* __restrict__ coff, co
       dsum += off[v + 1] - off[v];
     }
   }
-  cnt = wave_sum(cnt);
-  dsum = wave_sum(dsum);
-  if ((threadIdx.x & 63) == 0) {
-    if (cnt) atomicAdd(ncount, cnt);
-    if (dsum) atomicAdd(ndeg, (unsigned long long)dsum);
-  }
+  block_add2(cnt, ncount, static_cast<unsigned long long>(dsum), ndeg);
 }
 
 // bitmap -> queue (order within the queue does not matter to BFS levels)
@@ -145,23 +139,8 @@ __global__ void k_bits_to_queue(const uint32_t* __restrict__ bits, uint32_t nwor
   }
 }
 
-__global__ void k_reached_edges(const int32_t* __restrict__ level, const uint64_t* __restrict__ off, uint32_t nV,
-                                unsigned long long* __restrict__ out, int32_t* __restrict__ depth) {
-  uint64_t s = 0;
-  int32_t dmax = -1;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    if (level[v] >= 0) { s += off[v + 1] - off[v]; dmax = level[v] > dmax ? level[v] : dmax; }
-  }
-  s = wave_sum(s);
-  for (int o = 32; o > 0; o >>= 1) { const int32_t t = __shfl_xor(dmax, o, 64); dmax = t > dmax ? t : dmax; }
-  if ((threadIdx.x & 63) == 0) {
-    if (s) atomicAdd(out, (unsigned long long)s);
-    atomicMax(depth, dmax);
-  }
-}
-
 struct BfsBufs {
-  DevBuf q[2], qdeg, qpos, bits[2], cnt, deg, tail, lev;
+  DevBuf q[2], qdeg, qpos, bits[2], cnt, deg, lev;
   uint32_t nV = 0;
 };
 BfsBufs& bfs_bufs(uint32_t nV) {
@@ -176,7 +155,6 @@ BfsBufs& bfs_bufs(uint32_t nV) {
     b.bits[1].alloc(4 * nw);
     b.cnt.alloc(64);
     b.deg.alloc(64);
-    b.tail.alloc(64);
     b.lev.alloc(4 * (size_t)nV);   // levels when the caller keeps none on the device
     b.nV = nV;
   }
@@ -202,6 +180,7 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
     dsrc = o2[1] - o2[0];
   }
   uint32_t qn = 1;
+  uint64_t reached_deg = dsrc;        // out-degree sum of the vertices reached so far
   uint64_t mf = dsrc;                 // edges out of the frontier
   uint64_t mu = g->nE - dsrc;         // edges of unvisited vertices (approx.)
   int cur = 0;
@@ -253,21 +232,16 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
     prev_n = qn;
     qn = nn;
     mf = nd;
+    reached_deg += nd;
     mu = mu > nd ? mu - nd : 0;
     cur = nxt;
     ++L;
   }
-  DevBuf& tmp = b.tail;
-  GRE_HIP(hipMemsetAsync(tmp.p, 0, 8, s));
-  hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(1), 0, s, (int32_t*)((char*)tmp.p + 8), (uint64_t)1, -1);
-  hipLaunchKernelGGL(k_reached_edges, dim3(gcap(nV)), dim3(kB), 0, s, level, off, nV, (unsigned long long*)tmp.p,
-                     (int32_t*)((char*)tmp.p + 8));
-  GRE_HIP(hipGetLastError());
-  uint64_t h[2];
-  GRE_HIP(hipMemcpyAsync(h, tmp.p, 16, hipMemcpyDeviceToHost, s));
-  GRE_HIP(hipStreamSynchronize(s));
-  if (edges_out) *edges_out = h[0];
-  if (depth_out) *depth_out = (int32_t)(h[1] & 0xffffffffULL);
+  // every reached vertex entered exactly one frontier, whose out-degree sum
+  // the level reported: the reached edges are their total; the deepest level
+  // is the last one that found vertices
+  if (edges_out) *edges_out = reached_deg;
+  if (depth_out) *depth_out = L - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -355,7 +329,7 @@ __global__ __launch_bounds__(kB) void k_ms_td(const uint64_t* __restrict__ off, 
       const unsigned long long bits = fr[u] & ~vis[v];
       if (bits) take = atomicOr(nxt + v, bits) == 0ULL;
     }
-    wave_append(take, v, nq, ncount);
+    block_append(take, v, nq, ncount);
   }
 }
 
@@ -377,12 +351,7 @@ __global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, unsign
     if ((seen | m) == full) fulldeg += d;
     if (lev) ms_record(lev, nV, v, m, L);
   }
-  deg = wave_sum(deg);
-  fulldeg = wave_sum(fulldeg);
-  if ((threadIdx.x & 63) == 0) {
-    if (deg) atomicAdd(sums, (unsigned long long)deg);
-    if (fulldeg) atomicAdd(sums + 1, (unsigned long long)fulldeg);
-  }
+  block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
 }
 
 // bottom-up: every vertex still missing source bits ORs its in-neighbours'
@@ -431,14 +400,9 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
         }
       }
     }
-    wave_append(take, (uint32_t)v, nq, ncount);
+    block_append(take, (uint32_t)v, nq, ncount);
   }
-  deg = wave_sum(deg);
-  fulldeg = wave_sum(fulldeg);
-  if ((threadIdx.x & 63) == 0) {
-    if (deg) atomicAdd(sums, (unsigned long long)deg);
-    if (fulldeg) atomicAdd(sums + 1, (unsigned long long)fulldeg);
-  }
+  block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
 }
 
 // clears the frontier masks of the queued vertices
@@ -452,8 +416,7 @@ __global__ void k_ms_edges(const unsigned long long* __restrict__ vis, const uin
   uint64_t s = 0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x)
     s += (off[v + 1] - off[v]) * (uint64_t)__popcll(vis[v]);
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
+  block_add(static_cast<unsigned long long>(s), out);
 }
 
 // ---------------------------------------------------------------------------
@@ -631,7 +594,7 @@ __global__ __launch_bounds__(kB) void k_bc_td(const uint64_t* __restrict__ off, 
         }
       }
     }
-    wave_append(take, v, nq, ncount);
+    block_append(take, v, nq, ncount);
   }
 }
 
@@ -652,8 +615,7 @@ __global__ void k_bc_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, const 
     }
     deg += off[v + 1] - off[v];
   }
-  deg = wave_sum(deg);
-  if ((threadIdx.x & 63) == 0 && deg) atomicAdd(sums, (unsigned long long)deg);
+  block_add(static_cast<unsigned long long>(deg), sums);
 }
 
 // after a bottom-up level: the new frontier (nonzero masks) as a queue, with
@@ -667,10 +629,9 @@ __global__ __launch_bounds__(kB) void k_bc_queue(uint32_t nV, const unsigned lon
     const uint64_t v = base + threadIdx.x;
     const bool take = v < nV && nfr[v] != 0ULL;
     if (take) deg += off[v + 1] - off[v];
-    wave_append(take, (uint32_t)v, nq, ncount);
+    block_append(take, (uint32_t)v, nq, ncount);
   }
-  deg = wave_sum(deg);
-  if ((threadIdx.x & 63) == 0 && deg) atomicAdd(sums, (unsigned long long)deg);
+  block_add(static_cast<unsigned long long>(deg), sums);
 }
 
 // mask of lanes whose level is L
