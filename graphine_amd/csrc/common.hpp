@@ -168,6 +168,36 @@ __device__ inline void block_add2(A a, A* pa, B b, B* pb) {
   __syncthreads();
 }
 
+// reserves n slots of *qn for the calling thread, one counter update per
+// block; returns the first of them
+__device__ inline uint32_t block_reserve(uint32_t n, uint32_t* __restrict__ qn) {
+  __shared__ uint32_t wtot[16];
+  __shared__ uint32_t bbase;
+  const int lane = (int)(threadIdx.x & 63);
+  const int w = (int)(threadIdx.x >> 6);
+  uint32_t x = n;  // inclusive scan over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wtot[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (unsigned i = 0; i < (blockDim.x >> 6); ++i) {
+      const uint32_t c = wtot[i];
+      wtot[i] = t;
+      t += c;
+    }
+    bbase = t ? atomicAdd(qn, t) : 0u;
+  }
+  __syncthreads();
+  const uint32_t r = bbase + wtot[w] + x - n;
+  __syncthreads();
+  return r;
+}
+
 // appends v to q for every thread with take, one counter update per block
 // (order within q is unspecified)
 __device__ inline void block_append(bool take, uint32_t v, uint32_t* __restrict__ q, uint32_t* __restrict__ qn) {

@@ -71,20 +71,26 @@ __global__ __launch_bounds__(kB) void k_td(const uint64_t* __restrict__ off, con
   __syncthreads();
   const uint32_t i0 = range[0], i1 = range[1];
   uint64_t mydeg = 0;
-  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kB) {
-    uint32_t lo = i0, hi = i1 + 1;
-    while (hi - lo > 1) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (qpos[mid] <= k) lo = mid; else hi = mid;
+  // a fixed number of rounds for every thread keeps the block-level append uniform
+  for (int round = 0; round < kEB / kB; ++round) {
+    const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0;
+    if (k < k1) {
+      uint32_t lo = i0, hi = i1 + 1;
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (qpos[mid] <= k) lo = mid; else hi = mid;
+      }
+      const uint32_t u = q[lo];
+      v = idx[off[u] + (k - qpos[lo])];
+      if (level[v] < 0 && atomicCAS(level + v, -1, L + 1) == -1) {
+        take = true;
+        atomicOr(nbits + (v >> 5), 1u << (v & 31));
+        mydeg += off[v + 1] - off[v];
+      }
     }
-    const uint32_t u = q[lo];
-    const uint32_t v = idx[off[u] + (k - qpos[lo])];
-    if (level[v] < 0 && atomicCAS(level + v, -1, L + 1) == -1) {
-      const uint32_t p = atomicAdd(ncount, 1u);
-      nq[p] = v;
-      atomicOr(nbits + (v >> 5), 1u << (v & 31));
-      mydeg += off[v + 1] - off[v];
-    }
+    block_append(take, v, nq, ncount);
   }
   block_add(static_cast<unsigned long long>(mydeg), ndeg);
 }
@@ -126,10 +132,12 @@ __global__ __launch_bounds__(kB) void k_bu(const uint64_t* __restrict__ coff, co
 // bitmap -> queue (order within the queue does not matter to BFS levels)
 __global__ void k_bits_to_queue(const uint32_t* __restrict__ bits, uint32_t nwords, uint32_t* __restrict__ q,
                                 uint32_t* __restrict__ qn) {
-  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t m = bits[w];
-    if (!m) continue;
-    const uint32_t p = atomicAdd(qn, (uint32_t)__popc(m));
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // every thread runs the same number of rounds (block-level reservation)
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nwords; base += stride) {
+    const uint64_t w = base + threadIdx.x;
+    uint32_t m = w < nwords ? bits[w] : 0u;
+    const uint32_t p = block_reserve((uint32_t)__popc(m), qn);
     uint32_t i = 0;
     while (m) {
       const int b = __ffs(m) - 1;
