@@ -127,6 +127,11 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 __device__ inline uint32_t lane_id() { return __lane_id(); }
+__device__ inline unsigned long long wave_or(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
 
 // Block-level counter updates.  A counter that every wave of a large grid
 // updates serialises in L2 (about 12 ns per update at one address on MI355X,

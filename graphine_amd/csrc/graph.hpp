@@ -59,6 +59,16 @@ struct PullRows {
   DevBuf mid, lng;
   uint32_t n_mid = 0, n_long = 0;
 };
+// CSC rows too long for one thread of the 64-source BFS's bottom-up step, cut
+// into chunks of consecutive in-edges (one wave each)
+struct MsLongRows {
+  bool built = false;
+  DevBuf rows;       // uint32 [n] vertex ids
+  DevBuf cv;         // uint32 [nchunks] index into rows
+  DevBuf cb;         // uint64 [nchunks] first in-edge of the chunk
+  DevBuf acc;        // uint64 [n] OR of the masks the chunks found (zero between levels)
+  uint32_t n = 0, nchunks = 0;
+};
 }  // namespace gre
 
 struct gre_graph {
@@ -82,6 +92,7 @@ struct gre_graph {
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
   // engine pull row classes by direction (built on demand)
   std::map<int, gre::PullRows> pull_rows;
+  gre::MsLongRows ms_long;
 };
 
 namespace gre {

@@ -362,8 +362,15 @@ __global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, unsign
   block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
 }
 
+// Rows with more than kMsLong in-edges are handled apart in the bottom-up step:
+// one thread would walk a hub's million in-edges alone, and a vertex rarely
+// gains all 64 bits in one level, so early levels scan whole rows.
+constexpr uint64_t kMsLong = 1024;
+constexpr uint64_t kMsChunk = 4096;   // in-edges per chunk, one wave each
+
 // bottom-up: every vertex still missing source bits ORs its in-neighbours'
-// frontier masks, stopping once it has them all
+// frontier masks, stopping once it has them all (rows above kMsLong in-edges:
+// k_ms_bu_chunks and k_ms_bu_long_fin)
 __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
                                               const uint64_t* __restrict__ off, uint32_t nV, unsigned long long full,
                                               const unsigned long long* __restrict__ fr,
@@ -379,9 +386,9 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
     bool take = false;
     if (v < nV) {
       const unsigned long long seen = vis[v];
-      if (seen != full) {
+      const uint64_t b = coff[v], e = coff[v + 1];
+      if (seen != full && e - b <= kMsLong) {
         unsigned long long m = 0;
-        const uint64_t b = coff[v], e = coff[v + 1];
         // 8 in-neighbours per step, their masks gathered together (past the
         // row's end: the first neighbour again); once (m | seen) == full the new
         // bits m & ~seen are all the missing ones, whichever neighbours were ORed
@@ -411,6 +418,159 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
     block_append(take, (uint32_t)v, nq, ncount);
   }
   block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
+}
+
+// bottom-up over the long rows' chunks, one wave per chunk: the OR of the
+// chunk's in-neighbours' frontier masks goes to acc[row] (stopping once the
+// row has every bit)
+__global__ __launch_bounds__(kB) void k_ms_bu_chunks(const uint64_t* __restrict__ coff,
+                                                     const uint32_t* __restrict__ cidx,
+                                                     const uint32_t* __restrict__ rows,
+                                                     const uint32_t* __restrict__ cv,
+                                                     const uint64_t* __restrict__ cb, uint32_t nchunks,
+                                                     unsigned long long full,
+                                                     const unsigned long long* __restrict__ fr,
+                                                     const unsigned long long* __restrict__ vis,
+                                                     unsigned long long* __restrict__ acc) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
+    const uint32_t i = cv[c];
+    const uint32_t v = rows[i];
+    const unsigned long long seen = vis[v];
+    if (seen == full) continue;  // the same chunk, so the same answer, on every lane
+    const uint64_t b = cb[c], rowend = coff[v + 1];
+    const uint64_t e = b + kMsChunk < rowend ? b + kMsChunk : rowend;
+    unsigned long long m = 0;
+    for (uint64_t p = b; p < e; p += 4 * 64) {
+      // 4 in-neighbours per lane per step, gathered together (past the end:
+      // the chunk's first in-neighbour again, which changes nothing)
+      uint32_t w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t q = p + (uint64_t)(j * 64 + lane);
+        w[j] = cidx[q < e ? q : b];
+      }
+      unsigned long long f[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) f[j] = fr[w[j]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) m |= f[j];
+      if ((wave_or(m) | seen) == full) break;
+    }
+    m = wave_or(m) & ~seen;
+    if (lane == 0 && m) atomicOr(acc + i, m);
+  }
+}
+
+// the long rows' new bits (acc, cleared for the next level), as k_ms_bu does
+// for the others
+__global__ __launch_bounds__(kB) void k_ms_bu_long_fin(const uint32_t* __restrict__ rows, uint32_t n,
+                                                       unsigned long long full,
+                                                       unsigned long long* __restrict__ acc,
+                                                       unsigned long long* __restrict__ nfr,
+                                                       unsigned long long* __restrict__ vis,
+                                                       const uint64_t* __restrict__ off, int32_t* __restrict__ lev,
+                                                       uint32_t nV, int32_t L, uint32_t* __restrict__ nq,
+                                                       uint32_t* __restrict__ ncount,
+                                                       unsigned long long* __restrict__ sums) {
+  uint64_t deg = 0, fulldeg = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0;
+    if (i < n) {
+      v = rows[i];
+      const unsigned long long seen = vis[v];
+      const unsigned long long m = acc[i] & ~seen;
+      acc[i] = 0ULL;
+      if (m) {
+        nfr[v] = m;
+        vis[v] = seen | m;
+        take = true;
+        const uint64_t d = off[v + 1] - off[v];
+        deg += d;
+        if ((seen | m) == full) fulldeg += d;
+        if (lev) ms_record(lev, nV, v, m, L);
+      }
+    }
+    block_append(take, v, nq, ncount);
+  }
+  block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
+}
+
+// vertices with more than kMsLong in-edges
+__global__ void k_ms_long_rows(const uint64_t* __restrict__ coff, uint32_t nV, uint32_t* __restrict__ rows,
+                               uint32_t* __restrict__ n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nV; base += stride) {
+    const uint64_t v = base + threadIdx.x;
+    const bool take = v < nV && coff[v + 1] - coff[v] > kMsLong;
+    block_append(take, (uint32_t)v, rows, n);
+  }
+}
+
+__global__ void k_ms_row_bounds(const uint32_t* __restrict__ rows, uint32_t n, const uint64_t* __restrict__ coff,
+                                uint64_t* __restrict__ be) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    be[2 * i] = coff[rows[i]];
+    be[2 * i + 1] = coff[rows[i] + 1];
+  }
+}
+
+// the long rows and their chunks, built once per graph
+MsLongRows& ms_long_rows(gre_graph* g, hipStream_t s) {
+  MsLongRows& lr = g->ms_long;
+  if (lr.built) return lr;
+  const uint32_t nV = g->nV;
+  const uint64_t* coff = g->coff.as<uint64_t>();
+  DevBuf rows, cnt;
+  rows.alloc(4 * (size_t)nV);
+  cnt.alloc(4);
+  GRE_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+  hipLaunchKernelGGL(k_ms_long_rows, dim3(gcap(nV)), dim3(kB), 0, s, coff, nV, rows.as<uint32_t>(),
+                     cnt.as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+  uint32_t n = 0;
+  GRE_HIP(hipMemcpyAsync(&n, cnt.p, 4, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> hr(n);
+  std::vector<uint64_t> hbe(2 * (size_t)n);
+  if (n) {
+    DevBuf be;
+    be.alloc(16 * (size_t)n);
+    hipLaunchKernelGGL(k_ms_row_bounds, dim3(gcap(n)), dim3(kB), 0, s, rows.as<uint32_t>(), n, coff,
+                       be.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipMemcpyAsync(hr.data(), rows.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipMemcpyAsync(hbe.data(), be.p, 16 * (size_t)n, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<uint32_t> cv;
+  std::vector<uint64_t> cb;
+  for (uint32_t i = 0; i < n; ++i) {
+    for (uint64_t p = hbe[2 * (size_t)i]; p < hbe[2 * (size_t)i + 1]; p += kMsChunk) {
+      cv.push_back(i);
+      cb.push_back(p);
+    }
+  }
+  GRE_CHECK(cv.size() < 0xffffffffULL, GRE_DEV_FAILURE, "too many long-row chunks");
+  lr.n = n;
+  lr.nchunks = (uint32_t)cv.size();
+  lr.rows.alloc(4 * std::max<size_t>(n, 1));
+  lr.acc.alloc(8 * std::max<size_t>(n, 1));
+  lr.cv.alloc(4 * std::max<size_t>(cv.size(), 1));
+  lr.cb.alloc(8 * std::max<size_t>(cb.size(), 1));
+  GRE_HIP(hipMemsetAsync(lr.acc.p, 0, 8 * std::max<size_t>(n, 1), s));
+  if (n) GRE_HIP(hipMemcpyAsync(lr.rows.p, hr.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
+  if (!cv.empty()) {
+    GRE_HIP(hipMemcpyAsync(lr.cv.p, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice, s));
+    GRE_HIP(hipMemcpyAsync(lr.cb.p, cb.data(), 8 * cb.size(), hipMemcpyHostToDevice, s));
+  }
+  GRE_HIP(hipStreamSynchronize(s));
+  lr.built = true;
+  return lr;
 }
 
 // clears the frontier masks of the queued vertices
@@ -750,6 +910,7 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
                          (uint64_t)nV * nsrc, -1);
     }
     GRE_HIP(hipMemsetAsync(vis.p, 0, 8 * (size_t)nV, s));
+    if (g->ms_long.built) GRE_HIP(hipMemsetAsync(g->ms_long.acc.p, 0, 8 * std::max<size_t>(g->ms_long.n, 1), s));
     GRE_HIP(hipMemsetAsync(fr[0].p, 0, 8 * (size_t)nV, s));
     GRE_HIP(hipMemsetAsync(fr[1].p, 0, 8 * (size_t)nV, s));
     GRE_HIP(hipMemsetAsync(nxt.p, 0, 8 * (size_t)nV, s));
@@ -800,11 +961,24 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
                              (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
                              off, level_out ? lev.as<int32_t>() : nullptr, nV, L, (unsigned long long*)sums.p);
       } else {
+        MsLongRows& lr = ms_long_rows(g, s);
         ProfScope ps("msbfs_bottomup");
+        // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
+        if (lr.nchunks)
+          hipLaunchKernelGGL(k_ms_bu_chunks, dim3(std::min<uint64_t>(((uint64_t)lr.nchunks + 3) / 4, 1u << 16)),
+                             dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), lr.rows.as<uint32_t>(),
+                             lr.cv.as<uint32_t>(), lr.cb.as<uint64_t>(), lr.nchunks, full,
+                             (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                             (unsigned long long*)lr.acc.p);
         hipLaunchKernelGGL(k_ms_bu, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
                            off, nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
                            (unsigned long long*)vis.p, level_out ? lev.as<int32_t>() : nullptr, L,
                            q[nx].as<uint32_t>(), flag.as<uint32_t>(), (unsigned long long*)sums.p);
+        if (lr.n)
+          hipLaunchKernelGGL(k_ms_bu_long_fin, dim3(gcap(lr.n)), dim3(kB), 0, s, lr.rows.as<uint32_t>(), lr.n, full,
+                             (unsigned long long*)lr.acc.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
+                             off, level_out ? lev.as<int32_t>() : nullptr, nV, L, q[nx].as<uint32_t>(),
+                             flag.as<uint32_t>(), (unsigned long long*)sums.p);
         GRE_HIP(hipGetLastError());
         GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
       }
