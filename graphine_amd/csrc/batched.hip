@@ -46,6 +46,8 @@ struct BFwd {  // forward: gather x (CSC), finalise ranks (raw sums go to xout)
   double damp; int k;
   __device__ TT* sums() const { return xout; }
   __device__ TT gather(uint64_t e, uint32_t u, int j) const { return (TT)w[e] * xin[(uint64_t)u * k + j]; }
+  __device__ TT wt(uint64_t e) const { return (TT)w[e]; }
+  __device__ TT val(uint32_t u, int j) const { return xin[(uint64_t)u * k + j]; }
   __device__ void finalize(uint32_t r, int j, TT s, double& a0, double& a1) const {
     const uint64_t o = (uint64_t)r * k + j;
     const TT pv = p[o];
@@ -64,6 +66,8 @@ struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda (raw sums go to 
   double damp; int k;
   __device__ TT* sums() const { return wl; }
   __device__ TT gather(uint64_t e, uint32_t v, int j) const { return (TT)w[e] * lin[(uint64_t)v * k + j]; }
+  __device__ TT wt(uint64_t e) const { return (TT)w[e]; }
+  __device__ TT val(uint32_t v, int j) const { return lin[(uint64_t)v * k + j]; }
   __device__ void finalize(uint32_t u, int j, TT s, double& a0, double& a1) const {
     const uint64_t o = (uint64_t)u * k + j;
     wl[o] = s;
@@ -115,34 +119,37 @@ __global__ __launch_bounds__(kBlk) void k_bwalk(const uint64_t* __restrict__ off
     }
     rend = __shfl(endw, (int)(r - wr), G);
   };
-  // ids of the chunk after the current one are loaded before the current
-  // chunk is folded
+  // ids and weights of the chunk after the current one are loaded before the
+  // current chunk is folded
   uint32_t myn = e0 + gl < e1 ? idx[e0 + gl] : 0u;
+  TT mywn = e0 + gl < e1 ? m.wt(e0 + gl) : (TT)0;
   for (uint64_t cb = e0; cb < e1; cb += G) {
-    // the group's next G entries: ids one per lane (coalesced), then G
-    // weighted gathers of k contiguous values (a weight is one broadcast
-    // load, served by L1 for the G entries of a chunk)
+    // the group's next G entries: ids and weights one per lane (coalesced),
+    // then G gathers of k contiguous values; each entry's id and weight reach
+    // the group by shuffles, so a lane keeps one address per gather in flight
     const uint32_t myu = myn;
-    // unconditional gathers (past the tile's end: entry e1 - 1, neighbour 0;
-    // idle lanes: component 0), so all G are in flight before the first use
+    const TT myw = mywn;
+    // unconditional gathers (past the tile's end: neighbour 0; idle lanes:
+    // component 0), so all G are in flight before the first use
     TT g[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const uint32_t u = __shfl(myu, q, G);
       const bool ok = cb + q < e1;
-      const TT v = m.gather(ok ? cb + q : e1 - 1, u, act ? gl : 0);
+      const TT v = m.val(u, act ? gl : 0);
       g[q] = (act && ok) ? v : (TT)0;
     }
     {
       const uint64_t nx = cb + G + gl;
       myn = nx < e1 ? idx[nx] : 0u;
+      mywn = nx < e1 ? m.wt(nx) : (TT)0;
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const uint64_t e = cb + q;
       if (e >= e1) break;
       while (e >= rend) flush_next();  // rows ending before e, empty rows included
-      acc += g[q];
+      acc += __shfl(myw, q, G) * g[q];
     }
   }
   // rows still open at the tile end, plus empty rows that start in the tile
@@ -164,22 +171,25 @@ __global__ __launch_bounds__(kBlk) void k_brows(const uint64_t* __restrict__ off
     const uint64_t b = off[r], e = off[r + 1];
     TT acc = (TT)0;
     uint32_t myn = b + gl < e ? idx[b + gl] : 0u;
+    TT mywn = b + gl < e ? m.wt(b + gl) : (TT)0;
     for (uint64_t cb = b; cb < e; cb += G) {
       const uint32_t myu = myn;
+      const TT myw = mywn;
       TT g[G];
 #pragma unroll
       for (int q = 0; q < G; ++q) {
         const uint32_t u = __shfl(myu, q, G);
         const bool ok = cb + q < e;
-        const TT v = m.gather(ok ? cb + q : e - 1, u, act ? gl : 0);
+        const TT v = m.val(u, act ? gl : 0);
         g[q] = (act && ok) ? v : (TT)0;
       }
       {
         const uint64_t nx = cb + G + gl;
         myn = nx < e ? idx[nx] : 0u;
+        mywn = nx < e ? m.wt(nx) : (TT)0;
       }
 #pragma unroll
-      for (int q = 0; q < G; ++q) acc += g[q];
+      for (int q = 0; q < G; ++q) acc += __shfl(myw, q, G) * g[q];
     }
     if (act) sums[r * k + gl] = acc;
   }
