@@ -661,6 +661,10 @@ __global__ void k_write(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, c
   const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
   uint32_t base_c = bcnt[blockIdx.x];
   uint64_t base_d = bdeg[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) qpos[bcnt[nb]] = bdeg[nb];
+  // bcnt holds the exclusive prefix of the blocks' counts: a block with no
+  // active vertex has nothing to write (small frontiers skip most blocks)
+  if (bcnt[blockIdx.x + 1] == base_c) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i0 = 0; i0 < kVB; i0 += kB) {
     const uint64_t v = v0 + i0 + threadIdx.x;
@@ -690,7 +694,6 @@ __global__ void k_write(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, c
     base_d += td;
     __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) qpos[bcnt[nb]] = bdeg[nb];
 }
 
 // edge-balanced push: virtual edge k of the frontier -> (u, v, CSR position)
