@@ -66,6 +66,23 @@ def test_msbfs_equals_single_source(N, O, nsrc):
     assert edges == total
 
 
+def test_msbfs_multi_chunk_rows(N, O):
+    # rows above 1024 in-edges take the chunked bottom-up path (4096 in-edges
+    # per wave, OR-combined); scale 16 has rows of up to ~9,600 in-edges
+    g, og = graphs(N, O, 16, 16, 4)
+    assert int(og.outdeg.max()) > 2 * 4096
+    rng = np.random.default_rng(16)
+    srcs = rng.choice(og.nV, 64, replace=False).astype(np.uint32)
+    for rep in range(2):  # the second call reuses the long-row accumulators
+        lev, edges = N.msbfs(g, srcs)
+        total = 0
+        for i, s in enumerate(srcs):
+            ref = O.bfs_levels(og, int(s))
+            assert np.array_equal(lev[i], ref)
+            total += int(og.outdeg[ref >= 0].sum())
+        assert edges == total
+
+
 @pytest.mark.parametrize("nsrc", [1, 5, 70])
 def test_bc_matches_brandes(N, O, nsrc):
     g, og = graphs(N, O, 12, 16, 4)
