@@ -59,14 +59,15 @@ struct PullRows {
   DevBuf mid, lng;
   uint32_t n_mid = 0, n_long = 0;
 };
-// CSC rows too long for one thread of the 64-source BFS's bottom-up step, cut
-// into chunks of consecutive in-edges (one wave each)
-struct MsLongRows {
-  bool built = false;
+// Rows of one adjacency (CSC or CSR) longer than a threshold, cut into chunks
+// of consecutive entries (one wave each) for the per-vertex traversal kernels;
+// chunks are listed row by row, in entry order.
+struct LongRows {
   DevBuf rows;       // uint32 [n] vertex ids
   DevBuf cv;         // uint32 [nchunks] index into rows
-  DevBuf cb;         // uint64 [nchunks] first in-edge of the chunk
-  DevBuf acc;        // uint64 [n] OR of the masks the chunks found (zero between levels)
+  DevBuf cb;         // uint64 [nchunks] first entry of the chunk
+  DevBuf cstart;     // uint32 [n+1] first chunk of each row
+  DevBuf acc;        // uint64 [n] per-row accumulator of the users (zero between uses)
   uint32_t n = 0, nchunks = 0;
 };
 }  // namespace gre
@@ -92,7 +93,7 @@ struct gre_graph {
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
   // engine pull row classes by direction (built on demand)
   std::map<int, gre::PullRows> pull_rows;
-  gre::MsLongRows ms_long;
+  std::map<uint64_t, gre::LongRows> long_rows;  // by (csc, threshold, chunk entries)
 };
 
 namespace gre {

@@ -500,36 +500,38 @@ __global__ __launch_bounds__(kB) void k_ms_bu_long_fin(const uint32_t* __restric
   block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
 }
 
-// vertices with more than kMsLong in-edges
-__global__ void k_ms_long_rows(const uint64_t* __restrict__ coff, uint32_t nV, uint32_t* __restrict__ rows,
-                               uint32_t* __restrict__ n) {
+// vertices whose row in off has more than thr entries
+__global__ void k_long_rows(const uint64_t* __restrict__ off, uint32_t nV, uint64_t thr, uint32_t* __restrict__ rows,
+                            uint32_t* __restrict__ n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nV; base += stride) {
     const uint64_t v = base + threadIdx.x;
-    const bool take = v < nV && coff[v + 1] - coff[v] > kMsLong;
+    const bool take = v < nV && off[v + 1] - off[v] > thr;
     block_append(take, (uint32_t)v, rows, n);
   }
 }
 
-__global__ void k_ms_row_bounds(const uint32_t* __restrict__ rows, uint32_t n, const uint64_t* __restrict__ coff,
+__global__ void k_row_bounds_of(const uint32_t* __restrict__ rows, uint32_t n, const uint64_t* __restrict__ off,
                                 uint64_t* __restrict__ be) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    be[2 * i] = coff[rows[i]];
-    be[2 * i + 1] = coff[rows[i] + 1];
+    be[2 * i] = off[rows[i]];
+    be[2 * i + 1] = off[rows[i] + 1];
   }
 }
 
-// the long rows and their chunks, built once per graph
-MsLongRows& ms_long_rows(gre_graph* g, hipStream_t s) {
-  MsLongRows& lr = g->ms_long;
-  if (lr.built) return lr;
+// the rows of the CSC (csc) or CSR with more than thr entries and their
+// chunks of up to chunk entries, built once per graph and (csc, thr, chunk)
+LongRows& long_rows(gre_graph* g, bool csc, uint64_t thr, uint64_t chunk, hipStream_t s) {
+  const uint64_t key = ((uint64_t)csc << 63) | (thr << 32) | chunk;
+  auto it = g->long_rows.find(key);
+  if (it != g->long_rows.end()) return it->second;
   const uint32_t nV = g->nV;
-  const uint64_t* coff = g->coff.as<uint64_t>();
+  const uint64_t* off = (csc ? g->coff : g->off).as<uint64_t>();
   DevBuf rows, cnt;
   rows.alloc(4 * (size_t)nV);
   cnt.alloc(4);
   GRE_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
-  hipLaunchKernelGGL(k_ms_long_rows, dim3(gcap(nV)), dim3(kB), 0, s, coff, nV, rows.as<uint32_t>(),
+  hipLaunchKernelGGL(k_long_rows, dim3(gcap(nV)), dim3(kB), 0, s, off, nV, thr, rows.as<uint32_t>(),
                      cnt.as<uint32_t>());
   GRE_HIP(hipGetLastError());
   uint32_t n = 0;
@@ -540,36 +542,40 @@ MsLongRows& ms_long_rows(gre_graph* g, hipStream_t s) {
   if (n) {
     DevBuf be;
     be.alloc(16 * (size_t)n);
-    hipLaunchKernelGGL(k_ms_row_bounds, dim3(gcap(n)), dim3(kB), 0, s, rows.as<uint32_t>(), n, coff,
+    hipLaunchKernelGGL(k_row_bounds_of, dim3(gcap(n)), dim3(kB), 0, s, rows.as<uint32_t>(), n, off,
                        be.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     GRE_HIP(hipMemcpyAsync(hr.data(), rows.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipMemcpyAsync(hbe.data(), be.p, 16 * (size_t)n, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
   }
-  std::vector<uint32_t> cv;
+  std::vector<uint32_t> cv, cs;
   std::vector<uint64_t> cb;
   for (uint32_t i = 0; i < n; ++i) {
-    for (uint64_t p = hbe[2 * (size_t)i]; p < hbe[2 * (size_t)i + 1]; p += kMsChunk) {
+    cs.push_back((uint32_t)cv.size());
+    for (uint64_t p = hbe[2 * (size_t)i]; p < hbe[2 * (size_t)i + 1]; p += chunk) {
       cv.push_back(i);
       cb.push_back(p);
     }
   }
   GRE_CHECK(cv.size() < 0xffffffffULL, GRE_DEV_FAILURE, "too many long-row chunks");
+  cs.push_back((uint32_t)cv.size());
+  LongRows& lr = g->long_rows[key];
   lr.n = n;
   lr.nchunks = (uint32_t)cv.size();
   lr.rows.alloc(4 * std::max<size_t>(n, 1));
   lr.acc.alloc(8 * std::max<size_t>(n, 1));
   lr.cv.alloc(4 * std::max<size_t>(cv.size(), 1));
   lr.cb.alloc(8 * std::max<size_t>(cb.size(), 1));
+  lr.cstart.alloc(4 * cs.size());
   GRE_HIP(hipMemsetAsync(lr.acc.p, 0, 8 * std::max<size_t>(n, 1), s));
   if (n) GRE_HIP(hipMemcpyAsync(lr.rows.p, hr.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
   if (!cv.empty()) {
     GRE_HIP(hipMemcpyAsync(lr.cv.p, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice, s));
     GRE_HIP(hipMemcpyAsync(lr.cb.p, cb.data(), 8 * cb.size(), hipMemcpyHostToDevice, s));
   }
+  GRE_HIP(hipMemcpyAsync(lr.cstart.p, cs.data(), 4 * cs.size(), hipMemcpyHostToDevice, s));
   GRE_HIP(hipStreamSynchronize(s));
-  lr.built = true;
   return lr;
 }
 
@@ -587,6 +593,191 @@ __global__ void k_ms_edges(const unsigned long long* __restrict__ vis, const uin
   block_add(static_cast<unsigned long long>(s), out);
 }
 
+// Rows above kBcLong entries are cut into chunks of kBcChunk (one wave each):
+// one wave walking a hub's tens of thousands of neighbours set the time of a
+// whole level.  A chunk leaves per-lane partial sums; the row's finish adds
+// them in chunk order (path counts are integers, exact in any order).
+constexpr uint64_t kBcLong = 512;
+constexpr uint64_t kBcChunk = 512;
+
+// forward fold of entries [b, e) of v's in-row: each lane adds the sigma of
+// the in-neighbours that reach v for its source; gotl collects those lanes
+__device__ inline void bc_fwd_run(const uint32_t* __restrict__ cidx, uint64_t b, uint64_t e,
+                                  unsigned long long seen, const unsigned long long* __restrict__ fr,
+                                  const double* __restrict__ sigma, int lane, double& acc,
+                                  unsigned long long& gotl) {
+  for (uint64_t base = b; base < e; base += 64) {
+    // 64 in-neighbours and their frontier masks at once
+    const uint64_t p = base + lane;
+    const uint32_t w = p < e ? cidx[p] : 0u;
+    const unsigned long long f = p < e ? (fr[w] & ~seen) : 0ULL;
+    gotl |= f;
+    unsigned long long act = __ballot(f != 0ULL);
+    // sigma rows of the contributing neighbours, 8 gathers in flight, added
+    // in row order
+    while (act) {
+      uint32_t wj[8];
+      unsigned long long fj[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = act ? __ffsll(act) - 1 : 0;
+        fj[t] = act ? __shfl(f, j, 64) : 0ULL;
+        wj[t] = __shfl(w, j, 64);
+        act &= act - 1ULL;
+      }
+      double sv[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) sv[t] = sigma[(uint64_t)wj[t] * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if ((fj[t] >> lane) & 1ULL) acc += sv[t];
+    }
+  }
+}
+
+// backward fold of entries [b, e) of v's out-row: each lane adds q of the
+// out-neighbours one level down for its source (a = v's lanes at this level)
+__device__ inline void bc_bwd_run(const uint32_t* __restrict__ idx, uint64_t b, uint64_t e, unsigned long long a,
+                                  const unsigned long long* __restrict__ nxt, const double* __restrict__ q, int lane,
+                                  double& acc) {
+  for (uint64_t base = b; base < e; base += 64) {
+    // 64 out-neighbours and the lanes they hold at L+1, restricted to v's lanes at L
+    const uint64_t p = base + lane;
+    const uint32_t w = p < e ? idx[p] : 0u;
+    const unsigned long long m = p < e ? (nxt[w] & a) : 0ULL;
+    unsigned long long act = __ballot(m != 0ULL);
+    while (act) {
+      uint32_t wj[8];
+      unsigned long long mj[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = act ? __ffsll(act) - 1 : 0;
+        mj[t] = act ? __shfl(m, j, 64) : 0ULL;
+        wj[t] = __shfl(w, j, 64);
+        act &= act - 1ULL;
+      }
+      double qv[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) qv[t] = q[(uint64_t)wj[t] * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if ((mj[t] >> lane) & 1ULL) acc += qv[t];
+    }
+  }
+}
+
+// forward level over the long rows' chunks: per chunk, each lane's partial
+// sum and the lanes found
+__global__ __launch_bounds__(kB) void k_bc_fwd_chunks(const uint64_t* __restrict__ coff,
+                                                      const uint32_t* __restrict__ cidx,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint32_t* __restrict__ cv,
+                                                      const uint64_t* __restrict__ cb, uint32_t nchunks, int nsrc,
+                                                      const unsigned long long* __restrict__ fr,
+                                                      const unsigned long long* __restrict__ vis,
+                                                      const double* __restrict__ sigma, double* __restrict__ psum,
+                                                      unsigned long long* __restrict__ pgot) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
+    const uint32_t v = rows[cv[c]];
+    const unsigned long long seen = vis[v];
+    double acc = 0.0;
+    unsigned long long gotl = 0;
+    if (seen != full) {  // the same chunk, so the same answer, on every lane
+      const uint64_t b = cb[c], rowend = coff[v + 1];
+      bc_fwd_run(cidx, b, b + kBcChunk < rowend ? b + kBcChunk : rowend, seen, fr, sigma, lane, acc, gotl);
+    }
+    psum[c * 64 + lane] = acc;
+    const unsigned long long got = wave_or(gotl);
+    if (lane == 0) pgot[c] = got;
+  }
+}
+
+// the long rows' forward results from their chunks (as k_bc_fwd does for the rest)
+__global__ __launch_bounds__(kB) void k_bc_fwd_long_fin(const uint32_t* __restrict__ rows,
+                                                        const uint32_t* __restrict__ cstart, uint32_t n, int nsrc,
+                                                        const double* __restrict__ psum,
+                                                        const unsigned long long* __restrict__ pgot,
+                                                        unsigned long long* __restrict__ nfr,
+                                                        unsigned long long* __restrict__ vis,
+                                                        double* __restrict__ sigma, uint8_t* __restrict__ lev,
+                                                        int32_t L, unsigned int* __restrict__ any) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  bool changed = false;
+  for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
+    const uint32_t v = rows[i];
+    const unsigned long long seen = vis[v];
+    if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
+    double acc = 0.0;
+    unsigned long long got = 0;
+    for (uint32_t c = cstart[i]; c < cstart[i + 1]; ++c) {
+      acc += psum[(uint64_t)c * 64 + lane];
+      got |= pgot[c];
+    }
+    if (lane == 0) nfr[v] = got;
+    if (got) {
+      if ((got >> lane) & 1ULL) {
+        sigma[(uint64_t)v * 64 + lane] = acc;
+        lev[(uint64_t)v * 64 + lane] = (uint8_t)(L + 1);
+      }
+      if (lane == 0) vis[v] = seen | got;
+      changed = true;
+    }
+  }
+  if (__any(changed) && lane == 0) atomicOr(any, 1u);
+}
+
+// backward level over the long rows' chunks: each lane's partial sum
+__global__ __launch_bounds__(kB) void k_bc_bwd_chunks(const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ idx,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint32_t* __restrict__ cv,
+                                                      const uint64_t* __restrict__ cb, uint32_t nchunks,
+                                                      const double* __restrict__ q,
+                                                      const unsigned long long* __restrict__ atl,
+                                                      const unsigned long long* __restrict__ nxt,
+                                                      double* __restrict__ psum) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
+    const uint32_t v = rows[cv[c]];
+    const unsigned long long a = atl[v];
+    double acc = 0.0;
+    if (a) {
+      const uint64_t b = cb[c], rowend = off[v + 1];
+      bc_bwd_run(idx, b, b + kBcChunk < rowend ? b + kBcChunk : rowend, a, nxt, q, lane, acc);
+    }
+    psum[c * 64 + lane] = acc;
+  }
+}
+
+// the long rows' delta and q from their chunks' partial sums, in chunk order
+__global__ __launch_bounds__(kB) void k_bc_bwd_long_fin(const uint32_t* __restrict__ rows,
+                                                        const uint32_t* __restrict__ cstart, uint32_t n,
+                                                        const double* __restrict__ psum,
+                                                        const double* __restrict__ sigma,
+                                                        double* __restrict__ delta, double* __restrict__ q,
+                                                        const unsigned long long* __restrict__ atl) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
+    const uint32_t v = rows[i];
+    const unsigned long long a = atl[v];
+    if (!((a >> lane) & 1ULL)) continue;
+    double acc = 0.0;
+    for (uint32_t c = cstart[i]; c < cstart[i + 1]; ++c) acc += psum[(uint64_t)c * 64 + lane];
+    const uint64_t o = (uint64_t)v * 64 + lane;
+    const double sv = sigma[o];
+    const double d = sv * acc;
+    delta[o] = d;
+    q[o] = (1.0 + d) / sv;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Brandes BC, 64 sources per chunk, one wave per vertex (lane = source)
 // ---------------------------------------------------------------------------
@@ -602,38 +793,13 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   bool changed = false;
   for (uint64_t v = wid; v < nV; v += nw) {
+    const uint64_t b = coff[v], e = coff[v + 1];
+    if (e - b > kBcLong) continue;  // k_bc_fwd_chunks + k_bc_fwd_long_fin
     const unsigned long long seen = vis[v];
     if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
     double acc = 0.0;
     unsigned long long gotl = 0;
-    const uint64_t b = coff[v], e = coff[v + 1];
-    for (uint64_t base = b; base < e; base += 64) {
-      // 64 in-neighbours and their frontier masks at once
-      const uint64_t p = base + lane;
-      const uint32_t w = p < e ? cidx[p] : 0u;
-      const unsigned long long f = p < e ? (fr[w] & ~seen) : 0ULL;
-      gotl |= f;
-      unsigned long long act = __ballot(f != 0ULL);
-      // sigma rows of the contributing neighbours, 8 gathers in flight, added
-      // in row order
-      while (act) {
-        uint32_t wj[8];
-        unsigned long long fj[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const int j = act ? __ffsll(act) - 1 : 0;
-          fj[t] = act ? __shfl(f, j, 64) : 0ULL;
-          wj[t] = __shfl(w, j, 64);
-          act &= act - 1ULL;
-        }
-        double sv[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) sv[t] = sigma[(uint64_t)wj[t] * 64 + lane];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if ((fj[t] >> lane) & 1ULL) acc += sv[t];
-      }
-    }
+    bc_fwd_run(cidx, b, e, seen, fr, sigma, lane, acc, gotl);
     // got = OR over the lanes of gotl
     unsigned long long got = gotl;
     for (int o = 32; o > 0; o >>= 1) got |= __shfl_xor(got, o, 64);
@@ -664,33 +830,11 @@ __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off,
   for (uint64_t v = wid; v < nV; v += nw) {
     const unsigned long long a = atl[v];
     if (!a) continue;
+    const uint64_t b = off[v], e = off[v + 1];
+    if (e - b > kBcLong) continue;  // k_bc_bwd_chunks + k_bc_bwd_long_fin
     const bool mine = (a >> lane) & 1ULL;
     double acc = 0.0;
-    const uint64_t b = off[v], e = off[v + 1];
-    for (uint64_t base = b; base < e; base += 64) {
-      // 64 out-neighbours and the lanes they hold at L+1, restricted to v's lanes at L
-      const uint64_t p = base + lane;
-      const uint32_t w = p < e ? idx[p] : 0u;
-      const unsigned long long m = p < e ? (nxt[w] & a) : 0ULL;
-      unsigned long long act = __ballot(m != 0ULL);
-      while (act) {
-        uint32_t wj[8];
-        unsigned long long mj[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const int j = act ? __ffsll(act) - 1 : 0;
-          mj[t] = act ? __shfl(m, j, 64) : 0ULL;
-          wj[t] = __shfl(w, j, 64);
-          act &= act - 1ULL;
-        }
-        double qv[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) qv[t] = q[(uint64_t)wj[t] * 64 + lane];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if ((mj[t] >> lane) & 1ULL) acc += qv[t];
-      }
-    }
+    bc_bwd_run(idx, b, e, a, nxt, q, lane, acc);
     if (mine) {
       const uint64_t o = (uint64_t)v * 64 + lane;
       const double sv = sigma[o];
@@ -910,7 +1054,8 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
                          (uint64_t)nV * nsrc, -1);
     }
     GRE_HIP(hipMemsetAsync(vis.p, 0, 8 * (size_t)nV, s));
-    if (g->ms_long.built) GRE_HIP(hipMemsetAsync(g->ms_long.acc.p, 0, 8 * std::max<size_t>(g->ms_long.n, 1), s));
+    LongRows& lr = long_rows(g, true, kMsLong, kMsChunk, s);
+    GRE_HIP(hipMemsetAsync(lr.acc.p, 0, 8 * std::max<size_t>(lr.n, 1), s));
     GRE_HIP(hipMemsetAsync(fr[0].p, 0, 8 * (size_t)nV, s));
     GRE_HIP(hipMemsetAsync(fr[1].p, 0, 8 * (size_t)nV, s));
     GRE_HIP(hipMemsetAsync(nxt.p, 0, 8 * (size_t)nV, s));
@@ -961,7 +1106,6 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
                              (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
                              off, level_out ? lev.as<int32_t>() : nullptr, nV, L, (unsigned long long*)sums.p);
       } else {
-        MsLongRows& lr = ms_long_rows(g, s);
         ProfScope ps("msbfs_bottomup");
         // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
         if (lr.nchunks)
@@ -1036,6 +1180,15 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     GRE_HIP(hipMemsetAsync(bc.p, 0, bc.bytes, s));
     GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
     const unsigned gw = (unsigned)std::min<uint64_t>(((uint64_t)nV * 64 + kB - 1) / kB, 1u << 20);
+    // long rows of the forward (CSC) and backward (CSR) pulls, and their chunks' partials
+    LongRows& lf = long_rows(g, true, kBcLong, kBcChunk, s);
+    LongRows& lb = long_rows(g, false, kBcLong, kBcChunk, s);
+    DevBuf psum, pgot;
+    psum.alloc(8 * 64 * (size_t)std::max<uint32_t>(std::max(lf.nchunks, lb.nchunks), 1));
+    pgot.alloc(8 * (size_t)std::max<uint32_t>(lf.nchunks, 1));
+    auto wgrid = [](uint64_t waves) {
+      return (unsigned)std::min<uint64_t>(std::max<uint64_t>((waves + 3) / 4, 1), 1u << 16);
+    };
     for (int c0 = 0; c0 < nsrc; c0 += 64) {
       const int nc = std::min(64, nsrc - c0);
       const uint32_t* cs = dsrc.as<uint32_t>() + c0;
@@ -1090,10 +1243,22 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
                                g->off.as<uint64_t>(), (unsigned long long*)bsum.p);
         } else {
           ProfScope ps("bc_forward");
+          if (lf.nchunks)
+            hipLaunchKernelGGL(k_bc_fwd_chunks, dim3(wgrid(lf.nchunks)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
+                               g->cidx.as<uint32_t>(), lf.rows.as<uint32_t>(), lf.cv.as<uint32_t>(),
+                               lf.cb.as<uint64_t>(), lf.nchunks, nc, (const unsigned long long*)fr[cur].p,
+                               (const unsigned long long*)vis.p, sigma.as<double>(), psum.as<double>(),
+                               (unsigned long long*)pgot.p);
           hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), nV, nc,
                              (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
                              (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
                              flag.as<unsigned int>() + 1);
+          if (lf.n)
+            hipLaunchKernelGGL(k_bc_fwd_long_fin, dim3(wgrid(lf.n)), dim3(kB), 0, s, lf.rows.as<uint32_t>(),
+                               lf.cstart.as<uint32_t>(), lf.n, nc, psum.as<double>(),
+                               (const unsigned long long*)pgot.p, (unsigned long long*)fr[nx].p,
+                               (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
+                               flag.as<unsigned int>() + 1);
           hipLaunchKernelGGL(k_bc_queue, dim3(gcap(nV)), dim3(kB), 0, s, nV, (const unsigned long long*)fr[nx].p,
                              g->off.as<uint64_t>(), bq[nx].as<uint32_t>(), flag.as<uint32_t>(),
                              (unsigned long long*)bsum.p);
@@ -1122,9 +1287,19 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
         hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l,
                            (unsigned long long*)msk[mc].p);
         ProfScope ps("bc_backward");
+        if (lb.nchunks)
+          hipLaunchKernelGGL(k_bc_bwd_chunks, dim3(wgrid(lb.nchunks)), dim3(kB), 0, s, g->off.as<uint64_t>(),
+                             g->idx.as<uint32_t>(), lb.rows.as<uint32_t>(), lb.cv.as<uint32_t>(),
+                             lb.cb.as<uint64_t>(), lb.nchunks, qv.as<double>(),
+                             (const unsigned long long*)msk[mc].p, (const unsigned long long*)msk[mc ^ 1].p,
+                             psum.as<double>());
         hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), nV,
                            sigma.as<double>(), delta.as<double>(), qv.as<double>(),
                            (const unsigned long long*)msk[mc].p, (const unsigned long long*)msk[mc ^ 1].p);
+        if (lb.n)
+          hipLaunchKernelGGL(k_bc_bwd_long_fin, dim3(wgrid(lb.n)), dim3(kB), 0, s, lb.rows.as<uint32_t>(),
+                             lb.cstart.as<uint32_t>(), lb.n, psum.as<double>(), sigma.as<double>(),
+                             delta.as<double>(), qv.as<double>(), (const unsigned long long*)msk[mc].p);
         mc ^= 1;
       }
       hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, nV, nc, cs, delta.as<double>(), bc.as<double>());
