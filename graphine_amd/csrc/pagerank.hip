@@ -41,21 +41,25 @@ struct ModeFwd {
   const uint32_t* cid;    // compact source ids (contrib indexed by cid[v]) or nullptr
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
+  bool resid = true;      // accumulate the L1 change (fixed-count runs need it on the last iteration only)
   // what row r's update reads besides its sum, loaded ahead by staged pulls
+  // (R: with the old rank, for the L1 change)
   struct Pre {
     TT pv, old;
     int32_t d;
     uint32_t c;
   };
-  __device__ Pre pre(uint32_t r) const {
+  template <bool R>
+  __device__ Pre pre_r(uint32_t r) const {
     Pre q;
     q.pv = p ? p[r] : (TT)invn;
-    q.old = xin[r];
+    q.old = R ? xin[r] : (TT)0;
     q.d = outdeg[r];
     q.c = (!w && contrib_out && cid) ? cid[r] : 0u;
     return q;
   }
-  __device__ void finalize_pre(uint32_t r, TT s, const Pre& q, double& a0, double& a1) const {
+  template <bool R>
+  __device__ void finalize_r(uint32_t r, TT s, const Pre& q, double& a0, double& a1) const {
     const double dang = scal_in[0];
     const TT y = (TT)damp * (s + (TT)dang * q.pv) + (TT)(1.0 - damp) * q.pv;
     xout[r] = y;
@@ -65,25 +69,36 @@ struct ModeFwd {
       else contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
     }
     if (d == 0) a0 += (double)y;
-    a1 += fabs((double)y - (double)q.old);
+    if (R) a1 += fabs((double)y - (double)q.old);
   }
-  __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const { finalize_pre(r, s, pre(r), a0, a1); }
 };
 
-// The weighting is a compile-time property of the launched mode: a run-time
-// test inside gather() would put each weighted gather in its own branch, where
-// the multiply waits for the loads before the next gather is issued.
-template <typename TT, bool W>
+// The weighting and the L1 change are compile-time properties of the launched
+// mode: a run-time test inside gather() would put each weighted gather in its
+// own branch, where the multiply waits for the loads before the next gather is
+// issued, and a conditional old-rank load slows the row pass the same way.
+template <typename TT, bool W, bool R>
 struct ModeFwdG : ModeFwd<TT> {
+  using Pre = typename ModeFwd<TT>::Pre;
   __device__ TT gather(uint64_t e, uint32_t u) const {
     if (W) return (TT)this->w[e] * this->gin[u];
     return this->gin[u];
   }
+  __device__ Pre pre(uint32_t r) const { return this->template pre_r<R>(r); }
+  __device__ void finalize_pre(uint32_t r, TT s, const Pre& q, double& a0, double& a1) const {
+    this->template finalize_r<R>(r, s, q, a0, a1);
+  }
+  __device__ void finalize(uint32_t r, TT s, double& a0, double& a1) const { finalize_pre(r, s, pre(r), a0, a1); }
 };
 template <typename TT>
 void fwd_pull(const PullPlan& p, const ModeFwd<TT>& m, double* scal_out, hipStream_t s) {
-  if (m.w) pull_pass(p, ModeFwdG<TT, true>{m}, scal_out, s);
-  else pull_pass(p, ModeFwdG<TT, false>{m}, scal_out, s);
+  if (m.w) {
+    if (m.resid) pull_pass(p, ModeFwdG<TT, true, true>{m}, scal_out, s);
+    else pull_pass(p, ModeFwdG<TT, true, false>{m}, scal_out, s);
+  } else {
+    if (m.resid) pull_pass(p, ModeFwdG<TT, false, true>{m}, scal_out, s);
+    else pull_pass(p, ModeFwdG<TT, false, false>{m}, scal_out, s);
+  }
 }
 
 template <typename TT>
@@ -344,9 +359,10 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
 
 // one forward iteration; scal_in/scal_out are device pointers to (acc0, acc1)
 template <typename TT>
-void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, hipStream_t s) {
+void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, hipStream_t s, bool resid = true) {
   ModeFwd<TT> m = fwd_mode<TT>(P, damp);
   m.scal_in = scal_in;
+  m.resid = resid;
   if (use_part(P)) {
     // segment sums over XCD-local source ranges, then each row's segments
     ModeSeg<TT> ms{m.gin, P->xsum.as<TT>()};
@@ -422,7 +438,8 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
       GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       double* sc = P->scal.as<double>();
       fwd_init<TT>(P, s);
-      for (int it = 0; it < iters; ++it) fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s);
+      // only the last iteration's L1 change is returned
+      for (int it = 0; it < iters; ++it) fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s, it + 1 == iters);
       GRE_HIP(hipStreamEndCapture(s, &P->fwd_graph));
       GRE_HIP(hipGraphInstantiate(&P->fwd_exec, P->fwd_graph, nullptr, nullptr, 0));
       P->fwd_key = key;
@@ -443,7 +460,7 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
   int it = 0;
   double res = 0.0;
   for (it = 0; it < iters; ++it) {
-    fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s);
+    fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s, tol > 0.0 || it + 1 == iters);
     if (tol > 0.0) {
       double h[2];
       GRE_HIP(hipMemcpyAsync(h, sc + 2 * (it + 1), sizeof(h), hipMemcpyDeviceToHost, s));

@@ -31,6 +31,27 @@ def test_pagerank_f64_matches_oracle(N, O, schedule, tile_items):
     assert abs(x.sum() - 1.0) < 1e-12
 
 
+@pytest.mark.parametrize("dtype,schedule", [("f64", 6), ("f32", 6), ("f64", 1)])
+def test_fixed_iteration_residual_is_last_change(N, dtype, schedule):
+    # fixed-count runs accumulate the L1 change on the last iteration only; the
+    # returned residual is the change between the last two iterates, replayed
+    # from a captured graph or eager
+    g = N.Graph.rmat(15, 16, 3, False)
+    dt = N.F64 if dtype == "f64" else N.F32
+    for graphs in (True, False):
+        pr = N.PageRank(g, dt)
+        pr.set_schedule(schedule, 8)
+        pr.set_graphs(graphs)
+        pr.run(0.85, 19)
+        x19 = pr.ranks().astype(np.float64)
+        it, res = pr.run(0.85, 20)
+        x20 = pr.ranks().astype(np.float64)
+        assert it == 20
+        ref = np.abs(x20 - x19).sum()
+        assert ref > 0
+        assert abs(res - ref) <= 1e-9 * ref
+
+
 @pytest.mark.parametrize("ranges,shared", [(8, 0), (16, 0), (32, 0), (64, 0), (128, 0), (256, 0),
                                            (64, 1000), (128, 1 << 12), (8, 1 << 30)])
 def test_pagerank_partitioned_ranges(N, O, ranges, shared):
