@@ -777,6 +777,7 @@ struct gre_engine {
   int64_t superstep = 0;
   uint32_t qn = 0;
   uint64_t qedges = 0;
+  bool queue_stale = false;    // counts and prefixes are current, the queue itself is not written yet
   double times[5] = {0, 0, 0, 0, 0};
   int mode = 0;                // GRE_ENGINE_AUTO / _PUSH / _PULL
   int64_t pull_steps = 0;      // supersteps of the last run done in the pull form
@@ -898,12 +899,14 @@ void default_state(gre_engine* e) {
   GRE_HIP(hipGetLastError());
 }
 
+// The frontier's size and out-degree total, from the blocks' counts.  The
+// queue itself (k_write) is written only when a push step needs it: pull
+// steps read the currSet bitmap, which stays unchanged until the next apply.
 void rebuild_queue(gre_engine* e, const Adj& a, hipStream_t s) {
+  (void)a;
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb);
-  hipLaunchKernelGGL(k_write, dim3(e->nb), dim3(kB), 0, s, e->curr.as<uint32_t>(), e->g->nV, a,
-                     e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb, e->queue.as<uint32_t>(),
-                     e->qpos.as<uint64_t>());
   GRE_HIP(hipGetLastError());
+  e->queue_stale = true;
   uint32_t qn = 0;
   uint64_t qe = 0;
   GRE_HIP(hipMemcpyAsync(&qn, e->bcnt.as<uint32_t>() + e->nb, 4, hipMemcpyDeviceToHost, s));
@@ -1053,6 +1056,13 @@ int64_t engine_step(gre_engine* e) {
   } else if (e->qedges > 0) {
     const uint64_t nblk = (e->qedges + kEB - 1) / kEB;
     GRE_CHECK(nblk < 0x7fffffffULL, GRE_DEV_FAILURE, "frontier too large for one launch");
+    if (e->queue_stale) {
+      hipLaunchKernelGGL(k_write, dim3(e->nb), dim3(kB), 0, s, e->curr.as<uint32_t>(), e->g->nV, a,
+                         e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb, e->queue.as<uint32_t>(),
+                         e->qpos.as<uint64_t>());
+      GRE_HIP(hipGetLastError());
+      e->queue_stale = false;
+    }
     ProfScope ps("push_expand");
     dispatch(e->prog, [&](auto P) {
       hipLaunchKernelGGL(k_expand<decltype(P)::value>, dim3((unsigned)nblk), dim3(kB), 0, s, c, a,
