@@ -778,6 +778,8 @@ struct gre_engine {
   uint32_t qn = 0;
   uint64_t qedges = 0;
   bool queue_stale = false;    // counts and prefixes are current, the queue itself is not written yet
+  bool sc_valid = false;       // Sc holds the current scatter data (GRE's PageRank never changes it after init)
+  gre::DevBuf rows_mark;       // nextSet of the rows with a non-empty CSC row (the partitioned pull's)
   double times[5] = {0, 0, 0, 0, 0};
   int mode = 0;                // GRE_ENGINE_AUTO / _PUSH / _PULL
   int64_t pull_steps = 0;      // supersteps of the last run done in the pull form
@@ -932,6 +934,7 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
       GRE_CHECK(srcs[i] < e->g->nV, GRE_DEV_NULL, "source id >= nV");
   hipStream_t s = stream();
   e->dir = dir;
+  e->sc_valid = false;  // init writes the scatter data
   const double t0 = now();
   // run(): superstep != 0 -> currSet.clear(), nextSet.clear() (Engine.hpp:182-193)
   GRE_HIP(hipMemsetAsync(e->curr.p, 0, e->curr.bytes, s));
@@ -1011,10 +1014,21 @@ int64_t engine_step(gre_engine* e) {
       e->xsum.alloc(sizeof(double) * (size_t)e->xp->nseg);
     }
     const uint32_t nvpad = e->nwords * 32;
-    hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->nV, nvpad,
-                       e->next.as<uint32_t>());
-    hipLaunchKernelGGL(k_to_compact, dim3(gcap(g->nV)), dim3(kB), 0, s, (const double*)c.S, g->cid.as<uint32_t>(),
-                       g->nV, e->Sc.as<double>());
+    // every vertex with a non-empty row gets a message: the same words every
+    // superstep, computed once per engine
+    if (!e->rows_mark.p) {
+      e->rows_mark.alloc(4 * (size_t)e->nwords);
+      hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->nV, nvpad,
+                         e->rows_mark.as<uint32_t>());
+      GRE_HIP(hipGetLastError());
+    }
+    GRE_HIP(hipMemcpyAsync(e->next.p, e->rows_mark.p, 4 * (size_t)e->nwords, hipMemcpyDeviceToDevice, s));
+    // GRE's PageRank scatters its init value every superstep (PageRank.hpp:38-47, apply leaves
+    // pr_scatter alone), so its compact copy is made once per run
+    if (!(e->sc_valid && e->prog == GRE_PROG_PAGERANK))
+      hipLaunchKernelGGL(k_to_compact, dim3(gcap(g->nV)), dim3(kB), 0, s, (const double*)c.S, g->cid.as<uint32_t>(),
+                         g->nV, e->Sc.as<double>());
+    e->sc_valid = e->prog == GRE_PROG_PAGERANK;
     pull_pass(e->seg_plan, EngSegMode{e->Sc.as<double>(), e->xsum.as<double>()}, nullptr, s);
     pull_pass(e->row_plan, EngRowMode{e->xsum.as<double>(), (double*)c.C}, nullptr, s);
     ++e->pull_steps;
@@ -1175,6 +1189,7 @@ int gre_engine_upload(gre_engine* e, int which, const void* host) {
     gre::DevBuf& b = arr(e, which);
     GRE_HIP(hipMemcpyAsync(b.p, host, e->es[which] * (size_t)e->g->nV, hipMemcpyHostToDevice, stream()));
     GRE_HIP(hipStreamSynchronize(stream()));
+    e->sc_valid = false;  // the scatter data may have changed
   });
 }
 
