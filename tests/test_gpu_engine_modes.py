@@ -124,6 +124,40 @@ def test_modes_hub_rows(N, O):
         assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
 
 
+def test_partitioned_pagerank_scatter_upload_between_steps(N):
+    # the partitioned pull keeps a compact copy of GRE PageRank's scatter data
+    # for the whole run (the program never changes it); a scatter upload
+    # between supersteps must reach the next superstep, as in the push form
+    import ctypes as C
+    g = N.Graph.rmat(19, 16, 1, False)
+    out = {}
+    for mode in (N.Engine.PULL, N.Engine.PUSH):
+        e = N.Engine(g, N.PAGERANK)
+        e.set_mode(mode)
+        L = N.lib()
+        act = C.c_int64()
+        assert L.gre_engine_begin(e.h, N.DIR_OUT, None, 0) == 0
+        for step in range(4):
+            if step == 2:
+                sc = e.download(N.ARR_SCATTER)
+                e.upload(N.ARR_SCATTER, sc * 0.5)
+            assert L.gre_engine_step(e.h, C.byref(act)) == 0
+        out[mode] = e.download(N.ARR_VERTEX)
+        if mode == N.Engine.PULL:
+            assert e.pull_steps() == 4
+        e.close()
+    g.close()
+    assert np.allclose(out[N.Engine.PULL], out[N.Engine.PUSH], rtol=1e-12, atol=0)
+    # and the halving took effect: the ranks differ from an unmodified run
+    e = N.Engine(g2 := N.Graph.rmat(19, 16, 1, False), N.PAGERANK)
+    e.config_max_supersteps(4)
+    e.run(N.DIR_OUT)
+    plain = e.download(N.ARR_VERTEX)
+    e.close()
+    g2.close()
+    assert not np.allclose(plain, out[N.Engine.PULL], rtol=1e-6, atol=0)
+
+
 def test_modes_pagerank_partitioned(N, O):
     # from 2^22 edges the complete-frontier pull runs over the XCD source
     # partition (segment sums, then each row's segments)
