@@ -244,6 +244,9 @@ def main():
                 "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4),
                 "pull_tile_avg_ms": round(avg_tile_s * 1e3, 4), "fixup_avg_ms": round(avg_fix_s * 1e3, 4),
                 "combine_avg_ms": round(avg_cmb_s * 1e3, 4)}
+    if traffic:
+        # the profiled HBM bytes of an iteration over this run's iteration time
+        roofline["traffic_gbs"] = round(traffic / iter_s / 1e9, 1)
     log(f"iteration kernels {iter_s * 1e3:.3f} ms (tile {avg_tile_s * 1e3:.3f}, fixup {avg_fix_s * 1e3:.3f}, "
         f"combine {avg_cmb_s * 1e3:.3f}) -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
     roofline["schedule"] = (f"{sched}, {tile_items} entries/thread"
