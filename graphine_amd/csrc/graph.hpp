@@ -22,6 +22,7 @@ struct XPart {
   uint32_t H = 0;    // sources [0, H) form one range shared by all XCDs (cut by rows)
   uint64_t nseg = 0;
   DevBuf idx;        // uint32 [E]   compact source ids, grouped by (class, range, row)
+  DevBuf perm;       // uint32 [E]   CSC position of each regrouped entry (edge weights follow it)
   DevBuf seg_off;    // uint32 [nseg+1] (E < 2^32)
   DevBuf pair_off;   // uint32 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
   DevBuf pair_list;  // uint32 [nseg] segment ids, by row then class-major range

@@ -42,6 +42,7 @@ struct ModeFwd {
   const double* scal_in;  // [0] = dangling mass of xin
   double damp, invn;
   bool resid = true;      // accumulate the L1 change (fixed-count runs need it on the last iteration only)
+  bool craw = false;      // contrib_out takes the rank itself (weighted), not rank / out-degree
   // what row r's update reads besides its sum, loaded ahead by staged pulls
   // (R: with the old rank, for the L1 change)
   struct Pre {
@@ -55,7 +56,7 @@ struct ModeFwd {
     q.pv = p ? p[r] : (TT)invn;
     q.old = R ? xin[r] : (TT)0;
     q.d = outdeg[r];
-    q.c = (!w && contrib_out && cid) ? cid[r] : 0u;
+    q.c = (contrib_out && cid) ? cid[r] : 0u;
     return q;
   }
   template <bool R>
@@ -64,9 +65,9 @@ struct ModeFwd {
     const TT y = (TT)damp * (s + (TT)dang * q.pv) + (TT)(1.0 - damp) * q.pv;
     xout[r] = y;
     const int32_t d = q.d;
-    if (!w && contrib_out) {
-      if (cid) { if (d > 0) contrib_out[q.c] = y / (TT)d; }
-      else contrib_out[r] = d > 0 ? y / (TT)d : (TT)0;
+    if (contrib_out) {
+      if (cid) { if (d > 0) contrib_out[q.c] = craw ? y : y / (TT)d; }
+      else contrib_out[r] = craw ? y : (d > 0 ? y / (TT)d : (TT)0);
     }
     if (d == 0) a0 += (double)y;
     if (R) a1 += fabs((double)y - (double)q.old);
@@ -152,12 +153,16 @@ void adj_pull(const PullPlan& p, const ModeAdj<TT>& m, double* scal_out, hipStre
 }
 
 // partitioned pull, first pass: a segment's sum of contrib over its sources
-template <typename TT>
+template <typename TT, bool W>
 struct ModeSeg {
   using T = TT;
   const TT* gin;  // contrib, compact source ids
   TT* sum;        // per segment
-  __device__ TT gather(uint64_t, uint32_t u) const { return gin[u]; }
+  const TT* w;    // W: normalised weights in the partition's entry order
+  __device__ TT gather(uint64_t e, uint32_t u) const {
+    if (W) return w[e] * gin[u];
+    return gin[u];
+  }
   __device__ void finalize(uint32_t seg, TT s, double&, double&) const { sum[seg] = s; }
 };
 
@@ -173,7 +178,7 @@ struct ModeRowSum {
 template <typename TT>
 __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t* __restrict__ outdeg,
                            TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part,
-                           const uint32_t* __restrict__ cid) {
+                           const uint32_t* __restrict__ cid, bool craw) {
   __shared__ double red[8];
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
@@ -182,8 +187,8 @@ __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t*
     x[v] = xv;
     const int32_t d = outdeg[v];
     if (contrib) {
-      if (cid) { if (d > 0) contrib[cid[v]] = xv / (TT)d; }
-      else contrib[v] = d > 0 ? xv / (TT)d : (TT)0;
+      if (cid) { if (d > 0) contrib[cid[v]] = craw ? xv : xv / (TT)d; }
+      else contrib[v] = craw ? xv : (d > 0 ? xv / (TT)d : (TT)0);
     }
     if (d == 0) a0 += (double)xv;
   }
@@ -232,6 +237,14 @@ __global__ void k_normalise(const uint64_t* __restrict__ off, uint32_t nV, uint6
 template <typename TT>
 __global__ void k_permute(const TT* __restrict__ in, const uint64_t* __restrict__ perm, uint64_t n,
                           TT* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[perm[i]];
+}
+
+template <typename TT>
+__global__ void k_permute32(const TT* __restrict__ in, const uint32_t* __restrict__ perm, uint64_t n,
+                            TT* __restrict__ out) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = in[perm[i]];
@@ -292,10 +305,11 @@ void ensure_scal(gre_pr* P, int iters) {
   P->scal.alloc(sizeof(double) * 2 * (size_t)P->scal_cap);
 }
 
-// The compact schedule applies to the unweighted single-vector forward only:
-// its gathers read contrib[], which then holds one entry per vertex with
-// out-edges, hubs first (graph_compact_sources).
-bool use_compact(const gre_pr* P) { return P->compact && !P->weighted && P->k == 1; }
+// The compact schedules apply to the single-vector forward: its gathers read
+// contrib[], which then holds one entry per vertex with out-edges, hubs first
+// (graph_compact_sources); weighted runs multiply by the edge's weight, in CSC
+// order (compact) or in the partition's entry order (part).
+bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
@@ -315,6 +329,23 @@ void ensure_part_plan(gre_pr* P, int ipt) {
   P->plan_px.nt = false;
   const size_t need = tsize(P->dtype) * (size_t)P->xp->nseg;
   if (P->xsum.bytes < need) P->xsum.alloc(need);
+  // weighted: the normalised weights follow the partition's entry order
+  if (P->weighted && (P->wpart_xp != P->xp || P->wpart_ver != P->wver)) {
+    const uint64_t nE = P->g->nE;
+    hipStream_t s = stream();
+    P->wpart.alloc(tsize(P->dtype) * nE);
+    if (nE > 0) {
+      if (P->dtype == GRE_F64)
+        hipLaunchKernelGGL(k_permute32<double>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsc.as<double>(),
+                           P->xp->perm.as<uint32_t>(), nE, P->wpart.as<double>());
+      else
+        hipLaunchKernelGGL(k_permute32<float>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsc.as<float>(),
+                           P->xp->perm.as<uint32_t>(), nE, P->wpart.as<float>());
+      GRE_HIP(hipGetLastError());
+    }
+    P->wpart_xp = P->xp;
+    P->wpart_ver = P->wver;
+  }
 }
 
 void ensure_plan(gre_pr* P, bool csc, int ipt) {
@@ -344,9 +375,12 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
   // gathered x/outdeg ping-pongs between contrib[0] and contrib[1]
   const int xc = P->xcur, cc = P->ccur;
   ModeFwd<TT> m;
+  // weighted and not compact: the ranks themselves are gathered
+  const bool direct = P->weighted && !use_compact(P);
   m.w = P->weighted ? P->wcsc.as<TT>() : nullptr;
-  m.gin = P->weighted ? P->x[xc].as<TT>() : P->contrib[cc].as<TT>();
-  m.contrib_out = P->weighted ? nullptr : P->contrib[cc ^ 1].as<TT>();
+  m.gin = direct ? P->x[xc].as<TT>() : P->contrib[cc].as<TT>();
+  m.contrib_out = direct ? nullptr : P->contrib[cc ^ 1].as<TT>();
+  m.craw = P->weighted;
   m.xin = P->x[xc].as<TT>();
   m.xout = P->weighted ? P->x[xc ^ 1].as<TT>() : P->x[xc].as<TT>();
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
@@ -365,17 +399,19 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
   m.resid = resid;
   if (use_part(P)) {
     // segment sums over XCD-local source ranges, then each row's segments
-    ModeSeg<TT> ms{m.gin, P->xsum.as<TT>()};
-    pull_pass(P->plan_x, ms, nullptr, s);
+    if (P->weighted) pull_pass(P->plan_x, ModeSeg<TT, true>{m.gin, P->xsum.as<TT>(), P->wpart.as<TT>()}, nullptr, s);
+    else pull_pass(P->plan_x, ModeSeg<TT, false>{m.gin, P->xsum.as<TT>(), nullptr}, nullptr, s);
     m.gin = P->xsum.as<TT>();
+    m.w = nullptr;  // the row pass adds segment sums, weights already applied
     fwd_pull(P->plan_px, m, scal_out, s);
+    if (P->weighted) P->xcur ^= 1;
     P->ccur ^= 1;
     return;
   }
   // in-place x for the unweighted form: residual reads xin before xout writes
   fwd_pull(P->plan_csc, m, scal_out, s);
   if (P->weighted) P->xcur ^= 1;
-  else P->ccur ^= 1;
+  if (m.contrib_out) P->ccur ^= 1;
 }
 
 template <typename TT>
@@ -386,8 +422,9 @@ void fwd_init(gre_pr* P, hipStream_t s) {
   P->xcur = 0;
   P->ccur = 0;
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
-                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(), P->weighted ? nullptr : P->contrib[0].as<TT>(),
-                     1.0 / (double)nV, tmp, use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr);
+                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(),
+                     (P->weighted && !use_compact(P)) ? nullptr : P->contrib[0].as<TT>(), 1.0 / (double)nV, tmp,
+                     use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0,
                      P->scal.as<double>());
   GRE_HIP(hipGetLastError());
@@ -402,7 +439,8 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters) {
   std::vector<uint64_t> k = {dbits, (uint64_t)iters, (uint64_t)P->dtype, (uint64_t)P->schedule, (uint64_t)P->ipt,
                              (uint64_t)P->nt, (uint64_t)P->compact, (uint64_t)P->part, (uint64_t)P->weighted,
                              (uint64_t)P->has_p, ptr(P->p.p), ptr(P->scal.p), ptr(P->x[0].p), ptr(P->x[1].p),
-                             ptr(P->contrib[0].p), ptr(P->contrib[1].p), ptr(P->wcsc.p), ptr(P->xsum.p)};
+                             ptr(P->contrib[0].p), ptr(P->contrib[1].p), ptr(P->wcsc.p), ptr(P->xsum.p),
+                             ptr(P->wpart.p), P->wver};
   for (const PullPlan* pl : {&P->plan_csc, &P->plan_x, &P->plan_px}) {
     k.push_back(ptr(pl->off));
     k.push_back(ptr(pl->off32));
@@ -498,11 +536,11 @@ void autotune_forward(gre_pr* P, double damp) {
   hipStream_t s = stream();
   const int ipts[3] = {4, 8, 16};
   std::vector<int> scheds = {GRE_SCHED_TILE, GRE_SCHED_TILE_NT};
-  if (!P->weighted && P->k == 1) {
+  if (P->k == 1) {
     scheds.push_back(GRE_SCHED_COMPACT);
     scheds.push_back(GRE_SCHED_COMPACT_NT);
   }
-  const bool can_part = !P->weighted && P->k == 1 && P->g->nE > 0 && P->g->nE < 0xffffffffULL;
+  const bool can_part = P->k == 1 && P->g->nE > 0 && P->g->nE < 0xffffffffULL;
   if (can_part) {
     scheds.push_back(GRE_SCHED_PART);
     scheds.push_back(GRE_SCHED_PART_NT);
@@ -619,7 +657,7 @@ void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* g
   DevBuf dummy;
   dummy.alloc(sizeof(TT) * (size_t)g->nV);
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, g->nV, x, g->outdeg.as<int32_t>(), dummy.as<TT>(),
-                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr);
+                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr, false);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dsc);
   if (gp_host) {
     hipLaunchKernelGGL(k_grad_p<TT>, dim3(gb), dim3(kB), 0, s, g->nV, lam, dsc, damp, dummy.as<TT>());
@@ -652,7 +690,7 @@ void alloc_state(gre_pr* P) {
   const size_t ts = tsize(P->dtype);
   P->x[0].alloc(ts * n);
   if (P->weighted || P->k > 1) P->x[1].alloc(ts * n);
-  if (!P->weighted && P->k == 1) { P->contrib[0].alloc(ts * n); P->contrib[1].alloc(ts * n); }
+  if (P->k == 1) { P->contrib[0].alloc(ts * n); P->contrib[1].alloc(ts * n); }
 }
 
 void alloc_adjoint(gre_pr* P) {
@@ -819,6 +857,7 @@ int gre_pr_set_edge_weights(gre_pr* P, const float* w, uint64_t seed) {
       }
       GRE_HIP(hipGetLastError());
     }
+    ++P->wver;
     GRE_HIP(hipStreamSynchronize(s));
   });
 }

@@ -45,13 +45,18 @@ struct gre_pr {
   uint32_t shared = 1u << 18;  // hottest sources processed as one range by all XCDs
   std::shared_ptr<gre::XPart> xp;  // the partition in use
   gre::DevBuf xsum;       // per-segment partial sums (compute type)
+  gre::DevBuf wpart;      // weighted: normalised weights in the partition's entry order
+  std::shared_ptr<gre::XPart> wpart_xp;  // the partition and weights version wpart was made for
+  uint64_t wpart_ver = ~0ULL;
+  uint64_t wver = 0;      // bumped by every gre_pr_set_edge_weights
   gre::PullPlan plan_x, plan_px;   // segment pull; per-row pull over the segment sums
   gre::PullBuffers buf_x, buf_px;
   int plan_ipt_x = 0;
   bool has_p = false;
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place
-  gre::DevBuf contrib[2]; // unweighted: x/outdeg ping-pong
+  gre::DevBuf contrib[2]; // gathered per-source values, ping-pong: x/outdeg (unweighted), or the
+                          // ranks in compact source order (weighted, compact and part schedules)
   int xcur = 0;           // x[xcur] holds the current ranks (always 0 when unweighted, k == 1)
   int ccur = 0;           // contrib[ccur] holds the current x/outdeg (unweighted, k == 1)
   gre::DevBuf p;          // personalisation (nV*k), optional

@@ -193,6 +193,12 @@ __global__ void k_xpcount(const uint32_t* __restrict__ seg_row, uint64_t nseg, u
     atomicAdd(&cnt[seg_row[s]], 1u);
 }
 
+__global__ void k_gather_u64(const uint64_t* __restrict__ in, const uint32_t* __restrict__ at, uint64_t n,
+                             uint64_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[at[i]];
+}
+
 __global__ void k_iota32(uint32_t* __restrict__ o, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     o[i] = (uint32_t)i;
@@ -373,13 +379,23 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
                        g->cidx_c.as<uint32_t>(), E, bnd.as<uint32_t>(), R, H, hub_rows.as<uint32_t>(),
                        key.as<uint16_t>(), val.as<uint64_t>());
     GRE_HIP(hipGetLastError());
+    // a stable sort of the entries' positions by key, then the values in that
+    // order (the positions stay: they carry edge weights to the new order)
+    DevBuf pos;
+    pos.alloc(4 * E);
+    xp->perm.alloc(4 * E);
+    hipLaunchKernelGGL(k_iota32, dim3(gcap(E)), dim3(kB), 0, s, pos.as<uint32_t>(), E);
+    GRE_HIP(hipGetLastError());
     size_t tb = 0;
     const unsigned end_bit = bits_of((uint64_t)(8 * per));
-    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), val.as<uint64_t>(),
-                                      val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), pos.as<uint32_t>(),
+                                      xp->perm.as<uint32_t>(), (size_t)E, 0, end_bit, s));
     void* tmp = scratch(0, tb);
-    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), val.as<uint64_t>(),
-                                      val_s.as<uint64_t>(), (size_t)E, 0, end_bit, s));
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, key.as<uint16_t>(), key_s.as<uint16_t>(), pos.as<uint32_t>(),
+                                      xp->perm.as<uint32_t>(), (size_t)E, 0, end_bit, s));
+    hipLaunchKernelGGL(k_gather_u64, dim3(gcap(E)), dim3(kB), 0, s, val.as<uint64_t>(), xp->perm.as<uint32_t>(), E,
+                       val_s.as<uint64_t>());
+    GRE_HIP(hipGetLastError());
     GRE_HIP(hipStreamSynchronize(s));
   }
 

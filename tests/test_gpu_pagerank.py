@@ -191,6 +191,43 @@ def test_weighted_ppr_adjoint_gradients_match_oracle(N, O):
     assert rel_l1(gp, gpref) < 1e-11
 
 
+@pytest.mark.parametrize("schedule,tile_items", [(1, 8), (3, 8), (4, 16), (5, 4), (5, 8), (6, 8)])
+def test_weighted_ppr_schedules_match_oracle(N, O, schedule, tile_items):
+    # compact (3, 4) and partitioned (5, 6) schedules of the weighted forward
+    # gather the ranks in compact source order and multiply by the weights in
+    # CSC order (compact) or in the partition's entry order (partitioned); new
+    # weights between runs reach the partition's copy
+    scale = 14
+    g = N.Graph.rmat(scale, 16, 5, False)
+    og = O.rmat_graph(scale, 16, 5, False)
+    p = ppr_problem(O, og)
+    pr = N.PageRank(g, N.F64, 1, weighted=True)
+    pr.set_schedule(schedule, tile_items)
+    if schedule in (5, 6):
+        pr.set_partition(16, 1 << 10)
+    pr.set_personalization(p)
+    src = np.repeat(np.arange(og.nV), og.outdeg)
+    for seed in (5, 9):
+        pr.set_edge_weights(None, seed=seed)
+        wraw, _ = pr.edge_weights()
+        pr.set_graphs(seed == 9)
+        it, _ = pr.run(0.85, 40)
+        assert it == 40
+        assert pr.schedule() == (N.PageRank.SCHEDULES[schedule], tile_items)
+        # an FP64 run normalises in FP64 (the returned normalised copy is FP32)
+        wr = wraw.astype(np.float64)
+        rs = np.bincount(src, weights=wr, minlength=og.nV)
+        wn = np.where(rs[src] > 0, wr / rs[src], 0.0)
+        xref, _ = O.ppr(og, wn[og.ceid.astype(np.int64)], p, 0.85, tol=0.0, maxit=40)
+        assert rel_l1(pr.ranks(), xref) < 1e-12
+    p32 = N.PageRank(g, N.F32, 1, weighted=True)
+    p32.set_schedule(schedule, tile_items)
+    p32.set_edge_weights(None, seed=9)
+    p32.set_personalization(p)
+    p32.run(0.85, 40)
+    assert rel_l1(p32.ranks(), xref) < 1e-6
+
+
 def test_ppr_f32_forward_with_f64_residual(N, O):
     scale = 14
     g = N.Graph.rmat(scale, 16, 5, False)
