@@ -187,11 +187,13 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
 /* Pull schedules of the forward pass.  All are tile-segmented reductions with
  * tile_items (4, 8 or 16) adjacency entries per thread; _NT streams the
  * neighbour ids with a non-temporal hint; COMPACT gathers from a contrib array
- * indexed by hub-sorted compact source ids (unweighted, single vector).
+ * indexed by hub-sorted compact source ids (single vector; weighted handles
+ * gather the ranks there and multiply by the edge weight).
  * PART additionally cuts the compact sources into ranges, each pinned to one
  * XCD so its gathers stay in that XCD's L2, and sums a row's per-range partial
- * sums in a second pull (unweighted, single vector, fewer than 2^32 edges;
- * ranges set with gre_pr_set_partition).
+ * sums in a second pull (single vector, fewer than 2^32 edges; ranges set with
+ * gre_pr_set_partition; weighted handles keep a copy of the weights in the
+ * partition's entry order, remade when the weights or the partition change).
  * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
  * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
  * ROWS (one group of lanes per destination row, wavefront per vertex).
