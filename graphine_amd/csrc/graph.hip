@@ -306,6 +306,10 @@ __global__ void k_outdeg_u32(const int32_t* __restrict__ d, uint32_t n, uint32_t
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = (uint32_t)d[i];
 }
+__global__ void k_indeg_u32(const uint64_t* __restrict__ coff, uint32_t n, uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)(coff[i + 1] - coff[i]);
+}
 // cid[order[i]] = i for the nsrc vertices with out-edges, ~0 for the others
 __global__ void k_rank(const uint32_t* __restrict__ order, const uint32_t* __restrict__ deg_sorted, uint32_t n,
                        uint32_t* __restrict__ cid, uint32_t* __restrict__ nsrc) {
@@ -322,8 +326,10 @@ __global__ void k_map_idx(const uint32_t* __restrict__ idx, const uint32_t* __re
 }
 }  // namespace
 
-void graph_compact_sources(gre_graph* g) {
-  if (g->cid.p) return;
+// Ranks the vertices by descending degree (ties by vertex id: the sort is
+// stable); id[v] = rank for vertices of non-zero degree, ~0 for the others;
+// idx_c[p] = id[idx[p]]; returns the number of ranked vertices.
+static uint32_t compact_ids(gre_graph* g, bool out_degree, const uint32_t* idx, DevBuf& id, DevBuf& idx_c) {
   hipStream_t s = stream();
   const uint32_t n = g->nV;
   DevBuf keys, keys_sorted, vals, order, cnt;
@@ -333,29 +339,42 @@ void graph_compact_sources(gre_graph* g) {
   order.alloc(4 * (size_t)n);
   cnt.alloc(8);
   GRE_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
-  hipLaunchKernelGGL(k_outdeg_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, g->outdeg.as<int32_t>(), n,
-                     keys.as<uint32_t>());
+  if (out_degree)
+    hipLaunchKernelGGL(k_outdeg_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, g->outdeg.as<int32_t>(), n,
+                       keys.as<uint32_t>());
+  else
+    hipLaunchKernelGGL(k_indeg_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, g->coff.as<uint64_t>(), n,
+                       keys.as<uint32_t>());
   hipLaunchKernelGGL(k_iota_u32, dim3(grid_cap(n)), dim3(kBlock), 0, s, vals.as<uint32_t>(), n);
   GRE_HIP(hipGetLastError());
-  // descending out-degree, ties by vertex id (the sort is stable)
   size_t tb = 0;
   GRE_HIP(rocprim::radix_sort_pairs_desc(nullptr, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
                                          vals.as<uint32_t>(), order.as<uint32_t>(), (size_t)n, 0, 32, s));
   void* tmp = scratch(0, tb);
   GRE_HIP(rocprim::radix_sort_pairs_desc(tmp, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
                                          vals.as<uint32_t>(), order.as<uint32_t>(), (size_t)n, 0, 32, s));
-  g->cid.alloc(4 * (size_t)n);
+  id.alloc(4 * (size_t)n);
   hipLaunchKernelGGL(k_rank, dim3(grid_cap(n)), dim3(kBlock), 0, s, order.as<uint32_t>(), keys_sorted.as<uint32_t>(), n,
-                     g->cid.as<uint32_t>(), cnt.as<uint32_t>());
-  g->cidx_c.alloc(4 * (size_t)g->nE);
+                     id.as<uint32_t>(), cnt.as<uint32_t>());
+  idx_c.alloc(4 * (size_t)g->nE);
   if (g->nE)
-    hipLaunchKernelGGL(k_map_idx, dim3(grid_cap(g->nE)), dim3(kBlock), 0, s, g->cidx.as<uint32_t>(),
-                       g->cid.as<uint32_t>(), g->nE, g->cidx_c.as<uint32_t>());
+    hipLaunchKernelGGL(k_map_idx, dim3(grid_cap(g->nE)), dim3(kBlock), 0, s, idx, id.as<uint32_t>(), g->nE,
+                       idx_c.as<uint32_t>());
   GRE_HIP(hipGetLastError());
   uint32_t ns = 0;
   GRE_HIP(hipMemcpyAsync(&ns, cnt.p, 4, hipMemcpyDeviceToHost, s));
   GRE_HIP(hipStreamSynchronize(s));
-  g->nsrc = ns;
+  return ns;
+}
+
+void graph_compact_sources(gre_graph* g) {
+  if (g->cid.p) return;
+  g->nsrc = compact_ids(g, true, g->cidx.as<uint32_t>(), g->cid, g->cidx_c);
+}
+
+void graph_compact_targets(gre_graph* g) {
+  if (g->did.p) return;
+  g->ntgt = compact_ids(g, false, g->idx.as<uint32_t>(), g->did, g->idx_c);
 }
 
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles) {

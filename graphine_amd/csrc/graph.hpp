@@ -90,6 +90,10 @@ struct gre_graph {
   // cidx_c[p] = cid[cidx[p]] (CSC in compact source ids)
   gre::DevBuf cid, cidx_c;
   uint32_t nsrc = 0;
+  // did[v] = rank of v among vertices with in-edges, by descending in-degree;
+  // idx_c[p] = did[idx[p]] (CSR in compact target ids)
+  gre::DevBuf did, idx_c;
+  uint32_t ntgt = 0;
   // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with handles)
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
   // engine pull row classes by direction (built on demand)
@@ -106,6 +110,8 @@ const uint32_t* graph_w_csc(gre_graph* g);
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles);
 // Builds g->cid / g->cidx_c / g->nsrc once (hub-sorted compact source ids).
 void graph_compact_sources(gre_graph* g);
+// Builds g->did / g->idx_c / g->ntgt once (hub-sorted compact target ids).
+void graph_compact_targets(gre_graph* g);
 // tile_rb over an arbitrary offset array of nrows rows (see graph_tiles).
 void build_tile_rows(const uint64_t* off, uint32_t nrows, uint64_t ntiles, uint32_t te, uint32_t* rb,
                      hipStream_t s);

@@ -105,8 +105,11 @@ void fwd_pull(const PullPlan& p, const ModeFwd<TT>& m, double* scal_out, hipStre
 template <typename TT>
 struct ModeAdj {
   using T = TT;
-  const TT* lin;
+  const TT* lin;          // previous lambda (residual)
   TT* lout;
+  const TT* gin;          // gathered lambda: lin, or its compact-target-order copy
+  TT* lc_out;             // compact schedules: lout in compact target order, else nullptr
+  const uint32_t* tid;    // compact target ids (lc_out indexed by tid[u])
   const TT* w;            // normalised weights, CSR order (weighted), compute type
   const TT* c;
   const TT* p;
@@ -118,9 +121,11 @@ struct ModeAdj {
   struct Pre {
     TT cu, old, pv;
     int32_t d;
+    uint32_t t;
   };
   __device__ Pre pre(uint32_t u) const {
     Pre q;
+    q.t = lc_out ? tid[u] : 0xffffffffu;
     q.d = outdeg[u];
     q.cu = c[u];
     q.old = lin[u];
@@ -133,6 +138,7 @@ struct ModeAdj {
     wl[u] = s;
     const TT y = q.cu + (TT)damp * (s + (d == 0 ? (TT)scal_in[0] : (TT)0));
     lout[u] = y;
+    if (q.t != 0xffffffffu) lc_out[q.t] = y;
     a0 += (double)q.pv * (double)y;
     a1 += fabs((double)y - (double)q.old);
   }
@@ -142,8 +148,8 @@ struct ModeAdj {
 template <typename TT, bool W>
 struct ModeAdjG : ModeAdj<TT> {
   __device__ TT gather(uint64_t e, uint32_t v) const {
-    if (W) return (TT)this->w[e] * this->lin[v];
-    return this->lin[v];
+    if (W) return (TT)this->w[e] * this->gin[v];
+    return this->gin[v];
   }
 };
 template <typename TT>
@@ -198,12 +204,17 @@ __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t*
 
 template <typename TT>
 __global__ void k_adj_init(uint32_t nV, const TT* __restrict__ c, const TT* __restrict__ p, TT* __restrict__ lam,
-                           double invn, double* __restrict__ part) {
+                           double invn, double* __restrict__ part, const uint32_t* __restrict__ tid,
+                           TT* __restrict__ lc) {
   __shared__ double red[8];
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
        v += (uint64_t)gridDim.x * blockDim.x) {
     lam[v] = c[v];
+    if (lc) {
+      const uint32_t t = tid[v];
+      if (t != 0xffffffffu) lc[t] = c[v];
+    }
     a0 += (double)(p ? p[v] : (TT)invn) * (double)c[v];
   }
   block_sum2(a0, a1, red);
@@ -365,6 +376,11 @@ void ensure_plan(gre_pr* P, bool csc, int ipt) {
   if (csc && use_compact(P)) {
     graph_compact_sources(P->g);
     plan.idx = P->g->cidx_c.as<uint32_t>();
+  }
+  // the adjoint of the compact schedules gathers lambda by compact target id
+  if (!csc && use_compact(P)) {
+    graph_compact_targets(P->g);
+    plan.idx = P->g->idx_c.as<uint32_t>();
   }
 }
 
@@ -603,10 +619,16 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
   ensure_plan(P, false, P->ipt);
   ensure_scal(P, iters);
+  const bool cpt = use_compact(P);
+  if (cpt && P->lc[0].bytes < tsize(P->dtype) * (size_t)nV) {
+    P->lc[0].alloc(tsize(P->dtype) * (size_t)nV);
+    P->lc[1].alloc(tsize(P->dtype) * (size_t)nV);
+  }
+  const uint32_t* tid = cpt ? P->g->did.as<uint32_t>() : nullptr;
   double* sc = P->scal.as<double>();
   P->lcur = 0;
   hipLaunchKernelGGL(k_adj_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->c.as<TT>(), P->has_p ? P->p.as<TT>() : nullptr,
-                     P->lam[0].as<TT>(), 1.0 / (double)nV, tmp);
+                     P->lam[0].as<TT>(), 1.0 / (double)nV, tmp, tid, cpt ? P->lc[0].as<TT>() : nullptr);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
   GRE_HIP(hipGetLastError());
   int it = 0;
@@ -615,6 +637,9 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     ModeAdj<TT> m;
     m.lin = P->lam[P->lcur].as<TT>();
     m.lout = P->lam[P->lcur ^ 1].as<TT>();
+    m.gin = cpt ? P->lc[P->lcur].as<TT>() : m.lin;
+    m.lc_out = cpt ? P->lc[P->lcur ^ 1].as<TT>() : nullptr;
+    m.tid = tid;
     m.w = P->weighted ? P->wcsr.as<TT>() : nullptr;
     m.c = P->c.as<TT>();
     m.p = P->has_p ? P->p.as<TT>() : nullptr;

@@ -63,6 +63,7 @@ struct gre_pr {
   gre::DevBuf wraw, wcsr, wcsc, rowsum;  // weights (CSR raw, CSR norm, CSC norm), FP64 row sums
   // adjoint
   gre::DevBuf lam[2], c, wl;
+  gre::DevBuf lc[2];      // adjoint, compact schedules: lambda in compact target order, ping-pong
   int lcur = 0;
   bool have_adj = false;
   bool have_fwd = false;
