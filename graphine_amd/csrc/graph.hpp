@@ -94,7 +94,8 @@ struct gre_graph {
   // idx_c[p] = did[idx[p]] (CSR in compact target ids)
   gre::DevBuf did, idx_c;
   uint32_t ntgt = 0;
-  // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with handles)
+  // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with
+  // handles); over the CSC by (R, H), over the CSR (compact targets) by (-R, H)
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
   // engine pull row classes by direction (built on demand)
   std::map<int, gre::PullRows> pull_rows;
@@ -119,7 +120,8 @@ void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint3
                      hipStream_t s);
 // The source partition of g: the H hottest compact sources as one shared range
 // and the rest in R tail ranges (R in {8, 16, ..., 256}); built once and cached.
-std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H);
+// csc = false partitions the CSR the same way, over compact target ids.
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = true);
 // Its tiles of te entries (cached on the partition).
 const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s);
 // Tiles of te entries over the per-row pair lists (rows = vertices).

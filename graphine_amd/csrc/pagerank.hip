@@ -110,7 +110,8 @@ struct ModeAdj {
   const TT* gin;          // gathered lambda: lin, or its compact-target-order copy
   TT* lc_out;             // compact schedules: lout in compact target order, else nullptr
   const uint32_t* tid;    // compact target ids (lc_out indexed by tid[u])
-  const TT* w;            // normalised weights, CSR order (weighted), compute type
+  const TT* w;            // normalised weights, CSR order (weighted and gathered here), compute type
+  bool unw;               // unweighted: the sum is divided by the out-degree
   const TT* c;
   const TT* p;
   const int32_t* outdeg;
@@ -134,7 +135,7 @@ struct ModeAdj {
   }
   __device__ void finalize_pre(uint32_t u, TT s, const Pre& q, double& a0, double& a1) const {
     const int32_t d = q.d;
-    if (!w && d > 0) s = s / (TT)d;
+    if (unw && d > 0) s = s / (TT)d;
     wl[u] = s;
     const TT y = q.cu + (TT)damp * (s + (d == 0 ? (TT)scal_in[0] : (TT)0));
     lout[u] = y;
@@ -359,10 +360,51 @@ void ensure_part_plan(gre_pr* P, int ipt) {
   }
 }
 
+// the adjoint's partition: the same (R, H) over the CSR in compact target ids
+void ensure_part_plan_adj(gre_pr* P, int ipt) {
+  if (!P->xpa || P->xpa->R != P->ranges || P->xpa->H != std::min(P->shared, P->g->ntgt)) {
+    P->xpa = graph_xpart(P->g, P->ranges, P->shared, false);
+    P->plan_ipt_xa = 0;
+  }
+  if (P->plan_ipt_xa != ipt) {
+    P->buf_xa.make(P->plan_xa, *P->xpa, ipt, tsize(P->dtype));
+    P->buf_pxa.make_pairs(P->plan_pxa, *P->xpa, P->g->nV, 8, tsize(P->dtype));
+    P->plan_xa.tag = "adj_part_segments";
+    P->plan_pxa.tag = "adj_part_combine";
+    P->plan_pxa.fix_tag = "adj_part_combine_fixup";
+    P->plan_ipt_xa = ipt;
+  }
+  P->plan_xa.nt = P->nt;
+  P->plan_pxa.nt = false;
+  const size_t need = tsize(P->dtype) * (size_t)P->xpa->nseg;
+  if (P->asum.bytes < need) P->asum.alloc(need);
+  if (P->weighted && (P->wparta_xp != P->xpa || P->wparta_ver != P->wver)) {
+    const uint64_t nE = P->g->nE;
+    hipStream_t s = stream();
+    P->wparta.alloc(tsize(P->dtype) * nE);
+    if (nE > 0) {
+      if (P->dtype == GRE_F64)
+        hipLaunchKernelGGL(k_permute32<double>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<double>(),
+                           P->xpa->perm.as<uint32_t>(), nE, P->wparta.as<double>());
+      else
+        hipLaunchKernelGGL(k_permute32<float>, dim3(gcap(nE)), dim3(kB), 0, s, P->wcsr.as<float>(),
+                           P->xpa->perm.as<uint32_t>(), nE, P->wparta.as<float>());
+      GRE_HIP(hipGetLastError());
+    }
+    P->wparta_xp = P->xpa;
+    P->wparta_ver = P->wver;
+  }
+}
+
 void ensure_plan(gre_pr* P, bool csc, int ipt) {
   if (csc && use_part(P)) {
     graph_compact_sources(P->g);
     ensure_part_plan(P, ipt);
+    return;
+  }
+  if (!csc && use_part(P)) {
+    graph_compact_targets(P->g);
+    ensure_part_plan_adj(P, ipt);
     return;
   }
   int& have = csc ? P->plan_ipt_csc : P->plan_ipt_csr;
@@ -641,6 +683,7 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.lc_out = cpt ? P->lc[P->lcur ^ 1].as<TT>() : nullptr;
     m.tid = tid;
     m.w = P->weighted ? P->wcsr.as<TT>() : nullptr;
+    m.unw = !P->weighted;
     m.c = P->c.as<TT>();
     m.p = P->has_p ? P->p.as<TT>() : nullptr;
     m.outdeg = P->g->outdeg.as<int32_t>();
@@ -648,7 +691,18 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal_in = sc + 2 * it;
     m.damp = damp;
     m.invn = 1.0 / (double)nV;
-    adj_pull(P->plan_csr, m, sc + 2 * (it + 1), s);
+    if (use_part(P)) {
+      // segment sums over XCD-local ranges of compact targets, then each row's segments
+      if (P->weighted)
+        pull_pass(P->plan_xa, ModeSeg<TT, true>{m.gin, P->asum.as<TT>(), P->wparta.as<TT>()}, nullptr, s);
+      else
+        pull_pass(P->plan_xa, ModeSeg<TT, false>{m.gin, P->asum.as<TT>(), nullptr}, nullptr, s);
+      m.gin = P->asum.as<TT>();
+      m.w = nullptr;
+      adj_pull(P->plan_pxa, m, sc + 2 * (it + 1), s);
+    } else {
+      adj_pull(P->plan_csr, m, sc + 2 * (it + 1), s);
+    }
     P->lcur ^= 1;
     if (tol > 0.0) {
       double h[2];

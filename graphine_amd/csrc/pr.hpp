@@ -39,7 +39,7 @@ struct gre_pr {
   int ipt = 0;            // 0 = autotune on first run
   int schedule = 0;       // 0 = autotune; else GRE_SCHED_* (gre_dev.h)
   bool nt = false;        // non-temporal neighbour-id stream
-  bool compact = false;   // hub-sorted compact source ids (unweighted, k == 1)
+  bool compact = false;   // hub-sorted compact source ids (k == 1)
   bool part = false;      // source-range partitioned pull (implies compact)
   int ranges = 64;        // tail source ranges of the partitioned pull
   uint32_t shared = 1u << 18;  // hottest sources processed as one range by all XCDs
@@ -64,6 +64,15 @@ struct gre_pr {
   // adjoint
   gre::DevBuf lam[2], c, wl;
   gre::DevBuf lc[2];      // adjoint, compact schedules: lambda in compact target order, ping-pong
+  // adjoint, part schedules: the same partition over the CSR (rows = sources,
+  // ranges of compact targets), its segment sums and weights in its entry order
+  std::shared_ptr<gre::XPart> xpa;
+  gre::DevBuf asum, wparta;
+  std::shared_ptr<gre::XPart> wparta_xp;
+  uint64_t wparta_ver = ~0ULL;
+  gre::PullPlan plan_xa, plan_pxa;
+  gre::PullBuffers buf_xa, buf_pxa;
+  int plan_ipt_xa = 0;
   int lcur = 0;
   bool have_adj = false;
   bool have_fwd = false;

@@ -39,6 +39,14 @@ __global__ void k_deg_compact(const int32_t* __restrict__ outdeg, const uint32_t
     if (d > 0) degc[cid[v]] = (uint64_t)d;
   }
 }
+// the same for compact target ids: in-degrees from the CSC offsets
+__global__ void k_indeg_compact(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ did, uint32_t nV,
+                                uint64_t* __restrict__ degc) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = coff[v + 1] - coff[v];
+    if (d > 0) degc[did[v]] = d;
+  }
+}
 
 // Tail range bounds: bnd[r] = first compact id c in [H, nsrc] with
 // ex[c] >= ex[H] + r * (E - ex[H]) / R (ex: exclusive prefix of the compact
@@ -314,13 +322,19 @@ unsigned bits_of(uint64_t n) {
 
 }  // namespace
 
-std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc) {
   GRE_CHECK(R == 8 || R == 16 || R == 32 || R == 64 || R == 128 || R == 256, GRE_DEV_NULL,
             "source ranges must be 8, 16, 32, 64, 128 or 256");
-  graph_compact_sources(g);
-  const uint32_t nsrc = g->nsrc;
+  // csc: rows are destinations, gathering from compact sources (the forward);
+  // otherwise rows are sources, gathering from compact targets (the adjoint)
+  if (csc) graph_compact_sources(g);
+  else graph_compact_targets(g);
+  const uint32_t nsrc = csc ? g->nsrc : g->ntgt;
+  const uint64_t* roff = (csc ? g->coff : g->off).as<uint64_t>();
+  const uint32_t* ridx = (csc ? g->cidx_c : g->idx_c).as<uint32_t>();
   H = std::min(H, nsrc);
-  auto it = g->xpart.find(std::make_pair(R, H));
+  const auto mkey = std::make_pair(csc ? R : -R, H);
+  auto it = g->xpart.find(mkey);
   if (it != g->xpart.end()) return it->second;
   const uint64_t E = g->nE;
   const uint32_t nV = g->nV;
@@ -339,8 +353,12 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     degc.alloc(8 * ((size_t)nsrc + 1));
     ex.alloc(8 * ((size_t)nsrc + 1));
     GRE_HIP(hipMemsetAsync(degc.p, 0, 8 * ((size_t)nsrc + 1), s));
-    hipLaunchKernelGGL(k_deg_compact, dim3(gcap(nV)), dim3(kB), 0, s, g->outdeg.as<int32_t>(), g->cid.as<uint32_t>(),
-                       nV, degc.as<uint64_t>());
+    if (csc)
+      hipLaunchKernelGGL(k_deg_compact, dim3(gcap(nV)), dim3(kB), 0, s, g->outdeg.as<int32_t>(),
+                         g->cid.as<uint32_t>(), nV, degc.as<uint64_t>());
+    else
+      hipLaunchKernelGGL(k_indeg_compact, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
+                         g->did.as<uint32_t>(), nV, degc.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     scan_excl_u64(degc.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nsrc + 1, s);
     hipLaunchKernelGGL(k_bounds, dim3(1), dim3(320), 0, s, ex.as<uint64_t>(), H, nsrc, E, R, bnd.as<uint32_t>());
@@ -353,9 +371,9 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     ex.alloc(8 * ((size_t)nV + 1));
     GRE_HIP(hipMemsetAsync(cnt.p, 0, 8 * ((size_t)nV + 1), s));
     uint64_t nt = 0;
-    const uint32_t* rb = graph_tiles(g, true, 2048, &nt);
-    hipLaunchKernelGGL(k_hub_rowcnt, dim3(gcap((E + 7) / 8)), dim3(kB), 0, s, g->coff.as<uint64_t>(), rb, 2048u,
-                       g->cidx_c.as<uint32_t>(), E, H, cnt.as<unsigned long long>());
+    const uint32_t* rb = graph_tiles(g, csc, 2048, &nt);
+    hipLaunchKernelGGL(k_hub_rowcnt, dim3(gcap((E + 7) / 8)), dim3(kB), 0, s, roff, rb, 2048u, ridx, E, H,
+                       cnt.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
     scan_excl_u64(cnt.as<uint64_t>(), ex.as<uint64_t>(), (size_t)nV + 1, s);
     hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, s, ex.as<uint64_t>(), nV, hub_rows.as<uint32_t>());
@@ -374,9 +392,9 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
     val.alloc(8 * E);
     uint64_t nt = 0;
     const uint32_t te = 2048;
-    const uint32_t* rb = graph_tiles(g, true, te, &nt);
-    hipLaunchKernelGGL(k_xkeys, dim3(gcap(E)), dim3(kB), 0, s, g->coff.as<uint64_t>(), rb, te,
-                       g->cidx_c.as<uint32_t>(), E, bnd.as<uint32_t>(), R, H, hub_rows.as<uint32_t>(),
+    const uint32_t* rb = graph_tiles(g, csc, te, &nt);
+    hipLaunchKernelGGL(k_xkeys, dim3(gcap(E)), dim3(kB), 0, s, roff, rb, te, ridx, E, bnd.as<uint32_t>(), R, H,
+                       hub_rows.as<uint32_t>(),
                        key.as<uint16_t>(), val.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     // a stable sort of the entries' positions by key, then the values in that
@@ -475,7 +493,7 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H) {
                                       xp->pair_list.as<uint32_t>(), (size_t)nseg, 0, vb, s));
     GRE_HIP(hipStreamSynchronize(s));
   }
-  g->xpart[std::make_pair(R, H)] = xp;
+  g->xpart[mkey] = xp;
   return xp;
 }
 

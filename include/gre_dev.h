@@ -195,7 +195,8 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * gre_pr_set_partition; weighted handles keep a copy of the weights in the
  * partition's entry order, remade when the weights or the partition change).
  * Under COMPACT and PART the single-vector adjoint pulls over the CSR in
- * compact target ids (vertices with in-edges by descending in-degree).
+ * compact target ids (vertices with in-edges by descending in-degree); under
+ * PART with the same ranges, partitioned over the CSR.
  * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
  * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
  * ROWS (one group of lanes per destination row, wavefront per vertex).
