@@ -323,10 +323,24 @@ void ensure_scal(gre_pr* P, int iters) {
 // order (compact) or in the partition's entry order (part).
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
+// drops the cached partitions that no handle or engine holds any more
+void prune_xparts(gre_graph* g) {
+  for (auto it = g->xpart.begin(); it != g->xpart.end();) {
+    if (it->second.use_count() == 1) it = g->xpart.erase(it);
+    else ++it;
+  }
+}
+
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
   if (!P->xp || P->xp->R != P->ranges || P->xp->H != std::min(P->shared, P->g->nsrc)) {
+    // let go of the previous partition (and the weight copy that follows it),
+    // take the new one, then drop the old one unless someone else holds it
+    P->xp.reset();
+    P->wpart_xp.reset();
+    P->wpart_ver = ~0ULL;
     P->xp = graph_xpart(P->g, P->ranges, P->shared);
+    if (!P->tuning) prune_xparts(P->g);
     P->plan_ipt_x = 0;
   }
   if (P->plan_ipt_x != ipt) {
@@ -363,7 +377,11 @@ void ensure_part_plan(gre_pr* P, int ipt) {
 // the adjoint's partition: the same (R, H) over the CSR in compact target ids
 void ensure_part_plan_adj(gre_pr* P, int ipt) {
   if (!P->xpa || P->xpa->R != P->ranges || P->xpa->H != std::min(P->shared, P->g->ntgt)) {
+    P->xpa.reset();
+    P->wparta_xp.reset();
+    P->wparta_ver = ~0ULL;
     P->xpa = graph_xpart(P->g, P->ranges, P->shared, false);
+    prune_xparts(P->g);
     P->plan_ipt_xa = 0;
   }
   if (P->plan_ipt_xa != ipt) {
@@ -610,6 +628,7 @@ void autotune_forward(gre_pr* P, double damp) {
   GRE_HIP(hipEventCreate(&a));
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
+  P->tuning = true;
   for (int sc : scheds) {
     const bool prt = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
     // partitioned schedules also try (tail ranges, shared sources) pairs
@@ -645,12 +664,16 @@ void autotune_forward(gre_pr* P, double damp) {
   apply_schedule(P, best_s, best_i);
   P->ranges = best_r;
   P->shared = best_h;
-  if (!P->part) P->xp.reset();
-  // partitions of the candidates that lost, unless another handle holds them
-  for (auto it = P->g->xpart.begin(); it != P->g->xpart.end();) {
-    if (it->second.use_count() == 1) it = P->g->xpart.erase(it);
-    else ++it;
-  }
+  // the handle still holds the last candidate's partition: take the winner's
+  // from the cache, then drop the candidates that lost (unless another handle
+  // holds them)
+  P->tuning = false;
+  P->xp.reset();
+  P->wpart_xp.reset();
+  P->wpart_ver = ~0ULL;
+  P->plan_ipt_x = 0;
+  if (P->part) ensure_plan(P, true, best_i);
+  else prune_xparts(P->g);
 }
 
 template <typename TT>

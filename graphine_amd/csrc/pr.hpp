@@ -41,6 +41,7 @@ struct gre_pr {
   bool nt = false;        // non-temporal neighbour-id stream
   bool compact = false;   // hub-sorted compact source ids (k == 1)
   bool part = false;      // source-range partitioned pull (implies compact)
+  bool tuning = false;    // the autotuner is running: keep the candidates' partitions cached
   int ranges = 64;        // tail source ranges of the partitioned pull
   uint32_t shared = 1u << 18;  // hottest sources processed as one range by all XCDs
   std::shared_ptr<gre::XPart> xp;  // the partition in use
