@@ -44,6 +44,7 @@ struct BFwd {  // forward: gather x (CSC), finalise ranks (raw sums go to xout)
   const TT* xin; TT* xout; const TT* w; const TT* p; const int32_t* outdeg;
   const double* scal;  // [2k]: dang_j at scal[2j]
   double damp; int k;
+  bool resid = true;   // accumulate the L1 change (fixed-count runs need it on the last iteration only)
   __device__ TT* sums() const { return xout; }
   __device__ TT gather(uint64_t e, uint32_t u, int j) const { return (TT)w[e] * xin[(uint64_t)u * k + j]; }
   __device__ TT wt(uint64_t e) const { return (TT)w[e]; }
@@ -52,10 +53,9 @@ struct BFwd {  // forward: gather x (CSC), finalise ranks (raw sums go to xout)
     const uint64_t o = (uint64_t)r * k + j;
     const TT pv = p[o];
     const TT y = (TT)damp * (s + (TT)scal[2 * j] * pv) + (TT)(1.0 - damp) * pv;
-    const TT old = xin[o];
+    if (resid) a1 += fabs((double)y - (double)xin[o]);
     xout[o] = y;
     if (outdeg[r] == 0) a0 += (double)y;
-    a1 += fabs((double)y - (double)old);
   }
 };
 
@@ -64,18 +64,18 @@ struct BAdj {  // adjoint: gather lambda (CSR), finalise lambda (raw sums go to 
   const TT* lin; TT* lout; const TT* w; const TT* c; const TT* p; const int32_t* outdeg; TT* wl;
   const double* scal;  // [2k]: p.lambda_j at scal[2j]
   double damp; int k;
+  bool resid = true;   // accumulate the L1 change (fixed-count runs need it on the last iteration only)
   __device__ TT* sums() const { return wl; }
   __device__ TT gather(uint64_t e, uint32_t v, int j) const { return (TT)w[e] * lin[(uint64_t)v * k + j]; }
   __device__ TT wt(uint64_t e) const { return (TT)w[e]; }
   __device__ TT val(uint32_t v, int j) const { return lin[(uint64_t)v * k + j]; }
   __device__ void finalize(uint32_t u, int j, TT s, double& a0, double& a1) const {
+    // s was read from wl (sums()), where the pass left the raw sum for the gradients
     const uint64_t o = (uint64_t)u * k + j;
-    wl[o] = s;
     const TT y = c[o] + (TT)damp * (s + (outdeg[u] == 0 ? (TT)scal[2 * j] : (TT)0));
-    const TT old = lin[o];
+    if (resid) a1 += fabs((double)y - (double)lin[o]);
     lout[o] = y;
     a0 += (double)p[o] * (double)y;
-    a1 += fabs((double)y - (double)old);
   }
 };
 
@@ -544,6 +544,7 @@ double bforward(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal = sc + 2 * k * it;  // a0_j at index 2j
     m.damp = damp;
     m.k = k;
+    m.resid = tol > 0.0 || it + 1 >= iters;
     bpass_g<BFwd<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s, sched);
     P->xcur ^= 1;
   };
@@ -616,6 +617,7 @@ double badjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal = sc + 2 * k * it;
     m.damp = damp;
     m.k = k;
+    m.resid = tol > 0.0 || it + 1 >= iters;
     bpass_g<BAdj<TT>, TT>(b, P, m, sc + 2 * k * (it + 1), s, sched);
     P->lcur ^= 1;
   };
