@@ -137,12 +137,22 @@ def test_pagerank_edge_cases(N, O):
     dst = np.concatenate([hub_dst, np.zeros(3000, np.uint32), np.array([11, 10], np.uint32)])
     g = N.Graph.from_edges(n, src, dst)
     og = O.graph_from_edges(n, src, dst)
+    # the adjoint too (the transposed iteration: the hub's out-row is one long CSR row)
+    tg = np.array([0, 1, 10, 4000, 4999], np.uint32)
+    c = np.zeros(n)
+    c[tg] = 1.0
+    srcs = np.repeat(np.arange(og.nV), og.outdeg)
+    wn = 1.0 / og.outdeg[srcs].astype(np.float64)
+    uni = np.full(n, 1.0 / n)
+    lref, _ = O.ppr_adjoint(og, wn, uni, c, 0.85, tol=0.0, maxit=20)
     for sc in (1, 3, 5):
         for ti in (4, 8, 16):
             pr = N.PageRank(g, N.F64)
             pr.set_schedule(sc, ti)
             pr.run(0.85, 20)
             assert rel_l1(pr.ranks(), O.pagerank(og, 0.85, 20)) < 1e-12
+            pr.adjoint(tg, 0.85, 20)
+            assert rel_l1(pr.adjoint_vector(), lref) < 1e-12
 
 
 def ppr_problem(O, og, seed=5, nseeds=50):
