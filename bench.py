@@ -69,17 +69,32 @@ def sample_distinct(rng, pool, k):
     return pick if isinstance(pool, int) else np.asarray(pool)[pick]
 
 
-def pmc_traffic(scale, sched):
-    """HBM bytes per iteration and the profiled configuration, from the committed rocprofv3 --pmc
-    summary of this schedule family (plain or _nt), if present."""
-    f = os.path.join(ROOT, "profiles", f"pagerank_f32_{sched.replace('_nt', '')}_s{scale}_pmc.txt")
-    if not os.path.exists(f):
-        return None, None
+def read_pmc(f):
     vals = {}
     for line in open(f):
         if "=" in line and not line.startswith("#"):
             k, v = (x.strip() for x in line.split("=", 1))
             vals[k] = v
+    return vals
+
+
+def pmc_traffic(scale, sched, spec):
+    """HBM bytes per iteration and the profiled configuration, from the committed rocprofv3 --pmc
+    summaries: the one profiled with exactly this schedule (spec = NAME:IPT[:R:H]) if there is
+    one, else the one of this schedule family (plain or _nt), if present."""
+    import glob
+    cands = sorted(glob.glob(os.path.join(ROOT, "profiles", f"pagerank_f32_*_s{scale}_pmc.txt")))
+    fam = os.path.join(ROOT, "profiles", f"pagerank_f32_{sched.replace('_nt', '')}_s{scale}_pmc.txt")
+    pick = None
+    for f in cands:
+        if read_pmc(f).get("schedule") == spec:
+            pick = f
+            break
+    if pick is None and os.path.exists(fam):
+        pick = fam
+    if pick is None:
+        return None, None
+    vals = read_pmc(pick)
     return (float(vals["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in vals else None,
             vals.get("schedule"))
 
@@ -237,9 +252,13 @@ def main():
     sched, tile_items = pr.schedule()
     kern = ("k_pull_tile + k_pull_fixup (segments), then k_pull_staged + k_pull_fixup (per row)" if n_cmb
             else "k_pull_tile + k_pull_fixup")
-    traffic, traffic_sched = pmc_traffic(S, sched)
+    spec = f"{sched}:{tile_items}"
+    if sched.startswith("part"):
+        spec += ":{}:{}".format(*pr.partition())
+    traffic, traffic_sched = pmc_traffic(S, sched, spec)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_schedule": traffic_sched,
+                "traffic_schedule_is_measured_schedule": traffic_sched == spec,
                 "kernel": f"{kern} (one PageRank f32 iteration)",
                 "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4),
                 "pull_tile_avg_ms": round(avg_tile_s * 1e3, 4), "fixup_avg_ms": round(avg_fix_s * 1e3, 4),
