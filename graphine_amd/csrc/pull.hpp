@@ -274,7 +274,12 @@ __global__ __launch_bounds__(kPullBlock) void k_pull_staged(
   __shared__ uint32_t longj[TE / kLongRow + 2];
   __shared__ int nlong;
   __shared__ double red[8];
-  const uint64_t t = blockIdx.x;
+  // workgroup b runs on XCD b % 8; within each group of 64 tiles XCD x takes
+  // tiles 8x .. 8x+7, so the consecutive tiles that fetch the two ends of the
+  // same runs' lines share one L2 (a bijection; the last partial group keeps
+  // its order)
+  const uint64_t bid = blockIdx.x;
+  const uint64_t t = bid < (ntiles & ~(uint64_t)63) ? (bid & ~(uint64_t)63) + ((bid & 7) << 3) + ((bid & 63) >> 3) : bid;
   const int tid = (int)threadIdx.x;
   const uint64_t e0l = t * (uint64_t)TE;
   const Off e0 = (Off)e0l;
