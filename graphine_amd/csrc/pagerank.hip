@@ -617,10 +617,6 @@ void autotune_forward(gre_pr* P, double damp) {
     scheds.push_back(GRE_SCHED_COMPACT_NT);
   }
   const bool can_part = P->k == 1 && P->g->nE > 0 && P->g->nE < 0xffffffffULL;
-  if (can_part) {
-    scheds.push_back(GRE_SCHED_PART);
-    scheds.push_back(GRE_SCHED_PART_NT);
-  }
   int best_s = GRE_SCHED_TILE, best_i = 8, best_r = P->ranges;
   uint32_t best_h = P->shared;
   float best_ms = 1e30f;
@@ -629,34 +625,44 @@ void autotune_forward(gre_pr* P, double damp) {
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
   P->tuning = true;
-  for (int sc : scheds) {
-    const bool prt = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
-    // partitioned schedules also try (tail ranges, shared sources) pairs
-    std::vector<std::pair<int, uint32_t>> vars = {{P->ranges, P->shared}};
-    if (prt) vars = {{128, 0u}, {256, 0u}, {128, 1u << 17}, {256, 1u << 16}, {64, 1u << 18}};
+  auto trial = [&](int sc, int ipt, bool prt, const std::pair<int, uint32_t>& h) {
+    apply_schedule(P, sc, ipt);
+    if (prt) { P->ranges = h.first; P->shared = h.second; }
+    ensure_plan(P, true, ipt);
+    fwd_init<TT>(P, s);
+    double* scal = P->scal.as<double>();
+    fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
+    GRE_HIP(hipEventRecord(a, s));
+    fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
+    fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
+    GRE_HIP(hipEventRecord(b, s));
+    GRE_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GRE_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_s = sc;
+      best_i = ipt;
+      if (prt) { best_r = h.first; best_h = h.second; }
+    }
+  };
+  for (int sc : scheds)
+    for (int ipt : ipts) trial(sc, ipt, false, {P->ranges, P->shared});
+  // partitioned schedules try (tail ranges, shared sources) pairs, one
+  // partition at a time: only the best one so far stays cached beside it
+  std::shared_ptr<XPart> best_xp;
+  if (can_part) {
+    const std::pair<int, uint32_t> vars[5] = {{128, 0u}, {256, 0u}, {128, 1u << 17}, {256, 1u << 16}, {64, 1u << 18}};
     for (const auto& h : vars) {
-      for (int ipt : ipts) {
-        if (prt && ipt == 16) continue;
-        apply_schedule(P, sc, ipt);
-        if (prt) { P->ranges = h.first; P->shared = h.second; }
-        ensure_plan(P, true, ipt);
-        fwd_init<TT>(P, s);
-        double* scal = P->scal.as<double>();
-        fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
-        GRE_HIP(hipEventRecord(a, s));
-        fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
-        fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
-        GRE_HIP(hipEventRecord(b, s));
-        GRE_HIP(hipEventSynchronize(b));
-        float ms = 0.f;
-        GRE_HIP(hipEventElapsedTime(&ms, a, b));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best_s = sc;
-          best_i = ipt;
-          if (prt) { best_r = h.first; best_h = h.second; }
-        }
-      }
+      for (int sc : {GRE_SCHED_PART, GRE_SCHED_PART_NT})
+        for (int ipt : {4, 8}) trial(sc, ipt, true, h);
+      if ((best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT) && best_r == h.first && best_h == h.second)
+        best_xp = P->xp;
+      P->xp.reset();
+      P->wpart_xp.reset();
+      P->wpart_ver = ~0ULL;
+      P->plan_ipt_x = 0;
+      prune_xparts(P->g);
     }
   }
   (void)hipEventDestroy(a);
@@ -673,7 +679,8 @@ void autotune_forward(gre_pr* P, double damp) {
   P->wpart_ver = ~0ULL;
   P->plan_ipt_x = 0;
   if (P->part) ensure_plan(P, true, best_i);
-  else prune_xparts(P->g);
+  best_xp.reset();
+  prune_xparts(P->g);
 }
 
 template <typename TT>
@@ -855,7 +862,12 @@ int gre_pr_destroy(gre_pr* P) {
   return guard([&] {
     if (!P) return;
     GRE_HIP(hipStreamSynchronize(stream()));
+    // the graph keeps this handle's partitions cached for the next handle and
+    // drops the others that nothing holds any more
+    gre_graph* g = P->g;
+    std::shared_ptr<XPart> keep_f = P->xp, keep_a = P->xpa;
     delete P;
+    prune_xparts(g);
   });
 }
 
