@@ -624,7 +624,12 @@ void autotune_forward(gre_pr* P, double damp) {
   GRE_HIP(hipEventCreate(&a));
   GRE_HIP(hipEventCreate(&b));
   ensure_scal(P, 4);
-  P->tuning = true;
+  // candidates' partitions stay cached while tuning; cleared on every exit
+  struct Tuning {
+    gre_pr* P;
+    explicit Tuning(gre_pr* p) : P(p) { P->tuning = true; }
+    ~Tuning() { P->tuning = false; }
+  } tuning_scope(P);
   auto trial = [&](int sc, int ipt, bool prt, const std::pair<int, uint32_t>& h) {
     apply_schedule(P, sc, ipt);
     if (prt) { P->ranges = h.first; P->shared = h.second; }
