@@ -476,7 +476,15 @@ int batch_sched(gre_pr* P, int d, hipStream_t s, F&& pass) {
   float best = 1e30f;
   int pick = GRE_SCHED_TILE;
   for (int sc : {GRE_SCHED_TILE, GRE_SCHED_ROWS}) {
+    // the warm-up pass is timed too: a candidate whose warm-up alone takes
+    // twice the best's two timed passes is dropped without timing them
+    GRE_HIP(hipEventRecord(a, s));
     pass(sc, 0);
+    GRE_HIP(hipEventRecord(b, s));
+    GRE_HIP(hipEventSynchronize(b));
+    float warm = 0.f;
+    GRE_HIP(hipEventElapsedTime(&warm, a, b));
+    if (warm > 2.f * best) continue;
     GRE_HIP(hipEventRecord(a, s));
     pass(sc, 1);
     pass(sc, 2);
