@@ -296,7 +296,7 @@ def main():
         sweep = {}
         cand = [(sc, ti, 64, 0) for sc in (1, 2, 3, 4) for ti in (4, 8, 16)]
         # grouped by partition: the graph keeps the partition of the last closed handle cached
-        cand += [(sc, ti, r, h) for r, h in ((128, 0), (256, 0), (128, 1 << 17), (256, 1 << 16), (64, 1 << 18))
+        cand += [(sc, ti, r, h) for r, h in ((256, 1 << 12), (128, 1 << 12), (256, 0), (128, 1 << 17), (64, 1 << 18))
                  for sc in (5, 6) for ti in (4, 8)]
         for sc, ti, r, h in cand:
             q = N.PageRank(g, N.F32)

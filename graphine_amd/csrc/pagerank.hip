@@ -657,7 +657,7 @@ void autotune_forward(gre_pr* P, double damp) {
   // partition at a time: only the best one so far stays cached beside it
   std::shared_ptr<XPart> best_xp;
   if (can_part) {
-    const std::pair<int, uint32_t> vars[5] = {{128, 0u}, {256, 0u}, {128, 1u << 17}, {256, 1u << 16}, {64, 1u << 18}};
+    const std::pair<int, uint32_t> vars[5] = {{256, 1u << 12}, {128, 1u << 12}, {256, 0u}, {128, 1u << 17}, {64, 1u << 18}};
     for (const auto& h : vars) {
       for (int sc : {GRE_SCHED_PART, GRE_SCHED_PART_NT})
         for (int ipt : {4, 8}) trial(sc, ipt, true, h);

@@ -53,7 +53,7 @@ def test_fixed_iteration_residual_is_last_change(N, dtype, schedule):
 
 
 @pytest.mark.parametrize("ranges,shared", [(8, 0), (16, 0), (32, 0), (64, 0), (128, 0), (256, 0),
-                                           (64, 1000), (128, 1 << 12), (8, 1 << 30)])
+                                           (64, 1000), (128, 1 << 12), (256, 1 << 12), (8, 1 << 30)])
 def test_pagerank_partitioned_ranges(N, O, ranges, shared):
     # source partitions (shared hub range + tail ranges): every layout gives the
     # oracle's ranks, and two handles with the same partition give the same bits
