@@ -973,8 +973,8 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
 // hottest sources shared (2 MB of FP64 scatter data, resident in every L2);
 // used from 2^22 edges (below, the one-time partition build outweighs it) up to
 // the partition's 32-bit entry limit.
-constexpr int kEngRanges = 64;
-constexpr uint32_t kEngShared = 1u << 18;
+constexpr int kEngRanges = 256;
+constexpr uint32_t kEngShared = 1u << 12;
 bool engine_part_ok(const gre_graph* g) { return g->nE >= (1ull << 22) && g->nE < 0xffffffffULL; }
 
 int64_t engine_step(gre_engine* e) {
