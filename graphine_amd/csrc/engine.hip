@@ -969,10 +969,10 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
   e->times[0] = now() - t0;
 }
 
-// Source partition of the complete-frontier pull: 64 tail ranges and the 2^18
-// hottest sources shared (2 MB of FP64 scatter data, resident in every L2);
-// used from 2^22 edges (below, the one-time partition build outweighs it) up to
-// the partition's 32-bit entry limit.
+// Source partition of the complete-frontier pull: 256 tail ranges and the
+// 4,096 hottest sources shared (32 KB of FP64 scatter data, resident in every
+// L2); used from 2^22 edges (below, the one-time partition build outweighs it)
+// up to the partition's 32-bit entry limit.
 constexpr int kEngRanges = 256;
 constexpr uint32_t kEngShared = 1u << 12;
 bool engine_part_ok(const gre_graph* g) { return g->nE >= (1ull << 22) && g->nE < 0xffffffffULL; }
@@ -1157,7 +1157,11 @@ int gre_engine_destroy(gre_engine* e) {
   return guard([&] {
     if (!e) return;
     GRE_HIP(hipStreamSynchronize(stream()));
+    // like a PageRank handle: drop the graph's cached partitions that nothing
+    // holds once this engine is gone (about 9 GB each at 2^30 edges)
+    gre_graph* g = e->g;
     delete e;
+    prune_xparts(g);
   });
 }
 

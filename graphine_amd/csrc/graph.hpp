@@ -122,6 +122,8 @@ void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint3
 // and the rest in R tail ranges (R in {8, 16, ..., 256}); built once and cached.
 // csc = false partitions the CSR the same way, over compact target ids.
 std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = true);
+// Drops the cached partitions that no PageRank handle or engine holds any more.
+void prune_xparts(gre_graph* g);
 // Its tiles of te entries (cached on the partition).
 const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s);
 // Tiles of te entries over the per-row pair lists (rows = vertices).

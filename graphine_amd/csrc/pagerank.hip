@@ -323,13 +323,6 @@ void ensure_scal(gre_pr* P, int iters) {
 // order (compact) or in the partition's entry order (part).
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
-// drops the cached partitions that no handle or engine holds any more
-void prune_xparts(gre_graph* g) {
-  for (auto it = g->xpart.begin(); it != g->xpart.end();) {
-    if (it->second.use_count() == 1) it = g->xpart.erase(it);
-    else ++it;
-  }
-}
 
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
@@ -507,8 +500,10 @@ void fwd_init(gre_pr* P, hipStream_t s) {
 }
 
 // Everything a captured forward run bakes into its kernels' arguments and
-// grids: a replay is valid only while all of it is unchanged.
-std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters) {
+// grids: a replay is valid only while all of it is unchanged.  `tmp` is the
+// shared reduction scratch the init pass writes: another handle may grow (and
+// so move) it between replays.
+std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters, const void* tmp) {
   uint64_t dbits = 0;
   std::memcpy(&dbits, &damp, sizeof(dbits));
   auto ptr = [](const void* p) { return (uint64_t)(uintptr_t)p; };
@@ -516,7 +511,7 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters) {
                              (uint64_t)P->nt, (uint64_t)P->compact, (uint64_t)P->part, (uint64_t)P->weighted,
                              (uint64_t)P->has_p, ptr(P->p.p), ptr(P->scal.p), ptr(P->x[0].p), ptr(P->x[1].p),
                              ptr(P->contrib[0].p), ptr(P->contrib[1].p), ptr(P->wcsc.p), ptr(P->xsum.p),
-                             ptr(P->wpart.p), P->wver};
+                             ptr(P->wpart.p), P->wver, ptr(tmp)};
   for (const PullPlan* pl : {&P->plan_csc, &P->plan_x, &P->plan_px}) {
     k.push_back(ptr(pl->off));
     k.push_back(ptr(pl->off32));
@@ -544,11 +539,12 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
   if (tol <= 0.0 && iters > 0 && P->graphs && !prof_on()) {
     // fixed iteration count: the whole run is one captured graph, replayed
     // while nothing it depends on has changed
-    std::vector<uint64_t> key = fwd_graph_key(P, damp, iters);
+    // the scratch is sized (and so placed) before the key is taken
+    const void* tmp = scratch(3, sizeof(double) * 2 * gcap(P->g->nV));
+    std::vector<uint64_t> key = fwd_graph_key(P, damp, iters, tmp);
     if (!P->fwd_exec || key != P->fwd_key) {
       if (P->fwd_exec) { (void)hipGraphExecDestroy(P->fwd_exec); P->fwd_exec = nullptr; }
       if (P->fwd_graph) { (void)hipGraphDestroy(P->fwd_graph); P->fwd_graph = nullptr; }
-      (void)scratch(3, sizeof(double) * 2 * gcap(P->g->nV));  // allocated before the capture
       GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       double* sc = P->scal.as<double>();
       fwd_init<TT>(P, s);
@@ -606,7 +602,8 @@ void apply_schedule(gre_pr* P, int sc, int ipt) {
 // Autotuner: times every (schedule, tile size) candidate on two forward
 // iterations after a warm-up one and keeps the fastest.  Candidates: tile
 // sizes of 4/8/16 entries per thread, with and without the non-temporal id
-// stream, and (unweighted single vector) with hub-sorted compact sources.
+// stream, and for the single vector (weighted or not) the compact and the
+// partitioned schedules.
 template <typename TT>
 void autotune_forward(gre_pr* P, double damp) {
   hipStream_t s = stream();

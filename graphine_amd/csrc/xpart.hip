@@ -322,6 +322,13 @@ unsigned bits_of(uint64_t n) {
 
 }  // namespace
 
+void prune_xparts(gre_graph* g) {
+  for (auto it = g->xpart.begin(); it != g->xpart.end();) {
+    if (it->second.use_count() == 1) it = g->xpart.erase(it);
+    else ++it;
+  }
+}
+
 std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc) {
   GRE_CHECK(R == 8 || R == 16 || R == 32 || R == 64 || R == 128 || R == 256, GRE_DEV_NULL,
             "source ranges must be 8, 16, 32, 64, 128 or 256");

@@ -370,3 +370,126 @@ def test_ppr_adjoint_and_gradients(O):
     fdp = (loss(wr, hp) - loss(wr, hm)) / 2e-6
     gpj = ((1 - d) + d * x[g.outdeg == 0].sum()) * lam[j]
     assert abs(fdp - gpj) < 1e-7 * max(1.0, abs(fdp))
+
+
+# ---------------------------------------------------------------------------
+# PageRank_delta (include/Kernels/PageRank.hpp:129-234)
+# ---------------------------------------------------------------------------
+def delta_graph(O, n=110, sinks=10, m=650, seed=21):
+    """A random digraph on n vertices plus `sinks` vertices that have in-edges
+    but no out-edges (so the out-degree guard of the activation matters)."""
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, n, m)
+    d = rng.integers(0, n, m)
+    keep = s != d
+    s, d = s[keep], d[keep]
+    s = np.concatenate([s, rng.integers(0, n, 3 * sinks)])
+    d = np.concatenate([d, n + np.arange(3 * sinks) % sinks])
+    k = np.unique((s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64))
+    s, d = (k >> np.uint64(32)).astype(np.uint32), (k & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    return O.graph_from_edges(n + sinks, s, d)
+
+
+def transition(g):
+    """0/1 adjacency B and row-stochastic P (rows of out-degree 0 are zero)."""
+    A = to_scipy(g).toarray()
+    deg = g.outdeg.astype(np.float64)
+    P = np.zeros_like(A)
+    nz = deg > 0
+    P[nz] = A[nz] / deg[nz, None]
+    return (A > 0).astype(np.int64), P
+
+
+def test_gre_pagerank_delta_matrix_powers(O):
+    """Away from the 1e-5 threshold, superstep k adds delta_k = (0.85 P^T)^(k+1) 0.15 1
+    to pr, and the vertices active after it are those with out-edges at the end of
+    some walk of length k+1 (numpy matrix powers; no engine rule is restated)."""
+    g = delta_graph(O)
+    B, P = transition(g)
+    assert np.count_nonzero(g.outdeg == 0) >= 10
+    y = np.full(g.nV, 0.15)
+    pr = y.copy()
+    walk = np.ones(g.nV, np.int64)
+    for k in range(8):
+        y = 0.85 * (P.T @ y)
+        walk = (B.T @ walk > 0).astype(np.int64)
+        assert np.array_equal(y > 0, walk > 0)
+        assert y[y > 0].min() > 2e-5            # the regime this closed form holds in
+        pr += y
+        st = O.default_state(O.PAGERANK_DELTA, g.nV)
+        ss, counts, _ = O.gre_run(O.PAGERANK_DELTA, O.DIR_OUT, g, st, max_ss=k + 1)
+        assert ss == k + 1
+        assert np.allclose(st[2], pr, rtol=1e-13, atol=0)
+        assert counts[k] == np.count_nonzero((walk > 0) & (g.outdeg > 0))
+
+
+def test_gre_pagerank_delta_run_to_halt(O):
+    """The full run halts with every vertex's last delta at or below 1e-5. The
+    trajectory (pr after each superstep) splits into the deltas that propagated and
+    those dropped; the fixpoint x* = 0.15 (I - 0.85 P^T)^-1 1 then satisfies
+    x* - pr = (I - 0.85 P^T)^-1 0.85 P^T D exactly, D = the dropped deltas, and
+    each superstep's count is the number of vertices with out-edges and delta > 1e-5."""
+    g = delta_graph(O, seed=5)
+    _, P = transition(g)
+    I = np.eye(g.nV)
+    st = O.default_state(O.PAGERANK_DELTA, g.nV)
+    ss, counts, _ = O.gre_run(O.PAGERANK_DELTA, O.DIR_OUT, g, st)
+    assert 5 < ss < 1024 and counts[-1] == 0
+    prev = np.full(g.nV, 0.15)
+    D = np.zeros(g.nV)
+    for k in range(ss + 1):
+        sk = O.default_state(O.PAGERANK_DELTA, g.nV)
+        O.gre_run(O.PAGERANK_DELTA, O.DIR_OUT, g, sk, max_ss=k + 1)
+        delta = sk[2] - prev
+        prev = sk[2]
+        assert delta.min() >= 0.0
+        live = (delta > 1e-5 * (1 + 1e-9)) & (g.outdeg > 0)
+        dead = (delta > 0) & ~live
+        assert not np.any((np.abs(delta - 1e-5) < 1e-14) & (g.outdeg > 0))   # no knife edge
+        assert counts[k] == np.count_nonzero(live)
+        D += np.where(dead, delta, 0.0)
+    assert np.array_equal(prev, st[2])
+    xstar = np.linalg.solve(I - 0.85 * P.T, np.full(g.nV, 0.15))
+    gap = np.linalg.solve(I - 0.85 * P.T, 0.85 * (P.T @ D))
+    assert np.allclose(xstar - st[2], gap, rtol=1e-9, atol=1e-13)
+    assert np.all(xstar - st[2] > -1e-13)
+
+
+@pytest.mark.parametrize("deg,counts", [(76500, [0]), (76501, [0]), (76499, [76499, 0])])
+def test_gre_pagerank_delta_threshold(O, deg, counts):
+    """Six hubs point at the same `deg` targets, every target points at one sink.
+    A target's combine adds six messages 0.15/deg; at deg = 76500 that sum times
+    0.85 is exactly 1e-5 in IEEE double, and GRE's strict `delta > err`
+    (PageRank.hpp:182) leaves the targets inactive; one fewer target lifts the
+    delta above 1e-5. The sink gets a large delta but has no out-edges."""
+    m = 0.15 / deg
+    acc = 0.0
+    for _ in range(6):
+        acc += m
+    assert (0.85 * acc == 1e-5) == (deg == 76500)
+    assert (0.85 * acc > 1e-5) == (deg == 76499)
+    hubs = np.repeat(np.arange(6), deg)
+    tg = np.tile(np.arange(6, 6 + deg), 6)
+    sink = 6 + deg
+    src = np.concatenate([hubs, np.arange(6, 6 + deg)]).astype(np.uint32)
+    dst = np.concatenate([tg, np.full(deg, sink)]).astype(np.uint32)
+    g = O.graph_from_edges(sink + 1, src, dst)
+    st = O.default_state(O.PAGERANK_DELTA, g.nV)
+    ss, c, _ = O.gre_run(O.PAGERANK_DELTA, O.DIR_OUT, g, st)
+    assert c.tolist() == counts and ss == len(counts) - 1
+    assert st[2][6] == 0.15 + 0.85 * acc
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the multi-threaded GRE-style superstep
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("threads", [1, 4])
+def test_gre_pagerank_mt_equals_pagerank(O, threads):
+    """The timed CPU baseline computes the same FP64 power iteration as o_pagerank
+    (itself pinned by the dense Google matrix above), up to summation order."""
+    g = O.rmat_graph(12, 16, seed=1)
+    x, secs = O.gre_pagerank_mt(g, nthreads=threads, iters=20)
+    ref = O.pagerank(g, 0.85, 20)
+    assert secs > 0
+    assert np.abs(x - ref).sum() / ref.sum() < 1e-13
+    assert abs(x.sum() - 1.0) < 1e-12
