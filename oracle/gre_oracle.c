@@ -540,9 +540,12 @@ int o_ppr(uint32_t nV, const uint64_t* coff, const uint32_t* cidx, const double*
   int it;
   for (it = 0; it < maxit; ++it) {
     double dang = 0.0;
-    for (uint32_t v = 0; v < nV; ++v) if (outdeg[v] == 0) dang += x[v];
+#pragma omp parallel for reduction(+:dang) schedule(static)
+    for (int64_t v = 0; v < (int64_t)nV; ++v) if (outdeg[v] == 0) dang += x[v];
     double diff = 0.0;
-    for (uint32_t v = 0; v < nV; ++v) {
+    /* rows are independent; each row sums in CSC order whatever the thread count */
+#pragma omp parallel for reduction(+:diff) schedule(dynamic, 4096)
+    for (int64_t v = 0; v < (int64_t)nV; ++v) {
       double s = 0.0;
       for (uint64_t e = coff[v]; e < coff[v + 1]; ++e) s += wc[e] * x[cidx[e]];
       y[v] = damping * (s + dang * p[v]) + (1.0 - damping) * p[v];
@@ -563,9 +566,11 @@ int o_ppr_adjoint(uint32_t nV, const uint64_t* off, const uint32_t* idx, const d
   int it;
   for (it = 0; it < maxit; ++it) {
     double plam = 0.0;
-    for (uint32_t v = 0; v < nV; ++v) plam += p[v] * lam[v];
+#pragma omp parallel for reduction(+:plam) schedule(static)
+    for (int64_t v = 0; v < (int64_t)nV; ++v) plam += p[v] * lam[v];
     double diff = 0.0;
-    for (uint32_t u = 0; u < nV; ++u) {
+#pragma omp parallel for reduction(+:diff) schedule(dynamic, 4096)
+    for (int64_t u = 0; u < (int64_t)nV; ++u) {
       double s = 0.0;
       for (uint64_t e = off[u]; e < off[u + 1]; ++e) s += wr[e] * lam[idx[e]];
       if (outdeg[u] == 0) s += plam;

@@ -168,3 +168,55 @@ def test_modes_pagerank_partitioned(N, O):
         assert ss == oss == 10 and np.array_equal(counts, ocounts)
         assert e.pull_steps() == 10
         assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", [2, 0])
+def test_modes_pagerank_delta_partitioned(N, O, mode):
+    # PageRank_delta's first superstep has a complete frontier, so from 2^22
+    # edges it takes the partitioned pull (scatter data re-permuted every
+    # superstep: the program rewrites it in apply); later supersteps pull or
+    # push by frontier size.  Counts exact, ranks to 1e-12.
+    from test_gpu_graph_engine import delta_margin
+    g, og = N.Graph.rmat(19, 16, 1, False), O.rmat_graph(19, 16, 1, False)
+    assert len(og.idx) >= 1 << 22
+    assert delta_margin(N, g, og.outdeg, mode) > 1e-9
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK_DELTA, N.DIR_OUT, mode)
+    assert ss == oss and np.array_equal(counts, ocounts)
+    assert e.pull_steps() >= 1
+    assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
+
+
+def threshold_graph(N, O, deg):
+    # six hubs -> the same `deg` targets, every target -> one sink
+    hubs = np.repeat(np.arange(6), deg)
+    tg = np.tile(np.arange(6, 6 + deg), 6)
+    sink = 6 + deg
+    src = np.concatenate([hubs, np.arange(6, 6 + deg)]).astype(np.uint32)
+    dst = np.concatenate([tg, np.full(deg, sink)]).astype(np.uint32)
+    return N.Graph.from_edges(sink + 1, src, dst), O.graph_from_edges(sink + 1, src, dst)
+
+
+@pytest.mark.parametrize("deg", [76499, 76501])
+@pytest.mark.parametrize("mode_name,mode", MODES)
+def test_modes_pagerank_delta_near_threshold(N, O, deg, mode_name, mode):
+    # a target's delta is 0.85 * 6 * 0.15/deg: 1.3e-5 relative above 1e-5 at
+    # 76,499 (all targets activate), as far below at 76,501 (none do)
+    g, og = threshold_graph(N, O, deg)
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK_DELTA, N.DIR_OUT, mode)
+    assert counts.tolist() == ocounts.tolist() == ([76499, 0] if deg == 76499 else [0])
+    assert ss == oss
+    assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
+
+
+def test_pagerank_delta_knife_edge_push(N, O):
+    # 76,500 targets: six messages 0.15/76500 added one after another give
+    # 0.85 * sum == 1e-5 exactly, and GRE's strict `delta > err` keeps every
+    # target inactive.  GRE adds messages one at a time and so does the push
+    # form (equal addends: every arrival order gives the same sum).  The pull
+    # form sums a row split across two tiles as two partial sums, which can
+    # land a few ulps off 1e-5: a summation-order difference, not tested here.
+    g, og = threshold_graph(N, O, 76500)
+    e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK_DELTA, N.DIR_OUT, 1)
+    assert counts.tolist() == ocounts.tolist() == [0] and ss == oss == 0
+    pr = e.download(N.ARR_VERTEX)
+    assert np.allclose(pr, ost[2], rtol=1e-15, atol=0)

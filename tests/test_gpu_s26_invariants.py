@@ -2,16 +2,19 @@
 symmetrised entries; 1.06B directed).  The CPU oracle is too slow at this size,
 so each result is checked against the conditions that determine it exactly:
 
-  BFS levels     level[src] = 0; |level[u] - level[v]| <= 1 across every edge;
+  BFS levels     (from the hub and four random reached sources)
+                 level[src] = 0; |level[u] - level[v]| <= 1 across every edge;
                  every other reached vertex has a neighbour one level up; the
                  reached set is closed under edges.  These fix BFS levels.
-  CC labels      label constant along every edge, label[v] <= v and
-                 label[label[v]] = label[v]: the label of each component is a
-                 member no larger than any member, i.e. its minimum vertex id.
+  CC labels      label constant along every edge, label[v] <= v,
+                 label[label[v]] = label[v], and equal to an independent
+                 min-label propagation with pointer jumping computed by torch,
+                 whose fixpoint is each component's minimum vertex id.
   SSSP           dist[src] = 0; no edge can be relaxed; every other reached
                  vertex has a tight in-edge; reached set = BFS reached set.
                  These are the Bellman-Ford optimality conditions.
-  64-source BFS  equals the plain BFS for four sources.
+  64-source BFS  equals the plain BFS for four random sources, whose levels
+                 pass the BFS conditions above.
   PageRank       FP32 within 1e-6 relative L1 of FP64 (the FP64 path matches the
                  oracle to 1e-12 at smaller scales).
 
@@ -35,17 +38,24 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 @pytest.fixture(scope="module")
 def s26(N):
     g = N.Graph.rmat(26, 16, 6, True)
-    yield g
-    g.close()
-
-
-def test_s26_bfs_cc_sssp_optimality(N, s26):
-    g = s26
     od = g.outdeg()
     hub = int(np.argmax(od))
     lev, edges, depth = N.bfs(g, hub)
     assert edges == int(od[lev >= 0].astype(np.int64).sum())
     assert depth == int(lev.max())
+    rng = np.random.default_rng(26)
+    srcs = rng.choice(np.nonzero(lev >= 0)[0], 4, replace=False).astype(np.uint32)
+    levs, ecount = [lev], [edges]
+    for s in srcs:
+        lv, e, _ = N.bfs(g, int(s))
+        levs.append(lv)
+        ecount.append(e)
+    yield dict(g=g, hub=hub, srcs=srcs, levs=np.stack(levs), edges=ecount, checked=[])
+    g.close()
+
+
+def test_s26_bfs_cc_sssp_optimality(N, s26):
+    g, hub = s26["g"], s26["hub"]
     e = N.Engine(g, N.CC)
     e.run(N.DIR_INOUT)
     lab = e.download(N.ARR_VERTEX)
@@ -62,33 +72,30 @@ def test_s26_bfs_cc_sssp_optimality(N, s26):
         np.save(os.path.join(d, "idx.npy"), csr["idx"])
         del csr
         np.save(os.path.join(d, "w.npy"), g.weights_u32())
-        np.save(os.path.join(d, "meta.npy"), np.array([hub], np.int64))
-        np.save(os.path.join(d, "lev.npy"), lev)
+        np.save(os.path.join(d, "meta.npy"), np.array([hub, hub] + [int(x) for x in s26["srcs"]], np.int64))
+        np.save(os.path.join(d, "lev.npy"), s26["levs"])
         np.save(os.path.join(d, "lab.npy"), lab)
         np.save(os.path.join(d, "dist.npy"), dist)
         r = subprocess.run([sys.executable, os.path.join(HERE, "s26_checks.py"), d], capture_output=True, text=True,
-                           timeout=600)
+                           timeout=900)
     finally:
         shutil.rmtree(d, ignore_errors=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     for name in ("bfs", "cc", "sssp"):
         assert f"ok {name}" in r.stdout
+    assert r.stdout.count("ok bfs") == 1 + len(s26["srcs"])
+    s26["checked"].append(True)
 
 
 def test_s26_msbfs_equals_plain(N, s26):
-    g = s26
-    lev, _, _ = N.bfs(g, int(np.argmax(g.outdeg())))
-    reached = np.nonzero(lev >= 0)[0]
-    rng = np.random.default_rng(26)
-    srcs = rng.choice(reached, 4, replace=False).astype(np.uint32)
+    # the plain levels it is compared with passed the BFS conditions above
+    assert s26["checked"], "the plain BFS levels were not checked"
+    g, srcs = s26["g"], s26["srcs"]
     mlev, medges = N.msbfs(g, srcs)
-    total = 0
-    for i, s in enumerate(srcs):
-        plev, pe, _ = N.bfs(g, int(s))
-        assert np.array_equal(mlev[i], plev)
-        total += pe
-    assert medges == total
+    for i in range(len(srcs)):
+        assert np.array_equal(mlev[i], s26["levs"][i + 1])
+    assert medges == sum(s26["edges"][1:])
 
 
 def test_s26_pagerank_f32_vs_f64(N):
