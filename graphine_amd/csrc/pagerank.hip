@@ -13,8 +13,22 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
-
+// Device memory of the allocation-free forward (gre_pr_run_noalloc): the
+// init pass's dangling-mass partials (two doubles per block, at most 65,536
+// blocks), then each iteration's reduction pair (dangling mass, L1 change) for
+// at most kNoAllocIters iterations: 1,114,160 bytes per GPU, held until the
+// process ends.  Host code never passes this name to a kernel: it is a
+// host-side stand-in, and noalloc_scratch() looks up each device's own address.
+namespace gre {
+namespace {
+constexpr int kNoAllocIters = 4096;
+constexpr size_t kNoAllocTmp = 2 * 65536;
+constexpr size_t kNoAllocDoubles = kNoAllocTmp + 2 * (size_t)(kNoAllocIters + 1);
+}  // namespace
+}  // namespace gre
+static __device__ __align__(256) double scratch[gre::kNoAllocDoubles];
 
 namespace gre {
 namespace {
@@ -483,19 +497,22 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
   if (m.contrib_out) P->ccur ^= 1;
 }
 
+// Dangling-mass partials of the init pass: two doubles per block.
+size_t fwd_init_tmp_bytes(uint32_t nV) { return sizeof(double) * 2 * gcap(nV); }
+
+// The forward's initial state; its dangling mass goes to scal[0..1] through
+// the block partials in tmp (fwd_init_tmp_bytes).
 template <typename TT>
-void fwd_init(gre_pr* P, hipStream_t s) {
+void fwd_init(gre_pr* P, hipStream_t s, double* tmp, double* scal) {
   const uint32_t nV = P->g->nV;
   const unsigned gb = gcap(nV);
-  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
   P->xcur = 0;
   P->ccur = 0;
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
                      P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(),
                      (P->weighted && !use_compact(P)) ? nullptr : P->contrib[0].as<TT>(), 1.0 / (double)nV, tmp,
                      use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted);
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0,
-                     P->scal.as<double>());
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, scal);
   GRE_HIP(hipGetLastError());
 }
 
@@ -534,20 +551,23 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters, con
 }
 
 template <typename TT>
+double forward_eager(gre_pr* P, double damp, int iters, double tol, int* done, double* tmp, double* sc, bool sync);
+
+template <typename TT>
 double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
   hipStream_t s = stream();
   if (tol <= 0.0 && iters > 0 && P->graphs && !prof_on()) {
     // fixed iteration count: the whole run is one captured graph, replayed
     // while nothing it depends on has changed
     // the scratch is sized (and so placed) before the key is taken
-    const void* tmp = scratch(3, sizeof(double) * 2 * gcap(P->g->nV));
+    double* tmp = static_cast<double*>(scratch(3, fwd_init_tmp_bytes(P->g->nV)));
     std::vector<uint64_t> key = fwd_graph_key(P, damp, iters, tmp);
     if (!P->fwd_exec || key != P->fwd_key) {
       if (P->fwd_exec) { (void)hipGraphExecDestroy(P->fwd_exec); P->fwd_exec = nullptr; }
       if (P->fwd_graph) { (void)hipGraphDestroy(P->fwd_graph); P->fwd_graph = nullptr; }
       GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       double* sc = P->scal.as<double>();
-      fwd_init<TT>(P, s);
+      fwd_init<TT>(P, s, tmp, sc);
       // only the last iteration's L1 change is returned
       for (int it = 0; it < iters; ++it) fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s, it + 1 == iters);
       GRE_HIP(hipStreamEndCapture(s, &P->fwd_graph));
@@ -565,8 +585,18 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
     if (done) *done = iters;
     return h[1];
   }
-  fwd_init<TT>(P, s);
-  double* sc = P->scal.as<double>();
+  return forward_eager<TT>(P, damp, iters, tol, done, static_cast<double*>(scratch(3, fwd_init_tmp_bytes(P->g->nV))),
+                           P->scal.as<double>(), true);
+}
+
+// The forward launched kernel by kernel, with its reductions in tmp (the init
+// pass's partials) and sc (2 * (iters + 1) doubles: each iteration's dangling
+// mass and L1 change).  sync = false with tol <= 0 only enqueues the work and
+// returns 0 (no host wait, no read-back).
+template <typename TT>
+double forward_eager(gre_pr* P, double damp, int iters, double tol, int* done, double* tmp, double* sc, bool sync) {
+  hipStream_t s = stream();
+  fwd_init<TT>(P, s, tmp, sc);
   int it = 0;
   double res = 0.0;
   for (it = 0; it < iters; ++it) {
@@ -579,6 +609,8 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
       if (res < tol) { ++it; break; }
     }
   }
+  if (done) *done = it;
+  if (!sync && tol <= 0.0) return 0.0;
   if (tol <= 0.0 && iters > 0) {
     double h[2];
     GRE_HIP(hipMemcpyAsync(h, sc + 2 * iters, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -586,7 +618,6 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
     res = h[1];
   }
   GRE_HIP(hipStreamSynchronize(s));
-  if (done) *done = it;
   return res;
 }
 
@@ -631,8 +662,8 @@ void autotune_forward(gre_pr* P, double damp) {
     apply_schedule(P, sc, ipt);
     if (prt) { P->ranges = h.first; P->shared = h.second; }
     ensure_plan(P, true, ipt);
-    fwd_init<TT>(P, s);
     double* scal = P->scal.as<double>();
+    fwd_init<TT>(P, s, static_cast<double*>(scratch(3, fwd_init_tmp_bytes(P->g->nV))), scal);
     fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
     GRE_HIP(hipEventRecord(a, s));
     fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
@@ -827,6 +858,40 @@ void solve_adjoint(gre_pr* P, double damping, int iters, double tol, int* iters_
   P->have_adj = true;
 }
 
+// The address of `scratch` on the current device, looked up once per device
+// (each GPU holds its own copy of the array at its own address).
+double* noalloc_scratch() {
+  static std::mutex mu;
+  static double* ptr[64] = {};
+  int dev = 0;
+  GRE_HIP(hipGetDevice(&dev));
+  GRE_CHECK(dev >= 0 && dev < 64, GRE_DEV_FAILURE, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    GRE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    GRE_CHECK(p, GRE_DEV_FAILURE, "no device address for the allocation-free scratch");
+    ptr[dev] = static_cast<double*>(p);
+  }
+  return ptr[dev];
+}
+
+// True when the forward's pull plan for the handle's current schedule, tile
+// size, partition and weights is built, i.e. ensure_plan(P, true, P->ipt)
+// would allocate nothing and launch nothing.
+bool fwd_plan_ready(const gre_pr* P) {
+  if (P->ipt == 0) return false;                      // schedule not tuned yet
+  if (use_compact(P) && !P->g->cid.p) return false;
+  if (use_part(P)) {
+    if (!P->xp || P->xp->R != P->ranges || P->xp->H != std::min(P->shared, P->g->nsrc)) return false;
+    if (P->plan_ipt_x != P->ipt) return false;
+    if (P->xsum.bytes < tsize(P->dtype) * (size_t)P->xp->nseg) return false;
+    if (P->weighted && (P->wpart_xp != P->xp || P->wpart_ver != P->wver)) return false;
+    return true;
+  }
+  return P->plan_ipt_csc == P->ipt;
+}
+
 }  // namespace
 }  // namespace gre
 
@@ -954,7 +1019,7 @@ int gre_pr_set_edge_weights(gre_pr* P, const float* w, uint64_t seed) {
       PullBuffers pb;
       pb.make(pl, g, false, 8, sizeof(double));
       ModeRowSum m{P->wraw.as<float>(), P->rowsum.as<double>()};
-      double* sc = static_cast<double*>(scratch(4, 2 * sizeof(double)));
+      double* sc = static_cast<double*>(gre::scratch(4, 2 * sizeof(double)));
       pull_pass(pl, m, sc, s);
       GRE_HIP(hipStreamSynchronize(s));
     }
@@ -1031,6 +1096,30 @@ int gre_pr_run(gre_pr* P, double damping, int iters, double tol, int* iters_done
     ensure_scal(P, iters);
     const double r = P->dtype == GRE_F64 ? run_forward<double>(P, damping, iters, tol, iters_done)
                                          : run_forward<float>(P, damping, iters, tol, iters_done);
+    if (residual) *residual = r;
+    P->have_fwd = true;
+  });
+}
+
+int gre_pr_run_noalloc(gre_pr* P, double damping, int iters, double tol, int* iters_done, double* residual) {
+  return guard([&] {
+    GRE_CHECK(P, GRE_DEV_NULL, "null handle");
+    GRE_CHECK(iters >= 0, GRE_DEV_NULL, "iters must be >= 0");
+    GRE_CHECK(damping >= 0.0 && damping < 1.0, GRE_DEV_NULL, "damping must be in [0,1)");
+    GRE_CHECK(P->k == 1, GRE_DEV_NOT_IMPLEMENTED, "gre_pr_run_noalloc runs single-vector handles only (k = 1)");
+    GRE_CHECK(iters <= kNoAllocIters, GRE_DEV_NOBUFFER,
+              "gre_pr_run_noalloc runs at most 4096 iterations (its scratch holds 4097 reduction pairs)");
+    GRE_CHECK(!prof_on(), GRE_DEV_FAILURE, "gre_pr_run_noalloc does not run with kernel profiling on (it records events)");
+    GRE_CHECK(fwd_plan_ready(P), GRE_DEV_NOBUFFER,
+              "the handle's pull plan for its schedule is not built yet: one gre_pr_run on it (which allocates) "
+              "prepares it");
+    GRE_CHECK(fwd_init_tmp_bytes(P->g->nV) <= sizeof(double) * kNoAllocTmp, GRE_DEV_FAILURE, "init partials exceed scratch");
+    double* base = noalloc_scratch();
+    ensure_plan(P, true, P->ipt);   // ready: sets the plan's flags only
+    const bool sync = residual != nullptr || tol > 0.0;
+    const double r = P->dtype == GRE_F64
+                         ? forward_eager<double>(P, damping, iters, tol, iters_done, base, base + kNoAllocTmp, sync)
+                         : forward_eager<float>(P, damping, iters, tol, iters_done, base, base + kNoAllocTmp, sync);
     if (residual) *residual = r;
     P->have_fwd = true;
   });

@@ -97,6 +97,7 @@ def _declare(L):
         "gre_pr_get_edge_weights": ([vp, vp, vp], i32),
         "gre_pr_set_personalization": ([vp, vp], i32),
         "gre_pr_run": ([vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
+        "gre_pr_run_noalloc": ([vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_get_ranks": ([vp, vp], i32),
         "gre_pr_adjoint": ([vp, vp, i64, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl)], i32),
         "gre_pr_gradients": ([vp, dbl, vp, vp, vp], i32),

@@ -171,6 +171,19 @@ int gre_pr_run(gre_pr* p, double damping, int iters, double tol, int* iters_done
  * once into a hipGraph and replayed while the handle's schedule, buffers and
  * parameters are unchanged (on = 1, the default); on = 0 launches eagerly. */
 int gre_pr_set_graphs(gre_pr* p, int on);
+/* No counterpart in GRE: the same single-vector forward as gre_pr_run, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc/hipFree or allocating library routine: its reductions live in a
+ * static 1.06 MB device array per GPU, and it launches eagerly (no captured
+ * graph of its own).  Limits: k = 1 (GRE_DEV_NOT_IMPLEMENTED otherwise),
+ * iters <= 4096 (GRE_DEV_NOBUFFER), kernel profiling off, and the handle's pull
+ * plan already built for its current schedule, partition and weights by an
+ * earlier gre_pr_run (GRE_DEV_NOBUFFER otherwise).  With tol <= 0 and
+ * residual == NULL it only enqueues the work on the library stream (no host
+ * wait).  Calls that overlap on one GPU share the array and race. */
+int gre_pr_run_noalloc(gre_pr* p, double damping, int iters, double tol, int* iters_done,
+                       double* residual);
 int gre_pr_get_ranks(gre_pr* p, void* host_out);
 /* Adjoint of the converged forward: lambda = c + d (W lambda + g (p.lambda)),
  * a pull over CSR; c = indicator of targets (ntgt ids, or dense nV x k in cvec). */
