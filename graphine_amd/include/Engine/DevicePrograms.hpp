@@ -8,9 +8,16 @@
 // is a specialisation of ProgramTraits keyed on the program's class; the
 // shipped programs (include/Kernels in the GRE tree: PageRank, PageRank_delta,
 // BFS, ConnectedComponents, SSSP, Forward, Backward, CountIn) are bound here,
-// so they and their drivers compile unchanged.  A program without a binding
-// does not compile against SynchronousEngine (static_assert) rather than
-// silently running on the CPU.
+// so they and their drivers compile unchanged.
+//
+// A binding names rules; it does not prove that a class computes them.  At a
+// class's first run<dir>() the engine runs the class's own init / scatter /
+// combine / apply / assert_to_halt / reduce on the host over a small probe
+// graph, runs the bound device program on the same graph, and throws
+// DeviceError when the two differ (Engine/Engine.hpp).  A class with no
+// binding is probed against every device program whose element sizes match
+// and runs on the first one that reproduces it; if none does, run<dir>()
+// throws.  Nothing falls back to a CPU superstep.
 //
 // Per binding:
 //   id                      device program (GRE_PROG_*)
@@ -29,6 +36,8 @@
 #ifndef GRE_AMD_DEVICE_PROGRAMS_HPP
 #define GRE_AMD_DEVICE_PROGRAMS_HPP
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "gre_dev.h"
 
@@ -100,6 +109,105 @@ struct ProgramTraits< ::Backward> : TraitsBase {                 // BC.hpp:97-16
 template <>
 struct ProgramTraits< ::CountIn> : TraitsBase {                  // CountIn.hpp:18-106
   enum { id = GRE_PROG_COUNTIN, scatterBytes = 4, combineBytes = 4, vertexBytes = 4 };
+};
+
+// Runtime form of a binding (also what an unbound class is matched against).
+struct Binding {
+  int id;
+  int scatterBytes, combineBytes, vertexBytes;
+  bool reduceReadsVertexData, needsEdgeWeights, usesLevel, publishesState, needsForwardState;
+};
+
+template <class T>
+inline Binding bindingFromTraits() {
+  Binding b;
+  b.id = T::id;
+  b.scatterBytes = T::scatterBytes;
+  b.combineBytes = T::combineBytes;
+  b.vertexBytes = T::vertexBytes;
+  b.reduceReadsVertexData = T::reduceReadsVertexData != 0;
+  b.needsEdgeWeights = T::needsEdgeWeights != 0;
+  b.usesLevel = T::usesLevel != 0;
+  b.publishesState = T::publishesState != 0;
+  b.needsForwardState = T::needsForwardState != 0;
+  return b;
+}
+
+// The device programs, in GRE_PROG_* order.
+inline Binding deviceProgram(int id) {
+  switch (id) {
+    case GRE_PROG_PAGERANK: return bindingFromTraits<ProgramTraits< ::PageRank> >();
+    case GRE_PROG_PAGERANK_DELTA: return bindingFromTraits<ProgramTraits< ::PageRank_delta> >();
+    case GRE_PROG_BFS: return bindingFromTraits<ProgramTraits< ::BFS> >();
+    case GRE_PROG_CC: return bindingFromTraits<ProgramTraits< ::ConnectedComponents> >();
+    case GRE_PROG_SSSP: return bindingFromTraits<ProgramTraits< ::SSSP> >();
+    case GRE_PROG_BC_FORWARD: return bindingFromTraits<ProgramTraits< ::Forward> >();
+    case GRE_PROG_BC_BACKWARD: return bindingFromTraits<ProgramTraits< ::Backward> >();
+    case GRE_PROG_COUNTIN: return bindingFromTraits<ProgramTraits< ::CountIn> >();
+    default: {
+      Binding b = bindingFromTraits<ProgramTraits<void> >();
+      return b;
+    }
+  }
+}
+const int kNumDevicePrograms = 8;
+
+// ---- probe instances --------------------------------------------------------
+// How the binding check builds two instances of a program class on the probe
+// graph, and which runs it compares.  The default takes classes constructible
+// from the graph alone and makes one run<dir>() like the caller's.  GRE's
+// Forward and Backward take extra constructor arguments; their probes follow
+// GRE's BC driver (usr/test_bc.cpp:61-110): Forward from a source, Backward per
+// level over the Forward state of the same graph.
+
+// A typed view of an untyped pointer, for constructor arguments whose type
+// this header cannot name (GRE's vtxData*).
+struct AnyPtr {
+  void* p;
+  template <class P>
+  operator P*() const {
+    static_assert(sizeof(P) == 8, "a BC level/path-count record is two 32-bit ints");
+    return static_cast<P*>(p);
+  }
+};
+
+enum ProbeKind { ProbeNone, ProbeOneRun, ProbeBackwardLevels };
+
+template <class VC, class DG, class = void>
+struct ProbeMaker {
+  enum { kind = ProbeNone };
+};
+template <class VC, class DG>
+struct ProbeMaker<VC, DG, typename std::enable_if<std::is_constructible<VC, DG&>::value>::type> {
+  enum { kind = ProbeOneRun };
+  struct Holder {
+    VC vc;
+    Holder(DG& g, void*) : vc(g) {}
+  };
+};
+template <class DG>
+struct ProbeMaker< ::Forward, DG, void> {
+  enum { kind = ProbeOneRun };
+  template <class VC>
+  struct HolderT {
+    typename VC::VertexData_t* state;
+    VC vc;
+    HolderT(DG& g, void*) : state(NULL), vc(g, state, 1) {}
+    ~HolderT() { delete[] state; }
+  };
+  typedef HolderT< ::Forward> Holder;
+};
+template <class DG>
+struct ProbeMaker< ::Backward, DG, void> {
+  enum { kind = ProbeBackwardLevels };
+  template <class VC>
+  struct HolderT {
+    double* deps;
+    VC vc;
+    HolderT(DG& g, void* fwdState) : deps(NULL), vc(g, AnyPtr{fwdState}, deps, 0) {}
+    ~HolderT() { delete[] deps; }
+  };
+  typedef HolderT< ::Backward> Holder;
 };
 
 }  // namespace Device

@@ -32,7 +32,8 @@ class DistributedGraph {
   typedef DistributedGraph<LocalGraph> SELF_t;
   friend class GraphIngress<SELF_t>;
 
-  DistributedGraph() : isReady_(false), nV_(0), nE_(0), dev_(NULL), edgeProp_(NULL), edgePropBytes_(0) {}
+  DistributedGraph()
+      : isReady_(false), nV_(0), nE_(0), dev_(NULL), edgeProp_(NULL), edgePropBytes_(0), deterministic_(false) {}
   ~DistributedGraph() {
     if (dev_) gre_graph_destroy(dev_);
   }
@@ -132,17 +133,21 @@ class DistributedGraph {
     for (size_t e = 0; e < nE_; ++e) in >> eData[e];
     registerEdgeProperty(eData);
   }
+  // without a file GRE draws unseeded rand() % 65535 (DistributedGraph.hpp:207-226);
+  // a graph with deterministic properties (the binding check's probe graph)
+  // fills 1 + (i * 7919) % 97 and leaves rand()'s sequence alone
   template <typename T>
   void loadVertexProperty(T*& vData) {
     vData = new T[nV_];
-    for (size_t v = 0; v < nV_; ++v) vData[v] = std::rand() % 65535;
+    for (size_t v = 0; v < nV_; ++v) vData[v] = deterministic_ ? (T)(1 + (v * 7919) % 97) : (T)(std::rand() % 65535);
   }
   template <typename T>
   void loadEdgeProperty(T*& eData) {
     eData = new T[nE_];
-    for (size_t e = 0; e < nE_; ++e) eData[e] = std::rand() % 65535;
+    for (size_t e = 0; e < nE_; ++e) eData[e] = deterministic_ ? (T)(1 + (e * 7919) % 97) : (T)(std::rand() % 65535);
     registerEdgeProperty(eData);
   }
+  void setDeterministicProperties(bool on) { deterministic_ = on; }
   template <typename T>
   void unloadVertexProperty(T*& vData) {
     delete[] vData;
@@ -199,6 +204,7 @@ class DistributedGraph {
   std::vector<int> outdegrees_;
   const void* edgeProp_;
   size_t edgePropBytes_;
+  bool deterministic_;
   std::map<int, void*> progState_;
 
  public:

@@ -4,7 +4,7 @@
 // that walks edges on the host.  The device superstep never uses it.
 //
 // Deviation from GRE, kept on purpose: leid(it) is the CSR position of the
-// edge.  GRE divides that position by sizeof(lidx_t) (LocalGraph.hpp:173-176),
+// edge (also for an iterator of the reverse list).  GRE divides that position by sizeof(lidx_t) (LocalGraph.hpp:173-176),
 // so edge property e was read for edge 8e; and GRE's redge_begin/end index the
 // CSR offsets (:183-191), which is right only when every in-degree equals the
 // out-degree.  Here redge_* use the CSC offsets and rleid maps a CSC position
@@ -39,7 +39,15 @@ class GraphInCSR {
   edge_iterator edge_end() { ensure(); return edgeList_.end(); }
   edge_iterator edge_begin(lvid_t v) { ensure(); return edgeList_.begin() + edgeIndex_[v]; }
   edge_iterator edge_end(lvid_t v) { ensure(); return edgeList_.begin() + edgeIndex_[v + 1]; }
-  leid_t leid(const edge_iterator& it) { ensure(); return (leid_t)(it - edgeList_.begin()); }
+  // the CSR position of the edge; an iterator into the reverse list (a
+  // program scattering along In) is mapped through the CSC -> CSR table
+  leid_t leid(const edge_iterator& it) {
+    ensure();
+    const lvid_t* p = &*it;
+    if (nEdges_ > 0 && p >= redgeList_.data() && p < redgeList_.data() + nEdges_)
+      return (leid_t)redgeID_[p - redgeList_.data()];
+    return (leid_t)(it - edgeList_.begin());
+  }
 
   redge_iterator redge_begin() { ensure(); return redgeList_.begin(); }
   redge_iterator redge_end() { ensure(); return redgeList_.end(); }

@@ -128,3 +128,46 @@ def test_cpp_countin(O, tmp_path):
     raw, counts, ss, _ = run_driver("countin", prefix, tmp_path)
     cn = raw.view(np.int32)
     assert cn.sum() == og.nE and np.array_equal(cn, og.outdeg)   # run<In>: one message per out-edge
+
+
+def run_driver_rc(prog, prefix, tmp):
+    out = str(tmp / f"{prog}.bin")
+    return subprocess.run([DRIVER, prog, prefix, out], capture_output=True, text=True, timeout=120)
+
+
+def test_cpp_binding_check_rejects_a_program_with_other_rules(O, tmp_path):
+    # PageRank02 is bound to PageRank's device program but teleports 0.2: the
+    # check on the probe graph catches it before anything runs on the input
+    og = O.rmat_graph(10, 8, 1, False)
+    s, d = edges_of(og)
+    prefix = write_graph(tmp_path, og.nV, s, d)
+    r = run_driver_rc("pagerank02", prefix, tmp_path)
+    assert r.returncode == 5, r.stdout + r.stderr
+    assert "does not compute the rules of device program 0" in r.stderr
+    assert "Superstep-" not in r.stdout
+
+
+def test_cpp_unbound_program_is_matched_by_its_rules(O, tmp_path):
+    # Reach has no ProgramTraits binding; it computes BFS's rules in its own way
+    # and runs on the BFS device program, with GRE's BFS results
+    og = O.rmat_graph(12, 16, 2, True)
+    s, d = edges_of(og)
+    root = int(np.argmax(og.outdeg))
+    prefix = write_graph(tmp_path, og.nV, s, d, roots=[root, root])
+    raw, counts, ss, out = run_driver("reach", prefix, tmp_path)
+    assert "device program 2" in out
+    st = O.default_state(O.BFS, og.nV)
+    oss, ocounts, _ = O.gre_run(O.BFS, O.DIR_IN, og, st, srcs=[root])
+    assert ss == oss and counts == ocounts.tolist()
+    assert np.array_equal(raw.view(np.uint64), st[2])
+
+
+def test_cpp_unbound_program_without_a_match_is_refused(O, tmp_path):
+    # MaxLabel, run over every vertex, labels each component with its largest
+    # vertex id; no device program computes that
+    og = O.rmat_graph(10, 8, 2, True)
+    s, d = edges_of(og)
+    prefix = write_graph(tmp_path, og.nV, s, d, roots=[0, 0])
+    r = run_driver_rc("maxlabel", prefix, tmp_path)
+    assert r.returncode == 5, r.stdout + r.stderr
+    assert "matches no device program" in r.stderr
