@@ -281,7 +281,11 @@ def main():
     copy_bytes = 4 << 30
     ms_copy = N.copy_probe(copy_bytes, 10)
     roofline["copy_probe_gbs"] = round(2 * copy_bytes / (ms_copy / 1e3) / 1e9, 1)
-    log(f"copy probe: {roofline['copy_probe_gbs']} GB/s")
+    # and of a read-only stream (the iteration reads ~15 bytes per byte written)
+    ms_read = N.read_probe(copy_bytes, 10)
+    roofline["read_probe_gbs"] = round(copy_bytes / (ms_read / 1e3) / 1e9, 1)
+    roofline["frac_of_read_probe"] = round(achieved / roofline["read_probe_gbs"], 4)
+    log(f"copy probe: {roofline['copy_probe_gbs']} GB/s, read probe: {roofline['read_probe_gbs']} GB/s")
     # achievable rate of scattered 4-byte reads, by where the table fits
     ng = 1 << 28
     roofline["gather_probe_g_per_s"] = {

@@ -118,11 +118,50 @@ void PullBuffers::alloc_for(PullPlan& p, size_t tbytes) {
   p.part3 = part3.as<double>();
 }
 
-// plain streaming copy (16-byte vectors): the calibration kernel for HBM byte
-// counters and the achievable-bandwidth reference
-__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ in, uint4* __restrict__ out, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = in[i];
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kProbeU = 8;   // 16-byte loads in flight per thread
+
+// Streaming copy: each 256-thread block moves 256 * kProbeU 16-byte vectors,
+// all loads issued before the first store, stores non-temporal.  One block per
+// 32 KB, so a 4 GiB copy launches 131,072 workgroups.  The calibration kernel
+// for HBM byte counters and the achievable copy bandwidth.
+__global__ __launch_bounds__(256) void k_copy_probe(const u32x4* __restrict__ in, u32x4* __restrict__ out,
+                                                    uint64_t n) {
+  const uint64_t base = (uint64_t)blockIdx.x * 256 * kProbeU + threadIdx.x;
+  u32x4 v[kProbeU];
+  if (base + (uint64_t)(kProbeU - 1) * 256 < n) {
+#pragma unroll
+    for (int j = 0; j < kProbeU; ++j) v[j] = in[base + (uint64_t)j * 256];
+#pragma unroll
+    for (int j = 0; j < kProbeU; ++j) __builtin_nontemporal_store(v[j], out + base + (uint64_t)j * 256);
+  } else {
+    for (int j = 0; j < kProbeU; ++j)
+      if (base + (uint64_t)j * 256 < n) out[base + (uint64_t)j * 256] = in[base + (uint64_t)j * 256];
+  }
+}
+
+// Streaming read: the same access pattern without the stores; each thread
+// folds its vectors into one word so no load is dead.  The achievable rate of
+// a read-only stream (the PageRank iteration reads about 15 bytes per byte it
+// writes).
+__global__ __launch_bounds__(256) void k_read_probe(const u32x4* __restrict__ in, uint64_t n,
+                                                    unsigned* __restrict__ out) {
+  const uint64_t base = (uint64_t)blockIdx.x * 256 * kProbeU + threadIdx.x;
+  u32x4 v[kProbeU];
+  unsigned acc = 0;
+  if (base + (uint64_t)(kProbeU - 1) * 256 < n) {
+#pragma unroll
+    for (int j = 0; j < kProbeU; ++j) v[j] = in[base + (uint64_t)j * 256];
+#pragma unroll
+    for (int j = 0; j < kProbeU; ++j) acc ^= v[j].x ^ v[j].y ^ v[j].z ^ v[j].w;
+  } else {
+    for (int j = 0; j < kProbeU; ++j)
+      if (base + (uint64_t)j * 256 < n) {
+        const u32x4 w = in[base + (uint64_t)j * 256];
+        acc ^= w.x ^ w.y ^ w.z ^ w.w;
+      }
+  }
+  if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;   // practically never: keeps the loads live
 }
 
 // Gather probe: ids drawn uniformly from [0, n_tab) by a hash, then one
@@ -193,32 +232,51 @@ extern "C" int gre_gather_probe(uint64_t table_entries, uint64_t gathers, int re
   });
 }
 
-extern "C" int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy) {
+// Times `reps` launches of a streaming probe over `bytes` bytes (copy: in -> out;
+// read: in only) on the library stream.
+static int stream_probe(uint64_t bytes, int reps, bool copy, double* ms_per_pass) {
   using namespace gre;
   return guard([&] {
-    GRE_CHECK(ms_per_copy && reps > 0 && bytes >= 16, GRE_DEV_NULL, "bad argument");
+    GRE_CHECK(ms_per_pass && reps > 0 && bytes >= 16, GRE_DEV_NULL, "bad argument");
     hipStream_t s = stream();
     const uint64_t n = bytes / 16;
+    const uint64_t nblk = (n + 256 * kProbeU - 1) / (256 * kProbeU);
+    GRE_CHECK(nblk <= 0x7fffffffULL, GRE_DEV_NULL, "probe too large");
     DevBuf a, b;
     a.alloc(16 * n);
-    b.alloc(16 * n);
+    if (copy) b.alloc(16 * n);
+    else b.alloc(4 * nblk);
     GRE_HIP(hipMemsetAsync(a.p, 1, 16 * n, s));
     hipEvent_t e0, e1;
     GRE_HIP(hipEventCreate(&e0));
     GRE_HIP(hipEventCreate(&e1));
-    const unsigned grid = 256 * 8 * 4;  // 4 blocks of 256 per CU on 256 CUs, grid-stride
-    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n);
+    auto launch = [&] {
+      if (copy)
+        hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)nblk), dim3(256), 0, s, a.as<u32x4>(), b.as<u32x4>(), n);
+      else
+        hipLaunchKernelGGL(k_read_probe, dim3((unsigned)nblk), dim3(256), 0, s, a.as<u32x4>(), n, b.as<unsigned>());
+    };
+    launch();
+    GRE_HIP(hipGetLastError());
     GRE_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r) {
-      ProfScope ps("copy_probe");
-      hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n);
+      ProfScope ps(copy ? "copy_probe" : "read_probe");
+      launch();
     }
     GRE_HIP(hipEventRecord(e1, s));
     GRE_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     GRE_HIP(hipEventElapsedTime(&ms, e0, e1));
-    *ms_per_copy = ms / reps;
+    *ms_per_pass = ms / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   });
+}
+
+extern "C" int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy) {
+  return stream_probe(bytes, reps, true, ms_per_copy);
+}
+
+extern "C" int gre_read_probe(uint64_t bytes, int reps, double* ms_per_read) {
+  return stream_probe(bytes, reps, false, ms_per_read);
 }

@@ -67,6 +67,7 @@ def _declare(L):
         "gre_prof_get": ([C.c_char_p, C.POINTER(i64), C.POINTER(dbl)], i32),
         "gre_mem_in_use": ([C.POINTER(u64)], i32),
         "gre_copy_probe": ([u64, i32, C.POINTER(dbl)], i32),
+        "gre_read_probe": ([u64, i32, C.POINTER(dbl)], i32),
         "gre_gather_probe": ([u64, u64, i32, C.POINTER(dbl)], i32),
         "gre_pr_get_schedule": ([vp, C.POINTER(i32), C.POINTER(i32)], i32),
         "gre_graph_from_edges": ([pp, u32, u64, vp, vp], i32),
@@ -159,6 +160,12 @@ def prof_get(name):
 def copy_probe(nbytes, reps=10):
     ms = C.c_double()
     _ck(lib().gre_copy_probe(nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
+def read_probe(nbytes, reps=10):
+    ms = C.c_double()
+    _ck(lib().gre_read_probe(nbytes, reps, C.byref(ms)))
     return ms.value
 
 

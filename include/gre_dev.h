@@ -71,6 +71,10 @@ int gre_mem_in_use(uint64_t* bytes);
  * `bytes` bytes, reps times, on the library stream; the kernel that calibrates
  * HBM byte counters and gives the achievable copy bandwidth. */
 int gre_copy_probe(uint64_t bytes, int reps, double* ms_per_copy);
+/* Measurement aid (no GRE counterpart): times a read-only stream over `bytes`
+ * bytes (16-byte loads, eight in flight per thread), reps times: the
+ * achievable HBM read bandwidth. */
+int gre_read_probe(uint64_t bytes, int reps, double* ms_per_read);
 /* Measurement aid (no GRE counterpart): times `gathers` uniform random 4-byte
  * gathers from a table of table_entries floats (ids streamed from HBM), reps
  * times: the ceiling of scattered reads served from L1, L2, the Infinity
