@@ -212,6 +212,7 @@ template <> struct Prog<GRE_PROG_COUNTIN> {             // CountIn.hpp:18-106
 //   fin    merges the folded value into C[v]; any = at least one message;
 //   skip   v cannot change this superstep (no need to read its row);
 //   kFirst the first active neighbour in row order decides (first writer);
+//   kFuseApply  apply(v) may run in the pulling thread (see PullFirstU64);
 //   kStale an inactive neighbour's message was already merged into C[v] by the
 //          superstep after its last change, so folding it again cannot change
 //          fin's result -- true for min-label CC when every vertex began
@@ -221,6 +222,7 @@ struct PullSumF64 {                  // PageRank, PageRank_delta, BC Backward: l
   using T = double;
   static constexpr bool kFirst = false;
   static constexpr bool kStale = false;
+  static constexpr bool kFuseApply = false;
   __device__ static T id() { return 0.0; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const double*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a + b; }
@@ -235,6 +237,7 @@ struct PullSumI32 {                  // BC Forward, CountIn: int += (wraps like 
   using T = int32_t;
   static constexpr bool kFirst = false;
   static constexpr bool kStale = false;
+  static constexpr bool kFuseApply = false;
   __device__ static T id() { return 0; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const int32_t*)c.S)[u]; }
   __device__ static T op(T a, T b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
@@ -250,6 +253,7 @@ struct PullMinU64 {                  // ConnectedComponents: min label
   using T = unsigned long long;
   static constexpr bool kFirst = false;
   static constexpr bool kStale = true;
+  static constexpr bool kFuseApply = false;
   __device__ static T id() { return ~0ULL; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a < b ? a : b; }
@@ -265,6 +269,12 @@ struct PullFirstU64 {                // BFS: the first message into an unset com
   using T = unsigned long long;
   static constexpr bool kFirst = true;
   static constexpr bool kStale = false;
+  // apply(v) runs in the pulling thread right after v's combine, and the pull
+  // marks only the vertices it activates: v's combine is complete there (no
+  // other thread writes C[v] in the pull form), and BFS's apply writes v's
+  // scatter data only when v was never reached, i.e. not active, so no thread
+  // reads it as a message in this superstep
+  static constexpr bool kFuseApply = true;
   __device__ static T id() { return ~0ULL; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t) { return ((const unsigned long long*)c.S)[u]; }
   __device__ static T op(T a, T b) { return a != ~0ULL ? a : b; }
@@ -280,6 +290,7 @@ struct PullMinU32W {                 // SSSP: min of lastStates[u] + edgeWgt
   using T = uint32_t;
   static constexpr bool kFirst = false;
   static constexpr bool kStale = false;
+  static constexpr bool kFuseApply = false;
   __device__ static T id() { return 0xffffffffu; }
   __device__ static T msg(const EngCtx& c, uint32_t u, uint32_t w) { return ((const uint32_t*)c.S)[u] + w; }
   __device__ static T op(T a, T b) { return a < b ? a : b; }
@@ -331,7 +342,10 @@ __device__ inline void pull_entry(const PullAdj& a, uint32_t v, uint64_t j, uint
 }
 
 constexpr uint64_t kPullShort = 32;      // rows up to this many entries: one thread
+constexpr uint64_t kPullShortFirst = 1024;  // the same for first-writer programs (most rows stop early)
 constexpr uint64_t kPullLong = 65536;    // rows above this: one 1024-thread block; between: one wave
+template <typename R>
+constexpr uint64_t short_limit() { return R::kFirst ? kPullShortFirst : kPullShort; }
 
 __device__ inline bool bit_test(const uint32_t* b, uint32_t v) { return (b[v >> 5] >> (v & 31)) & 1u; }
 
@@ -340,18 +354,53 @@ __device__ inline void mark_next(uint32_t* next, uint32_t v) {
   if (!(next[v >> 5] & bit)) atomicOr(next + (v >> 5), bit);
 }
 
-// folds the active entries [b, e) of one adjacency array into acc; false once a
-// first-writer program has its message
+// folds the active entries [b, e) of one adjacency array into acc, in entry
+// order; false once a first-writer program has its message.  Eight entries per
+// step: their ids, then their frontier bits (and, for folding programs, their
+// messages) are loaded together, so a step waits for two or three memory
+// latencies instead of two per entry.  Past the row's end a step reads entry b
+// again (a valid address) and ignores it.
 template <typename R>
 __device__ inline bool fold_run(const EngCtx& c, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ w,
                                 uint64_t b, uint64_t e, const uint32_t* __restrict__ curr, bool all,
                                 typename R::T& acc, bool& any) {
-  for (uint64_t p = b; p < e; ++p) {
-    const uint32_t u = idx[p];
-    if (all || bit_test(curr, u)) {
-      acc = R::op(acc, R::msg(c, u, w ? w[p] : 0u));
-      any = true;
-      if (R::kFirst) return false;
+  using T = typename R::T;
+  for (uint64_t p = b; p < e; p += 8) {
+    uint64_t q[8];
+    uint32_t u[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = p + k < e ? p + k : b;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = idx[q[k]];
+    bool on[8];
+    if (all) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) on[k] = p + k < e;
+    } else {
+      uint32_t fw[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) fw[k] = curr[u[k] >> 5];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) on[k] = p + k < e && ((fw[k] >> (u[k] & 31)) & 1u);
+    }
+    if (R::kFirst) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (on[k]) {
+          acc = R::op(acc, R::msg(c, u[k], w ? w[q[k]] : 0u));
+          any = true;
+          return false;
+        }
+    } else {
+      T m[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) m[k] = R::msg(c, u[k], w ? w[q[k]] : 0u);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (on[k]) {
+          acc = R::op(acc, m[k]);
+          any = true;
+        }
     }
   }
   return true;
@@ -367,13 +416,14 @@ __global__ __launch_bounds__(kB) void k_pull_short(EngCtx c, PullAdj a, uint32_t
     if (a.dir != GRE_DIR_IN) { b0 = a.coff[v]; e0 = a.coff[v + 1]; }
     if (a.dir != GRE_DIR_OUT) { b1 = a.off[v]; e1 = a.off[v + 1]; }
     const uint64_t d = (e0 - b0) + (e1 - b1);
-    if (d > kPullShort || d == 0 || R::skip(c, (uint32_t)v)) continue;
+    if (d > short_limit<R>() || d == 0 || R::skip(c, (uint32_t)v)) continue;
     typename R::T acc = R::id();
     bool any = false;
     // CSC part first, then CSR (the row order of pull_entry)
     if (fold_run<R>(c, a.cidx, a.w_csc, b0, e0, curr, all, acc, any))
       fold_run<R>(c, a.idx, a.w_csr, b1, e1, curr, all, acc, any);
-    if (R::fin(c, (uint32_t)v, acc, any)) mark_next(next, (uint32_t)v);
+    if (R::fin(c, (uint32_t)v, acc, any) && (!R::kFuseApply || Prog<P>::apply(c, (uint32_t)v)))
+      mark_next(next, (uint32_t)v);
   }
 }
 
@@ -456,20 +506,20 @@ __global__ __launch_bounds__(W == 1 ? kB : 64 * W) void k_pull_rows(EngCtx c, Pu
       __syncthreads();
     }
     if ((W == 1 ? lane : (int)threadIdx.x) == 0 && !skip) {
-      if (R::fin(c, v, acc, first != ~0ULL)) mark_next(next, v);
+      if (R::fin(c, v, acc, first != ~0ULL) && (!R::kFuseApply || Prog<P>::apply(c, v))) mark_next(next, v);
     }
   }
 }
 
-// rows of pull degree in (kPullShort, kPullLong] and above kPullLong
-__global__ void k_pull_classify(PullAdj a, uint32_t nV, uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
-                                uint32_t* __restrict__ lng, uint32_t* __restrict__ nlng) {
+// rows of pull degree in (lim, kPullLong] and above kPullLong
+__global__ void k_pull_classify(PullAdj a, uint32_t nV, uint64_t lim, uint32_t* __restrict__ mid,
+                                uint32_t* __restrict__ nmid, uint32_t* __restrict__ lng, uint32_t* __restrict__ nlng) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t nvpad = (nV + 63) & ~63ULL;   // whole waves stay in the loop for the ballots
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nvpad; v += stride) {
     const uint64_t d = v < nV ? pull_deg(a, (uint32_t)v) : 0;
     wave_append(d > kPullLong, (uint32_t)v, lng, nlng);
-    wave_append(d > kPullShort && d <= kPullLong, (uint32_t)v, mid, nmid);
+    wave_append(d > lim && d <= kPullLong, (uint32_t)v, mid, nmid);
   }
 }
 
@@ -568,94 +618,207 @@ __global__ void k_init_src(EngCtx c, const uint32_t* __restrict__ src, int64_t n
     if (Prog<P>::init(c, src[i])) curr[src[i] >> 5] |= 1u << (src[i] & 31);
 }
 
-// per block of kVB vertices: number of set bits and their degree sum
-__device__ inline void block_count_write(uint32_t cnt, uint64_t deg, uint32_t* __restrict__ bcnt,
-                                         uint64_t* __restrict__ bdeg) {
-  __shared__ uint32_t sc[kB / 64];
-  __shared__ uint64_t sd[kB / 64];
-  for (int o = 32; o > 0; o >>= 1) {
-    cnt += __shfl_xor(cnt, o, 64);
-    deg += __shfl_xor(deg, o, 64);
+// ---- active-set kernels ---------------------------------------------------
+// One 256-thread block per kVB vertices (kVW bitmap words).  Work is spread
+// over the set bits only: the block takes the popcount prefix of its words,
+// then thread j handles the j-th set bit (word by binary search on the prefix,
+// bit by select), so a sparse bitmap costs little more than reading it and a
+// dense one keeps every thread busy.
+constexpr int kVW = kVB / 32;
+
+// the r-th (0-based) set bit of m; r < popc(m)
+__device__ inline uint32_t select_bit(uint32_t m, uint32_t r) {
+  uint32_t pos = 0, c;
+  c = (uint32_t)__popc(m & 0xffffu); if (r >= c) { r -= c; pos += 16; m >>= 16; }
+  c = (uint32_t)__popc(m & 0xffu);   if (r >= c) { r -= c; pos += 8;  m >>= 8; }
+  c = (uint32_t)__popc(m & 0xfu);    if (r >= c) { r -= c; pos += 4;  m >>= 4; }
+  c = (uint32_t)__popc(m & 0x3u);    if (r >= c) { r -= c; pos += 2;  m >>= 2; }
+  if (r >= (m & 1u)) pos += 1;
+  return pos;
+}
+
+// words[0..kVW) in LDS -> pre[0..kVW] exclusive popcount prefix (pre[kVW] = total);
+// every thread of the block calls it
+__device__ inline void word_prefix(const uint32_t* words, uint32_t* pre, uint32_t* wtot) {
+  const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = t < kVW ? (uint32_t)__popc(words[t]) : 0u;
+  const uint32_t own = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  if ((threadIdx.x & 63) == 0) { sc[threadIdx.x >> 6] = cnt; sd[threadIdx.x >> 6] = deg; }
+  if (lane == 63 && w < kVW / 64) wtot[w] = x;
+  __syncthreads();
+  if (t < kVW) {
+    uint32_t add = 0;
+    for (int k = 0; k < w; ++k) add += wtot[k];
+    pre[t] = add + x - own;
+    if (t == kVW - 1) pre[kVW] = add + x;
+  }
+  __syncthreads();
+}
+
+// word index of the j-th set bit: the last wi with pre[wi] <= j
+__device__ inline int word_of(const uint32_t* pre, uint32_t j) {
+  int lo = 0, hi = kVW;   // pre[lo] <= j < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// number of active vertices of the block and their degree sum, from the
+// block's active words in LDS (pre = their prefix), written to bcnt/bdeg
+__device__ inline void block_count_active(const uint32_t* act, const uint32_t* pre, uint32_t w0, Adj a,
+                                          uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg, uint64_t* red) {
+  const uint32_t total = pre[kVW];
+  uint64_t deg = 0;
+  for (uint32_t j = threadIdx.x; j < total; j += kB) {
+    const int wi = word_of(pre, j);
+    const uint32_t v = (w0 + (uint32_t)wi) * 32u + select_bit(act[wi], j - pre[wi]);
+    deg += degree(a, v);
+  }
+  for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = deg;
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t c = 0; uint64_t d = 0;
-    for (int w = 0; w < kB / 64; ++w) { c += sc[w]; d += sd[w]; }
-    bcnt[blockIdx.x] = c;
+    uint64_t d = 0;
+    for (int w = 0; w < kB / 64; ++w) d += red[w];
+    bcnt[blockIdx.x] = total;
     bdeg[blockIdx.x] = d;
   }
 }
 
-__global__ void k_count(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, uint32_t* __restrict__ bcnt,
-                        uint64_t* __restrict__ bdeg) {
-  const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
-  uint32_t cnt = 0;
-  uint64_t deg = 0;
-  for (int i = threadIdx.x; i < kVB; i += kB) {
-    const uint64_t v = v0 + i;
-    if (v < nV && bit_test(curr, (uint32_t)v)) { ++cnt; deg += degree(a, (uint32_t)v); }
-  }
-  block_count_write(cnt, deg, bcnt, bdeg);
+// count and degree sum of the active vertices (curr) per block
+__global__ __launch_bounds__(kB) void k_count(const uint32_t* __restrict__ curr, uint32_t nV, Adj a,
+                                              uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg) {
+  __shared__ uint32_t act[kVW];
+  __shared__ uint32_t pre[kVW + 1];
+  __shared__ uint32_t wtot[kVW / 64];
+  __shared__ uint64_t red[kB / 64];
+  const uint32_t w0 = blockIdx.x * (uint32_t)kVW;
+  (void)nV;   // bits past nV are never set
+  if (threadIdx.x < kVW) act[threadIdx.x] = curr[w0 + threadIdx.x];
+  __syncthreads();
+  word_prefix(act, pre, wtot);
+  block_count_active(act, pre, w0, a, bcnt, bdeg, red);
 }
 
-// apply + halt + count: new curr bit = (kHalt ? 0 : curr bit) | (next bit && apply(v))
+// apply + halt + count: new curr bit = (kHalt ? 0 : curr bit) | (next bit && apply(v)).
+// apply(v) touches only v's own data, so the order of the applies is free.
 template <int P>
-__global__ void k_apply(EngCtx c, const uint32_t* __restrict__ next, uint32_t* __restrict__ curr, uint32_t nV, Adj a,
-                        uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg) {
-  const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
-  uint32_t cnt = 0;
-  uint64_t deg = 0;
-  for (int i = threadIdx.x; i < kVB; i += kB) {
-    const uint64_t v = v0 + i;
-    bool on = false;
-    if (v < nV) {
-      on = Prog<P>::kHalt ? false : bit_test(curr, (uint32_t)v);
-      if (bit_test(next, (uint32_t)v) && Prog<P>::apply(c, (uint32_t)v)) on = true;
-    }
-    const uint64_t m = __ballot(on);
-    if ((threadIdx.x & 63) == 0) {
-      curr[v >> 5] = (uint32_t)m;
-      curr[(v >> 5) + 1] = (uint32_t)(m >> 32);
-    }
-    if (on) { ++cnt; deg += degree(a, (uint32_t)v); }
+__global__ __launch_bounds__(kB) void k_apply(EngCtx c, const uint32_t* __restrict__ next, uint32_t* __restrict__ curr,
+                                              uint32_t nV, Adj a, uint32_t* __restrict__ bcnt,
+                                              uint64_t* __restrict__ bdeg) {
+  __shared__ uint32_t nxt[kVW];
+  __shared__ uint32_t act[kVW];
+  __shared__ uint32_t pre[kVW + 1];
+  __shared__ uint32_t wtot[kVW / 64];
+  __shared__ uint64_t red[kB / 64];
+  const uint32_t w0 = blockIdx.x * (uint32_t)kVW;
+  (void)nV;   // next holds no bit past nV
+  if (threadIdx.x < kVW) {
+    nxt[threadIdx.x] = next[w0 + threadIdx.x];
+    act[threadIdx.x] = Prog<P>::kHalt ? 0u : curr[w0 + threadIdx.x];
   }
-  block_count_write(cnt, deg, bcnt, bdeg);
+  __syncthreads();
+  word_prefix(nxt, pre, wtot);
+  const uint32_t total = pre[kVW];
+  for (uint32_t j = threadIdx.x; j < total; j += kB) {
+    const int wi = word_of(pre, j);
+    const uint32_t b = select_bit(nxt[wi], j - pre[wi]);
+    if (Prog<P>::apply(c, (w0 + (uint32_t)wi) * 32u + b)) atomicOr(&act[wi], 1u << b);
+  }
+  __syncthreads();
+  if (threadIdx.x < kVW) curr[w0 + threadIdx.x] = act[threadIdx.x];
+  word_prefix(act, pre, wtot);
+  block_count_active(act, pre, w0, a, bcnt, bdeg, red);
 }
 
-// exclusive scan of block (count, degree) pairs; totals at [nb]
-__global__ void k_scan_blocks(uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg, uint32_t nb) {
-  __shared__ uint32_t sc[1024];
-  __shared__ uint64_t sd[1024];
+// after a pull that applied in place (kFuseApply): next holds exactly the
+// vertices apply() activated; new curr = (kHalt ? 0 : curr) | next, counted
+template <int P>
+__global__ __launch_bounds__(kB) void k_activate(const uint32_t* __restrict__ next, uint32_t* __restrict__ curr, Adj a,
+                                                 uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg) {
+  __shared__ uint32_t act[kVW];
+  __shared__ uint32_t pre[kVW + 1];
+  __shared__ uint32_t wtot[kVW / 64];
+  __shared__ uint64_t red[kB / 64];
+  const uint32_t w0 = blockIdx.x * (uint32_t)kVW;
+  if (threadIdx.x < kVW) {
+    const uint32_t m = next[w0 + threadIdx.x] | (Prog<P>::kHalt ? 0u : curr[w0 + threadIdx.x]);
+    act[threadIdx.x] = m;
+    curr[w0 + threadIdx.x] = m;
+  }
+  __syncthreads();
+  word_prefix(act, pre, wtot);
+  block_count_active(act, pre, w0, a, bcnt, bdeg, red);
+}
+
+// exclusive scan of block (count, degree) pairs; totals at [nb] and in
+// tot[0..1] (one read-back).  One 1024-thread block, kScanPer consecutive
+// pairs per thread and pass: every load of a pass is issued before any is
+// used, the per-thread sums are scanned with wave shuffles and one LDS step.
+constexpr int kScanPer = 16;
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* __restrict__ bcnt, uint64_t* __restrict__ bdeg,
+                                                      uint32_t nb, uint64_t* __restrict__ tot) {
+  __shared__ uint32_t wc[16];
+  __shared__ uint64_t wd[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t carry_c = 0;
   uint64_t carry_d = 0;
-  for (uint32_t base = 0; base < nb; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t c = i < nb ? bcnt[i] : 0;
-    const uint64_t d = i < nb ? bdeg[i] : 0;
-    sc[threadIdx.x] = c;
-    sd[threadIdx.x] = d;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      uint32_t tc = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
-      uint64_t td = threadIdx.x >= (unsigned)o ? sd[threadIdx.x - o] : 0;
-      __syncthreads();
-      sc[threadIdx.x] += tc;
-      sd[threadIdx.x] += td;
-      __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024 * kScanPer) {
+    const uint32_t b = base + threadIdx.x * kScanPer;
+    uint32_t cv[kScanPer];
+    uint64_t dv[kScanPer];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      const uint32_t i = b + k < nb ? b + k : nb - 1;   // a valid pair, ignored below
+      cv[k] = bcnt[i];
+      dv[k] = bdeg[i];
     }
-    if (i < nb) { bcnt[i] = carry_c + sc[threadIdx.x] - c; bdeg[i] = carry_d + sd[threadIdx.x] - d; }
-    carry_c += sc[1023];
-    carry_d += sd[1023];
+    uint32_t c = 0;
+    uint64_t d = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+      if (b + k < nb) { c += cv[k]; d += dv[k]; }
+    uint32_t ic = c;
+    uint64_t id = d;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t tc = __shfl_up(ic, o, 64);
+      const uint64_t td = __shfl_up(id, o, 64);
+      if (lane >= o) { ic += tc; id += td; }
+    }
+    if (lane == 63) { wc[w] = ic; wd[w] = id; }
+    __syncthreads();
+    uint32_t rc = carry_c + ic - c;
+    uint64_t rd = carry_d + id - d;
+    for (int k = 0; k < w; ++k) { rc += wc[k]; rd += wd[k]; }
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+      if (b + k < nb) {
+        bcnt[b + k] = rc;
+        bdeg[b + k] = rd;
+        rc += cv[k];
+        rd += dv[k];
+      }
+    for (int k = 0; k < 16; ++k) { carry_c += wc[k]; carry_d += wd[k]; }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { bcnt[nb] = carry_c; bdeg[nb] = carry_d; }
+  if (threadIdx.x == 0) {
+    bcnt[nb] = carry_c;
+    bdeg[nb] = carry_d;
+    tot[0] = carry_c;
+    tot[1] = carry_d;
+  }
 }
 
 // queue[pos] = v, qpos[pos] = edge prefix; qpos[total] = total edges
-__global__ void k_write(const uint32_t* __restrict__ curr, uint32_t nV, Adj a, const uint32_t* __restrict__ bcnt,
-                        const uint64_t* __restrict__ bdeg, uint32_t nb, uint32_t* __restrict__ queue,
-                        uint64_t* __restrict__ qpos) {
+__global__ __launch_bounds__(kB) void k_write(const uint32_t* __restrict__ curr, uint32_t nV, Adj a,
+                                               const uint32_t* __restrict__ bcnt, const uint64_t* __restrict__ bdeg,
+                                               uint32_t nb, uint32_t* __restrict__ queue,
+                                               uint64_t* __restrict__ qpos) {
   __shared__ uint32_t wc[kB / 64];
   __shared__ uint64_t wd[kB / 64];
   const uint64_t v0 = (uint64_t)blockIdx.x * kVB;
@@ -769,7 +932,10 @@ struct gre_engine {
   gre::DevBuf S, C, X, AUX;
   size_t es[4] = {0, 0, 0, 0};
   bool c_is_x = false;  // SSSP: combine and vertex data are one array
-  gre::DevBuf curr, next, queue, qpos, bcnt, bdeg;
+  gre::DevBuf curr, next, queue, qpos, bcnt, bdeg, tot;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // compute start, compute end, apply end
+  hipEvent_t ev_tot = nullptr;                       // the active totals have reached the host
+  uint64_t* tot_host = nullptr;                      // pinned: (active count, active edges)
   uint32_t nwords = 0, nb = 0;
   int32_t level = 1;
   int dir = GRE_DIR_OUT;
@@ -783,6 +949,8 @@ struct gre_engine {
   double times[5] = {0, 0, 0, 0, 0};
   int mode = 0;                // GRE_ENGINE_AUTO / _PUSH / _PULL
   int64_t pull_steps = 0;      // supersteps of the last run done in the pull form
+  uint64_t seen_edges = 0;     // first-writer programs: edges of every frontier so far
+  bool pulling = false;        // first-writer programs: the last superstep pulled
   bool all_began = false;      // the run began with every vertex active (no sources)
   int sum_dir = -1;            // direction of the tile plan below
   gre::PullPlan sum_plan;      // tile-segmented pull of complete-frontier FP64 sums
@@ -835,9 +1003,10 @@ PullAdj make_pull_adj(gre_engine* e) {
   return a;
 }
 
-// rows too long for one thread, listed once per graph and direction
-const PullRows& ensure_pull_rows(gre_engine* e, const PullAdj& a, hipStream_t s) {
-  PullRows& r = e->g->pull_rows[e->dir];
+// rows too long for one thread, listed once per graph, direction and short-row
+// limit (first-writer programs keep longer rows on one thread)
+const PullRows& ensure_pull_rows(gre_engine* e, const PullAdj& a, uint64_t lim, hipStream_t s) {
+  PullRows& r = e->g->pull_rows[e->dir + (lim == kPullShort ? 0 : 4)];
   if (r.mid.p) return r;
   const uint32_t nV = e->g->nV;
   DevBuf cnt;
@@ -845,7 +1014,7 @@ const PullRows& ensure_pull_rows(gre_engine* e, const PullAdj& a, hipStream_t s)
   r.mid.alloc(4 * (size_t)nV);
   r.lng.alloc(4 * (size_t)nV);
   GRE_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
-  hipLaunchKernelGGL(k_pull_classify, dim3(gcap(nV)), dim3(kB), 0, s, a, nV, r.mid.as<uint32_t>(),
+  hipLaunchKernelGGL(k_pull_classify, dim3(gcap(nV)), dim3(kB), 0, s, a, nV, lim, r.mid.as<uint32_t>(),
                      cnt.as<uint32_t>(), r.lng.as<uint32_t>(), cnt.as<uint32_t>() + 1);
   GRE_HIP(hipGetLastError());
   uint32_t h[2];
@@ -904,18 +1073,29 @@ void default_state(gre_engine* e) {
 // The frontier's size and out-degree total, from the blocks' counts.  The
 // queue itself (k_write) is written only when a push step needs it: pull
 // steps read the currSet bitmap, which stays unchanged until the next apply.
-void rebuild_queue(gre_engine* e, const Adj& a, hipStream_t s) {
+// done (optional) is recorded once the active set is rebuilt, before the
+// read-back the host waits for
+void rebuild_queue(gre_engine* e, const Adj& a, hipStream_t s, hipEvent_t done = nullptr) {
   (void)a;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb);
+  if (!e->tot.p) e->tot.alloc(16);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>(), e->nb,
+                     e->tot.as<uint64_t>());
   GRE_HIP(hipGetLastError());
+  if (done) GRE_HIP(hipEventRecord(done, s));
   e->queue_stale = true;
-  uint32_t qn = 0;
-  uint64_t qe = 0;
-  GRE_HIP(hipMemcpyAsync(&qn, e->bcnt.as<uint32_t>() + e->nb, 4, hipMemcpyDeviceToHost, s));
-  GRE_HIP(hipMemcpyAsync(&qe, e->bdeg.as<uint64_t>() + e->nb, 8, hipMemcpyDeviceToHost, s));
-  GRE_HIP(hipStreamSynchronize(s));
-  e->qn = qn;
-  e->qedges = qe;
+  // the totals go to pinned host memory and the host polls for them: a
+  // superstep waits here every time, and a sleeping wait adds tens of
+  // microseconds of wake-up to each
+  if (!e->tot_host) GRE_HIP(hipHostMalloc((void**)&e->tot_host, 16, hipHostMallocDefault));
+  if (!e->ev_tot) GRE_HIP(hipEventCreateWithFlags(&e->ev_tot, hipEventDisableTiming));
+  GRE_HIP(hipMemcpyAsync(e->tot_host, e->tot.p, 16, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipEventRecord(e->ev_tot, s));
+  hipError_t q;
+  while ((q = hipEventQuery(e->ev_tot)) == hipErrorNotReady) {
+  }
+  GRE_HIP(q);
+  e->qn = (uint32_t)e->tot_host[0];
+  e->qedges = e->tot_host[1];
 }
 
 double now() {
@@ -961,6 +1141,8 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
   hipLaunchKernelGGL(k_count, dim3(e->nb), dim3(kB), 0, s, e->curr.as<uint32_t>(), e->g->nV, a,
                      e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
   rebuild_queue(e, a, s);
+  e->seen_edges = e->qedges;
+  e->pulling = false;
   e->superstep = 0;
   e->pull_steps = 0;
   e->all_began = srcs == nullptr;
@@ -982,7 +1164,10 @@ int64_t engine_step(gre_engine* e) {
   hipStream_t s = stream();
   const Adj a = make_adj(e);
   EngCtx c = make_ctx(e);
-  const double t0 = now();
+  // phase times from events on the stream: no host wait between the phases
+  for (hipEvent_t& ev : e->ev)
+    if (!ev) GRE_HIP(hipEventCreate(&ev));
+  GRE_HIP(hipEventRecord(e->ev[0], s));
   // nextSet.clear() (Engine.hpp:322)
   hipLaunchKernelGGL(k_zero_u32, dim3(gcap(e->nwords)), dim3(kB), 0, s, e->next.as<uint32_t>(), (uint64_t)e->nwords);
   // push while the frontier's edges are few, pull once they are an eighth of
@@ -994,8 +1179,22 @@ int64_t engine_step(gre_engine* e) {
   // at every frontier size, the complete one included.
   const uint64_t epull = (e->dir == GRE_DIR_INOUT ? 2 : 1) * e->g->nE;
   const bool min_combine = e->prog == GRE_PROG_CC || e->prog == GRE_PROG_SSSP;
-  const bool pull = e->qedges > 0 && (e->mode == GRE_ENGINE_PULL ||
-                                      (e->mode == GRE_ENGINE_AUTO && !min_combine && e->qedges * 8 > epull));
+  // First-writer programs (BFS) reach each vertex once, so the edges of the
+  // frontiers so far are the reached vertices' edges: they switch like a
+  // direction-optimising BFS, to the pull once the frontier has a twentieth of
+  // the edges not yet reached, back to the push once it holds under 1/24 of
+  // the vertices.  Either form gives GRE's result; this only picks the cheaper.
+  const bool first_writer = e->prog == GRE_PROG_BFS;
+  bool auto_pull;
+  if (first_writer) {
+    const uint64_t unseen = epull > e->seen_edges ? epull - e->seen_edges : 0;
+    auto_pull = e->pulling ? (uint64_t)e->qn * 24 > e->g->nV : e->qedges * 20 > unseen;
+  } else {
+    auto_pull = !min_combine && e->qedges * 8 > epull;
+  }
+  const bool pull = e->qedges > 0 && (e->mode == GRE_ENGINE_PULL || (e->mode == GRE_ENGINE_AUTO && auto_pull));
+  e->pulling = pull;
+  bool fused = false;   // the compute phase applied in place (kFuseApply)
   const bool fsum = e->prog == GRE_PROG_PAGERANK || e->prog == GRE_PROG_PAGERANK_DELTA ||
                     e->prog == GRE_PROG_BC_BACKWARD;
   if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
@@ -1046,11 +1245,12 @@ int64_t engine_step(gre_engine* e) {
     ++e->pull_steps;
   } else if (pull) {
     const PullAdj pa = make_pull_adj(e);
-    const PullRows& rows = ensure_pull_rows(e, pa, s);
-    ProfScope ps("pull_compute");
     const uint32_t nV = e->g->nV;
     dispatch(e->prog, [&](auto P) {
       constexpr int PV = decltype(P)::value;
+      fused = Pull<PV>::kFuseApply;
+      const PullRows& rows = ensure_pull_rows(e, pa, short_limit<Pull<PV>>(), s);
+      ProfScope ps("pull_compute");
       // every neighbour's message is folded (no frontier test) when every
       // neighbour is active, or when the stale ones cannot change the result
       const bool all = e->qedges == epull || (Pull<PV>::kStale && e->all_began && e->qedges * 2 > epull);
@@ -1084,20 +1284,26 @@ int64_t engine_step(gre_engine* e) {
     });
     GRE_HIP(hipGetLastError());
   }
-  GRE_HIP(hipStreamSynchronize(s));
-  const double t1 = now();
+  GRE_HIP(hipEventRecord(e->ev[1], s));
   {
     ProfScope ps("apply");
     dispatch(e->prog, [&](auto P) {
-      hipLaunchKernelGGL(k_apply<decltype(P)::value>, dim3(e->nb), dim3(kB), 0, s, c, e->next.as<uint32_t>(),
-                         e->curr.as<uint32_t>(), e->g->nV, a, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
+      if (fused)
+        hipLaunchKernelGGL(k_activate<decltype(P)::value>, dim3(e->nb), dim3(kB), 0, s, e->next.as<uint32_t>(),
+                           e->curr.as<uint32_t>(), a, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
+      else
+        hipLaunchKernelGGL(k_apply<decltype(P)::value>, dim3(e->nb), dim3(kB), 0, s, c, e->next.as<uint32_t>(),
+                           e->curr.as<uint32_t>(), e->g->nV, a, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
     });
   }
   GRE_HIP(hipGetLastError());
-  rebuild_queue(e, a, s);
-  const double t2 = now();
-  e->times[2] += t1 - t0;
-  e->times[4] += t2 - t1;
+  rebuild_queue(e, a, s, e->ev[2]);   // waits for the stream
+  e->seen_edges += e->qedges;
+  float ms_c = 0.f, ms_a = 0.f;
+  GRE_HIP(hipEventElapsedTime(&ms_c, e->ev[0], e->ev[1]));
+  GRE_HIP(hipEventElapsedTime(&ms_a, e->ev[1], e->ev[2]));
+  e->times[2] += 1e-3 * ms_c;
+  e->times[4] += 1e-3 * ms_a;
   // the program's reduce() hook: Forward increments currLevel (BC.hpp:40-42)
   if (e->prog == GRE_PROG_BC_FORWARD) e->level++;
   return e->qn;
@@ -1160,6 +1366,10 @@ int gre_engine_destroy(gre_engine* e) {
     // like a PageRank handle: drop the graph's cached partitions that nothing
     // holds once this engine is gone (about 9 GB each at 2^30 edges)
     gre_graph* g = e->g;
+    for (hipEvent_t ev : e->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (e->ev_tot) (void)hipEventDestroy(e->ev_tot);
+    if (e->tot_host) (void)hipHostFree(e->tot_host);
     delete e;
     prune_xparts(g);
   });
