@@ -129,8 +129,9 @@ def bench_graph_replay(N):
     return out
 
 
-def bench_engine_pr(N, g):
-    """GRE's PageRank vertex program through the superstep engine (10 supersteps), push vs the automatic switch."""
+def bench_engine_pr(N, g, iter_ms):
+    """GRE's PageRank vertex program through the superstep engine (10 supersteps), push vs the automatic switch,
+    and its superstep against the headline iteration (iter_ms) scaled by the two's algorithmic bytes."""
     out = {"program": "GRE PageRank (FP64 scatter data), 10 supersteps", "graph": "the headline graph"}
     for name, mode in (("push", N.Engine.PUSH), ("auto", N.Engine.AUTO)):
         e = N.Engine(g, N.PAGERANK)
@@ -147,27 +148,47 @@ def bench_engine_pr(N, g):
         out[name + "_pull_steps"] = e.pull_steps()
         e.close()
     out["auto_vs_push"] = round(out["push_ms"] / out["auto_ms"], 2)
+    # one superstep pulls FP64 scatter data over the CSC (o(V+1) + (4+8)E) and
+    # applies over V (combine read + reset, vertex write, scatter read: 16 V
+    # beyond the offsets), the headline iteration o(V+1) + 8E + 16V
+    V, E = g.nV, g.nE
+    b_eng = 8 * (V + 1) + 12 * E + 16 * V
+    b_head = 8 * (V + 1) + 8 * E + 16 * V
+    ss_ms = out["auto_ms"] / 10
+    out["auto_ms_per_superstep"] = round(ss_ms, 3)
+    out["alg_bytes_per_superstep"] = b_eng
+    out["auto_gbs"] = round(b_eng / (ss_ms / 1e3) / 1e9, 1)
+    out["headline_iteration_ms_scaled"] = round(iter_ms * b_eng / b_head, 3)
+    out["vs_headline_scaled"] = round(ss_ms / (iter_ms * b_eng / b_head), 3)
     return out
 
 
-def bench_engine_bfs(N, gs, src):
-    """GRE's BFS vertex program through the engine from one source, push vs the automatic switch."""
-    out = {"program": "GRE BFS", "source": int(src)}
-    for name, mode in (("push", N.Engine.PUSH), ("auto", N.Engine.AUTO)):
+def bench_engine_bfs(N, gs, srcs, plain_ms):
+    """GRE's BFS vertex program through the engine (automatic switch) from each source, against the library's own
+    direction-optimising BFS from the same sources (plain_ms); the push form from the first source."""
+    out = {"program": "GRE BFS", "sources": [int(s_) for s_ in srcs]}
+    auto = []
+    for s_ in srcs:
         for rep in range(2):   # the first run builds the cached row classes
             e = N.Engine(gs, N.BFS)
-            e.set_mode(mode)
             N.sync()
             t = time.perf_counter()
-            ss, _ = e.run(N.DIR_IN, srcs=[src])
+            ss, _ = e.run(N.DIR_IN, srcs=[int(s_)])
             N.sync()
             dt = time.perf_counter() - t
-            pulls = e.pull_steps()
             e.close()
-        out[name + "_ms"] = round(1e3 * dt, 2)
-        out[name + "_pull_steps"] = pulls
-    out["supersteps"] = int(ss)
-    out["auto_vs_push"] = round(out["push_ms"] / out["auto_ms"], 2)
+        auto.append(1e3 * dt)
+    e = N.Engine(gs, N.BFS)
+    e.set_mode(N.Engine.PUSH)
+    N.sync()
+    t = time.perf_counter()
+    e.run(N.DIR_IN, srcs=[int(srcs[0])])
+    N.sync()
+    out["push_ms_first_source"] = round(1e3 * (time.perf_counter() - t), 2)
+    e.close()
+    out["auto_ms"] = [round(x, 2) for x in auto]
+    out["plain_bfs_ms"] = [round(x, 2) for x in plain_ms]
+    out["auto_vs_plain_bfs"] = round(sum(auto) / sum(plain_ms), 3)
     return out
 
 
@@ -411,7 +432,7 @@ def main():
         }
         log("ppr:", extra["ppr"])
         del pr
-        extra["engine_pagerank"] = bench_engine_pr(N, g)
+        extra["engine_pagerank"] = bench_engine_pr(N, g, iter_s * 1e3)
         log("engine pagerank:", extra["engine_pagerank"])
         del g
         # ---------------- BFS on the symmetrised graph ----------------------
@@ -424,13 +445,15 @@ def main():
         # 64-bit-mask BFS runs from all 64 in one launch
         srcs = sample_distinct(rng, reached, max(64, a.bfs_sources)).astype(np.uint32)
         N.bfs(gs, int(srcs[0]), want_levels=False)
-        rates = []
+        rates, plain_ms = [], []
         for s in srcs[:a.bfs_sources]:
             N.sync()
             t1 = time.perf_counter()
             _, er, _ = N.bfs(gs, int(s), want_levels=False)
             N.sync()
-            rates.append((er / 2) / (time.perf_counter() - t1) / 1e9)
+            dt_ = time.perf_counter() - t1
+            rates.append((er / 2) / dt_ / 1e9)
+            plain_ms.append(1e3 * dt_)
         hm = len(rates) / sum(1.0 / r for r in rates)
         extra["bfs"] = {"sources": len(rates), "graph": f"R-MAT scale {S} symmetrised, seed 6",
                         "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
@@ -445,7 +468,7 @@ def main():
                           "ms": round(1e3 * t_ms, 2),
                           "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
         log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
-        extra["engine_bfs"] = bench_engine_bfs(N, gs, int(srcs[0]))
+        extra["engine_bfs"] = bench_engine_bfs(N, gs, srcs[:4], plain_ms[:4])
         log("engine bfs:", extra["engine_bfs"])
         del gs
         extra["bc"] = bench_bc(N, rng)

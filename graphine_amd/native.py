@@ -112,6 +112,9 @@ def _declare(L):
         "gre_bfs_dev": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_msbfs": ([vp, vp, i32, vp, C.POINTER(u64)], i32),
         "gre_bc": ([vp, vp, i32, vp, vp, vp, vp], i32),
+        "gre_vertex_cut_ldg": ([u32, u64, i32, i32, vp, vp, vp, vp], i32),
+        "gre_vertex_cut_stats": ([u32, i32, i32, u64, vp, vp, vp, vp], i32),
+        "gre_vertex_cut_lvid2vid": ([u32, i32, i32, i32, u64, vp, vp, vp, vp, u64, C.POINTER(u64)], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -459,3 +462,31 @@ def bc(graph, srcs, want_parts=False):
     lv = np.zeros((len(s), graph.nV), np.int32) if want_parts else None
     _ck(lib().gre_bc(graph.h, _p(s), len(s), _p(out), _p(sig), _p(dl), _p(lv)))
     return out, sig, dl, lv
+
+
+def vertex_cut_ldg(nV, nE_meta, np_, files):
+    """LDG placement over np parts; files: list of (src, dst) arrays in file order.
+    Returns (src, dst, part) over all files concatenated."""
+    off = np.zeros(len(files) + 1, np.uint64)
+    off[1:] = np.cumsum([len(f[0]) for f in files])
+    src = np.ascontiguousarray(np.concatenate([f[0] for f in files]), np.uint32)
+    dst = np.ascontiguousarray(np.concatenate([f[1] for f in files]), np.uint32)
+    part = np.zeros(len(src), np.int32)
+    _ck(lib().gre_vertex_cut_ldg(nV, nE_meta, np_, len(files), _p(off), _p(src), _p(dst), _p(part)))
+    return src, dst, part
+
+
+def vertex_cut_stats(nV, np_, direction, src, dst, part):
+    st = np.zeros((np_, 7), np.int64)
+    _ck(lib().gre_vertex_cut_stats(nV, np_, direction, len(src), _p(src), _p(dst), _p(part), _p(st)))
+    return st
+
+
+def vertex_cut_lvid2vid(nV, np_, direction, p, src, dst, part):
+    n = C.c_uint64()
+    L = lib()
+    _ck(L.gre_vertex_cut_lvid2vid(nV, np_, direction, p, len(src), _p(src), _p(dst), _p(part), None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.uint64)
+    _ck(L.gre_vertex_cut_lvid2vid(nV, np_, direction, p, len(src), _p(src), _p(dst), _p(part), _p(out), n.value,
+                                  C.byref(n)))
+    return out

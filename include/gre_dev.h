@@ -248,6 +248,31 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out,
 int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double* sigma_out,
            double* delta_out, int32_t* level_out);
 
+/* ---- vertex-cut placement for np processes (GRE's MPI ingress) ------------
+ * Host-side, no device work: the placement a run over np GPUs would load.
+ * Replaces GraphIngress::LDGEdgeDecision (include/Graph/GraphIngress.hpp:
+ * 1126-1171) as loadEdgesFromFiles drives it (:511-723): process r reads the
+ * edge files r, r + np, ... in order and decides each edge's part with its own
+ * state; capacity = nE_meta / np / np (must be > 0, i.e. nE_meta >= np*np,
+ * where GRE would divide by zero).  file_off[nfiles + 1] cuts src/dst (all
+ * files concatenated, in file order) into the files; part receives each
+ * edge's part. */
+int gre_vertex_cut_ldg(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64_t* file_off,
+                       const uint32_t* src, const uint32_t* dst, int32_t* part);
+/* Per part p (stats[7 p ...]): masters, scatters, combiners, bi-agents,
+ * agents, local edges, aligned masters, as GraphIngress::ingress<dir> counts
+ * them (:111-153, 229-235); owner(v) = v % np.  The PageRank program prints
+ * the sums of combiners and scatters and their ratio to the edges
+ * (include/Kernels/PageRank.hpp:103-122). */
+int gre_vertex_cut_stats(uint32_t nV, int np, int dir, uint64_t nE, const uint32_t* src, const uint32_t* dst,
+                         const int32_t* part, int64_t* stats);
+/* Part p's lvid2vid: masters p, p+np, ..., padding to a multiple of 64 with
+ * all-ones, then its combiners (Out) or all agents (In, InOut) in ascending
+ * id (:154-178).  n_out receives the length; lvid2vid may be NULL. */
+int gre_vertex_cut_lvid2vid(uint32_t nV, int np, int dir, int p, uint64_t nE, const uint32_t* src,
+                            const uint32_t* dst, const int32_t* part, uint64_t* lvid2vid, uint64_t cap,
+                            uint64_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

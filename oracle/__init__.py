@@ -76,6 +76,9 @@ def _declare(L):
     L.o_ppr_edge_grad.argtypes = [u32, vp, vp, vp, vp, vp, vp, dbl, vp, vp]
     L.o_gre_pagerank_mt.argtypes = [C.c_int, u32, vp, vp, vp, dbl, C.c_int, vp]
     L.o_gre_pagerank_mt.restype = dbl
+    L.o_ldg_assign.argtypes = [u32, u64, C.c_int, C.c_int, vp, vp, vp, vp]
+    L.o_ldg_assign.restype = C.c_int
+    L.o_vertex_cut_stats.argtypes = [u32, C.c_int, C.c_int, u64, vp, vp, vp, vp]
 
 
 # --------------------------------------------------------------------------
@@ -286,3 +289,32 @@ def lognormal_csr(g, seed=5):
     for e in range(g.nE):
         w[e] = L.o_lognormal_weight(seed, int(src[e]), int(g.idx[e]))
     return w
+
+
+# --------------------------------------------------------------------------
+# vertex-cut ingress (GraphIngress.hpp)
+# --------------------------------------------------------------------------
+STAT_NAMES = ("masters", "scatters", "combiners", "bi_agents", "agents", "edges", "aligned_masters")
+
+
+def ldg_assign(nV, nE_meta, np_, files):
+    """LDG part of every edge: files is a list of (src, dst) arrays in file
+    order; reader r takes files r, r+np, ... (GraphIngress.hpp:511-600, 1126-1171)."""
+    off = np.zeros(len(files) + 1, np.uint64)
+    off[1:] = np.cumsum([len(f[0]) for f in files])
+    src = np.ascontiguousarray(np.concatenate([f[0] for f in files]) if files else np.zeros(0), np.uint32)
+    dst = np.ascontiguousarray(np.concatenate([f[1] for f in files]) if files else np.zeros(0), np.uint32)
+    part = np.zeros(len(src), np.int32)
+    rc = lib().o_ldg_assign(nV, nE_meta, np_, len(files), _p(off), _p(src), _p(dst), _p(part))
+    if rc != 0:
+        raise ValueError("capacity 0: fewer than np*np edges")
+    return src, dst, part
+
+
+def vertex_cut_stats(nV, np_, direction, src, dst, part):
+    st = np.zeros((np_, 7), np.int64)
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    part = np.ascontiguousarray(part, np.int32)
+    lib().o_vertex_cut_stats(nV, np_, direction, len(src), _p(src), _p(dst), _p(part), _p(st))
+    return st

@@ -647,3 +647,85 @@ double o_gre_pagerank_mt(int nthreads, uint32_t nV, const uint64_t* off, const u
   free(ps); free(cm); free((void*)locks);
   return t1 - t0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vertex-cut ingress at np processes (include/Graph/GraphIngress.hpp)      */
+/* ------------------------------------------------------------------------ */
+/* LDGEdgeDecision::getPartID (GraphIngress.hpp:1144-1170), one decision
+ * state per reading process; process r reads files r, r+np, ... in order
+ * (loadEdgesFromFiles :511-600), capacity nE/np/np (:560).  Returns -1 when
+ * the capacity is 0 (GRE divides by it). */
+int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64_t* file_off,
+                 const uint32_t* src, const uint32_t* dst, int32_t* part) {
+  const uint64_t capacity = nE_meta / (uint64_t)np / (uint64_t)np;
+  if (capacity == 0) return -1;
+  uint64_t* sbits = (uint64_t*)malloc((size_t)nV * 8);
+  uint64_t* dbits = (uint64_t*)malloc((size_t)nV * 8);
+  uint64_t cnt[64];
+  double sc[64];
+  int cand[64];
+  for (int reader = 0; reader < np; ++reader) {
+    memset(sbits, 0, (size_t)nV * 8);
+    memset(dbits, 0, (size_t)nV * 8);
+    memset(cnt, 0, sizeof(cnt));
+    for (int f = reader; f < nfiles; f += np) {
+      for (uint64_t e = file_off[f]; e < file_off[f + 1]; ++e) {
+        const uint64_t s = src[e], d = dst[e];
+        for (int i = 0; i < np; ++i) {
+          const double bal = fabs(1.0 - (double)cnt[i] / (double)capacity);
+          /* (srcMap.test(i) ? 1.0 : (src%np==i ? 0.5 : 0.0) + (dstMap.test(i) ? 1.0 : (dst%np==i ? 0.5 : 0.0))) */
+          double term;
+          if ((sbits[s] >> i) & 1ULL) {
+            term = 1.0;
+          } else {
+            term = ((int)(s % (uint64_t)np) == i) ? 0.5 : 0.0;
+            if ((dbits[d] >> i) & 1ULL) term += 1.0;
+            else if ((int)(d % (uint64_t)np) == i) term += 0.5;
+          }
+          sc[i] = bal * term;
+        }
+        double mx = sc[0];                             /* Util::getMax: first, then > */
+        for (int i = 0; i < np; ++i) if (sc[i] > mx) mx = sc[i];
+        int n = 0;
+        for (int i = 0; i < np; ++i) if (fabs(mx - sc[i]) < 1e-4) cand[n++] = i;
+        const int pid = cand[(s + d) % (uint64_t)n];
+        cnt[pid]++;
+        sbits[s] |= 1ULL << pid;
+        dbits[d] |= 1ULL << pid;
+        part[e] = pid;
+      }
+    }
+  }
+  free(sbits); free(dbits);
+  return 0;
+}
+
+/* Per part p: masters, scatters, combiners, bi-agents, agents, local edges,
+ * aligned masters (GraphIngress.hpp:111-153, 229-235; getPartID = vid % np). */
+void o_vertex_cut_stats(uint32_t nV, int np, int dir, uint64_t nE, const uint32_t* src, const uint32_t* dst,
+                        const int32_t* part, int64_t* stats) {
+  uint8_t* sc = (uint8_t*)malloc(nV ? nV : 1);
+  uint8_t* cb = (uint8_t*)malloc(nV ? nV : 1);
+  for (int p = 0; p < np; ++p) {
+    memset(sc, 0, nV); memset(cb, 0, nV);
+    int64_t ne = 0, ns = 0, nc = 0, nb = 0;
+    for (uint64_t e = 0; e < nE; ++e) {
+      if (part[e] != p) continue;
+      ne++;
+      if (src[e] % (uint32_t)np != (uint32_t)p) sc[src[e]] = 1;
+      if (dst[e] % (uint32_t)np != (uint32_t)p) cb[dst[e]] = 1;
+    }
+    for (uint32_t v = 0; v < nV; ++v) { ns += sc[v]; nc += cb[v]; nb += sc[v] & cb[v]; }
+    const int64_t nm = nV / (uint32_t)np + ((uint32_t)p < nV % (uint32_t)np);
+    const int64_t ag = nc + ns - nb;
+    int64_t* o = stats + 7 * p;
+    o[0] = nm;
+    o[1] = dir == O_DIR_OUT ? ns : ag;
+    o[2] = dir == O_DIR_OUT ? nc : ag;
+    o[3] = nb;
+    o[4] = ag;
+    o[5] = ne;
+    o[6] = (nm + 63) / 64 * 64;
+  }
+  free(sc); free(cb);
+}

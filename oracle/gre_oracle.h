@@ -103,4 +103,10 @@ double o_gre_pagerank_mt(int nthreads, uint32_t nV, const uint64_t* off, const u
 #ifdef __cplusplus
 }
 #endif
+/* vertex-cut ingress at np processes */
+int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64_t* file_off,
+                 const uint32_t* src, const uint32_t* dst, int32_t* part);
+void o_vertex_cut_stats(uint32_t nV, int np, int dir, uint64_t nE, const uint32_t* src, const uint32_t* dst,
+                        const int32_t* part, int64_t* stats);
+
 #endif
