@@ -13,6 +13,7 @@
 #include "graph.hpp"
 
 #include <algorithm>
+#include <map>
 #include <type_traits>
 #include <vector>
 
@@ -257,9 +258,37 @@ void bfs_run(gre_graph* g, uint32_t src, int32_t* level, uint64_t* edges_out, in
 // ---------------------------------------------------------------------------
 // Working arrays, grown to the largest graph seen and kept for the process
 // (like the single-source ones): a call allocates nothing of size V.
+// Level control of the 64-source BFS, in device memory: every level kernel
+// takes its sizes, its direction and the level number from here, and two
+// one-thread kernels advance it, so levels run back to back with no host
+// round trip (a captured graph of kMsBatch levels is replayed until mode is
+// kMsDone).
+enum : uint32_t { kMsTopDown = 0, kMsBottomUp = 1, kMsDone = 2 };
+struct MsCtl {
+  uint32_t L;          // level being expanded (frontier = vertices first reached at L)
+  uint32_t qn;         // frontier size
+  uint32_t nn;         // next frontier size (appended to by the level's kernels)
+  uint32_t prev_n;     // the frontier size of the level before
+  uint32_t mode;
+  uint32_t pad;
+  uint64_t T;          // out-edges of the frontier (top-down levels)
+  uint64_t mf;         // out-edges of the frontier, from the level before
+  uint64_t mu;         // edges of the vertices still missing source bits
+  unsigned long long sums[2];   // the level's (new frontier's edges, edges of vertices now complete)
+};
+constexpr int kMsBatch = 8;     // levels per replayed graph (even: queue parity repeats)
+constexpr int kScanB = 2048;    // elements per chunk of the device-length scan
+// Grids of the level kernels: grid-stride loops over their work, sized so a
+// level's kernels for the direction not taken (which return at once) cost
+// little to dispatch, while a full one still has many more waves than the
+// chip holds.
+constexpr unsigned kMsGrid = 8192;
+constexpr unsigned kMsScanGrid = 1024;
+
 struct MsBufs {
-  DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums, src;
+  DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums, src, ctl, bsum;
   uint32_t nV = 0;
+  MsCtl* ctl_host = nullptr;    // pinned
 };
 MsBufs& ms_bufs(uint32_t nV) {
   static MsBufs& b = *new MsBufs();  // lives until the process ends
@@ -275,9 +304,129 @@ MsBufs& ms_bufs(uint32_t nV) {
     b.flag.alloc(16);
     b.sums.alloc(16);
     b.src.alloc(4 * 64);
+    b.ctl.alloc(sizeof(MsCtl));
+    b.bsum.alloc(8 * (((size_t)nV + 1 + kScanB - 1) / kScanB + 1));
+    if (!b.ctl_host) GRE_HIP(hipHostMalloc((void**)&b.ctl_host, sizeof(MsCtl), hipHostMallocDefault));
     b.nV = nV;
   }
   return b;
+}
+
+// the direction of the level, as the host loop chose it before: bottom-up once
+// the frontier's edges exceed 1/14 of the edges still missing bits, top-down
+// again once the frontier shrinks under V/24; done once it is empty
+__global__ void k_ms_ctl_begin(MsCtl* c, uint32_t nV) {
+  if (c->mode == kMsDone) return;
+  if (c->qn == 0) { c->mode = kMsDone; return; }
+  bool td = c->mode == kMsTopDown;
+  if (td && c->L > 0 && c->mf * 14ull > c->mu) td = false;
+  else if (!td && (uint64_t)c->qn * 24ull < (uint64_t)nV && c->qn < c->prev_n) td = true;
+  c->mode = td ? kMsTopDown : kMsBottomUp;
+  c->nn = 0;
+  c->T = 0;
+  c->sums[0] = 0;
+  c->sums[1] = 0;
+}
+
+__global__ void k_ms_ctl_end(MsCtl* c) {
+  if (c->mode == kMsDone) return;
+  c->prev_n = c->qn;
+  c->qn = c->nn;
+  c->mf = c->sums[0];
+  c->mu = c->mu > c->sums[1] ? c->mu - c->sums[1] : 0;
+  c->L += 1;
+}
+
+// top-down levels: out-degrees of the frontier, then their exclusive prefix
+// (T = the total) by a three-pass scan whose length is read from the control
+__global__ void k_ms_qdeg(const MsCtl* __restrict__ c, const uint32_t* __restrict__ q,
+                          const uint64_t* __restrict__ off, uint64_t* __restrict__ d) {
+  if (c->mode != kMsTopDown) return;
+  const uint32_t n = c->qn;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x)
+    d[i] = i < n ? off[q[i] + 1] - off[q[i]] : 0;
+}
+
+__global__ __launch_bounds__(kB) void k_ms_scan_sums(const MsCtl* __restrict__ c, const uint64_t* __restrict__ in,
+                                                     uint64_t* __restrict__ bsum) {
+  __shared__ uint64_t red[kB / 64];
+  if (c->mode != kMsTopDown) return;
+  const uint64_t n = (uint64_t)c->qn + 1;
+  for (uint64_t b = blockIdx.x; b * kScanB < n; b += gridDim.x) {
+    const uint64_t b0 = b * kScanB;
+    uint64_t t = 0;
+    for (int k = 0; k < kScanB / kB; ++k) {
+      const uint64_t i = b0 + (uint64_t)threadIdx.x * (kScanB / kB) + k;
+      if (i < n) t += in[i];
+    }
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[b] = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_ms_scan_top(const MsCtl* __restrict__ c, uint64_t* __restrict__ bsum) {
+  __shared__ uint64_t wt[16];
+  if (c->mode != kMsTopDown) return;
+  const uint64_t nb = ((uint64_t)c->qn + 1 + kScanB - 1) / kScanB;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t carry = 0;
+  for (uint64_t base = 0; base < nb; base += 1024) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t v = i < nb ? bsum[i] : 0;
+    uint64_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wt[w] = x;
+    __syncthreads();
+    uint64_t pre = carry;
+    for (int k = 0; k < w; ++k) pre += wt[k];
+    if (i < nb) bsum[i] = pre + x - v;
+    for (int k = 0; k < 16; ++k) carry += wt[k];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_ms_scan_apply(MsCtl* __restrict__ c, const uint64_t* __restrict__ in,
+                                                      const uint64_t* __restrict__ bsum, uint64_t* __restrict__ out) {
+  __shared__ uint64_t wt[kB / 64];
+  if (c->mode != kMsTopDown) return;
+  const uint64_t n = (uint64_t)c->qn + 1;
+  for (uint64_t b = blockIdx.x; b * kScanB < n; b += gridDim.x) {
+    const uint64_t b0 = b * kScanB;
+    constexpr int P = kScanB / kB;
+    const uint64_t i0 = b0 + (uint64_t)threadIdx.x * P;
+    uint64_t v[P];
+    uint64_t t = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      v[k] = i0 + k < n ? in[i0 + k] : 0;
+      t += v[k];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wt[w] = x;
+    __syncthreads();
+    uint64_t r = bsum[b] + x - t;
+    for (int k = 0; k < w; ++k) r += wt[k];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      if (i0 + k < n) {
+        out[i0 + k] = r;
+        if (i0 + k == n - 1) c->T = r;   // qdeg[qn] = 0: the prefix there is the total
+      }
+      r += v[k];
+    }
+    __syncthreads();   // wt is rewritten by the next chunk
+  }
 }
 
 __global__ void k_ms_seed(const uint32_t* __restrict__ src, int n, unsigned long long* vis, unsigned long long* fr,
@@ -303,49 +452,60 @@ __device__ inline void ms_record(int32_t* __restrict__ lev, uint32_t nV, uint32_
 // source bits is queued by the first edge that reaches it (nxt was zero)
 __global__ __launch_bounds__(kB) void k_ms_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                               const uint32_t* __restrict__ q, const uint64_t* __restrict__ qpos,
-                                              uint32_t qn, uint64_t T, const unsigned long long* __restrict__ fr,
+                                              MsCtl* __restrict__ c, const unsigned long long* __restrict__ fr,
                                               const unsigned long long* __restrict__ vis,
-                                              unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nq,
-                                              uint32_t* __restrict__ ncount) {
+                                              unsigned long long* __restrict__ nxt, uint32_t* __restrict__ nq) {
   __shared__ uint32_t range[2];
-  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
-  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
-  if (threadIdx.x < 2) {
-    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
-    uint32_t lo = 0, hi = qn;
-    while (hi - lo > 1) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (qpos[mid] <= key) lo = mid; else hi = mid;
-    }
-    range[threadIdx.x] = lo;
-  }
-  __syncthreads();
-  const uint32_t i0 = range[0], i1 = range[1];
-  // a fixed number of rounds for every lane keeps the wave-level append uniform
-  for (int round = 0; round < kEB / kB; ++round) {
-    const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
-    bool take = false;
-    uint32_t v = 0;
-    if (k < k1) {
-      uint32_t lo = i0, hi = i1 + 1;
+  if (c->mode != kMsTopDown) return;
+  const uint32_t qn = c->qn;
+  const uint64_t T = c->T;
+  uint32_t* ncount = &c->nn;
+  // edge blocks of kEB, grid-stride (the same trip count on every thread of a block)
+  for (uint64_t blk = blockIdx.x; blk * kEB < T; blk += gridDim.x) {
+    const uint64_t k0 = blk * kEB;
+    const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+    if (threadIdx.x < 2) {
+      const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+      uint32_t lo = 0, hi = qn;
       while (hi - lo > 1) {
         const uint32_t mid = lo + (hi - lo) / 2;
-        if (qpos[mid] <= k) lo = mid; else hi = mid;
+        if (qpos[mid] <= key) lo = mid; else hi = mid;
       }
-      const uint32_t u = q[lo];
-      v = idx[off[u] + (k - qpos[lo])];
-      const unsigned long long bits = fr[u] & ~vis[v];
-      if (bits) take = atomicOr(nxt + v, bits) == 0ULL;
+      range[threadIdx.x] = lo;
     }
-    block_append(take, v, nq, ncount);
+    __syncthreads();
+    const uint32_t i0 = range[0], i1 = range[1];
+    // a fixed number of rounds for every lane keeps the wave-level append uniform
+    for (int round = 0; round < kEB / kB; ++round) {
+      const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
+      bool take = false;
+      uint32_t v = 0;
+      if (k < k1) {
+        uint32_t lo = i0, hi = i1 + 1;
+        while (hi - lo > 1) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (qpos[mid] <= k) lo = mid; else hi = mid;
+        }
+        const uint32_t u = q[lo];
+        v = idx[off[u] + (k - qpos[lo])];
+        const unsigned long long bits = fr[u] & ~vis[v];
+        if (bits) take = atomicOr(nxt + v, bits) == 0ULL;
+      }
+      block_append(take, v, nq, ncount);
+    }
+    __syncthreads();   // range[] is rewritten by the next edge block
   }
 }
 
 // top-down, second pass over the queued vertices: publish the new bits
-__global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, unsigned long long full,
+__global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, MsCtl* __restrict__ c, unsigned long long full,
                             unsigned long long* __restrict__ nxt, unsigned long long* __restrict__ nfr,
                             unsigned long long* __restrict__ vis, const uint64_t* __restrict__ off,
-                            int32_t* __restrict__ lev, uint32_t nV, int32_t L, unsigned long long* __restrict__ sums) {
+                            int32_t* __restrict__ lev, uint32_t nV) {
+  if (c->mode != kMsTopDown) return;
+  const uint32_t nn = c->nn;
+  const int32_t L = (int32_t)c->L;
+  unsigned long long* sums = c->sums;
   uint64_t deg = 0, fulldeg = 0;
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nn; j += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = nq[j];
@@ -376,8 +536,11 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
                                               const unsigned long long* __restrict__ fr,
                                               unsigned long long* __restrict__ nfr,
                                               unsigned long long* __restrict__ vis, int32_t* __restrict__ lev,
-                                              int32_t L, uint32_t* __restrict__ nq, uint32_t* __restrict__ ncount,
-                                              unsigned long long* __restrict__ sums) {
+                                              MsCtl* __restrict__ c, uint32_t* __restrict__ nq) {
+  if (c->mode != kMsBottomUp) return;
+  const int32_t L = (int32_t)c->L;
+  uint32_t* ncount = &c->nn;
+  unsigned long long* sums = c->sums;
   uint64_t deg = 0, fulldeg = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // every lane runs the same number of rounds so the wave-level append is uniform
@@ -431,7 +594,8 @@ __global__ __launch_bounds__(kB) void k_ms_bu_chunks(const uint64_t* __restrict_
                                                      unsigned long long full,
                                                      const unsigned long long* __restrict__ fr,
                                                      const unsigned long long* __restrict__ vis,
-                                                     unsigned long long* __restrict__ acc) {
+                                                     unsigned long long* __restrict__ acc, const MsCtl* __restrict__ c) {
+  if (c->mode != kMsBottomUp) return;
   const int lane = (int)(threadIdx.x & 63);
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
@@ -471,9 +635,11 @@ __global__ __launch_bounds__(kB) void k_ms_bu_long_fin(const uint32_t* __restric
                                                        unsigned long long* __restrict__ nfr,
                                                        unsigned long long* __restrict__ vis,
                                                        const uint64_t* __restrict__ off, int32_t* __restrict__ lev,
-                                                       uint32_t nV, int32_t L, uint32_t* __restrict__ nq,
-                                                       uint32_t* __restrict__ ncount,
-                                                       unsigned long long* __restrict__ sums) {
+                                                       uint32_t nV, MsCtl* __restrict__ c, uint32_t* __restrict__ nq) {
+  if (c->mode != kMsBottomUp) return;
+  const int32_t L = (int32_t)c->L;
+  uint32_t* ncount = &c->nn;
+  unsigned long long* sums = c->sums;
   uint64_t deg = 0, fulldeg = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -580,7 +746,10 @@ LongRows& long_rows(gre_graph* g, bool csc, uint64_t thr, uint64_t chunk, hipStr
 }
 
 // clears the frontier masks of the queued vertices
-__global__ void k_ms_clear(const uint32_t* __restrict__ q, uint32_t n, unsigned long long* __restrict__ fr) {
+__global__ void k_ms_clear(const uint32_t* __restrict__ q, const MsCtl* __restrict__ c,
+                           unsigned long long* __restrict__ fr) {
+  if (c->mode == kMsDone) return;
+  const uint32_t n = c->qn;
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
     fr[q[j]] = 0ULL;
 }
@@ -1070,74 +1239,95 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     GRE_HIP(hipMemcpyAsync(q[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
     GRE_HIP(hipStreamSynchronize(s));
     const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
-    uint32_t qn = (uint32_t)uq.size(), prev_n = 0;
-    uint64_t mf = 0;                // edges out of the frontier (known after the first scan)
-    uint64_t mu = g->nE;            // edges of vertices still missing source bits
-    bool topdown = true;
-    const double alpha = 14.0, beta = 24.0;
-    int cur = 0;
-    for (int32_t L = 0; qn > 0; ++L) {
-      GRE_CHECK(L <= (int32_t)nV, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+    // level 0: the distinct sources, top-down, mu = every edge
+    MsCtl* hc = mb.ctl_host;
+    *hc = MsCtl{};
+    hc->qn = (uint32_t)uq.size();
+    hc->mode = kMsTopDown;
+    hc->mu = g->nE;
+    MsCtl* dc = mb.ctl.as<MsCtl>();
+    GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(MsCtl), hipMemcpyHostToDevice, s));
+    int32_t* lv = level_out ? lev.as<int32_t>() : nullptr;
+    const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
+    GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for the multi-source BFS");
+    const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
+    const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
+    const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
+    // one level: control, then both directions' kernels (the one not chosen
+    // returns at once), the old frontier's masks cleared, control again
+    auto level = [&](int cur) {
       const int nx = cur ^ 1;
-      if (topdown && L > 0 && (double)mf > (double)mu / alpha) topdown = false;
-      else if (!topdown && (double)qn < (double)nV / beta && qn < prev_n) topdown = true;
-      GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-      GRE_HIP(hipMemsetAsync(sums.p, 0, 16, s));
-      uint32_t nn = 0;
-      if (topdown) {
-        hipLaunchKernelGGL(k_qdeg, dim3(gcap((uint64_t)qn + 1)), dim3(kB), 0, s, q[cur].as<uint32_t>(), qn, off,
-                           qdeg.as<uint64_t>());
-        scan_excl_u64(qdeg.as<uint64_t>(), qpos.as<uint64_t>(), (size_t)qn + 1, s);
-        uint64_t T = 0;
-        GRE_HIP(hipMemcpyAsync(&T, qpos.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
-        GRE_HIP(hipStreamSynchronize(s));
-        if (T > 0) {
-          ProfScope ps("msbfs_topdown");
-          hipLaunchKernelGGL(k_ms_td, dim3((unsigned)((T + kEB - 1) / kEB)), dim3(kB), 0, s, off,
-                             g->idx.as<uint32_t>(), q[cur].as<uint32_t>(), qpos.as<uint64_t>(), qn, T,
-                             (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
-                             (unsigned long long*)nxt.p, q[nx].as<uint32_t>(), flag.as<uint32_t>());
-        }
-        GRE_HIP(hipGetLastError());
-        GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
-        GRE_HIP(hipStreamSynchronize(s));
-        if (nn > 0)
-          hipLaunchKernelGGL(k_ms_td_fin, dim3(gcap(nn)), dim3(kB), 0, s, q[nx].as<uint32_t>(), nn, full,
-                             (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
-                             off, level_out ? lev.as<int32_t>() : nullptr, nV, L, (unsigned long long*)sums.p);
-      } else {
-        ProfScope ps("msbfs_bottomup");
-        // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
-        if (lr.nchunks)
-          hipLaunchKernelGGL(k_ms_bu_chunks, dim3(std::min<uint64_t>(((uint64_t)lr.nchunks + 3) / 4, 1u << 16)),
-                             dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), lr.rows.as<uint32_t>(),
-                             lr.cv.as<uint32_t>(), lr.cb.as<uint64_t>(), lr.nchunks, full,
-                             (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
-                             (unsigned long long*)lr.acc.p);
-        hipLaunchKernelGGL(k_ms_bu, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
-                           off, nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
-                           (unsigned long long*)vis.p, level_out ? lev.as<int32_t>() : nullptr, L,
-                           q[nx].as<uint32_t>(), flag.as<uint32_t>(), (unsigned long long*)sums.p);
-        if (lr.n)
-          hipLaunchKernelGGL(k_ms_bu_long_fin, dim3(gcap(lr.n)), dim3(kB), 0, s, lr.rows.as<uint32_t>(), lr.n, full,
-                             (unsigned long long*)lr.acc.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
-                             off, level_out ? lev.as<int32_t>() : nullptr, nV, L, q[nx].as<uint32_t>(),
-                             flag.as<uint32_t>(), (unsigned long long*)sums.p);
-        GRE_HIP(hipGetLastError());
-        GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
-      }
-      // the old frontier's masks go back to zero
-      hipLaunchKernelGGL(k_ms_clear, dim3(gcap(qn)), dim3(kB), 0, s, q[cur].as<uint32_t>(), qn,
+      hipLaunchKernelGGL(k_ms_ctl_begin, dim3(1), dim3(1), 0, s, dc, nV);
+      hipLaunchKernelGGL(k_ms_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q[cur].as<uint32_t>(), off,
+                         qdeg.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_sums, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+                         mb.bsum.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_top, dim3(1), dim3(1024), 0, s, dc, mb.bsum.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_apply, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+                         mb.bsum.as<uint64_t>(), qpos.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_td, dim3(tdgrid), dim3(kB), 0, s, off, g->idx.as<uint32_t>(), q[cur].as<uint32_t>(),
+                         qpos.as<uint64_t>(), dc, (const unsigned long long*)fr[cur].p,
+                         (const unsigned long long*)vis.p, (unsigned long long*)nxt.p, q[nx].as<uint32_t>());
+      hipLaunchKernelGGL(k_ms_td_fin, dim3(vgrid), dim3(kB), 0, s, q[nx].as<uint32_t>(), dc, full,
+                         (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, off,
+                         lv, nV);
+      // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
+      if (lr.nchunks)
+        hipLaunchKernelGGL(k_ms_bu_chunks, dim3(std::min<uint64_t>(((uint64_t)lr.nchunks + 3) / 4, kMsGrid)),
+                           dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), lr.rows.as<uint32_t>(),
+                           lr.cv.as<uint32_t>(), lr.cb.as<uint64_t>(), lr.nchunks, full,
+                           (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                           (unsigned long long*)lr.acc.p, dc);
+      hipLaunchKernelGGL(k_ms_bu, dim3(vgrid), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
+                         off, nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
+                         (unsigned long long*)vis.p, lv, dc, q[nx].as<uint32_t>());
+      if (lr.n)
+        hipLaunchKernelGGL(k_ms_bu_long_fin, dim3(std::min<unsigned>(gcap(lr.n), kMsGrid)), dim3(kB), 0, s,
+                           lr.rows.as<uint32_t>(), lr.n, full,
+                           (unsigned long long*)lr.acc.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
+                           off, lv, nV, dc, q[nx].as<uint32_t>());
+      hipLaunchKernelGGL(k_ms_clear, dim3(vgrid), dim3(kB), 0, s, q[cur].as<uint32_t>(), dc,
                          (unsigned long long*)fr[cur].p);
-      GRE_HIP(hipGetLastError());
-      uint64_t h[2];
-      GRE_HIP(hipMemcpyAsync(h, sums.p, 16, hipMemcpyDeviceToHost, s));
-      GRE_HIP(hipStreamSynchronize(s));
-      prev_n = qn;
-      qn = nn;
-      mf = h[0];
-      mu = mu > h[1] ? mu - h[1] : 0;
-      cur = nx;
+      hipLaunchKernelGGL(k_ms_ctl_end, dim3(1), dim3(1), 0, s, dc);
+    };
+    // kMsBatch levels captured once per launch configuration and replayed
+    // until the control says done: one read-back per batch, none per level
+    static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+    auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+    const std::vector<uint64_t> key = {P(g),         nV,           g->nE,        full,         P(lv),
+                                       P(dc),        P(vis.p),     P(fr[0].p),   P(fr[1].p),   P(nxt.p),
+                                       P(q[0].p),    P(q[1].p),    P(qdeg.p),    P(qpos.p),    P(mb.bsum.p),
+                                       P(lr.acc.p),  P(lr.rows.p), P(lr.cv.p),   P(lr.cb.p),   lr.n,
+                                       lr.nchunks,   P(off),       P(g->idx.p),  P(g->coff.p), P(g->cidx.p)};
+    hipGraphExec_t exec = nullptr;
+    auto it = graphs.find(key);
+    if (it != graphs.end()) {
+      exec = it->second;
+    } else {
+      // calls that return levels use a new output buffer each time: keep the
+      // cache small (the stream is idle here: no replay is in flight)
+      if (graphs.size() >= 16) {
+        GRE_HIP(hipStreamSynchronize(s));
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+      }
+      hipGraph_t graph = nullptr;
+      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < kMsBatch; ++k) level(k & 1);
+      GRE_HIP(hipStreamEndCapture(s, &graph));
+      GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      graphs[key] = exec;
+    }
+    {
+      ProfScope ps("msbfs_levels");
+      for (uint64_t done_levels = 0;; done_levels += kMsBatch) {
+        GRE_CHECK(done_levels <= (uint64_t)nV + kMsBatch, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+        GRE_HIP(hipGraphLaunch(exec, s));
+        GRE_HIP(hipMemcpyAsync(&hc->mode, &dc->mode, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        if (hc->mode == kMsDone) break;
+      }
     }
     if (edges_out) {
       GRE_HIP(hipMemsetAsync(flag.p, 0, 8, s));
