@@ -285,6 +285,35 @@ constexpr int kScanB = 2048;    // elements per chunk of the device-length scan
 constexpr unsigned kMsGrid = 8192;
 constexpr unsigned kMsScanGrid = 1024;
 
+// Control of the batched Brandes BC, in device memory.  A step is one forward
+// level, one backward level, or a chunk's set-up fused with its first forward
+// level; the control says which, every kernel of a step reads it and returns
+// at once when the step is not its kind, so one launch sequence of kBcBatch
+// steps is captured and replayed over every 64-source chunk until the
+// control reports done.
+enum : uint32_t { kBcInit = 0, kBcFwd = 1, kBcBwd = 2, kBcDone = 3, kBcTooDeep = 4 };
+constexpr uint32_t kBcMaxDepth = 250;  // levels are kept as uint8 (0xff: not reached)
+constexpr int kBcBatch = 16;           // steps per replayed graph
+struct BcCtl {
+  uint32_t phase;
+  uint32_t mode;       // forward direction: kMsTopDown or kMsBottomUp
+  uint32_t chunk;      // chunk being run: sources [64 chunk, 64 chunk + nc)
+  uint32_t nchunks;
+  uint32_t nsrc;
+  uint32_t nc;
+  uint32_t L;          // forward: level being expanded; backward: level being folded
+  uint32_t depth;      // deepest level of the chunk's forward pass
+  uint32_t qn;         // frontier size (the frontier is q[L & 1], its lane masks fr[L & 1])
+  uint32_t nn;         // next frontier size (appended to by the level's kernels)
+  uint64_t full;       // lane mask of the chunk's sources
+  uint64_t T;          // top-down: out-edges of the frontier (prefix total)
+  uint64_t mf;         // out-edges of the frontier
+  unsigned long long sums[2];   // sums[0]: out-edges of the next frontier
+};
+__device__ inline bool td_on(const MsCtl* c) { return c->mode == kMsTopDown; }
+__device__ inline bool td_on(const BcCtl* c) { return c->phase == kBcFwd && c->mode == kMsTopDown; }
+__device__ inline bool bu_on(const BcCtl* c) { return c->phase == kBcFwd && c->mode == kMsBottomUp; }
+
 struct MsBufs {
   DevBuf vis, fr[2], nxt, q[2], qdeg, qpos, flag, sums, src, ctl, bsum;
   uint32_t nV = 0;
@@ -341,16 +370,17 @@ __global__ void k_ms_ctl_end(MsCtl* c) {
 // (T = the total) by a three-pass scan whose length is read from the control
 __global__ void k_ms_qdeg(const MsCtl* __restrict__ c, const uint32_t* __restrict__ q,
                           const uint64_t* __restrict__ off, uint64_t* __restrict__ d) {
-  if (c->mode != kMsTopDown) return;
+  if (!td_on(c)) return;
   const uint32_t n = c->qn;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x)
     d[i] = i < n ? off[q[i] + 1] - off[q[i]] : 0;
 }
 
-__global__ __launch_bounds__(kB) void k_ms_scan_sums(const MsCtl* __restrict__ c, const uint64_t* __restrict__ in,
+template <class Ctl>
+__global__ __launch_bounds__(kB) void k_ms_scan_sums(const Ctl* __restrict__ c, const uint64_t* __restrict__ in,
                                                      uint64_t* __restrict__ bsum) {
   __shared__ uint64_t red[kB / 64];
-  if (c->mode != kMsTopDown) return;
+  if (!td_on(c)) return;
   const uint64_t n = (uint64_t)c->qn + 1;
   for (uint64_t b = blockIdx.x; b * kScanB < n; b += gridDim.x) {
     const uint64_t b0 = b * kScanB;
@@ -367,9 +397,10 @@ __global__ __launch_bounds__(kB) void k_ms_scan_sums(const MsCtl* __restrict__ c
   }
 }
 
-__global__ __launch_bounds__(1024) void k_ms_scan_top(const MsCtl* __restrict__ c, uint64_t* __restrict__ bsum) {
+template <class Ctl>
+__global__ __launch_bounds__(1024) void k_ms_scan_top(const Ctl* __restrict__ c, uint64_t* __restrict__ bsum) {
   __shared__ uint64_t wt[16];
-  if (c->mode != kMsTopDown) return;
+  if (!td_on(c)) return;
   const uint64_t nb = ((uint64_t)c->qn + 1 + kScanB - 1) / kScanB;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t carry = 0;
@@ -391,10 +422,11 @@ __global__ __launch_bounds__(1024) void k_ms_scan_top(const MsCtl* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(kB) void k_ms_scan_apply(MsCtl* __restrict__ c, const uint64_t* __restrict__ in,
+template <class Ctl>
+__global__ __launch_bounds__(kB) void k_ms_scan_apply(Ctl* __restrict__ c, const uint64_t* __restrict__ in,
                                                       const uint64_t* __restrict__ bsum, uint64_t* __restrict__ out) {
   __shared__ uint64_t wt[kB / 64];
-  if (c->mode != kMsTopDown) return;
+  if (!td_on(c)) return;
   const uint64_t n = (uint64_t)c->qn + 1;
   for (uint64_t b = blockIdx.x; b * kScanB < n; b += gridDim.x) {
     const uint64_t b0 = b * kScanB;
@@ -835,69 +867,80 @@ __device__ inline void bc_bwd_run(const uint32_t* __restrict__ idx, uint64_t b, 
   }
 }
 
+// The frontier of forward level L is q[L & 1] with lane masks fr[L & 1]; the
+// level builds q / fr of the other parity.  Backward level l reads the lane
+// masks of levels l and l + 1 from msk[l & 1] and msk[(l + 1) & 1].
+template <typename T>
+__device__ inline T* cur_of(const BcCtl* c, T* a0, T* a1) { return (c->L & 1) ? a1 : a0; }
+template <typename T>
+__device__ inline T* nxt_of(const BcCtl* c, T* a0, T* a1) { return (c->L & 1) ? a0 : a1; }
+
 // forward level over the long rows' chunks: per chunk, each lane's partial
 // sum and the lanes found
 __global__ __launch_bounds__(kB) void k_bc_fwd_chunks(const uint64_t* __restrict__ coff,
                                                       const uint32_t* __restrict__ cidx,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ cv,
-                                                      const uint64_t* __restrict__ cb, uint32_t nchunks, int nsrc,
-                                                      const unsigned long long* __restrict__ fr,
+                                                      const uint64_t* __restrict__ cb, uint32_t nchunks,
+                                                      const BcCtl* __restrict__ c, const unsigned long long* fr0,
+                                                      const unsigned long long* fr1,
                                                       const unsigned long long* __restrict__ vis,
                                                       const double* __restrict__ sigma, double* __restrict__ psum,
                                                       unsigned long long* __restrict__ pgot) {
+  if (!bu_on(c)) return;
+  const unsigned long long* fr = cur_of(c, fr0, fr1);
   const int lane = threadIdx.x & 63;
-  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const unsigned long long full = c->full;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
-    const uint32_t v = rows[cv[c]];
+  for (uint64_t k = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; k < nchunks; k += nw) {
+    const uint32_t v = rows[cv[k]];
     const unsigned long long seen = vis[v];
     double acc = 0.0;
     unsigned long long gotl = 0;
     if (seen != full) {  // the same chunk, so the same answer, on every lane
-      const uint64_t b = cb[c], rowend = coff[v + 1];
+      const uint64_t b = cb[k], rowend = coff[v + 1];
       bc_fwd_run(cidx, b, b + kBcChunk < rowend ? b + kBcChunk : rowend, seen, fr, sigma, lane, acc, gotl);
     }
-    psum[c * 64 + lane] = acc;
+    psum[k * 64 + lane] = acc;
     const unsigned long long got = wave_or(gotl);
-    if (lane == 0) pgot[c] = got;
+    if (lane == 0) pgot[k] = got;
   }
 }
 
 // the long rows' forward results from their chunks (as k_bc_fwd does for the rest)
 __global__ __launch_bounds__(kB) void k_bc_fwd_long_fin(const uint32_t* __restrict__ rows,
-                                                        const uint32_t* __restrict__ cstart, uint32_t n, int nsrc,
+                                                        const uint32_t* __restrict__ cstart, uint32_t n,
+                                                        const BcCtl* __restrict__ c,
                                                         const double* __restrict__ psum,
                                                         const unsigned long long* __restrict__ pgot,
-                                                        unsigned long long* __restrict__ nfr,
+                                                        unsigned long long* fr0, unsigned long long* fr1,
                                                         unsigned long long* __restrict__ vis,
-                                                        double* __restrict__ sigma, uint8_t* __restrict__ lev,
-                                                        int32_t L, unsigned int* __restrict__ any) {
+                                                        double* __restrict__ sigma, uint8_t* __restrict__ lev) {
+  if (!bu_on(c)) return;
+  unsigned long long* nfr = nxt_of(c, fr0, fr1);
   const int lane = threadIdx.x & 63;
-  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const unsigned long long full = c->full;
+  const uint8_t lnext = (uint8_t)(c->L + 1);
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  bool changed = false;
   for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
     const uint32_t v = rows[i];
     const unsigned long long seen = vis[v];
     if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
     double acc = 0.0;
     unsigned long long got = 0;
-    for (uint32_t c = cstart[i]; c < cstart[i + 1]; ++c) {
-      acc += psum[(uint64_t)c * 64 + lane];
-      got |= pgot[c];
+    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) {
+      acc += psum[(uint64_t)k * 64 + lane];
+      got |= pgot[k];
     }
     if (lane == 0) nfr[v] = got;
     if (got) {
       if ((got >> lane) & 1ULL) {
         sigma[(uint64_t)v * 64 + lane] = acc;
-        lev[(uint64_t)v * 64 + lane] = (uint8_t)(L + 1);
+        lev[(uint64_t)v * 64 + lane] = lnext;
       }
       if (lane == 0) vis[v] = seen | got;
-      changed = true;
     }
   }
-  if (__any(changed) && lane == 0) atomicOr(any, 1u);
 }
 
 // backward level over the long rows' chunks: each lane's partial sum
@@ -906,31 +949,36 @@ __global__ __launch_bounds__(kB) void k_bc_bwd_chunks(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ cv,
                                                       const uint64_t* __restrict__ cb, uint32_t nchunks,
-                                                      const double* __restrict__ q,
-                                                      const unsigned long long* __restrict__ atl,
-                                                      const unsigned long long* __restrict__ nxt,
+                                                      const BcCtl* __restrict__ c, const double* __restrict__ q,
+                                                      const unsigned long long* m0, const unsigned long long* m1,
                                                       double* __restrict__ psum) {
+  if (c->phase != kBcBwd) return;
+  const unsigned long long* atl = cur_of(c, m0, m1);
+  const unsigned long long* nxt = nxt_of(c, m0, m1);
   const int lane = threadIdx.x & 63;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t c = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
-    const uint32_t v = rows[cv[c]];
+  for (uint64_t k = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; k < nchunks; k += nw) {
+    const uint32_t v = rows[cv[k]];
     const unsigned long long a = atl[v];
     double acc = 0.0;
     if (a) {
-      const uint64_t b = cb[c], rowend = off[v + 1];
+      const uint64_t b = cb[k], rowend = off[v + 1];
       bc_bwd_run(idx, b, b + kBcChunk < rowend ? b + kBcChunk : rowend, a, nxt, q, lane, acc);
     }
-    psum[c * 64 + lane] = acc;
+    psum[k * 64 + lane] = acc;
   }
 }
 
 // the long rows' delta and q from their chunks' partial sums, in chunk order
 __global__ __launch_bounds__(kB) void k_bc_bwd_long_fin(const uint32_t* __restrict__ rows,
                                                         const uint32_t* __restrict__ cstart, uint32_t n,
+                                                        const BcCtl* __restrict__ c,
                                                         const double* __restrict__ psum,
                                                         const double* __restrict__ sigma,
                                                         double* __restrict__ delta, double* __restrict__ q,
-                                                        const unsigned long long* __restrict__ atl) {
+                                                        const unsigned long long* m0, const unsigned long long* m1) {
+  if (c->phase != kBcBwd) return;
+  const unsigned long long* atl = cur_of(c, m0, m1);
   const int lane = threadIdx.x & 63;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
@@ -938,7 +986,7 @@ __global__ __launch_bounds__(kB) void k_bc_bwd_long_fin(const uint32_t* __restri
     const unsigned long long a = atl[v];
     if (!((a >> lane) & 1ULL)) continue;
     double acc = 0.0;
-    for (uint32_t c = cstart[i]; c < cstart[i + 1]; ++c) acc += psum[(uint64_t)c * 64 + lane];
+    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) acc += psum[(uint64_t)k * 64 + lane];
     const uint64_t o = (uint64_t)v * 64 + lane;
     const double sv = sigma[o];
     const double d = sv * acc;
@@ -952,15 +1000,18 @@ __global__ __launch_bounds__(kB) void k_bc_bwd_long_fin(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // forward level L -> L+1: sigma_v[j] = sum over in-neighbours w at level L of sigma_w[j]
 __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
-                                               uint32_t nV, int nsrc, const unsigned long long* __restrict__ fr,
-                                               unsigned long long* __restrict__ nfr,
+                                               uint32_t nV, const BcCtl* __restrict__ c,
+                                               unsigned long long* fr0, unsigned long long* fr1,
                                                unsigned long long* __restrict__ vis, double* __restrict__ sigma,
-                                               uint8_t* __restrict__ lev, int32_t L, unsigned int* __restrict__ any) {
+                                               uint8_t* __restrict__ lev) {
+  if (!bu_on(c)) return;
+  const unsigned long long* fr = cur_of(c, fr0, fr1);
+  unsigned long long* nfr = nxt_of(c, fr0, fr1);
   const int lane = threadIdx.x & 63;
-  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  const unsigned long long full = c->full;
+  const uint8_t lnext = (uint8_t)(c->L + 1);
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  bool changed = false;
   for (uint64_t v = wid; v < nV; v += nw) {
     const uint64_t b = coff[v], e = coff[v + 1];
     if (e - b > kBcLong) continue;  // k_bc_fwd_chunks + k_bc_fwd_long_fin
@@ -969,30 +1020,30 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
     double acc = 0.0;
     unsigned long long gotl = 0;
     bc_fwd_run(cidx, b, e, seen, fr, sigma, lane, acc, gotl);
-    // got = OR over the lanes of gotl
-    unsigned long long got = gotl;
-    for (int o = 32; o > 0; o >>= 1) got |= __shfl_xor(got, o, 64);
+    const unsigned long long got = wave_or(gotl);
     if (lane == 0) nfr[v] = got;
     if (got) {
       if ((got >> lane) & 1ULL) {
         sigma[(uint64_t)v * 64 + lane] = acc;
-        lev[(uint64_t)v * 64 + lane] = (uint8_t)(L + 1);
+        lev[(uint64_t)v * 64 + lane] = lnext;
       }
       if (lane == 0) vis[v] = seen | got;
-      changed = true;
     }
   }
-  if (__any(changed) && lane == 0) atomicOr(any, 1u);
 }
 
-// backward level L: delta_v[j] = sigma_v[j] * sum over out-neighbours w at L+1 of q_w[j], where
+// backward level l: delta_v[j] = sigma_v[j] * sum over out-neighbours w at l+1 of q_w[j], where
 // q_w[j] = (1+delta_w[j])/sigma_w[j] was stored when w's level was done; v's own q follows.
-// atl[v] / nxt[v] = lanes of v at level L / L+1
+// atl[v] / nxt[v] = lanes of v at level l / l+1 (at the deepest level nxt is all zero:
+// delta = 0 and q = 1/sigma)
 __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
-                                               uint32_t nV, const double* __restrict__ sigma,
-                                               double* __restrict__ delta, double* __restrict__ q,
-                                               const unsigned long long* __restrict__ atl,
-                                               const unsigned long long* __restrict__ nxt) {
+                                               uint32_t nV, const BcCtl* __restrict__ c,
+                                               const double* __restrict__ sigma, double* __restrict__ delta,
+                                               double* __restrict__ q, const unsigned long long* m0,
+                                               const unsigned long long* m1) {
+  if (c->phase != kBcBwd) return;
+  const unsigned long long* atl = cur_of(c, m0, m1);
+  const unsigned long long* nxt = nxt_of(c, m0, m1);
   const int lane = threadIdx.x & 63;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -1014,76 +1065,80 @@ __global__ __launch_bounds__(kB) void k_bc_bwd(const uint64_t* __restrict__ off,
   }
 }
 
-// q of the deepest level's vertices (their delta is 0): q_v[j] = 1/sigma_v[j]
-__global__ void k_bc_q_last(uint32_t nV, const double* __restrict__ sigma, const double* __restrict__ delta,
-                            double* __restrict__ q, const unsigned long long* __restrict__ atl) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t v = wid; v < nV; v += nw) {
-    if (!((atl[v] >> lane) & 1ULL)) continue;
-    const uint64_t o = v * 64 + lane;
-    q[o] = (1.0 + delta[o]) / sigma[o];
-  }
-}
-
 // forward, top-down level L -> L+1 over the frontier's out-edges (edge-balanced
 // like k_ms_td): v gains the lanes nw = fr[u] & ~vis[v], and sigma_v[j] +=
 // sigma_u[j] for each of them -- path counts are integer-valued doubles below
 // 2^53, so the atomic sums are exact in any order; the first edge that reaches
 // v queues it.  vis stays the pre-level mask until k_bc_td_fin.
 __global__ __launch_bounds__(kB) void k_bc_td(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
-                                              const uint32_t* __restrict__ q, const uint64_t* __restrict__ qpos,
-                                              uint32_t qn, uint64_t T, const unsigned long long* __restrict__ fr,
-                                              const unsigned long long* __restrict__ vis, double* __restrict__ sigma,
-                                              unsigned long long* __restrict__ nfr, uint32_t* __restrict__ nq,
-                                              uint32_t* __restrict__ ncount) {
+                                              uint32_t* q0, uint32_t* q1, const uint64_t* __restrict__ qpos,
+                                              BcCtl* __restrict__ c, unsigned long long* fr0,
+                                              unsigned long long* fr1, const unsigned long long* __restrict__ vis,
+                                              double* __restrict__ sigma) {
   __shared__ uint32_t range[2];
-  const uint64_t k0 = (uint64_t)blockIdx.x * kEB;
-  const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
-  if (threadIdx.x < 2) {
-    const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
-    uint32_t lo = 0, hi = qn;
-    while (hi - lo > 1) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (qpos[mid] <= key) lo = mid; else hi = mid;
-    }
-    range[threadIdx.x] = lo;
-  }
-  __syncthreads();
-  const uint32_t i0 = range[0], i1 = range[1];
-  for (int round = 0; round < kEB / kB; ++round) {
-    const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
-    bool take = false;
-    uint32_t v = 0;
-    if (k < k1) {
-      uint32_t lo = i0, hi = i1 + 1;
+  if (!td_on(c)) return;
+  const uint32_t* q = cur_of(c, q0, q1);
+  uint32_t* nq = nxt_of(c, q0, q1);
+  const unsigned long long* fr = cur_of(c, fr0, fr1);
+  unsigned long long* nfr = nxt_of(c, fr0, fr1);
+  const uint32_t qn = c->qn;
+  const uint64_t T = c->T;
+  uint32_t* ncount = &c->nn;
+  // edge blocks of kEB, grid-stride (the same trip count on every thread of a block)
+  for (uint64_t blk = blockIdx.x; blk * kEB < T; blk += gridDim.x) {
+    const uint64_t k0 = blk * kEB;
+    const uint64_t k1 = k0 + kEB < T ? k0 + kEB : T;
+    if (threadIdx.x < 2) {
+      const uint64_t key = threadIdx.x == 0 ? k0 : k1 - 1;
+      uint32_t lo = 0, hi = qn;
       while (hi - lo > 1) {
         const uint32_t mid = lo + (hi - lo) / 2;
-        if (qpos[mid] <= k) lo = mid; else hi = mid;
+        if (qpos[mid] <= key) lo = mid; else hi = mid;
       }
-      const uint32_t u = q[lo];
-      v = idx[off[u] + (k - qpos[lo])];
-      const unsigned long long nw = fr[u] & ~vis[v];
-      if (nw) {
-        take = atomicOr(nfr + v, nw) == 0ULL;
-        unsigned long long m = nw;
-        while (m) {
-          const int j = __ffsll(m) - 1;
-          atomicAdd(sigma + (uint64_t)v * 64 + j, sigma[(uint64_t)u * 64 + j]);
-          m &= m - 1ULL;
+      range[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const uint32_t i0 = range[0], i1 = range[1];
+    for (int round = 0; round < kEB / kB; ++round) {
+      const uint64_t k = k0 + (uint64_t)round * kB + threadIdx.x;
+      bool take = false;
+      uint32_t v = 0;
+      if (k < k1) {
+        uint32_t lo = i0, hi = i1 + 1;
+        while (hi - lo > 1) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (qpos[mid] <= k) lo = mid; else hi = mid;
+        }
+        const uint32_t u = q[lo];
+        v = idx[off[u] + (k - qpos[lo])];
+        const unsigned long long nw = fr[u] & ~vis[v];
+        if (nw) {
+          take = atomicOr(nfr + v, nw) == 0ULL;
+          unsigned long long m = nw;
+          while (m) {
+            const int j = __ffsll(m) - 1;
+            atomicAdd(sigma + (uint64_t)v * 64 + j, sigma[(uint64_t)u * 64 + j]);
+            m &= m - 1ULL;
+          }
         }
       }
+      block_append(take, v, nq, ncount);
     }
-    block_append(take, v, nq, ncount);
+    __syncthreads();   // range[] is rewritten by the next edge block
   }
 }
 
 // after a top-down level: the queued vertices take their new lanes at level L+1;
 // sums[0] += their out-degrees
-__global__ void k_bc_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, const unsigned long long* __restrict__ nfr,
-                            unsigned long long* __restrict__ vis, uint8_t* __restrict__ lev, int32_t L,
-                            const uint64_t* __restrict__ off, unsigned long long* __restrict__ sums) {
+__global__ void k_bc_td_fin(BcCtl* __restrict__ c, const uint32_t* q0, const uint32_t* q1,
+                            const unsigned long long* fr0, const unsigned long long* fr1,
+                            unsigned long long* __restrict__ vis, uint8_t* __restrict__ lev,
+                            const uint64_t* __restrict__ off) {
+  if (!td_on(c)) return;
+  const uint32_t* nq = nxt_of(c, q0, q1);
+  const unsigned long long* nfr = nxt_of(c, fr0, fr1);
+  const uint32_t nn = c->nn;
+  const uint8_t lnext = (uint8_t)(c->L + 1);
   uint64_t deg = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = nq[i];
@@ -1091,87 +1146,231 @@ __global__ void k_bc_td_fin(const uint32_t* __restrict__ nq, uint32_t nn, const 
     vis[v] |= m;
     while (m) {
       const int j = __ffsll(m) - 1;
-      lev[(uint64_t)v * 64 + j] = (uint8_t)(L + 1);
+      lev[(uint64_t)v * 64 + j] = lnext;
       m &= m - 1ULL;
     }
     deg += off[v + 1] - off[v];
   }
-  block_add(static_cast<unsigned long long>(deg), sums);
+  block_add(static_cast<unsigned long long>(deg), c->sums);
 }
 
 // after a bottom-up level: the new frontier (nonzero masks) as a queue, with
 // sums[0] += its out-degrees
-__global__ __launch_bounds__(kB) void k_bc_queue(uint32_t nV, const unsigned long long* __restrict__ nfr,
-                                                 const uint64_t* __restrict__ off, uint32_t* __restrict__ nq,
-                                                 uint32_t* __restrict__ ncount, unsigned long long* __restrict__ sums) {
+__global__ __launch_bounds__(kB) void k_bc_queue(uint32_t nV, BcCtl* __restrict__ c, const unsigned long long* fr0,
+                                                 const unsigned long long* fr1, const uint64_t* __restrict__ off,
+                                                 uint32_t* q0, uint32_t* q1) {
+  if (!bu_on(c)) return;
+  const unsigned long long* nfr = nxt_of(c, fr0, fr1);
+  uint32_t* nq = nxt_of(c, q0, q1);
   uint64_t deg = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nV; base += stride) {
     const uint64_t v = base + threadIdx.x;
     const bool take = v < nV && nfr[v] != 0ULL;
     if (take) deg += off[v + 1] - off[v];
-    block_append(take, (uint32_t)v, nq, ncount);
+    block_append(take, (uint32_t)v, nq, &c->nn);
   }
-  block_add(static_cast<unsigned long long>(deg), sums);
+  block_add(static_cast<unsigned long long>(deg), c->sums);
 }
 
-// mask of lanes whose level is L
-__global__ void k_bc_mask(uint32_t nV, const uint8_t* __restrict__ lev, int32_t L, unsigned long long* __restrict__ m) {
+// top-down levels: out-degrees of the frontier (then their prefix by the
+// device-length scan)
+__global__ void k_bc_qdeg(const BcCtl* __restrict__ c, const uint32_t* q0, const uint32_t* q1,
+                          const uint64_t* __restrict__ off, uint64_t* __restrict__ d) {
+  if (!td_on(c)) return;
+  const uint32_t* q = cur_of(c, q0, q1);
+  const uint32_t n = c->qn;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x)
+    d[i] = i < n ? off[q[i] + 1] - off[q[i]] : 0;
+}
+
+// end of a forward level: the frontier's masks are cleared (fr of either
+// parity is zero outside its own queue, which a top-down level relies on)
+__global__ void k_bc_clear(const BcCtl* __restrict__ c, const uint32_t* q0, const uint32_t* q1,
+                           unsigned long long* fr0, unsigned long long* fr1) {
+  if (c->phase != kBcFwd) return;
+  const uint32_t* q = cur_of(c, q0, q1);
+  unsigned long long* fr = cur_of(c, fr0, fr1);
+  const uint32_t n = c->qn;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+    fr[q[j]] = 0ULL;
+}
+
+// backward level l: mask of the lanes whose level is l into msk[l & 1]; at the
+// deepest level msk[(l + 1) & 1] (level l + 1, empty) is cleared too
+__global__ void k_bc_mask(const BcCtl* __restrict__ c, uint32_t nV, const uint8_t* __restrict__ lev,
+                          unsigned long long* m0, unsigned long long* m1) {
+  if (c->phase != kBcBwd) return;
+  unsigned long long* m = cur_of(c, m0, m1);
+  unsigned long long* other = nxt_of(c, m0, m1);
+  const uint8_t l = (uint8_t)c->L;
+  const bool deepest = c->L == c->depth;
   const int lane = threadIdx.x & 63;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t v = wid; v < nV; v += nw) {
-    const unsigned long long b = __ballot(lev[(uint64_t)v * 64 + lane] == (uint8_t)L);
-    if (lane == 0) m[v] = b;
+    const unsigned long long b = __ballot(lev[v * 64 + lane] == l);
+    if (lane == 0) {
+      m[v] = b;
+      if (deepest) other[v] = 0ULL;
+    }
   }
 }
 
-// bc[v] += sum over lanes of delta (source itself excluded); optional copies out
-__global__ void k_bc_accum(uint32_t nV, int nsrc, const uint32_t* __restrict__ src, const double* __restrict__ delta,
-                           double* __restrict__ bc) {
+// after backward level 0: bc[v] += sum over the chunk's lanes of delta (the
+// lane's own source excluded)
+__global__ void k_bc_accum(const BcCtl* __restrict__ c, uint32_t nV, const uint32_t* __restrict__ src,
+                           const double* __restrict__ delta, double* __restrict__ bc) {
+  if (c->phase != kBcBwd || c->L != 0) return;
+  const int nc = (int)c->nc;
+  const uint32_t* cs = src + (uint64_t)c->chunk * 64;
   const int lane = threadIdx.x & 63;
   const uint64_t wid = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t v = wid; v < nV; v += nw) {
-    double d = (lane < nsrc && src[lane] != v) ? delta[(uint64_t)v * 64 + lane] : 0.0;
+    double d = (lane < nc && cs[lane] != v) ? delta[v * 64 + lane] : 0.0;
     d = wave_sum(d);
     if (lane == 0) bc[v] += d;
   }
 }
 
-__global__ void k_bc_init(uint32_t nV, const uint32_t* __restrict__ src, int nsrc, double* __restrict__ sigma,
+// set-up of a chunk: its per-source arrays cleared
+__global__ void k_bc_init(const BcCtl* __restrict__ c, uint32_t nV, double* __restrict__ sigma,
                           double* __restrict__ delta, uint8_t* __restrict__ lev, unsigned long long* __restrict__ vis,
-                          unsigned long long* __restrict__ fr) {
+                          unsigned long long* __restrict__ fr0, unsigned long long* __restrict__ fr1) {
+  if (c->phase != kBcInit) return;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * 64;
        i += (uint64_t)gridDim.x * blockDim.x) {
     sigma[i] = 0.0;
     delta[i] = 0.0;
     lev[i] = 0xff;
-    if ((i & 63) == 0) { vis[i >> 6] = 0; fr[i >> 6] = 0; }
+    if ((i & 63) == 0) { vis[i >> 6] = 0; fr0[i >> 6] = 0; fr1[i >> 6] = 0; }
   }
 }
 
-__global__ void k_bc_seed(const uint32_t* __restrict__ src, int nsrc, double* sigma, uint8_t* lev,
-                          unsigned long long* vis, unsigned long long* fr) {
-  for (int j = 0; j < nsrc; ++j) {
-    const uint64_t o = (uint64_t)src[j] * 64 + j;
-    sigma[o] = 1.0;
-    lev[o] = 0;
-    vis[src[j]] |= 1ULL << j;
-    fr[src[j]] |= 1ULL << j;
+// first kernel of a step: a chunk's seeds and first frontier (its distinct
+// sources, uq / uqn, prepared by the host), then the forward level's direction
+__global__ void k_bc_begin(BcCtl* __restrict__ c, uint64_t nE, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ uq,
+                           const uint32_t* __restrict__ uqn, double* __restrict__ sigma, uint8_t* __restrict__ lev,
+                           unsigned long long* __restrict__ vis, unsigned long long* __restrict__ fr0,
+                           uint32_t* __restrict__ q0) {
+  if (c->phase == kBcInit) {
+    const uint32_t c0 = c->chunk * 64;
+    const uint32_t nc = c->nsrc - c0 < 64 ? c->nsrc - c0 : 64;
+    for (uint32_t j = 0; j < nc; ++j) {
+      const uint64_t s = src[c0 + j];
+      sigma[s * 64 + j] = 1.0;
+      lev[s * 64 + j] = 0;
+      vis[s] |= 1ULL << j;
+      fr0[s] |= 1ULL << j;
+    }
+    const uint32_t nu = uqn[c->chunk];
+    uint64_t mf = 0;
+    for (uint32_t i = 0; i < nu; ++i) {
+      const uint32_t v = uq[c0 + i];
+      q0[i] = v;
+      mf += off[v + 1] - off[v];
+    }
+    c->nc = nc;
+    c->full = nc == 64 ? ~0ULL : ((1ULL << nc) - 1);
+    c->qn = nu;
+    c->mf = mf;
+    c->L = 0;
+    c->phase = kBcFwd;
+  }
+  if (c->phase != kBcFwd) return;
+  if (c->L >= kBcMaxDepth) { c->phase = kBcTooDeep; return; }
+  // top-down over the frontier's out-edges while they are few (under E / 14),
+  // else the bottom-up pull over every vertex
+  c->mode = c->mf * 14 < nE ? kMsTopDown : kMsBottomUp;
+  c->nn = 0;
+  c->T = 0;
+  c->sums[0] = 0;
+  c->sums[1] = 0;
+}
+
+// last kernel of a step: the next forward level, the turn to the backward
+// pass once a level finds nothing, the next backward level, the next chunk
+__global__ void k_bc_end(BcCtl* c) {
+  if (c->phase == kBcFwd) {
+    if (c->nn == 0) {
+      c->phase = kBcBwd;
+      c->depth = c->L;
+    } else {
+      c->qn = c->nn;
+      c->mf = c->sums[0];
+      c->L += 1;
+    }
+  } else if (c->phase == kBcBwd) {
+    if (c->L == 0) {
+      c->chunk += 1;
+      c->phase = c->chunk < c->nchunks ? kBcInit : kBcDone;
+    } else {
+      c->L -= 1;
+    }
   }
 }
 
-// lane-major (v*64 + j) -> source-major (j*nV + v)
+// after backward level 0: the chunk's lanes, lane-major (v*64 + j), into
+// rows 64 chunk + j of the source-major output (j*nV + v)
 template <typename T, typename U>
-__global__ void k_bc_export(uint32_t nV, int nsrc, const T* __restrict__ in, U* __restrict__ out, bool lev8) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * nsrc;
+__global__ void k_bc_export(const BcCtl* __restrict__ c, uint32_t nV, const T* __restrict__ in, U* __restrict__ out,
+                            bool lev8) {
+  if (c->phase != kBcBwd || c->L != 0) return;
+  const uint64_t nc = c->nc;
+  U* o = out + (uint64_t)c->chunk * 64 * nV;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (uint64_t)nV * nc;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t j = i / nV, v = i % nV;
     const T x = in[v * 64 + j];
-    if (lev8) out[i] = (U)(x == (T)0xff ? -1 : (int)x);
-    else out[i] = (U)x;
+    if (lev8) o[i] = (U)(x == (T)0xff ? -1 : (int)x);
+    else o[i] = (U)x;
   }
+}
+
+// Working arrays of the batched BC, grown to the largest graph, source count
+// and long-row chunk count seen and kept for the process (like the BFS ones):
+// a call allocates only the per-source outputs it is asked for.
+struct BcBufs {
+  DevBuf sigma, delta, qv, lev, vis, fr[2], msk[2], q[2], qdeg, qpos, bsum, bc, ctl, psum, pgot, src, uq, uqn;
+  uint32_t nV = 0, nchunks = 0, pchunks = 0;
+  BcCtl* ctl_host = nullptr;   // pinned
+};
+BcBufs& bc_bufs(uint32_t nV, uint32_t nchunks, uint32_t pchunks) {
+  static BcBufs& b = *new BcBufs();  // lives until the process ends
+  if (b.nV < nV || !b.sigma.p) {
+    const size_t n64 = (size_t)nV * 64;
+    b.sigma.alloc(8 * n64);
+    b.delta.alloc(8 * n64);
+    b.qv.alloc(8 * n64);
+    b.lev.alloc(n64);
+    b.vis.alloc(8 * (size_t)nV);
+    for (int k = 0; k < 2; ++k) {
+      b.fr[k].alloc(8 * (size_t)nV);
+      b.msk[k].alloc(8 * (size_t)nV);
+      b.q[k].alloc(4 * (size_t)nV + 4 * 64);
+    }
+    b.qdeg.alloc(8 * ((size_t)nV + 1));
+    b.qpos.alloc(8 * ((size_t)nV + 1));
+    b.bsum.alloc(8 * (((size_t)nV + 1 + kScanB - 1) / kScanB + 1));
+    b.bc.alloc(8 * (size_t)nV);
+    b.nV = nV;
+  }
+  if (b.nchunks < nchunks) {
+    b.src.alloc(4 * 64 * (size_t)nchunks);
+    b.uq.alloc(4 * 64 * (size_t)nchunks);
+    b.uqn.alloc(4 * (size_t)nchunks);
+    b.nchunks = nchunks;
+  }
+  if (b.pchunks < pchunks) {
+    b.psum.alloc(8 * 64 * (size_t)pchunks);
+    b.pgot.alloc(8 * (size_t)pchunks);
+    b.pchunks = pchunks;
+  }
+  if (!b.ctl.p) b.ctl.alloc(sizeof(BcCtl));
+  if (!b.ctl_host) GRE_HIP(hipHostMalloc((void**)&b.ctl_host, sizeof(BcCtl), hipHostMallocDefault));
+  return b;
 }
 
 }  // namespace
@@ -1260,10 +1459,10 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
       hipLaunchKernelGGL(k_ms_ctl_begin, dim3(1), dim3(1), 0, s, dc, nV);
       hipLaunchKernelGGL(k_ms_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q[cur].as<uint32_t>(), off,
                          qdeg.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_sums, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+      hipLaunchKernelGGL(k_ms_scan_sums<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
                          mb.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_top, dim3(1), dim3(1024), 0, s, dc, mb.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_apply, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+      hipLaunchKernelGGL(k_ms_scan_top<MsCtl>, dim3(1), dim3(1024), 0, s, dc, mb.bsum.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_apply<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
                          mb.bsum.as<uint64_t>(), qpos.as<uint64_t>());
       hipLaunchKernelGGL(k_ms_td, dim3(tdgrid), dim3(kB), 0, s, off, g->idx.as<uint32_t>(), q[cur].as<uint32_t>(),
                          qpos.as<uint64_t>(), dc, (const unsigned long long*)fr[cur].p,
@@ -1348,169 +1547,165 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
-    const uint64_t n64 = (uint64_t)nV * 64;
-    DevBuf sigma, delta, qv, lev, vis, fr[2], msk[2], bc, dsrc, flag, bq[2], bqdeg, bqpos, bsum;
-    bq[0].alloc(4 * (size_t)nV + 4 * 64);
-    bq[1].alloc(4 * (size_t)nV + 4 * 64);
-    bqdeg.alloc(8 * ((size_t)nV + 1));
-    bqpos.alloc(8 * ((size_t)nV + 1));
-    bsum.alloc(8);
-    sigma.alloc(8 * n64);
-    delta.alloc(8 * n64);
-    qv.alloc(8 * n64);
-    lev.alloc(n64);
-    vis.alloc(8 * (size_t)nV);
-    fr[0].alloc(8 * (size_t)nV);
-    fr[1].alloc(8 * (size_t)nV);
-    msk[0].alloc(8 * (size_t)nV);
-    msk[1].alloc(8 * (size_t)nV);
-    bc.alloc(8 * (size_t)nV);
-    dsrc.alloc(4 * (size_t)nsrc);
-    flag.alloc(8);
-    GRE_HIP(hipMemsetAsync(bc.p, 0, bc.bytes, s));
-    GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
-    const unsigned gw = (unsigned)std::min<uint64_t>(((uint64_t)nV * 64 + kB - 1) / kB, 1u << 20);
-    // long rows of the forward (CSC) and backward (CSR) pulls, and their chunks' partials
+    const uint32_t nchunks = (uint32_t)((nsrc + 63) / 64);
     LongRows& lf = long_rows(g, true, kBcLong, kBcChunk, s);
     LongRows& lb = long_rows(g, false, kBcLong, kBcChunk, s);
-    DevBuf psum, pgot;
-    psum.alloc(8 * 64 * (size_t)std::max<uint32_t>(std::max(lf.nchunks, lb.nchunks), 1));
-    pgot.alloc(8 * (size_t)std::max<uint32_t>(lf.nchunks, 1));
+    BcBufs& b = bc_bufs(nV, nchunks, std::max<uint32_t>(std::max(lf.nchunks, lb.nchunks), 1));
+    // each chunk's distinct sources: its first frontier
+    std::vector<uint32_t> uq(64 * (size_t)nchunks, 0), uqn(nchunks, 0);
+    for (uint32_t k = 0; k < nchunks; ++k) {
+      const int c0 = 64 * (int)k, nc = std::min(64, nsrc - c0);
+      std::vector<uint32_t> u(srcs + c0, srcs + c0 + nc);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      std::copy(u.begin(), u.end(), uq.begin() + c0);
+      uqn[k] = (uint32_t)u.size();
+    }
+    // per-source outputs: filled on the device, chunk by chunk, copied once at the end
+    DevBuf xs, xd, xl;
+    if (sigma_out) xs.alloc(8 * (size_t)nV * nsrc);
+    if (delta_out) xd.alloc(8 * (size_t)nV * nsrc);
+    if (level_out) xl.alloc(4 * (size_t)nV * nsrc);
+    GRE_HIP(hipMemsetAsync(b.bc.p, 0, 8 * (size_t)nV, s));
+    GRE_HIP(hipMemcpyAsync(b.src.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+    GRE_HIP(hipMemcpyAsync(b.uq.p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
+    GRE_HIP(hipMemcpyAsync(b.uqn.p, uqn.data(), 4 * uqn.size(), hipMemcpyHostToDevice, s));
+    BcCtl* hc = b.ctl_host;
+    *hc = BcCtl{};
+    hc->phase = kBcInit;
+    hc->nchunks = nchunks;
+    hc->nsrc = (uint32_t)nsrc;
+    BcCtl* dc = b.ctl.as<BcCtl>();
+    GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(BcCtl), hipMemcpyHostToDevice, s));
+
+    const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
+    GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for gre_bc");
+    const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
+    const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
+    const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
     auto wgrid = [](uint64_t waves) {
-      return (unsigned)std::min<uint64_t>(std::max<uint64_t>((waves + 3) / 4, 1), 1u << 16);
+      return (unsigned)std::min<uint64_t>(std::max<uint64_t>((waves + 3) / 4, 1), kMsGrid);
     };
-    for (int c0 = 0; c0 < nsrc; c0 += 64) {
-      const int nc = std::min(64, nsrc - c0);
-      const uint32_t* cs = dsrc.as<uint32_t>() + c0;
-      hipLaunchKernelGGL(k_bc_init, dim3(gcap(n64)), dim3(kB), 0, s, nV, cs, nc, sigma.as<double>(), delta.as<double>(),
-                         lev.as<uint8_t>(), (unsigned long long*)vis.p, (unsigned long long*)fr[0].p);
-      hipLaunchKernelGGL(k_bc_seed, dim3(1), dim3(1), 0, s, cs, nc, sigma.as<double>(), lev.as<uint8_t>(),
-                         (unsigned long long*)vis.p, (unsigned long long*)fr[0].p);
-      GRE_HIP(hipGetLastError());
-      // forward levels: top-down over the frontier's out-edges while they are
-      // few (under E / 14), else the bottom-up pull over every vertex
-      std::vector<uint32_t> uq;
-      for (int i = 0; i < nc; ++i) uq.push_back(srcs[c0 + i]);
-      std::sort(uq.begin(), uq.end());
-      uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
-      GRE_HIP(hipMemcpyAsync(bq[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
-      uint32_t qn = (uint32_t)uq.size();
-      uint64_t mf = 0;  // out-edges of the frontier (measured on the device)
-      bool first = true;
-      int cur = 0;
-      int32_t L = 0;
-      for (;; ++L) {
-        GRE_CHECK(L < 250, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
-        const int nx = cur ^ 1;
-        uint64_t T = 0;
-        if (first || mf * 14 < g->nE) {
-          hipLaunchKernelGGL(k_qdeg, dim3(gcap((uint64_t)qn + 1)), dim3(kB), 0, s, bq[cur].as<uint32_t>(), qn,
-                             g->off.as<uint64_t>(), bqdeg.as<uint64_t>());
-          scan_excl_u64(bqdeg.as<uint64_t>(), bqpos.as<uint64_t>(), (size_t)qn + 1, s);
-          GRE_HIP(hipMemcpyAsync(&T, bqpos.as<uint64_t>() + qn, 8, hipMemcpyDeviceToHost, s));
-          GRE_HIP(hipStreamSynchronize(s));
-          mf = T;
-        }
-        first = false;
-        GRE_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-        GRE_HIP(hipMemsetAsync(bsum.p, 0, 8, s));
-        uint32_t nn = 0;
-        if (mf * 14 < g->nE) {
-          ProfScope ps("bc_forward");
-          GRE_HIP(hipMemsetAsync(fr[nx].p, 0, 8 * (size_t)nV, s));
-          if (T > 0)
-            hipLaunchKernelGGL(k_bc_td, dim3((unsigned)((T + kEB - 1) / kEB)), dim3(kB), 0, s, g->off.as<uint64_t>(),
-                               g->idx.as<uint32_t>(), bq[cur].as<uint32_t>(), bqpos.as<uint64_t>(), qn, T,
-                               (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
-                               sigma.as<double>(), (unsigned long long*)fr[nx].p, bq[nx].as<uint32_t>(),
-                               flag.as<uint32_t>());
-          GRE_HIP(hipGetLastError());
-          GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
-          GRE_HIP(hipStreamSynchronize(s));
-          if (nn > 0)
-            hipLaunchKernelGGL(k_bc_td_fin, dim3(gcap(nn)), dim3(kB), 0, s, bq[nx].as<uint32_t>(), nn,
-                               (const unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, lev.as<uint8_t>(), L,
-                               g->off.as<uint64_t>(), (unsigned long long*)bsum.p);
-        } else {
-          ProfScope ps("bc_forward");
-          if (lf.nchunks)
-            hipLaunchKernelGGL(k_bc_fwd_chunks, dim3(wgrid(lf.nchunks)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
-                               g->cidx.as<uint32_t>(), lf.rows.as<uint32_t>(), lf.cv.as<uint32_t>(),
-                               lf.cb.as<uint64_t>(), lf.nchunks, nc, (const unsigned long long*)fr[cur].p,
-                               (const unsigned long long*)vis.p, sigma.as<double>(), psum.as<double>(),
-                               (unsigned long long*)pgot.p);
-          hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), nV, nc,
-                             (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
-                             (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
-                             flag.as<unsigned int>() + 1);
-          if (lf.n)
-            hipLaunchKernelGGL(k_bc_fwd_long_fin, dim3(wgrid(lf.n)), dim3(kB), 0, s, lf.rows.as<uint32_t>(),
-                               lf.cstart.as<uint32_t>(), lf.n, nc, psum.as<double>(),
-                               (const unsigned long long*)pgot.p, (unsigned long long*)fr[nx].p,
-                               (unsigned long long*)vis.p, sigma.as<double>(), lev.as<uint8_t>(), L,
-                               flag.as<unsigned int>() + 1);
-          hipLaunchKernelGGL(k_bc_queue, dim3(gcap(nV)), dim3(kB), 0, s, nV, (const unsigned long long*)fr[nx].p,
-                             g->off.as<uint64_t>(), bq[nx].as<uint32_t>(), flag.as<uint32_t>(),
-                             (unsigned long long*)bsum.p);
-          GRE_HIP(hipGetLastError());
-          GRE_HIP(hipMemcpyAsync(&nn, flag.p, 4, hipMemcpyDeviceToHost, s));
-        }
-        GRE_HIP(hipGetLastError());
-        unsigned long long ms = 0;
-        GRE_HIP(hipMemcpyAsync(&ms, bsum.p, 8, hipMemcpyDeviceToHost, s));
+    const unsigned gw = wgrid(nV);
+    const uint64_t* off = g->off.as<uint64_t>();
+    const uint64_t* coff = g->coff.as<uint64_t>();
+    const uint32_t* idx = g->idx.as<uint32_t>();
+    const uint32_t* cidx = g->cidx.as<uint32_t>();
+    double* sigma = b.sigma.as<double>();
+    double* delta = b.delta.as<double>();
+    double* qv = b.qv.as<double>();
+    uint8_t* lev = b.lev.as<uint8_t>();
+    auto* vis = (unsigned long long*)b.vis.p;
+    auto* fr0 = (unsigned long long*)b.fr[0].p;
+    auto* fr1 = (unsigned long long*)b.fr[1].p;
+    auto* m0 = (unsigned long long*)b.msk[0].p;
+    auto* m1 = (unsigned long long*)b.msk[1].p;
+    uint32_t* q0 = b.q[0].as<uint32_t>();
+    uint32_t* q1 = b.q[1].as<uint32_t>();
+    // one step: set-up (chunk start), a forward level in either direction, a
+    // backward level, the chunk's accumulation and outputs; all but the
+    // step's own kernels return at once
+    auto step = [&]() {
+      hipLaunchKernelGGL(k_bc_init, dim3(std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid)), dim3(kB), 0, s, dc,
+                         nV, sigma, delta, lev, vis, fr0, fr1);
+      hipLaunchKernelGGL(k_bc_begin, dim3(1), dim3(1), 0, s, dc, g->nE, off, b.src.as<uint32_t>(),
+                         b.uq.as<uint32_t>(), b.uqn.as<uint32_t>(), sigma, lev, vis, fr0, q0);
+      // forward, top-down
+      hipLaunchKernelGGL(k_bc_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, off, b.qdeg.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_sums<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
+                         b.bsum.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_top<BcCtl>, dim3(1), dim3(1024), 0, s, dc, b.bsum.as<uint64_t>());
+      hipLaunchKernelGGL(k_ms_scan_apply<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
+                         b.bsum.as<uint64_t>(), b.qpos.as<uint64_t>());
+      hipLaunchKernelGGL(k_bc_td, dim3(tdgrid), dim3(kB), 0, s, off, idx, q0, q1, b.qpos.as<uint64_t>(), dc, fr0, fr1,
+                         vis, sigma);
+      hipLaunchKernelGGL(k_bc_td_fin, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1, vis, lev, off);
+      // forward, bottom-up (the long rows' chunks read fr and vis only; k_bc_fwd skips those rows)
+      if (lf.nchunks)
+        hipLaunchKernelGGL(k_bc_fwd_chunks, dim3(wgrid(lf.nchunks)), dim3(kB), 0, s, coff, cidx,
+                           lf.rows.as<uint32_t>(), lf.cv.as<uint32_t>(), lf.cb.as<uint64_t>(), lf.nchunks, dc, fr0,
+                           fr1, vis, sigma, b.psum.as<double>(), (unsigned long long*)b.pgot.p);
+      hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, coff, cidx, nV, dc, fr0, fr1, vis, sigma, lev);
+      if (lf.n)
+        hipLaunchKernelGGL(k_bc_fwd_long_fin, dim3(wgrid(lf.n)), dim3(kB), 0, s, lf.rows.as<uint32_t>(),
+                           lf.cstart.as<uint32_t>(), lf.n, dc, b.psum.as<double>(),
+                           (const unsigned long long*)b.pgot.p, fr0, fr1, vis, sigma, lev);
+      hipLaunchKernelGGL(k_bc_queue, dim3(vgrid), dim3(kB), 0, s, nV, dc, fr0, fr1, off, q0, q1);
+      hipLaunchKernelGGL(k_bc_clear, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1);
+      // backward
+      hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, dc, nV, lev, m0, m1);
+      if (lb.nchunks)
+        hipLaunchKernelGGL(k_bc_bwd_chunks, dim3(wgrid(lb.nchunks)), dim3(kB), 0, s, off, idx,
+                           lb.rows.as<uint32_t>(), lb.cv.as<uint32_t>(), lb.cb.as<uint64_t>(), lb.nchunks, dc, qv, m0,
+                           m1, b.psum.as<double>());
+      hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, off, idx, nV, dc, sigma, delta, qv, m0, m1);
+      if (lb.n)
+        hipLaunchKernelGGL(k_bc_bwd_long_fin, dim3(wgrid(lb.n)), dim3(kB), 0, s, lb.rows.as<uint32_t>(),
+                           lb.cstart.as<uint32_t>(), lb.n, dc, b.psum.as<double>(), sigma, delta, qv, m0, m1);
+      hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, dc, nV, b.src.as<uint32_t>(), delta,
+                         b.bc.as<double>());
+      const unsigned xg = std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid);
+      if (sigma_out)
+        hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, sigma,
+                           xs.as<double>(), false);
+      if (delta_out)
+        hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, delta,
+                           xd.as<double>(), false);
+      if (level_out)
+        hipLaunchKernelGGL((k_bc_export<uint8_t, int32_t>), dim3(xg), dim3(kB), 0, s, dc, nV, lev,
+                           xl.as<int32_t>(), true);
+      hipLaunchKernelGGL(k_bc_end, dim3(1), dim3(1), 0, s, dc);
+    };
+    // kBcBatch steps captured once per launch configuration and replayed
+    // until the control says done: one read-back per batch, none per level
+    // or chunk
+    static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+    auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+    const std::vector<uint64_t> key = {
+        P(g),         nV,           g->nE,        P(dc),        P(sigma),      P(delta),     P(qv),
+        P(lev),       P(vis),       P(fr0),       P(fr1),       P(m0),         P(m1),        P(q0),
+        P(q1),        P(b.qdeg.p),  P(b.qpos.p),  P(b.bsum.p),  P(b.psum.p),   P(b.pgot.p),  P(b.bc.p),
+        P(b.src.p),   P(b.uq.p),    P(b.uqn.p),   P(lf.rows.p), P(lf.cv.p),    P(lf.cb.p),   P(lf.cstart.p),
+        lf.n,         lf.nchunks,   P(lb.rows.p), P(lb.cv.p),   P(lb.cb.p),    P(lb.cstart.p), lb.n,
+        lb.nchunks,   P(off),       P(idx),       P(coff),      P(cidx),       P(xs.p),      P(xd.p),
+        P(xl.p)};
+    hipGraphExec_t exec = nullptr;
+    auto it = graphs.find(key);
+    if (it != graphs.end()) {
+      exec = it->second;
+    } else {
+      // calls that return per-source arrays use new output buffers each time:
+      // keep the cache small (the stream is idle here: no replay is in flight)
+      if (graphs.size() >= 16) {
         GRE_HIP(hipStreamSynchronize(s));
-        mf = ms;
-        qn = nn;
-        cur = nx;
-        if (!nn) break;
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
       }
-      // deepest level reached is L; dependencies from L-1 down to 0, with the
-      // lane masks of level l (mc) and l + 1 (mn, the previous level's mc)
-      int mc = 0;
-      if (L > 0) {
-        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), L,
-                           (unsigned long long*)msk[1].p);
-        hipLaunchKernelGGL(k_bc_q_last, dim3(gw), dim3(kB), 0, s, nV, sigma.as<double>(), delta.as<double>(),
-                           qv.as<double>(), (const unsigned long long*)msk[1].p);
-      }
-      for (int32_t l = L - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, nV, lev.as<uint8_t>(), l,
-                           (unsigned long long*)msk[mc].p);
-        ProfScope ps("bc_backward");
-        if (lb.nchunks)
-          hipLaunchKernelGGL(k_bc_bwd_chunks, dim3(wgrid(lb.nchunks)), dim3(kB), 0, s, g->off.as<uint64_t>(),
-                             g->idx.as<uint32_t>(), lb.rows.as<uint32_t>(), lb.cv.as<uint32_t>(),
-                             lb.cb.as<uint64_t>(), lb.nchunks, qv.as<double>(),
-                             (const unsigned long long*)msk[mc].p, (const unsigned long long*)msk[mc ^ 1].p,
-                             psum.as<double>());
-        hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), nV,
-                           sigma.as<double>(), delta.as<double>(), qv.as<double>(),
-                           (const unsigned long long*)msk[mc].p, (const unsigned long long*)msk[mc ^ 1].p);
-        if (lb.n)
-          hipLaunchKernelGGL(k_bc_bwd_long_fin, dim3(wgrid(lb.n)), dim3(kB), 0, s, lb.rows.as<uint32_t>(),
-                             lb.cstart.as<uint32_t>(), lb.n, psum.as<double>(), sigma.as<double>(),
-                             delta.as<double>(), qv.as<double>(), (const unsigned long long*)msk[mc].p);
-        mc ^= 1;
-      }
-      hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, nV, nc, cs, delta.as<double>(), bc.as<double>());
-      GRE_HIP(hipGetLastError());
-      if (sigma_out || delta_out || level_out) {
-        DevBuf tmp;
-        tmp.alloc(8 * (size_t)nV * nc);
-        auto dl = [&](auto* dev, auto* host, bool lev8, size_t esz) {
-          using T = std::remove_pointer_t<decltype(dev)>;
-          using U = std::remove_pointer_t<decltype(host)>;
-          hipLaunchKernelGGL((k_bc_export<T, U>), dim3(gcap((uint64_t)nV * nc)), dim3(kB), 0, s, nV, nc, dev,
-                             (U*)tmp.p, lev8);
-          GRE_HIP(hipMemcpyAsync(host + (size_t)c0 * nV, tmp.p, esz * (size_t)nV * nc, hipMemcpyDeviceToHost, s));
-          GRE_HIP(hipStreamSynchronize(s));
-        };
-        if (sigma_out) dl(sigma.as<double>(), sigma_out, false, 8);
-        if (delta_out) dl(delta.as<double>(), delta_out, false, 8);
-        if (level_out) dl(lev.as<uint8_t>(), level_out, true, 4);
+      hipGraph_t graph = nullptr;
+      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < kBcBatch; ++k) step();
+      GRE_HIP(hipStreamEndCapture(s, &graph));
+      GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      graphs[key] = exec;
+    }
+    {
+      ProfScope ps("bc_steps");
+      // a chunk takes at most 2 (kBcMaxDepth + 1) steps
+      const uint64_t max_steps = (uint64_t)nchunks * 2 * (kBcMaxDepth + 1) + kBcBatch;
+      for (uint64_t done_steps = 0;; done_steps += kBcBatch) {
+        GRE_CHECK(done_steps <= max_steps, GRE_DEV_FAILURE, "gre_bc did not terminate");
+        GRE_HIP(hipGraphLaunch(exec, s));
+        GRE_HIP(hipMemcpyAsync(&hc->phase, &dc->phase, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        GRE_HIP(hipStreamSynchronize(s));
+        if (hc->phase >= kBcDone) break;
       }
     }
-    if (bc_out) GRE_HIP(hipMemcpyAsync(bc_out, bc.p, 8 * (size_t)nV, hipMemcpyDeviceToHost, s));
+    GRE_CHECK(hc->phase == kBcDone, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
+    if (sigma_out) GRE_HIP(hipMemcpyAsync(sigma_out, xs.p, xs.bytes, hipMemcpyDeviceToHost, s));
+    if (delta_out) GRE_HIP(hipMemcpyAsync(delta_out, xd.p, xd.bytes, hipMemcpyDeviceToHost, s));
+    if (level_out) GRE_HIP(hipMemcpyAsync(level_out, xl.p, xl.bytes, hipMemcpyDeviceToHost, s));
+    if (bc_out) GRE_HIP(hipMemcpyAsync(bc_out, b.bc.p, 8 * (size_t)nV, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
   });
 }

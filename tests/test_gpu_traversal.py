@@ -99,3 +99,54 @@ def test_bc_matches_brandes(N, O, nsrc):
         odl[s] = 0.0
         acc += odl
     assert np.abs(bc - acc).sum() <= 1e-9 * np.abs(acc).sum()
+
+
+def _chain(N, O, n):
+    """Undirected path 0 - 1 - ... - n-1: BFS depth n-1 from an end."""
+    a = np.arange(n - 1, dtype=np.uint32)
+    src = np.concatenate([a, a + 1])
+    dst = np.concatenate([a + 1, a])
+    return N.Graph.from_edges(n, src, dst), O.graph_from_edges(n, src, dst)
+
+
+def _check_bc(N, O, g, og, srcs):
+    bc, sig, dl, lv = N.bc(g, srcs, want_parts=True)
+    acc = np.zeros(og.nV)
+    for i, s in enumerate(srcs):
+        osig, odl, olv = O.brandes_one(og, int(s))
+        assert np.array_equal(lv[i], olv), i
+        assert np.allclose(sig[i], osig, rtol=1e-12, atol=0), i
+        assert np.allclose(dl[i], odl, rtol=1e-9, atol=1e-12), i
+        odl = odl.copy()
+        odl[s] = 0.0
+        acc += odl
+    assert np.abs(bc - acc).sum() <= 1e-9 * np.abs(acc).sum()
+
+
+def test_bc_deep_chain(N, O):
+    # 2 x 240 steps per chunk: many replayed batches, forward and backward
+    g, og = _chain(N, O, 241)
+    _check_bc(N, O, g, og, np.array([0, 240, 120, 7], np.uint32))
+
+
+def test_bc_depth_limit(N, O):
+    # levels are kept as uint8: a depth of 250 or more is refused, not wrapped
+    g, og = _chain(N, O, 260)
+    with pytest.raises(N.GreError, match="250"):
+        N.bc(g, np.array([0], np.uint32))
+    # the same graph from its middle stays within the limit
+    _check_bc(N, O, g, og, np.array([130], np.uint32))
+
+
+def test_bc_repeated_sources_three_chunks(N, O):
+    # 150 sources in chunks of 64, 64, 22, with repeats inside and across chunks
+    g, og = graphs(N, O, 11, 8, 7)
+    rng = np.random.default_rng(11)
+    srcs = rng.choice(og.nV, 150, replace=True).astype(np.uint32)
+    srcs[5] = srcs[3]
+    srcs[140] = srcs[3]
+    _check_bc(N, O, g, og, srcs)
+    # a second call of another size on the same graph reuses the captured steps
+    bc2, _, _, _ = N.bc(g, srcs[:70])
+    bc3, _, _, _ = N.bc(g, srcs[:70])
+    assert np.array_equal(bc2, bc3)
