@@ -10,6 +10,10 @@ GTEPS = stored edges x iterations / time.  The same JSON line carries:
   cpu_baseline  GRE's scatter/combine/apply superstep restated in C with 12
                 worker threads and 8192 striped locks, timed on this host on a
                 bounded sample (a few PageRank iterations of the same graph);
+                beside it the same superstep at np = 4 processes over GRE's
+                LDG vertex cut (thread groups exchanging packed scatter and
+                combiner messages), with 12 workers each and with the cores
+                this process may use shared among the four;
   bfs / msbfs   16-source BFS on the symmetrised scale-26 graph (seed 6):
                 plain direction-optimising BFS per source vs one 64-bit-mask
                 multi-source launch (Graph500 GTEPS);
@@ -365,6 +369,17 @@ def main():
             cpu["all_cores"] = {"value": round(E / secs2 / 1e9, 4), "cores": allc,
                                 "sample": f"1 iteration, {secs2:.1f}s"}
             log(f"cpu baseline, {allc} threads: {E / secs2 / 1e9:.3f} GTEPS")
+        # np = 4 over GRE's LDG vertex cut: four processes of 12 workers (GRE's
+        # own count), then four processes sharing the cores this process may use
+        wl = [nthreads, max(allc // 4, 1)]
+        _, s4, info = oracle.gre_pagerank_np(og, np_=4, workers=wl, iters=1)
+        cpu["np4"] = [{"value": round(E / s / 1e9, 4), "unit": "GTEPS", "ranks": 4, "workers_per_rank": w,
+                       "threads": 4 * w, "cores": allc,
+                       "sample": f"1 iteration, {s:.1f}s, after a {info['ingress_s']:.1f}s LDG ingress"}
+                      for w, s in zip(wl, s4)]
+        cpu["np4_messages"] = {"scatter": info["scatter_msgs"], "combiner": info["combiner_msgs"],
+                               "part_edges": [int(v) for v in info["part_edges"]]}
+        log("cpu baseline np=4:", cpu["np4"])
         del d
 
     extra = {}

@@ -79,6 +79,8 @@ def _declare(L):
     L.o_ldg_assign.argtypes = [u32, u64, C.c_int, C.c_int, vp, vp, vp, vp]
     L.o_ldg_assign.restype = C.c_int
     L.o_vertex_cut_stats.argtypes = [u32, C.c_int, C.c_int, u64, vp, vp, vp, vp]
+    L.o_gre_pagerank_np.argtypes = [C.c_int, C.c_int, vp, u32, vp, vp, vp, dbl, C.c_int, vp, vp, vp]
+    L.o_gre_pagerank_np.restype = C.c_int
 
 
 # --------------------------------------------------------------------------
@@ -278,6 +280,30 @@ def gre_pagerank_mt(g, nthreads=12, damping=0.85, iters=20):
     t = lib().o_gre_pagerank_mt(nthreads, g.nV, _p(g.off), _p(g.idx), _p(g.outdeg), damping,
                                 iters, _p(x))
     return x, t
+
+
+def gre_pagerank_np(g, np_=4, workers=12, damping=0.85, iters=20):
+    """GRE's PageRank superstep at np_ processes on one host, over the LDG
+    vertex cut of the CSR rows read as np_ edge files (GraphIngress.hpp:511-600,
+    1126-1171; Engine.hpp:427-700).  workers: threads per process, or a list
+    (one ingress, one timed run per entry).  Returns (x, seconds, info) with
+    seconds a float or a list, and info: the last run's phase seconds,
+    ingress seconds, scatter and combiner messages per superstep, edges per
+    part."""
+    ws = [int(w) for w in (workers if isinstance(workers, (list, tuple)) else [workers])]
+    x = np.zeros(g.nV, np.float64)
+    secs = np.zeros(len(ws), np.float64)
+    wa = np.array(ws, np.int32)
+    info = np.zeros(8 + np_, np.float64)
+    rc = lib().o_gre_pagerank_np(np_, len(ws), _p(wa), g.nV, _p(g.off), _p(g.idx), _p(g.outdeg), damping,
+                                 iters, _p(x), _p(secs), _p(info))
+    if rc != 0:
+        raise ValueError("np out of range, a worker count below 1, or fewer than np*np edges")
+    phases = {k: float(v) for k, v in
+              zip(("refresh", "scatter_sync", "compute", "combiner_sync", "apply"), info[:5])}
+    t = [float(v) for v in secs] if isinstance(workers, (list, tuple)) else float(secs[0])
+    return x, t, {"phases": phases, "ingress_s": float(info[5]), "scatter_msgs": int(info[6]),
+                  "combiner_msgs": int(info[7]), "part_edges": info[8:].astype(np.int64)}
 
 
 def lognormal_csr(g, seed=5):

@@ -17,7 +17,10 @@
  *   o_pagerank / o_ppr     dense Google-matrix iteration and linear solve (numpy)
  *   o_ppr_adjoint / grad   numpy linear solve of the transposed system and
  *                          central finite differences of an FP64 forward
- *   PageRank_delta near its 1e-5 activation threshold: parity unpinned.
+ *   GRE engine, PageRank_delta  numpy matrix powers, the linear-solve fixpoint
+ *                          of a run to halt, a hand-built 1e-5 knife edge
+ *   o_gre_pagerank_mt/_np  textbook o_pagerank ranks; the np run's message
+ *                          counts equal o_vertex_cut_stats' agent counts
  */
 #define _POSIX_C_SOURCE 200809L
 #include "gre_oracle.h"
@@ -25,6 +28,7 @@
 #include <string.h>
 #include <math.h>
 #include <time.h>
+#include <pthread.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -655,6 +659,32 @@ double o_gre_pagerank_mt(int nthreads, uint32_t nV, const uint64_t* off, const u
  * state per reading process; process r reads files r, r+np, ... in order
  * (loadEdgesFromFiles :511-600), capacity nE/np/np (:560).  Returns -1 when
  * the capacity is 0 (GRE divides by it). */
+/* One decision: the part with the highest balance-weighted locality score,
+ * ties (within 1e-4) broken by (src + dst) % #tied (GraphIngress.hpp:1144-1170). */
+static int ldg_pick(int np, uint64_t capacity, const uint64_t* cnt, uint64_t sb, uint64_t db, uint64_t s,
+                    uint64_t d) {
+  double sc[64];
+  int cand[64];
+  for (int i = 0; i < np; ++i) {
+    const double bal = fabs(1.0 - (double)cnt[i] / (double)capacity);
+    /* (srcMap.test(i) ? 1.0 : (src%np==i ? 0.5 : 0.0) + (dstMap.test(i) ? 1.0 : (dst%np==i ? 0.5 : 0.0))) */
+    double term;
+    if ((sb >> i) & 1ULL) {
+      term = 1.0;
+    } else {
+      term = ((int)(s % (uint64_t)np) == i) ? 0.5 : 0.0;
+      if ((db >> i) & 1ULL) term += 1.0;
+      else if ((int)(d % (uint64_t)np) == i) term += 0.5;
+    }
+    sc[i] = bal * term;
+  }
+  double mx = sc[0];                             /* Util::getMax: first, then > */
+  for (int i = 0; i < np; ++i) if (sc[i] > mx) mx = sc[i];
+  int n = 0;
+  for (int i = 0; i < np; ++i) if (fabs(mx - sc[i]) < 1e-4) cand[n++] = i;
+  return cand[(s + d) % (uint64_t)n];
+}
+
 int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64_t* file_off,
                  const uint32_t* src, const uint32_t* dst, int32_t* part) {
   const uint64_t capacity = nE_meta / (uint64_t)np / (uint64_t)np;
@@ -662,8 +692,6 @@ int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64
   uint64_t* sbits = (uint64_t*)malloc((size_t)nV * 8);
   uint64_t* dbits = (uint64_t*)malloc((size_t)nV * 8);
   uint64_t cnt[64];
-  double sc[64];
-  int cand[64];
   for (int reader = 0; reader < np; ++reader) {
     memset(sbits, 0, (size_t)nV * 8);
     memset(dbits, 0, (size_t)nV * 8);
@@ -671,24 +699,7 @@ int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64
     for (int f = reader; f < nfiles; f += np) {
       for (uint64_t e = file_off[f]; e < file_off[f + 1]; ++e) {
         const uint64_t s = src[e], d = dst[e];
-        for (int i = 0; i < np; ++i) {
-          const double bal = fabs(1.0 - (double)cnt[i] / (double)capacity);
-          /* (srcMap.test(i) ? 1.0 : (src%np==i ? 0.5 : 0.0) + (dstMap.test(i) ? 1.0 : (dst%np==i ? 0.5 : 0.0))) */
-          double term;
-          if ((sbits[s] >> i) & 1ULL) {
-            term = 1.0;
-          } else {
-            term = ((int)(s % (uint64_t)np) == i) ? 0.5 : 0.0;
-            if ((dbits[d] >> i) & 1ULL) term += 1.0;
-            else if ((int)(d % (uint64_t)np) == i) term += 0.5;
-          }
-          sc[i] = bal * term;
-        }
-        double mx = sc[0];                             /* Util::getMax: first, then > */
-        for (int i = 0; i < np; ++i) if (sc[i] > mx) mx = sc[i];
-        int n = 0;
-        for (int i = 0; i < np; ++i) if (fabs(mx - sc[i]) < 1e-4) cand[n++] = i;
-        const int pid = cand[(s + d) % (uint64_t)n];
+        const int pid = ldg_pick(np, capacity, cnt, sbits[s], dbits[d], s, d);
         cnt[pid]++;
         sbits[s] |= 1ULL << pid;
         dbits[d] |= 1ULL << pid;
@@ -728,4 +739,325 @@ void o_vertex_cut_stats(uint32_t nV, int np, int dir, uint64_t nE, const uint32_
     o[6] = (nm + 63) / 64 * 64;
   }
   free(sc); free(cb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* GRE's PageRank superstep at np processes on one host (CPU baseline)      */
+/* ------------------------------------------------------------------------ */
+/* Ingress: edge file i holds the CSR rows u with u % np == i, in order (the
+ * layout GRE's out-degree counting assumes, GraphIngress.hpp:591,596);
+ * reader r takes file r and places each edge by LDG (ldg_pick), capacity
+ * E/np/np.  Part p keeps its masters (v % np == p, ascending) then its agents
+ * (ascending), a local CSR of its edges, and the send lists of the two syncs.
+ * A superstep follows Engine.hpp:427-700: masters refresh their scatter data,
+ * scatter sync (each master's value to the parts holding it as a scatter
+ * agent), local push over the part's edges (dynamic grain-8 fetch, striped
+ * locks), combiner sync (each combiner agent's sum to its master, then reset),
+ * apply on the masters.  Processes are thread groups with private arrays; a
+ * message is (vid, value) packed into the sender's buffer and unpacked by the
+ * receiver through its own vid -> local id map, with a barrier between
+ * phases, as MPI moves them through shared memory on one host.  The apply is
+ * the textbook one with the dangling mass (as o_gre_pagerank_mt). */
+typedef struct { uint64_t vid; double val; } o_msg;
+typedef struct {
+  uint32_t nl, nm;
+  uint32_t* l2g;
+  uint32_t* g2l;          /* [nV], UINT32_MAX: not local */
+  uint64_t* off;          /* [nl+1] local out-edges by local source */
+  uint32_t* adj;
+  uint64_t ss_off[65];    /* scatter sends to part q: ss_lv[ss_off[q] .. ss_off[q+1]) */
+  uint32_t* ss_lv;
+  uint64_t cs_off[65];    /* combiner sends to master part q */
+  uint32_t* cs_lv;
+  o_msg* sbuf;            /* [ss_off[np]] */
+  o_msg* cbuf;            /* [cs_off[np]] */
+  double *ps, *acc, *x;
+  int32_t* deg;           /* masters' out-degree */
+  volatile int32_t* locks;
+  volatile int64_t next;
+} o_part;
+
+typedef struct {
+  int np, W, iters;
+  uint32_t nV;
+  double damping;
+  o_part* parts;
+  double* dang;           /* [np*W] per-thread dangling partials */
+  pthread_barrier_t bar;
+  double phase[5];        /* refresh, scatter sync, compute, combiner sync, apply (thread 0 of part 0) */
+} o_np_run;
+
+typedef struct { o_np_run* r; int p, t; } o_np_arg;
+
+static void lock_add(volatile int32_t* locks, uint32_t k, double* a, double v) {
+  volatile int32_t* lk = &locks[k & 8191];
+  while (!__sync_bool_compare_and_swap(lk, 0, 1)) { while (*lk == 1) {} }
+  *a += v;
+  __sync_synchronize();
+  *lk = 0;
+}
+
+static void* np_worker(void* argp) {
+  o_np_arg* a = (o_np_arg*)argp;
+  o_np_run* r = a->r;
+  const int np = r->np, W = r->W, p = a->p, t = a->t;
+  o_part* P = &r->parts[p];
+  const int lead = p == 0 && t == 0;
+  const double invn = 1.0 / (double)r->nV;
+#define SLICE(n, lo, hi) const uint64_t lo = (uint64_t)(n) * t / W, hi = (uint64_t)(n) * (t + 1) / W
+  for (int it = 0; it < r->iters; ++it) {
+    double t0 = lead ? now_s() : 0.0;
+    /* masters: scatter data and dangling mass */
+    {
+      SLICE(P->nm, lo, hi);
+      double dg = 0.0;
+      for (uint64_t i = lo; i < hi; ++i) {
+        if (P->deg[i] > 0) P->ps[i] = P->x[i] / (double)P->deg[i];
+        else { P->ps[i] = 0.0; dg += P->x[i]; }
+      }
+      r->dang[p * W + t] = dg;
+      if (t == 0) P->next = 0;
+    }
+    pthread_barrier_wait(&r->bar);
+    double t1 = lead ? now_s() : 0.0;
+    /* scatter sync: pack, then unpack the messages addressed to this part */
+    {
+      SLICE(P->ss_off[np], lo, hi);
+      for (uint64_t i = lo; i < hi; ++i) {
+        const uint32_t lv = P->ss_lv[i];
+        P->sbuf[i].vid = P->l2g[lv];
+        P->sbuf[i].val = P->ps[lv];
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    for (int q = 0; q < np; ++q) {
+      if (q == p) continue;
+      const o_part* Q = &r->parts[q];
+      const uint64_t b = Q->ss_off[p], n = Q->ss_off[p + 1] - b;
+      SLICE(n, lo, hi);
+      for (uint64_t i = lo; i < hi; ++i) {
+        const o_msg m = Q->sbuf[b + i];
+        P->ps[P->g2l[m.vid]] = m.val;
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    double t2 = lead ? now_s() : 0.0;
+    /* local push over the part's edges */
+    for (;;) {
+      const int64_t u0 = __sync_fetch_and_add(&P->next, 8);
+      if (u0 >= (int64_t)P->nl) break;
+      const int64_t u1 = u0 + 8 < (int64_t)P->nl ? u0 + 8 : (int64_t)P->nl;
+      for (int64_t u = u0; u < u1; ++u) {
+        if (P->off[u] == P->off[u + 1]) continue;
+        const double m = P->ps[u];
+        for (uint64_t e = P->off[u]; e < P->off[u + 1]; ++e) {
+          const uint32_t v = P->adj[e];
+          lock_add(P->locks, v, &P->acc[v], m);
+        }
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    double t3 = lead ? now_s() : 0.0;
+    /* combiner sync: pack and reset the agents' sums, then add the received ones */
+    {
+      SLICE(P->cs_off[np], lo, hi);
+      for (uint64_t i = lo; i < hi; ++i) {
+        const uint32_t lv = P->cs_lv[i];
+        P->cbuf[i].vid = P->l2g[lv];
+        P->cbuf[i].val = P->acc[lv];
+        P->acc[lv] = 0.0;
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    for (int q = 0; q < np; ++q) {
+      if (q == p) continue;
+      const o_part* Q = &r->parts[q];
+      const uint64_t b = Q->cs_off[p], n = Q->cs_off[p + 1] - b;
+      SLICE(n, lo, hi);
+      for (uint64_t i = lo; i < hi; ++i) {
+        const o_msg m = Q->cbuf[b + i];
+        const uint32_t lv = (uint32_t)(m.vid / (uint64_t)np);
+        lock_add(P->locks, lv, &P->acc[lv], m.val);
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    double t4 = lead ? now_s() : 0.0;
+    /* apply on the masters (the dangling mass summed in a fixed order) */
+    {
+      double dang = 0.0;
+      for (int k = 0; k < np * W; ++k) dang += r->dang[k];
+      const double base = (1.0 - r->damping) * invn + r->damping * dang * invn;
+      SLICE(P->nm, lo, hi);
+      for (uint64_t i = lo; i < hi; ++i) {
+        P->x[i] = base + r->damping * P->acc[i];
+        P->acc[i] = 0.0;
+      }
+    }
+    pthread_barrier_wait(&r->bar);
+    if (lead) {
+      const double t5 = now_s();
+      r->phase[0] += t1 - t0; r->phase[1] += t2 - t1; r->phase[2] += t3 - t2;
+      r->phase[3] += t4 - t3; r->phase[4] += t5 - t4;
+    }
+  }
+#undef SLICE
+  return NULL;
+}
+
+/* One ingress, then `iters` supersteps from x = 1/V once per worker count
+ * workers[k] (k < nw), secs[k] their seconds; x holds the last run's ranks.
+ * info[0..4]: the last run's phase seconds (refresh, scatter sync, compute,
+ * combiner sync, apply); info[5]: ingress seconds; info[6], info[7]: scatter
+ * and combiner messages per superstep; info[8 + p]: edges of part p.
+ * Returns 0, or -1 (np out of [1, 64], a worker count below 1, capacity 0,
+ * or no memory). */
+int o_gre_pagerank_np(int np, int nw, const int* workers, uint32_t nV, const uint64_t* off, const uint32_t* idx,
+                      const int32_t* outdeg, double damping, int iters, double* x, double* secs, double* info) {
+  const uint64_t nE = off[nV];
+  if (np < 1 || np > 64 || nw < 1) return -1;
+  for (int k = 0; k < nw; ++k) if (workers[k] < 1) return -1;
+  const uint64_t capacity = nE / (uint64_t)np / (uint64_t)np;
+  if (capacity == 0) return -1;
+  const double ti0 = now_s();
+  int32_t* part = (int32_t*)malloc((nE ? nE : 1) * sizeof(int32_t));
+  uint8_t* smark = (uint8_t*)calloc((size_t)np * nV + 1, 1);   /* v is a source of part p's edges */
+  uint8_t* dmark = (uint8_t*)calloc((size_t)np * nV + 1, 1);   /* v is a target of part p's edges */
+  o_part* parts = (o_part*)calloc((size_t)np, sizeof(o_part));
+  if (!part || !smark || !dmark || !parts) { free(part); free(smark); free(dmark); free(parts); return -1; }
+  /* reader r: file r = rows u = r, r + np, ... */
+#pragma omp parallel for num_threads(np) schedule(static, 1)
+  for (int rd = 0; rd < np; ++rd) {
+    uint64_t* sb = (uint64_t*)calloc(nV ? nV : 1, 8);
+    uint64_t* db = (uint64_t*)calloc(nV ? nV : 1, 8);
+    uint64_t cnt[64] = {0};
+    for (uint64_t u = (uint64_t)rd; u < nV; u += (uint64_t)np)
+      for (uint64_t e = off[u]; e < off[u + 1]; ++e) {
+        const uint64_t d = idx[e];
+        const int pid = ldg_pick(np, capacity, cnt, sb[u], db[d], u, d);
+        cnt[pid]++;
+        sb[u] |= 1ULL << pid;
+        db[d] |= 1ULL << pid;
+        part[e] = pid;
+      }
+    free(sb); free(db);
+  }
+#pragma omp parallel for num_threads(np) schedule(static, 1)
+  for (int p = 0; p < np; ++p) {
+    uint8_t* sm = smark + (size_t)p * nV;
+    uint8_t* dm = dmark + (size_t)p * nV;
+    uint64_t ne = 0;
+    for (uint64_t u = 0; u < nV; ++u)
+      for (uint64_t e = off[u]; e < off[u + 1]; ++e)
+        if (part[e] == p) { sm[u] = 1; dm[idx[e]] = 1; ne++; }
+    o_part* P = &parts[p];
+    P->nm = nV / (uint32_t)np + ((uint32_t)p < nV % (uint32_t)np);
+    uint32_t nl = P->nm;
+    for (uint32_t v = 0; v < nV; ++v)
+      if ((int)(v % (uint32_t)np) != p && (sm[v] || dm[v])) nl++;
+    P->nl = nl;
+    P->l2g = (uint32_t*)malloc(((size_t)nl + 1) * 4);
+    P->g2l = (uint32_t*)malloc(((size_t)nV + 1) * 4);
+    memset(P->g2l, 0xff, ((size_t)nV + 1) * 4);
+    uint32_t k = 0;
+    for (uint64_t v = (uint64_t)p; v < nV; v += (uint64_t)np) { P->l2g[k] = (uint32_t)v; P->g2l[v] = k; k++; }
+    for (uint32_t v = 0; v < nV; ++v)
+      if ((int)(v % (uint32_t)np) != p && (sm[v] || dm[v])) { P->l2g[k] = v; P->g2l[v] = k; k++; }
+    P->off = (uint64_t*)calloc((size_t)nl + 1, 8);
+    P->adj = (uint32_t*)malloc((ne ? ne : 1) * 4);
+    for (uint64_t u = 0; u < nV; ++u)
+      for (uint64_t e = off[u]; e < off[u + 1]; ++e)
+        if (part[e] == p) P->off[P->g2l[u] + 1]++;
+    for (uint32_t i = 0; i < nl; ++i) P->off[i + 1] += P->off[i];
+    uint64_t* fill = (uint64_t*)malloc(((size_t)nl + 1) * 8);
+    memcpy(fill, P->off, ((size_t)nl + 1) * 8);
+    for (uint64_t u = 0; u < nV; ++u)
+      for (uint64_t e = off[u]; e < off[u + 1]; ++e)
+        if (part[e] == p) P->adj[fill[P->g2l[u]]++] = P->g2l[idx[e]];
+    free(fill);
+    P->ps = (double*)calloc((size_t)nl + 1, 8);
+    P->acc = (double*)calloc((size_t)nl + 1, 8);
+    P->x = (double*)malloc(((size_t)P->nm + 1) * 8);
+    P->deg = (int32_t*)malloc(((size_t)P->nm + 1) * 4);
+    for (uint32_t i = 0; i < P->nm; ++i) { P->x[i] = 1.0 / (double)nV; P->deg[i] = outdeg[P->l2g[i]]; }
+    P->locks = (volatile int32_t*)calloc(8192, sizeof(int32_t));
+    if (info) info[8 + p] = (double)ne;
+  }
+  /* send lists (every part's marks are known now) */
+  double nsm = 0.0, ncm = 0.0;
+  for (int p = 0; p < np; ++p) {
+    o_part* P = &parts[p];
+    uint64_t n = 0;
+    P->ss_off[0] = 0;
+    for (int q = 0; q < np; ++q) {
+      if (q != p)
+        for (uint32_t i = 0; i < P->nm; ++i) n += smark[(size_t)q * nV + P->l2g[i]];
+      P->ss_off[q + 1] = n;
+    }
+    P->ss_lv = (uint32_t*)malloc((n ? n : 1) * 4);
+    n = 0;
+    for (int q = 0; q < np; ++q)
+      if (q != p)
+        for (uint32_t i = 0; i < P->nm; ++i)
+          if (smark[(size_t)q * nV + P->l2g[i]]) P->ss_lv[n++] = i;
+    P->sbuf = (o_msg*)malloc((n ? n : 1) * sizeof(o_msg));
+    nsm += (double)n;
+    /* combiner agents, grouped by master part, ascending within */
+    uint64_t c = 0;
+    P->cs_off[0] = 0;
+    const uint8_t* dm = dmark + (size_t)p * nV;
+    for (int q = 0; q < np; ++q) {
+      if (q != p)
+        for (uint64_t v = (uint64_t)q; v < nV; v += (uint64_t)np) c += dm[v];
+      P->cs_off[q + 1] = c;
+    }
+    P->cs_lv = (uint32_t*)malloc((c ? c : 1) * 4);
+    c = 0;
+    for (int q = 0; q < np; ++q)
+      if (q != p)
+        for (uint64_t v = (uint64_t)q; v < nV; v += (uint64_t)np)
+          if (dm[v]) P->cs_lv[c++] = P->g2l[v];
+    P->cbuf = (o_msg*)malloc((c ? c : 1) * sizeof(o_msg));
+    ncm += (double)c;
+  }
+  free(part); free(smark); free(dmark);
+  const double ti1 = now_s();
+
+  o_np_run r;
+  for (int k = 0; k < nw; ++k) {
+    const int W = workers[k];
+    for (int p = 0; p < np; ++p) {
+      o_part* P = &parts[p];
+      for (uint32_t i = 0; i < P->nm; ++i) P->x[i] = 1.0 / (double)nV;
+      memset(P->acc, 0, ((size_t)P->nl + 1) * 8);
+    }
+    memset(&r, 0, sizeof(r));
+    r.np = np; r.W = W; r.iters = iters; r.nV = nV; r.damping = damping; r.parts = parts;
+    r.dang = (double*)calloc((size_t)np * W, 8);
+    pthread_barrier_init(&r.bar, NULL, (unsigned)(np * W));
+    pthread_t* th = (pthread_t*)malloc((size_t)np * W * sizeof(pthread_t));
+    o_np_arg* args = (o_np_arg*)malloc((size_t)np * W * sizeof(o_np_arg));
+    const double t0 = now_s();
+    for (int j = 0; j < np * W; ++j) {
+      args[j].r = &r; args[j].p = j / W; args[j].t = j % W;
+      pthread_create(&th[j], NULL, np_worker, &args[j]);
+    }
+    for (int j = 0; j < np * W; ++j) pthread_join(th[j], NULL);
+    secs[k] = now_s() - t0;
+    pthread_barrier_destroy(&r.bar);
+    free(r.dang); free(th); free(args);
+  }
+  for (int p = 0; p < np; ++p) {
+    o_part* P = &parts[p];
+    for (uint32_t i = 0; i < P->nm; ++i) x[P->l2g[i]] = P->x[i];
+    free(P->l2g); free(P->g2l); free(P->off); free(P->adj); free(P->ss_lv); free(P->cs_lv);
+    free(P->sbuf); free(P->cbuf); free(P->ps); free(P->acc); free(P->x); free(P->deg); free((void*)P->locks);
+  }
+  if (info) {
+    for (int k = 0; k < 5; ++k) info[k] = r.phase[k];
+    info[5] = ti1 - ti0;
+    info[6] = nsm;
+    info[7] = ncm;
+  }
+  free(parts);
+  return 0;
 }

@@ -99,14 +99,18 @@ void o_ppr_edge_grad(uint32_t nV, const uint64_t* off, const uint32_t* idx, cons
  * then apply.  Returns seconds spent in the supersteps. */
 double o_gre_pagerank_mt(int nthreads, uint32_t nV, const uint64_t* off, const uint32_t* idx,
                          const int32_t* outdeg, double damping, int iters, double* x);
-
-#ifdef __cplusplus
-}
-#endif
 /* vertex-cut ingress at np processes */
 int o_ldg_assign(uint32_t nV, uint64_t nE_meta, int np, int nfiles, const uint64_t* file_off,
                  const uint32_t* src, const uint32_t* dst, int32_t* part);
 void o_vertex_cut_stats(uint32_t nV, int np, int dir, uint64_t nE, const uint32_t* src, const uint32_t* dst,
                         const int32_t* part, int64_t* stats);
+/* PageRank supersteps at np processes of W workers each on one host, over the
+ * LDG vertex cut (CPU baseline timing; see gre_oracle.c) */
+int o_gre_pagerank_np(int np, int nw, const int* workers, uint32_t nV, const uint64_t* off, const uint32_t* idx,
+                      const int32_t* outdeg, double damping, int iters, double* x, double* secs, double* info);
+
+#ifdef __cplusplus
+}
+#endif
 
 #endif

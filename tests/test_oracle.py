@@ -493,3 +493,46 @@ def test_gre_pagerank_mt_equals_pagerank(O, threads):
     assert secs > 0
     assert np.abs(x - ref).sum() / ref.sum() < 1e-13
     assert abs(x.sum() - 1.0) < 1e-12
+
+
+def _file_order(g, np_):
+    """Edge files of the np-process ingress: file i = CSR rows u with u % np == i."""
+    src = np.repeat(np.arange(g.nV, dtype=np.uint32), g.outdeg)
+    files = []
+    for i in range(np_):
+        m = (src % np_) == i
+        files.append((src[m], g.idx[m]))
+    return files
+
+
+@pytest.mark.parametrize("np_,workers", [(1, 3), (2, 2), (4, 3), (3, 1)])
+def test_gre_pagerank_np_equals_pagerank(O, np_, workers):
+    """The np-process CPU baseline: the same FP64 ranks as o_pagerank up to
+    summation order; each superstep sends one scatter message per scatter agent
+    and one combiner message per combiner agent of the LDG cut, whose counts the
+    vertex-cut statistics give independently; every edge lands in one part."""
+    g = O.rmat_graph(11, 16, seed=1)
+    x, secs, info = O.gre_pagerank_np(g, np_=np_, workers=workers, iters=20)
+    ref = O.pagerank(g, 0.85, 20)
+    assert secs > 0
+    assert np.abs(x - ref).sum() / ref.sum() < 1e-13
+    src, dst, part = O.ldg_assign(g.nV, g.nE, np_, _file_order(g, np_))
+    st = O.vertex_cut_stats(g.nV, np_, O.DIR_OUT, src, dst, part)
+    assert info["scatter_msgs"] == st[:, 1].sum()      # scatters (Out direction)
+    assert info["combiner_msgs"] == st[:, 2].sum()     # combiners
+    assert np.array_equal(info["part_edges"], st[:, 5])
+    assert info["part_edges"].sum() == g.nE
+    if np_ == 1:
+        assert info["scatter_msgs"] == 0 and info["combiner_msgs"] == 0
+
+
+
+def test_gre_pagerank_np_worker_list(O):
+    """One ingress, one run per worker count: every run starts afresh and ends
+    at the same ranks."""
+    g = O.rmat_graph(10, 16, seed=2)
+    ref = O.pagerank(g, 0.85, 10)
+    for ws in ([1, 3], [3, 1]):
+        x, secs, info = O.gre_pagerank_np(g, np_=4, workers=ws, iters=10)
+        assert len(secs) == 2 and min(secs) > 0
+        assert np.abs(x - ref).sum() / ref.sum() < 1e-13
