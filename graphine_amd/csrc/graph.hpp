@@ -100,6 +100,11 @@ struct gre_graph {
   // engine pull row classes by direction (built on demand)
   std::map<int, gre::PullRows> pull_rows;
   std::map<uint64_t, gre::LongRows> long_rows;  // by (csc, threshold, chunk entries)
+  // long-row (threshold, chunk) pairs of the 64-source BFS's bottom-up pass and
+  // of the BC forward pass: 0 until the first call times the candidates (or the
+  // caller sets them)
+  uint64_t ms_long = 0, ms_chunk = 0;
+  uint64_t bc_flong = 0, bc_fchunk = 0;
 };
 
 namespace gre {

@@ -554,21 +554,24 @@ __global__ void k_ms_td_fin(const uint32_t* __restrict__ nq, MsCtl* __restrict__
   block_add2(static_cast<unsigned long long>(deg), sums, static_cast<unsigned long long>(fulldeg), sums + 1);
 }
 
-// Rows with more than kMsLong in-edges are handled apart in the bottom-up step:
+// Rows with more than `lng` in-edges are handled apart in the bottom-up step:
 // one thread would walk a hub's million in-edges alone, and a vertex rarely
 // gains all 64 bits in one level, so early levels scan whole rows.
-constexpr uint64_t kMsLong = 1024;
-constexpr uint64_t kMsChunk = 4096;   // in-edges per chunk, one wave each
+// The threshold and the chunk length (in-edges per chunk, one wave each) are
+// picked per graph by timing the candidates below on the first call.
+constexpr uint64_t kMsCand[][2] = {{1024, 4096}, {256, 1024}, {4096, 16384}, {512, 2048}};
+// calls with fewer sources run the first pair and leave the pick to a later call
+constexpr int kTuneMinSources = 32;
 
 // bottom-up: every vertex still missing source bits ORs its in-neighbours'
-// frontier masks, stopping once it has them all (rows above kMsLong in-edges:
+// frontier masks, stopping once it has them all (rows above lng in-edges:
 // k_ms_bu_chunks and k_ms_bu_long_fin)
 __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
                                               const uint64_t* __restrict__ off, uint32_t nV, unsigned long long full,
                                               const unsigned long long* __restrict__ fr,
                                               unsigned long long* __restrict__ nfr,
                                               unsigned long long* __restrict__ vis, int32_t* __restrict__ lev,
-                                              MsCtl* __restrict__ c, uint32_t* __restrict__ nq) {
+                                              MsCtl* __restrict__ c, uint32_t* __restrict__ nq, uint64_t lng) {
   if (c->mode != kMsBottomUp) return;
   const int32_t L = (int32_t)c->L;
   uint32_t* ncount = &c->nn;
@@ -582,7 +585,7 @@ __global__ __launch_bounds__(kB) void k_ms_bu(const uint64_t* __restrict__ coff,
     if (v < nV) {
       const unsigned long long seen = vis[v];
       const uint64_t b = coff[v], e = coff[v + 1];
-      if (seen != full && e - b <= kMsLong) {
+      if (seen != full && e - b <= lng) {
         unsigned long long m = 0;
         // 8 in-neighbours per step, their masks gathered together (past the
         // row's end: the first neighbour again); once (m | seen) == full the new
@@ -626,7 +629,8 @@ __global__ __launch_bounds__(kB) void k_ms_bu_chunks(const uint64_t* __restrict_
                                                      unsigned long long full,
                                                      const unsigned long long* __restrict__ fr,
                                                      const unsigned long long* __restrict__ vis,
-                                                     unsigned long long* __restrict__ acc, const MsCtl* __restrict__ c) {
+                                                     unsigned long long* __restrict__ acc, const MsCtl* __restrict__ c,
+                                                     uint64_t chunk) {
   if (c->mode != kMsBottomUp) return;
   const int lane = (int)(threadIdx.x & 63);
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -636,7 +640,7 @@ __global__ __launch_bounds__(kB) void k_ms_bu_chunks(const uint64_t* __restrict_
     const unsigned long long seen = vis[v];
     if (seen == full) continue;  // the same chunk, so the same answer, on every lane
     const uint64_t b = cb[c], rowend = coff[v + 1];
-    const uint64_t e = b + kMsChunk < rowend ? b + kMsChunk : rowend;
+    const uint64_t e = b + chunk < rowend ? b + chunk : rowend;
     unsigned long long m = 0;
     for (uint64_t p = b; p < e; p += 4 * 64) {
       // 4 in-neighbours per lane per step, gathered together (past the end:
@@ -800,6 +804,11 @@ __global__ void k_ms_edges(const unsigned long long* __restrict__ vis, const uin
 // them in chunk order (path counts are integers, exact in any order).
 constexpr uint64_t kBcLong = 512;
 constexpr uint64_t kBcChunk = 512;
+// Those are the backward pass's; the forward pass's pair (whose chunking
+// leaves the exact integer path counts unchanged) is picked per graph by
+// timing these candidates on the first call.  The backward pass keeps its
+// pair: its chunk partial sums fix the rounding of the dependencies.
+constexpr uint64_t kBcCand[][2] = {{512, 512}, {256, 256}, {2048, 2048}, {1024, 4096}};
 
 // forward fold of entries [b, e) of v's in-row: each lane adds the sigma of
 // the in-neighbours that reach v for its source; gotl collects those lanes
@@ -881,7 +890,7 @@ __global__ __launch_bounds__(kB) void k_bc_fwd_chunks(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ cidx,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ cv,
-                                                      const uint64_t* __restrict__ cb, uint32_t nchunks,
+                                                      const uint64_t* __restrict__ cb, uint32_t nchunks, uint64_t chunk,
                                                       const BcCtl* __restrict__ c, const unsigned long long* fr0,
                                                       const unsigned long long* fr1,
                                                       const unsigned long long* __restrict__ vis,
@@ -899,7 +908,7 @@ __global__ __launch_bounds__(kB) void k_bc_fwd_chunks(const uint64_t* __restrict
     unsigned long long gotl = 0;
     if (seen != full) {  // the same chunk, so the same answer, on every lane
       const uint64_t b = cb[k], rowend = coff[v + 1];
-      bc_fwd_run(cidx, b, b + kBcChunk < rowend ? b + kBcChunk : rowend, seen, fr, sigma, lane, acc, gotl);
+      bc_fwd_run(cidx, b, b + chunk < rowend ? b + chunk : rowend, seen, fr, sigma, lane, acc, gotl);
     }
     psum[k * 64 + lane] = acc;
     const unsigned long long got = wave_or(gotl);
@@ -1000,7 +1009,7 @@ __global__ __launch_bounds__(kB) void k_bc_bwd_long_fin(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // forward level L -> L+1: sigma_v[j] = sum over in-neighbours w at level L of sigma_w[j]
 __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cidx,
-                                               uint32_t nV, const BcCtl* __restrict__ c,
+                                               uint32_t nV, uint64_t lng, const BcCtl* __restrict__ c,
                                                unsigned long long* fr0, unsigned long long* fr1,
                                                unsigned long long* __restrict__ vis, double* __restrict__ sigma,
                                                uint8_t* __restrict__ lev) {
@@ -1014,7 +1023,7 @@ __global__ __launch_bounds__(kB) void k_bc_fwd(const uint64_t* __restrict__ coff
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t v = wid; v < nV; v += nw) {
     const uint64_t b = coff[v], e = coff[v + 1];
-    if (e - b > kBcLong) continue;  // k_bc_fwd_chunks + k_bc_fwd_long_fin
+    if (e - b > lng) continue;  // k_bc_fwd_chunks + k_bc_fwd_long_fin
     const unsigned long long seen = vis[v];
     if (seen == full) { if (lane == 0) nfr[v] = 0; continue; }
     double acc = 0.0;
@@ -1373,6 +1382,330 @@ BcBufs& bc_bufs(uint32_t nV, uint32_t nchunks, uint32_t pchunks) {
   return b;
 }
 
+// One 64-source BFS over g from srcs (uq: their distinct ids, the first
+// frontier), the bottom-up pass cutting rows above lng in-edges into chunks of
+// `chunk`; levels into lv when non-null.  Leaves the visited masks in the
+// working arrays.
+void ms_run(gre_graph* g, const uint32_t* srcs, int nsrc, const std::vector<uint32_t>& uq, int32_t* lv,
+            uint64_t lng, uint64_t chunk) {
+  hipStream_t s = stream();
+  const uint32_t nV = g->nV;
+  const uint64_t* off = g->off.as<uint64_t>();
+  MsBufs& mb = ms_bufs(nV);
+  DevBuf* fr = mb.fr;
+  DevBuf* q = mb.q;
+  DevBuf &vis = mb.vis, &nxt = mb.nxt, &qdeg = mb.qdeg, &qpos = mb.qpos;
+  LongRows& lr = long_rows(g, true, lng, chunk, s);
+  GRE_HIP(hipMemsetAsync(vis.p, 0, 8 * (size_t)nV, s));
+  GRE_HIP(hipMemsetAsync(lr.acc.p, 0, 8 * std::max<size_t>(lr.n, 1), s));
+  GRE_HIP(hipMemsetAsync(fr[0].p, 0, 8 * (size_t)nV, s));
+  GRE_HIP(hipMemsetAsync(fr[1].p, 0, 8 * (size_t)nV, s));
+  GRE_HIP(hipMemsetAsync(nxt.p, 0, 8 * (size_t)nV, s));
+  GRE_HIP(hipMemcpyAsync(mb.src.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(1), 0, s, mb.src.as<uint32_t>(), nsrc, (unsigned long long*)vis.p,
+                     (unsigned long long*)fr[0].p, lv, nV);
+  GRE_HIP(hipGetLastError());
+  GRE_HIP(hipMemcpyAsync(q[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
+  // level 0: the distinct sources, top-down, mu = every edge
+  MsCtl* hc = mb.ctl_host;
+  *hc = MsCtl{};
+  hc->qn = (uint32_t)uq.size();
+  hc->mode = kMsTopDown;
+  hc->mu = g->nE;
+  MsCtl* dc = mb.ctl.as<MsCtl>();
+  GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(MsCtl), hipMemcpyHostToDevice, s));
+  const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
+  GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for the multi-source BFS");
+  const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
+  const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
+  const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
+  // one level: control, then both directions' kernels (the one not chosen
+  // returns at once), the old frontier's masks cleared, control again
+  auto level = [&](int cur) {
+    const int nx = cur ^ 1;
+    hipLaunchKernelGGL(k_ms_ctl_begin, dim3(1), dim3(1), 0, s, dc, nV);
+    hipLaunchKernelGGL(k_ms_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q[cur].as<uint32_t>(), off, qdeg.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_sums<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+                       mb.bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_top<MsCtl>, dim3(1), dim3(1024), 0, s, dc, mb.bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_apply<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
+                       mb.bsum.as<uint64_t>(), qpos.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_td, dim3(tdgrid), dim3(kB), 0, s, off, g->idx.as<uint32_t>(), q[cur].as<uint32_t>(),
+                       qpos.as<uint64_t>(), dc, (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                       (unsigned long long*)nxt.p, q[nx].as<uint32_t>());
+    hipLaunchKernelGGL(k_ms_td_fin, dim3(vgrid), dim3(kB), 0, s, q[nx].as<uint32_t>(), dc, full,
+                       (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, off, lv,
+                       nV);
+    // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
+    if (lr.nchunks)
+      hipLaunchKernelGGL(k_ms_bu_chunks, dim3(std::min<uint64_t>(((uint64_t)lr.nchunks + 3) / 4, kMsGrid)), dim3(kB),
+                         0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), lr.rows.as<uint32_t>(),
+                         lr.cv.as<uint32_t>(), lr.cb.as<uint64_t>(), lr.nchunks, full,
+                         (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
+                         (unsigned long long*)lr.acc.p, dc, chunk);
+    hipLaunchKernelGGL(k_ms_bu, dim3(vgrid), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), off, nV,
+                       full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
+                       (unsigned long long*)vis.p, lv, dc, q[nx].as<uint32_t>(), lng);
+    if (lr.n)
+      hipLaunchKernelGGL(k_ms_bu_long_fin, dim3(std::min<unsigned>(gcap(lr.n), kMsGrid)), dim3(kB), 0, s,
+                         lr.rows.as<uint32_t>(), lr.n, full, (unsigned long long*)lr.acc.p,
+                         (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, off, lv, nV, dc,
+                         q[nx].as<uint32_t>());
+    hipLaunchKernelGGL(k_ms_clear, dim3(vgrid), dim3(kB), 0, s, q[cur].as<uint32_t>(), dc,
+                       (unsigned long long*)fr[cur].p);
+    hipLaunchKernelGGL(k_ms_ctl_end, dim3(1), dim3(1), 0, s, dc);
+  };
+  // kMsBatch levels captured once per launch configuration and replayed
+  // until the control says done: one read-back per batch, none per level
+  static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+  auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  const std::vector<uint64_t> key = {P(g),         nV,          g->nE,        full,         P(lv),        P(dc),
+                                     P(vis.p),     P(fr[0].p),  P(fr[1].p),   P(nxt.p),     P(q[0].p),    P(q[1].p),
+                                     P(qdeg.p),    P(qpos.p),   P(mb.bsum.p), P(lr.acc.p),  P(lr.rows.p), P(lr.cv.p),
+                                     P(lr.cb.p),   lr.n,        lr.nchunks,   lng,          chunk,        P(off),
+                                     P(g->idx.p),  P(g->coff.p), P(g->cidx.p)};
+  hipGraphExec_t exec = nullptr;
+  auto it = graphs.find(key);
+  if (it != graphs.end()) {
+    exec = it->second;
+  } else {
+    // calls that return levels use a new output buffer each time: keep the
+    // cache small (the stream is idle here: no replay is in flight)
+    if (graphs.size() >= 16) {
+      GRE_HIP(hipStreamSynchronize(s));
+      for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+      graphs.clear();
+    }
+    hipGraph_t graph = nullptr;
+    GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kMsBatch; ++k) level(k & 1);
+    GRE_HIP(hipStreamEndCapture(s, &graph));
+    GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    graphs[key] = exec;
+  }
+  ProfScope ps("msbfs_levels");
+  for (uint64_t done_levels = 0;; done_levels += kMsBatch) {
+    GRE_CHECK(done_levels <= (uint64_t)nV + kMsBatch, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
+    GRE_HIP(hipGraphLaunch(exec, s));
+    GRE_HIP(hipMemcpyAsync(&hc->mode, &dc->mode, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    if (hc->mode == kMsDone) break;
+  }
+}
+
+// Times fn() once on the library stream (after one untimed call that builds
+// and captures what it needs), in ms.
+template <typename F>
+float time_twice(F&& fn) {
+  hipStream_t s = stream();
+  fn();
+  hipEvent_t a, b;
+  GRE_HIP(hipEventCreate(&a));
+  GRE_HIP(hipEventCreate(&b));
+  float ms = 0.f;
+  try {
+    GRE_HIP(hipEventRecord(a, s));
+    fn();
+    GRE_HIP(hipEventRecord(b, s));
+    GRE_HIP(hipEventSynchronize(b));
+    GRE_HIP(hipEventElapsedTime(&ms, a, b));
+  } catch (...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+// The first search of a graph picks the bottom-up pass's (threshold, chunk)
+// pair: each candidate runs on this call's sources and the fastest is kept.
+void ms_tune(gre_graph* g, const uint32_t* srcs, int nsrc, const std::vector<uint32_t>& uq) {
+  float best = 1e30f;
+  for (const auto& c : kMsCand) {
+    const float ms = time_twice([&] { ms_run(g, srcs, nsrc, uq, nullptr, c[0], c[1]); });
+    if (ms < best) {
+      best = ms;
+      g->ms_long = c[0];
+      g->ms_chunk = c[1];
+    }
+  }
+}
+
+// Batched Brandes BC of every 64-source chunk of srcs in one device-driven
+// launch sequence, the forward pass cutting rows above flong in-edges into
+// chunks of fchunk; bc sums are left in the working array, per-source arrays
+// go to xs / xd / xl (device, nsrc x nV) when non-null.
+void bc_run(gre_graph* g, const uint32_t* srcs, int nsrc, double* xs, double* xd, int32_t* xl, uint64_t flong,
+          uint64_t fchunk) {
+  hipStream_t s = stream();
+  const uint32_t nV = g->nV;
+  const uint32_t nchunks = (uint32_t)((nsrc + 63) / 64);
+  LongRows& lf = long_rows(g, true, flong, fchunk, s);
+  LongRows& lb = long_rows(g, false, kBcLong, kBcChunk, s);
+  BcBufs& b = bc_bufs(nV, nchunks, std::max<uint32_t>(std::max(lf.nchunks, lb.nchunks), 1));
+  // each chunk's distinct sources: its first frontier
+  std::vector<uint32_t> uq(64 * (size_t)nchunks, 0), uqn(nchunks, 0);
+  for (uint32_t k = 0; k < nchunks; ++k) {
+    const int c0 = 64 * (int)k, nc = std::min(64, nsrc - c0);
+    std::vector<uint32_t> u(srcs + c0, srcs + c0 + nc);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    std::copy(u.begin(), u.end(), uq.begin() + c0);
+    uqn[k] = (uint32_t)u.size();
+  }
+  GRE_HIP(hipMemsetAsync(b.bc.p, 0, 8 * (size_t)nV, s));
+  GRE_HIP(hipMemcpyAsync(b.src.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
+  GRE_HIP(hipMemcpyAsync(b.uq.p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
+  GRE_HIP(hipMemcpyAsync(b.uqn.p, uqn.data(), 4 * uqn.size(), hipMemcpyHostToDevice, s));
+  BcCtl* hc = b.ctl_host;
+  *hc = BcCtl{};
+  hc->phase = kBcInit;
+  hc->nchunks = nchunks;
+  hc->nsrc = (uint32_t)nsrc;
+  BcCtl* dc = b.ctl.as<BcCtl>();
+  GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(BcCtl), hipMemcpyHostToDevice, s));
+
+  const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
+  GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for gre_bc");
+  const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
+  const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
+  const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
+  auto wgrid = [](uint64_t waves) {
+    return (unsigned)std::min<uint64_t>(std::max<uint64_t>((waves + 3) / 4, 1), kMsGrid);
+  };
+  const unsigned gw = wgrid(nV);
+  const uint64_t* off = g->off.as<uint64_t>();
+  const uint64_t* coff = g->coff.as<uint64_t>();
+  const uint32_t* idx = g->idx.as<uint32_t>();
+  const uint32_t* cidx = g->cidx.as<uint32_t>();
+  double* sigma = b.sigma.as<double>();
+  double* delta = b.delta.as<double>();
+  double* qv = b.qv.as<double>();
+  uint8_t* lev = b.lev.as<uint8_t>();
+  auto* vis = (unsigned long long*)b.vis.p;
+  auto* fr0 = (unsigned long long*)b.fr[0].p;
+  auto* fr1 = (unsigned long long*)b.fr[1].p;
+  auto* m0 = (unsigned long long*)b.msk[0].p;
+  auto* m1 = (unsigned long long*)b.msk[1].p;
+  uint32_t* q0 = b.q[0].as<uint32_t>();
+  uint32_t* q1 = b.q[1].as<uint32_t>();
+  // one step: set-up (chunk start), a forward level in either direction, a
+  // backward level, the chunk's accumulation and outputs; all but the
+  // step's own kernels return at once
+  auto step = [&]() {
+    hipLaunchKernelGGL(k_bc_init, dim3(std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid)), dim3(kB), 0, s, dc,
+                       nV, sigma, delta, lev, vis, fr0, fr1);
+    hipLaunchKernelGGL(k_bc_begin, dim3(1), dim3(1), 0, s, dc, g->nE, off, b.src.as<uint32_t>(),
+                       b.uq.as<uint32_t>(), b.uqn.as<uint32_t>(), sigma, lev, vis, fr0, q0);
+    // forward, top-down
+    hipLaunchKernelGGL(k_bc_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, off, b.qdeg.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_sums<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
+                       b.bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_top<BcCtl>, dim3(1), dim3(1024), 0, s, dc, b.bsum.as<uint64_t>());
+    hipLaunchKernelGGL(k_ms_scan_apply<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
+                       b.bsum.as<uint64_t>(), b.qpos.as<uint64_t>());
+    hipLaunchKernelGGL(k_bc_td, dim3(tdgrid), dim3(kB), 0, s, off, idx, q0, q1, b.qpos.as<uint64_t>(), dc, fr0, fr1,
+                       vis, sigma);
+    hipLaunchKernelGGL(k_bc_td_fin, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1, vis, lev, off);
+    // forward, bottom-up (the long rows' chunks read fr and vis only; k_bc_fwd skips those rows)
+    if (lf.nchunks)
+      hipLaunchKernelGGL(k_bc_fwd_chunks, dim3(wgrid(lf.nchunks)), dim3(kB), 0, s, coff, cidx,
+                         lf.rows.as<uint32_t>(), lf.cv.as<uint32_t>(), lf.cb.as<uint64_t>(), lf.nchunks, fchunk, dc, fr0,
+                         fr1, vis, sigma, b.psum.as<double>(), (unsigned long long*)b.pgot.p);
+    hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, coff, cidx, nV, flong, dc, fr0, fr1, vis, sigma, lev);
+    if (lf.n)
+      hipLaunchKernelGGL(k_bc_fwd_long_fin, dim3(wgrid(lf.n)), dim3(kB), 0, s, lf.rows.as<uint32_t>(),
+                         lf.cstart.as<uint32_t>(), lf.n, dc, b.psum.as<double>(),
+                         (const unsigned long long*)b.pgot.p, fr0, fr1, vis, sigma, lev);
+    hipLaunchKernelGGL(k_bc_queue, dim3(vgrid), dim3(kB), 0, s, nV, dc, fr0, fr1, off, q0, q1);
+    hipLaunchKernelGGL(k_bc_clear, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1);
+    // backward
+    hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, dc, nV, lev, m0, m1);
+    if (lb.nchunks)
+      hipLaunchKernelGGL(k_bc_bwd_chunks, dim3(wgrid(lb.nchunks)), dim3(kB), 0, s, off, idx,
+                         lb.rows.as<uint32_t>(), lb.cv.as<uint32_t>(), lb.cb.as<uint64_t>(), lb.nchunks, dc, qv, m0,
+                         m1, b.psum.as<double>());
+    hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, off, idx, nV, dc, sigma, delta, qv, m0, m1);
+    if (lb.n)
+      hipLaunchKernelGGL(k_bc_bwd_long_fin, dim3(wgrid(lb.n)), dim3(kB), 0, s, lb.rows.as<uint32_t>(),
+                         lb.cstart.as<uint32_t>(), lb.n, dc, b.psum.as<double>(), sigma, delta, qv, m0, m1);
+    hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, dc, nV, b.src.as<uint32_t>(), delta,
+                       b.bc.as<double>());
+    const unsigned xg = std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid);
+    if (xs) hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, sigma, xs, false);
+    if (xd) hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, delta, xd, false);
+    if (xl) hipLaunchKernelGGL((k_bc_export<uint8_t, int32_t>), dim3(xg), dim3(kB), 0, s, dc, nV, lev, xl, true);
+    hipLaunchKernelGGL(k_bc_end, dim3(1), dim3(1), 0, s, dc);
+  };
+  // kBcBatch steps captured once per launch configuration and replayed
+  // until the control says done: one read-back per batch, none per level
+  // or chunk
+  static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+  auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  const std::vector<uint64_t> key = {
+      P(g),         nV,           g->nE,        P(dc),        P(sigma),      P(delta),     P(qv),
+      P(lev),       P(vis),       P(fr0),       P(fr1),       P(m0),         P(m1),        P(q0),
+      P(q1),        P(b.qdeg.p),  P(b.qpos.p),  P(b.bsum.p),  P(b.psum.p),   P(b.pgot.p),  P(b.bc.p),
+      P(b.src.p),   P(b.uq.p),    P(b.uqn.p),   P(lf.rows.p), P(lf.cv.p),    P(lf.cb.p),   P(lf.cstart.p),
+      lf.n,         lf.nchunks,   P(lb.rows.p), P(lb.cv.p),   P(lb.cb.p),    P(lb.cstart.p), lb.n,
+      lb.nchunks,   P(off),       P(idx),       P(coff),      P(cidx),       P(xs),        P(xd),
+      P(xl),        flong,        fchunk};
+  hipGraphExec_t exec = nullptr;
+  auto it = graphs.find(key);
+  if (it != graphs.end()) {
+    exec = it->second;
+  } else {
+    // calls that return per-source arrays use new output buffers each time:
+    // keep the cache small (the stream is idle here: no replay is in flight)
+    if (graphs.size() >= 16) {
+      GRE_HIP(hipStreamSynchronize(s));
+      for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+      graphs.clear();
+    }
+    hipGraph_t graph = nullptr;
+    GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kBcBatch; ++k) step();
+    GRE_HIP(hipStreamEndCapture(s, &graph));
+    GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    graphs[key] = exec;
+  }
+  {
+    ProfScope ps("bc_steps");
+    // a chunk takes at most 2 (kBcMaxDepth + 1) steps
+    const uint64_t max_steps = (uint64_t)nchunks * 2 * (kBcMaxDepth + 1) + kBcBatch;
+    for (uint64_t done_steps = 0;; done_steps += kBcBatch) {
+      GRE_CHECK(done_steps <= max_steps, GRE_DEV_FAILURE, "gre_bc did not terminate");
+      GRE_HIP(hipGraphLaunch(exec, s));
+      GRE_HIP(hipMemcpyAsync(&hc->phase, &dc->phase, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      if (hc->phase >= kBcDone) break;
+    }
+  }
+  GRE_CHECK(hc->phase == kBcDone, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
+}
+
+// The first BC of a graph picks the forward pass's (threshold, chunk) pair:
+// each candidate runs on this call's first chunk of sources and the fastest is
+// kept.  Path counts are integers, exact in any order, so the pick changes no
+// result.
+void bc_tune(gre_graph* g, const uint32_t* srcs, int nsrc) {
+  float best = 1e30f;
+  for (const auto& c : kBcCand) {
+    const float ms = time_twice([&] { bc_run(g, srcs, std::min(nsrc, 64), nullptr, nullptr, nullptr, c[0], c[1]); });
+    if (ms < best) {
+      best = ms;
+      g->bc_flong = c[0];
+      g->bc_fchunk = c[1];
+    }
+  }
+}
+
 }  // namespace
 }  // namespace gre
 
@@ -1409,130 +1742,30 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out, 
     for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
-    const uint64_t* off = g->off.as<uint64_t>();
-    MsBufs& mb = ms_bufs(nV);
-    DevBuf* fr = mb.fr;
-    DevBuf* q = mb.q;
-    DevBuf &vis = mb.vis, &nxt = mb.nxt, &qdeg = mb.qdeg, &qpos = mb.qpos, &flag = mb.flag, &sums = mb.sums;
-    DevBuf lev;
-    DevBuf& dsrc = mb.src;
-    if (level_out) {
-      lev.alloc(4 * (size_t)nV * nsrc);
-      hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
-                         (uint64_t)nV * nsrc, -1);
-    }
-    GRE_HIP(hipMemsetAsync(vis.p, 0, 8 * (size_t)nV, s));
-    LongRows& lr = long_rows(g, true, kMsLong, kMsChunk, s);
-    GRE_HIP(hipMemsetAsync(lr.acc.p, 0, 8 * std::max<size_t>(lr.n, 1), s));
-    GRE_HIP(hipMemsetAsync(fr[0].p, 0, 8 * (size_t)nV, s));
-    GRE_HIP(hipMemsetAsync(fr[1].p, 0, 8 * (size_t)nV, s));
-    GRE_HIP(hipMemsetAsync(nxt.p, 0, 8 * (size_t)nV, s));
-    GRE_HIP(hipMemcpyAsync(dsrc.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(1), 0, s, dsrc.as<uint32_t>(), nsrc, (unsigned long long*)vis.p,
-                       (unsigned long long*)fr[0].p, level_out ? lev.as<int32_t>() : nullptr, nV);
-    GRE_HIP(hipGetLastError());
     // the first frontier: the distinct sources
     std::vector<uint32_t> uq(srcs, srcs + nsrc);
     std::sort(uq.begin(), uq.end());
     uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
-    GRE_HIP(hipMemcpyAsync(q[0].p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
-    GRE_HIP(hipStreamSynchronize(s));
-    const unsigned long long full = nsrc == 64 ? ~0ULL : ((1ULL << nsrc) - 1);
-    // level 0: the distinct sources, top-down, mu = every edge
-    MsCtl* hc = mb.ctl_host;
-    *hc = MsCtl{};
-    hc->qn = (uint32_t)uq.size();
-    hc->mode = kMsTopDown;
-    hc->mu = g->nE;
-    MsCtl* dc = mb.ctl.as<MsCtl>();
-    GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(MsCtl), hipMemcpyHostToDevice, s));
-    int32_t* lv = level_out ? lev.as<int32_t>() : nullptr;
-    const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
-    GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for the multi-source BFS");
-    const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
-    const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
-    const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
-    // one level: control, then both directions' kernels (the one not chosen
-    // returns at once), the old frontier's masks cleared, control again
-    auto level = [&](int cur) {
-      const int nx = cur ^ 1;
-      hipLaunchKernelGGL(k_ms_ctl_begin, dim3(1), dim3(1), 0, s, dc, nV);
-      hipLaunchKernelGGL(k_ms_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q[cur].as<uint32_t>(), off,
-                         qdeg.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_sums<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
-                         mb.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_top<MsCtl>, dim3(1), dim3(1024), 0, s, dc, mb.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_apply<MsCtl>, dim3(sgrid), dim3(kB), 0, s, dc, qdeg.as<uint64_t>(),
-                         mb.bsum.as<uint64_t>(), qpos.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_td, dim3(tdgrid), dim3(kB), 0, s, off, g->idx.as<uint32_t>(), q[cur].as<uint32_t>(),
-                         qpos.as<uint64_t>(), dc, (const unsigned long long*)fr[cur].p,
-                         (const unsigned long long*)vis.p, (unsigned long long*)nxt.p, q[nx].as<uint32_t>());
-      hipLaunchKernelGGL(k_ms_td_fin, dim3(vgrid), dim3(kB), 0, s, q[nx].as<uint32_t>(), dc, full,
-                         (unsigned long long*)nxt.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p, off,
-                         lv, nV);
-      // the long rows' chunks read vis and fr[cur] only; k_ms_bu skips those rows
-      if (lr.nchunks)
-        hipLaunchKernelGGL(k_ms_bu_chunks, dim3(std::min<uint64_t>(((uint64_t)lr.nchunks + 3) / 4, kMsGrid)),
-                           dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(), lr.rows.as<uint32_t>(),
-                           lr.cv.as<uint32_t>(), lr.cb.as<uint64_t>(), lr.nchunks, full,
-                           (const unsigned long long*)fr[cur].p, (const unsigned long long*)vis.p,
-                           (unsigned long long*)lr.acc.p, dc);
-      hipLaunchKernelGGL(k_ms_bu, dim3(vgrid), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->cidx.as<uint32_t>(),
-                         off, nV, full, (const unsigned long long*)fr[cur].p, (unsigned long long*)fr[nx].p,
-                         (unsigned long long*)vis.p, lv, dc, q[nx].as<uint32_t>());
-      if (lr.n)
-        hipLaunchKernelGGL(k_ms_bu_long_fin, dim3(std::min<unsigned>(gcap(lr.n), kMsGrid)), dim3(kB), 0, s,
-                           lr.rows.as<uint32_t>(), lr.n, full,
-                           (unsigned long long*)lr.acc.p, (unsigned long long*)fr[nx].p, (unsigned long long*)vis.p,
-                           off, lv, nV, dc, q[nx].as<uint32_t>());
-      hipLaunchKernelGGL(k_ms_clear, dim3(vgrid), dim3(kB), 0, s, q[cur].as<uint32_t>(), dc,
-                         (unsigned long long*)fr[cur].p);
-      hipLaunchKernelGGL(k_ms_ctl_end, dim3(1), dim3(1), 0, s, dc);
-    };
-    // kMsBatch levels captured once per launch configuration and replayed
-    // until the control says done: one read-back per batch, none per level
-    static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
-    auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
-    const std::vector<uint64_t> key = {P(g),         nV,           g->nE,        full,         P(lv),
-                                       P(dc),        P(vis.p),     P(fr[0].p),   P(fr[1].p),   P(nxt.p),
-                                       P(q[0].p),    P(q[1].p),    P(qdeg.p),    P(qpos.p),    P(mb.bsum.p),
-                                       P(lr.acc.p),  P(lr.rows.p), P(lr.cv.p),   P(lr.cb.p),   lr.n,
-                                       lr.nchunks,   P(off),       P(g->idx.p),  P(g->coff.p), P(g->cidx.p)};
-    hipGraphExec_t exec = nullptr;
-    auto it = graphs.find(key);
-    if (it != graphs.end()) {
-      exec = it->second;
-    } else {
-      // calls that return levels use a new output buffer each time: keep the
-      // cache small (the stream is idle here: no replay is in flight)
-      if (graphs.size() >= 16) {
-        GRE_HIP(hipStreamSynchronize(s));
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
-      }
-      hipGraph_t graph = nullptr;
-      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < kMsBatch; ++k) level(k & 1);
-      GRE_HIP(hipStreamEndCapture(s, &graph));
-      GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      graphs[key] = exec;
+    // the pair is timed on the first call with a batch wide enough to show
+    // the lanes' cost (a few sources finish their rows early); until then the
+    // first candidate runs
+    if (!g->ms_long && nsrc >= kTuneMinSources) ms_tune(g, srcs, nsrc, uq);
+    DevBuf lev;
+    if (level_out) {
+      lev.alloc(4 * (size_t)nV * nsrc);
+      hipLaunchKernelGGL(k_fill<int32_t>, dim3(gcap((uint64_t)nV * nsrc)), dim3(kB), 0, s, lev.as<int32_t>(),
+                         (uint64_t)nV * nsrc, -1);
+      GRE_HIP(hipGetLastError());
     }
-    {
-      ProfScope ps("msbfs_levels");
-      for (uint64_t done_levels = 0;; done_levels += kMsBatch) {
-        GRE_CHECK(done_levels <= (uint64_t)nV + kMsBatch, GRE_DEV_FAILURE, "multi-source BFS did not terminate");
-        GRE_HIP(hipGraphLaunch(exec, s));
-        GRE_HIP(hipMemcpyAsync(&hc->mode, &dc->mode, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        GRE_HIP(hipStreamSynchronize(s));
-        if (hc->mode == kMsDone) break;
-      }
-    }
+    ms_run(g, srcs, nsrc, uq, level_out ? lev.as<int32_t>() : nullptr, g->ms_long ? g->ms_long : kMsCand[0][0],
+           g->ms_long ? g->ms_chunk : kMsCand[0][1]);
+    MsBufs& mb = ms_bufs(nV);
     if (edges_out) {
-      GRE_HIP(hipMemsetAsync(flag.p, 0, 8, s));
-      hipLaunchKernelGGL(k_ms_edges, dim3(gcap(nV)), dim3(kB), 0, s, (const unsigned long long*)vis.p,
-                         g->off.as<uint64_t>(), nV, (unsigned long long*)flag.p);
-      GRE_HIP(hipMemcpyAsync(edges_out, flag.p, 8, hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipMemsetAsync(mb.flag.p, 0, 8, s));
+      hipLaunchKernelGGL(k_ms_edges, dim3(gcap(nV)), dim3(kB), 0, s, (const unsigned long long*)mb.vis.p,
+                         g->off.as<uint64_t>(), nV, (unsigned long long*)mb.flag.p);
+      GRE_HIP(hipGetLastError());
+      GRE_HIP(hipMemcpyAsync(edges_out, mb.flag.p, 8, hipMemcpyDeviceToHost, s));
     }
     if (level_out)
       GRE_HIP(hipMemcpyAsync(level_out, lev.p, 4 * (size_t)nV * nsrc, hipMemcpyDeviceToHost, s));
@@ -1547,166 +1780,42 @@ int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double*
     for (int i = 0; i < nsrc; ++i) GRE_CHECK(srcs[i] < g->nV, GRE_DEV_NULL, "source id >= nV");
     hipStream_t s = stream();
     const uint32_t nV = g->nV;
-    const uint32_t nchunks = (uint32_t)((nsrc + 63) / 64);
-    LongRows& lf = long_rows(g, true, kBcLong, kBcChunk, s);
-    LongRows& lb = long_rows(g, false, kBcLong, kBcChunk, s);
-    BcBufs& b = bc_bufs(nV, nchunks, std::max<uint32_t>(std::max(lf.nchunks, lb.nchunks), 1));
-    // each chunk's distinct sources: its first frontier
-    std::vector<uint32_t> uq(64 * (size_t)nchunks, 0), uqn(nchunks, 0);
-    for (uint32_t k = 0; k < nchunks; ++k) {
-      const int c0 = 64 * (int)k, nc = std::min(64, nsrc - c0);
-      std::vector<uint32_t> u(srcs + c0, srcs + c0 + nc);
-      std::sort(u.begin(), u.end());
-      u.erase(std::unique(u.begin(), u.end()), u.end());
-      std::copy(u.begin(), u.end(), uq.begin() + c0);
-      uqn[k] = (uint32_t)u.size();
-    }
+    if (!g->bc_flong && nsrc >= kTuneMinSources) bc_tune(g, srcs, nsrc);
     // per-source outputs: filled on the device, chunk by chunk, copied once at the end
     DevBuf xs, xd, xl;
     if (sigma_out) xs.alloc(8 * (size_t)nV * nsrc);
     if (delta_out) xd.alloc(8 * (size_t)nV * nsrc);
     if (level_out) xl.alloc(4 * (size_t)nV * nsrc);
-    GRE_HIP(hipMemsetAsync(b.bc.p, 0, 8 * (size_t)nV, s));
-    GRE_HIP(hipMemcpyAsync(b.src.p, srcs, 4 * (size_t)nsrc, hipMemcpyHostToDevice, s));
-    GRE_HIP(hipMemcpyAsync(b.uq.p, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice, s));
-    GRE_HIP(hipMemcpyAsync(b.uqn.p, uqn.data(), 4 * uqn.size(), hipMemcpyHostToDevice, s));
-    BcCtl* hc = b.ctl_host;
-    *hc = BcCtl{};
-    hc->phase = kBcInit;
-    hc->nchunks = nchunks;
-    hc->nsrc = (uint32_t)nsrc;
-    BcCtl* dc = b.ctl.as<BcCtl>();
-    GRE_HIP(hipMemcpyAsync(dc, hc, sizeof(BcCtl), hipMemcpyHostToDevice, s));
-
-    const uint64_t nbmax = ((uint64_t)nV + 1 + kScanB - 1) / kScanB;
-    GRE_CHECK(nbmax <= 0x7fffffffULL, GRE_DEV_FAILURE, "graph too large for gre_bc");
-    const unsigned tdgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(g->nE / kEB, 1), 2 * kMsGrid);
-    const unsigned vgrid = std::min<unsigned>(gcap((uint64_t)nV + 1), kMsGrid);
-    const unsigned sgrid = (unsigned)std::min<uint64_t>(nbmax, kMsScanGrid);
-    auto wgrid = [](uint64_t waves) {
-      return (unsigned)std::min<uint64_t>(std::max<uint64_t>((waves + 3) / 4, 1), kMsGrid);
-    };
-    const unsigned gw = wgrid(nV);
-    const uint64_t* off = g->off.as<uint64_t>();
-    const uint64_t* coff = g->coff.as<uint64_t>();
-    const uint32_t* idx = g->idx.as<uint32_t>();
-    const uint32_t* cidx = g->cidx.as<uint32_t>();
-    double* sigma = b.sigma.as<double>();
-    double* delta = b.delta.as<double>();
-    double* qv = b.qv.as<double>();
-    uint8_t* lev = b.lev.as<uint8_t>();
-    auto* vis = (unsigned long long*)b.vis.p;
-    auto* fr0 = (unsigned long long*)b.fr[0].p;
-    auto* fr1 = (unsigned long long*)b.fr[1].p;
-    auto* m0 = (unsigned long long*)b.msk[0].p;
-    auto* m1 = (unsigned long long*)b.msk[1].p;
-    uint32_t* q0 = b.q[0].as<uint32_t>();
-    uint32_t* q1 = b.q[1].as<uint32_t>();
-    // one step: set-up (chunk start), a forward level in either direction, a
-    // backward level, the chunk's accumulation and outputs; all but the
-    // step's own kernels return at once
-    auto step = [&]() {
-      hipLaunchKernelGGL(k_bc_init, dim3(std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid)), dim3(kB), 0, s, dc,
-                         nV, sigma, delta, lev, vis, fr0, fr1);
-      hipLaunchKernelGGL(k_bc_begin, dim3(1), dim3(1), 0, s, dc, g->nE, off, b.src.as<uint32_t>(),
-                         b.uq.as<uint32_t>(), b.uqn.as<uint32_t>(), sigma, lev, vis, fr0, q0);
-      // forward, top-down
-      hipLaunchKernelGGL(k_bc_qdeg, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, off, b.qdeg.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_sums<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
-                         b.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_top<BcCtl>, dim3(1), dim3(1024), 0, s, dc, b.bsum.as<uint64_t>());
-      hipLaunchKernelGGL(k_ms_scan_apply<BcCtl>, dim3(sgrid), dim3(kB), 0, s, dc, b.qdeg.as<uint64_t>(),
-                         b.bsum.as<uint64_t>(), b.qpos.as<uint64_t>());
-      hipLaunchKernelGGL(k_bc_td, dim3(tdgrid), dim3(kB), 0, s, off, idx, q0, q1, b.qpos.as<uint64_t>(), dc, fr0, fr1,
-                         vis, sigma);
-      hipLaunchKernelGGL(k_bc_td_fin, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1, vis, lev, off);
-      // forward, bottom-up (the long rows' chunks read fr and vis only; k_bc_fwd skips those rows)
-      if (lf.nchunks)
-        hipLaunchKernelGGL(k_bc_fwd_chunks, dim3(wgrid(lf.nchunks)), dim3(kB), 0, s, coff, cidx,
-                           lf.rows.as<uint32_t>(), lf.cv.as<uint32_t>(), lf.cb.as<uint64_t>(), lf.nchunks, dc, fr0,
-                           fr1, vis, sigma, b.psum.as<double>(), (unsigned long long*)b.pgot.p);
-      hipLaunchKernelGGL(k_bc_fwd, dim3(gw), dim3(kB), 0, s, coff, cidx, nV, dc, fr0, fr1, vis, sigma, lev);
-      if (lf.n)
-        hipLaunchKernelGGL(k_bc_fwd_long_fin, dim3(wgrid(lf.n)), dim3(kB), 0, s, lf.rows.as<uint32_t>(),
-                           lf.cstart.as<uint32_t>(), lf.n, dc, b.psum.as<double>(),
-                           (const unsigned long long*)b.pgot.p, fr0, fr1, vis, sigma, lev);
-      hipLaunchKernelGGL(k_bc_queue, dim3(vgrid), dim3(kB), 0, s, nV, dc, fr0, fr1, off, q0, q1);
-      hipLaunchKernelGGL(k_bc_clear, dim3(vgrid), dim3(kB), 0, s, dc, q0, q1, fr0, fr1);
-      // backward
-      hipLaunchKernelGGL(k_bc_mask, dim3(gw), dim3(kB), 0, s, dc, nV, lev, m0, m1);
-      if (lb.nchunks)
-        hipLaunchKernelGGL(k_bc_bwd_chunks, dim3(wgrid(lb.nchunks)), dim3(kB), 0, s, off, idx,
-                           lb.rows.as<uint32_t>(), lb.cv.as<uint32_t>(), lb.cb.as<uint64_t>(), lb.nchunks, dc, qv, m0,
-                           m1, b.psum.as<double>());
-      hipLaunchKernelGGL(k_bc_bwd, dim3(gw), dim3(kB), 0, s, off, idx, nV, dc, sigma, delta, qv, m0, m1);
-      if (lb.n)
-        hipLaunchKernelGGL(k_bc_bwd_long_fin, dim3(wgrid(lb.n)), dim3(kB), 0, s, lb.rows.as<uint32_t>(),
-                           lb.cstart.as<uint32_t>(), lb.n, dc, b.psum.as<double>(), sigma, delta, qv, m0, m1);
-      hipLaunchKernelGGL(k_bc_accum, dim3(gw), dim3(kB), 0, s, dc, nV, b.src.as<uint32_t>(), delta,
-                         b.bc.as<double>());
-      const unsigned xg = std::min<unsigned>(gcap((uint64_t)nV * 64), kMsGrid);
-      if (sigma_out)
-        hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, sigma,
-                           xs.as<double>(), false);
-      if (delta_out)
-        hipLaunchKernelGGL((k_bc_export<double, double>), dim3(xg), dim3(kB), 0, s, dc, nV, delta,
-                           xd.as<double>(), false);
-      if (level_out)
-        hipLaunchKernelGGL((k_bc_export<uint8_t, int32_t>), dim3(xg), dim3(kB), 0, s, dc, nV, lev,
-                           xl.as<int32_t>(), true);
-      hipLaunchKernelGGL(k_bc_end, dim3(1), dim3(1), 0, s, dc);
-    };
-    // kBcBatch steps captured once per launch configuration and replayed
-    // until the control says done: one read-back per batch, none per level
-    // or chunk
-    static std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
-    auto P = [](const void* p) { return (uint64_t)(uintptr_t)p; };
-    const std::vector<uint64_t> key = {
-        P(g),         nV,           g->nE,        P(dc),        P(sigma),      P(delta),     P(qv),
-        P(lev),       P(vis),       P(fr0),       P(fr1),       P(m0),         P(m1),        P(q0),
-        P(q1),        P(b.qdeg.p),  P(b.qpos.p),  P(b.bsum.p),  P(b.psum.p),   P(b.pgot.p),  P(b.bc.p),
-        P(b.src.p),   P(b.uq.p),    P(b.uqn.p),   P(lf.rows.p), P(lf.cv.p),    P(lf.cb.p),   P(lf.cstart.p),
-        lf.n,         lf.nchunks,   P(lb.rows.p), P(lb.cv.p),   P(lb.cb.p),    P(lb.cstart.p), lb.n,
-        lb.nchunks,   P(off),       P(idx),       P(coff),      P(cidx),       P(xs.p),      P(xd.p),
-        P(xl.p)};
-    hipGraphExec_t exec = nullptr;
-    auto it = graphs.find(key);
-    if (it != graphs.end()) {
-      exec = it->second;
-    } else {
-      // calls that return per-source arrays use new output buffers each time:
-      // keep the cache small (the stream is idle here: no replay is in flight)
-      if (graphs.size() >= 16) {
-        GRE_HIP(hipStreamSynchronize(s));
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
-      }
-      hipGraph_t graph = nullptr;
-      GRE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < kBcBatch; ++k) step();
-      GRE_HIP(hipStreamEndCapture(s, &graph));
-      GRE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      graphs[key] = exec;
-    }
-    {
-      ProfScope ps("bc_steps");
-      // a chunk takes at most 2 (kBcMaxDepth + 1) steps
-      const uint64_t max_steps = (uint64_t)nchunks * 2 * (kBcMaxDepth + 1) + kBcBatch;
-      for (uint64_t done_steps = 0;; done_steps += kBcBatch) {
-        GRE_CHECK(done_steps <= max_steps, GRE_DEV_FAILURE, "gre_bc did not terminate");
-        GRE_HIP(hipGraphLaunch(exec, s));
-        GRE_HIP(hipMemcpyAsync(&hc->phase, &dc->phase, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        GRE_HIP(hipStreamSynchronize(s));
-        if (hc->phase >= kBcDone) break;
-      }
-    }
-    GRE_CHECK(hc->phase == kBcDone, GRE_DEV_FAILURE, "BFS depth above 250 levels is not supported by gre_bc");
+    bc_run(g, srcs, nsrc, sigma_out ? xs.as<double>() : nullptr, delta_out ? xd.as<double>() : nullptr,
+           level_out ? xl.as<int32_t>() : nullptr, g->bc_flong ? g->bc_flong : kBcCand[0][0],
+           g->bc_flong ? g->bc_fchunk : kBcCand[0][1]);
     if (sigma_out) GRE_HIP(hipMemcpyAsync(sigma_out, xs.p, xs.bytes, hipMemcpyDeviceToHost, s));
     if (delta_out) GRE_HIP(hipMemcpyAsync(delta_out, xd.p, xd.bytes, hipMemcpyDeviceToHost, s));
     if (level_out) GRE_HIP(hipMemcpyAsync(level_out, xl.p, xl.bytes, hipMemcpyDeviceToHost, s));
-    if (bc_out) GRE_HIP(hipMemcpyAsync(bc_out, b.bc.p, 8 * (size_t)nV, hipMemcpyDeviceToHost, s));
+    if (bc_out) GRE_HIP(hipMemcpyAsync(bc_out, bc_bufs(nV, 1, 1).bc.p, 8 * (size_t)nV, hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int gre_traversal_set_chunks(gre_graph* g, int which, uint64_t long_rows, uint64_t chunk) {
+  return guard([&] {
+    GRE_CHECK(g, GRE_DEV_NULL, "null graph");
+    GRE_CHECK(which == GRE_CHUNKS_MSBFS || which == GRE_CHUNKS_BC_FORWARD, GRE_DEV_NULL, "unknown traversal pass");
+    GRE_CHECK(long_rows == 0 || (long_rows < (1ull << 31) && chunk >= 1 && chunk < (1ull << 32)), GRE_DEV_NULL,
+              "long_rows must be below 2^31 and chunk in [1, 2^32)");
+    uint64_t* p = which == GRE_CHUNKS_MSBFS ? &g->ms_long : &g->bc_flong;
+    uint64_t* q = which == GRE_CHUNKS_MSBFS ? &g->ms_chunk : &g->bc_fchunk;
+    *p = long_rows;
+    *q = long_rows ? chunk : 0;
+  });
+}
+
+int gre_traversal_get_chunks(gre_graph* g, int which, uint64_t* long_rows, uint64_t* chunk) {
+  return guard([&] {
+    GRE_CHECK(g && long_rows && chunk, GRE_DEV_NULL, "null argument");
+    GRE_CHECK(which == GRE_CHUNKS_MSBFS || which == GRE_CHUNKS_BC_FORWARD, GRE_DEV_NULL, "unknown traversal pass");
+    *long_rows = which == GRE_CHUNKS_MSBFS ? g->ms_long : g->bc_flong;
+    *chunk = which == GRE_CHUNKS_MSBFS ? g->ms_chunk : g->bc_fchunk;
   });
 }
 

@@ -112,6 +112,8 @@ def _declare(L):
         "gre_bfs_dev": ([vp, u32, vp, C.POINTER(u64), C.POINTER(C.c_int32)], i32),
         "gre_msbfs": ([vp, vp, i32, vp, C.POINTER(u64)], i32),
         "gre_bc": ([vp, vp, i32, vp, vp, vp, vp], i32),
+        "gre_traversal_set_chunks": ([vp, i32, u64, u64], i32),
+        "gre_traversal_get_chunks": ([vp, i32, C.POINTER(u64), C.POINTER(u64)], i32),
         "gre_vertex_cut_ldg": ([u32, u64, i32, i32, vp, vp, vp, vp], i32),
         "gre_vertex_cut_stats": ([u32, i32, i32, u64, vp, vp, vp, vp], i32),
         "gre_vertex_cut_lvid2vid": ([u32, i32, i32, i32, u64, vp, vp, vp, vp, u64, C.POINTER(u64)], i32),
@@ -452,6 +454,21 @@ def msbfs(graph, srcs, want_levels=True):
     e = C.c_uint64()
     _ck(lib().gre_msbfs(graph.h, _p(s), len(s), _p(lev), C.byref(e)))
     return lev, e.value
+
+
+CHUNKS_MSBFS, CHUNKS_BC_FORWARD = 0, 1
+
+
+def set_traversal_chunks(graph, which, long_rows, chunk=0):
+    """Long-row (threshold, chunk) pair of a batched traversal on this graph;
+    long_rows = 0 leaves it to the first call's timing of the candidates."""
+    _ck(lib().gre_traversal_set_chunks(graph.h, which, long_rows, chunk))
+
+
+def traversal_chunks(graph, which):
+    a, b = C.c_uint64(), C.c_uint64()
+    _ck(lib().gre_traversal_get_chunks(graph.h, which, C.byref(a), C.byref(b)))
+    return a.value, b.value
 
 
 def bc(graph, srcs, want_parts=False):

@@ -247,6 +247,20 @@ int gre_msbfs(gre_graph* g, const uint32_t* srcs, int nsrc, int32_t* level_out,
  * sigma_out/delta_out: nsrc x nV. */
 int gre_bc(gre_graph* g, const uint32_t* srcs, int nsrc, double* bc_out, double* sigma_out,
            double* delta_out, int32_t* level_out);
+/* Long-row chunking of the batched traversals, per graph (no GRE counterpart:
+ * GRE's engine has no such kernels; the knob is to these what
+ * gre_pr_set_schedule is to PageRank).  which: GRE_CHUNKS_MSBFS (the in-rows
+ * of gre_msbfs's bottom-up levels) or GRE_CHUNKS_BC_FORWARD (gre_bc's forward
+ * pull).  Rows with more than long_rows entries are cut into chunks of
+ * `chunk` entries, one wavefront each.  long_rows = 0 (the default) leaves the
+ * pair to the first call on the graph with at least 32 sources, which times a
+ * few candidates on them and keeps the fastest (narrower calls before it run
+ * a fixed default pair).  Levels and path counts do not depend on the pair; the BC
+ * backward pass keeps a fixed one (its partial sums set the rounding of the
+ * dependencies). */
+enum { GRE_CHUNKS_MSBFS = 0, GRE_CHUNKS_BC_FORWARD = 1 };
+int gre_traversal_set_chunks(gre_graph* g, int which, uint64_t long_rows, uint64_t chunk);
+int gre_traversal_get_chunks(gre_graph* g, int which, uint64_t* long_rows, uint64_t* chunk);
 
 /* ---- vertex-cut placement for np processes (GRE's MPI ingress) ------------
  * Host-side, no device work: the placement a run over np GPUs would load.

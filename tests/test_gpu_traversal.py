@@ -150,3 +150,58 @@ def test_bc_repeated_sources_three_chunks(N, O):
     bc2, _, _, _ = N.bc(g, srcs[:70])
     bc3, _, _, _ = N.bc(g, srcs[:70])
     assert np.array_equal(bc2, bc3)
+
+
+def test_msbfs_chunk_pairs_change_nothing(N, O):
+    """Every long-row pair of the bottom-up pass, tiny ones included (which
+    chunk most rows of this graph), gives the oracle's levels; a narrow call
+    leaves the pair untuned, the first call with 32 or more sources times the
+    candidates and keeps one of them."""
+    g, og = graphs(N, O, 12, 16, 3)
+    rng = np.random.default_rng(3)
+    srcs = rng.choice(og.nV, 40, replace=False).astype(np.uint32)
+    assert N.traversal_chunks(g, N.CHUNKS_MSBFS) == (0, 0)
+    lev4, _ = N.msbfs(g, srcs[:4])
+    assert N.traversal_chunks(g, N.CHUNKS_MSBFS) == (0, 0)
+    lev0, e0 = N.msbfs(g, srcs)
+    assert np.array_equal(lev0[:4], lev4)
+    picked = N.traversal_chunks(g, N.CHUNKS_MSBFS)
+    assert picked in {(1024, 4096), (256, 1024), (4096, 16384), (512, 2048)}
+    refs = [O.bfs_levels(og, int(s)) for s in srcs]
+    for i in range(len(srcs)):
+        assert np.array_equal(lev0[i], refs[i])
+    for pair in [(8, 16), (16, 300), (64, 64), (1024, 4096)]:
+        N.set_traversal_chunks(g, N.CHUNKS_MSBFS, *pair)
+        lev, e = N.msbfs(g, srcs)
+        assert N.traversal_chunks(g, N.CHUNKS_MSBFS) == pair
+        assert np.array_equal(lev, lev0) and e == e0, pair
+    N.set_traversal_chunks(g, N.CHUNKS_MSBFS, 0)
+    assert N.traversal_chunks(g, N.CHUNKS_MSBFS) == (0, 0)
+
+
+def test_bc_forward_chunk_pairs_change_nothing(N, O):
+    """The forward pair moves only exact integer path counts: every pair gives
+    bit-identical path counts, levels, dependencies and BC."""
+    g, og = graphs(N, O, 12, 16, 4)
+    rng = np.random.default_rng(4)
+    srcs = rng.choice(og.nV, 70, replace=False).astype(np.uint32)
+    ref = N.bc(g, srcs, want_parts=True)
+    assert N.traversal_chunks(g, N.CHUNKS_BC_FORWARD)[0] in (256, 512, 1024, 2048)
+    for pair in [(8, 8), (32, 100), (512, 512), (2048, 2048)]:
+        N.set_traversal_chunks(g, N.CHUNKS_BC_FORWARD, *pair)
+        got = N.bc(g, srcs, want_parts=True)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b), pair
+    osig, odl, olv = O.brandes_one(og, int(srcs[0]))
+    assert np.array_equal(ref[3][0], olv)
+    assert np.allclose(ref[2][0], odl, rtol=1e-9, atol=1e-12)
+
+
+def test_traversal_chunks_argument_checks(N):
+    g = N.Graph.rmat(8, 8, 1, True)
+    with pytest.raises(N.GreError):
+        N.set_traversal_chunks(g, 5, 16, 16)
+    with pytest.raises(N.GreError):
+        N.set_traversal_chunks(g, N.CHUNKS_MSBFS, 16, 0)
+    with pytest.raises(N.GreError):
+        N.set_traversal_chunks(g, N.CHUNKS_BC_FORWARD, 1 << 31, 16)
