@@ -223,6 +223,56 @@ def bench_bc(N, rng):
             "teps_batched": round(256 * g.nE / tb / 1e9, 2)}
 
 
+def bench_scrambled(N, S, spec, iters, nbfs):
+    """The headline PageRank (its schedule fixed to the headline's pick), the plain BFS and the 64-source BFS on the
+    same R-MAT graphs with their vertex labels scrambled (as Graph500's generator does), so the hubs are not the low
+    ids: how much the unscrambled numbers owe to label locality."""
+    out = {"graph": f"R-MAT scale {S}, edge factor 16, seed 6, labels scrambled", "pagerank_schedule": spec}
+    g = N.Graph.rmat(S, 16, 6, False, scramble=True)
+    pr = N.PageRank(g, N.F32)
+    parts = spec.split(":")
+    code = {v: k for k, v in N.PageRank.SCHEDULES.items()}[parts[0]]
+    pr.set_schedule(code, int(parts[1]))
+    if len(parts) > 2:
+        pr.set_partition(int(parts[2]), int(parts[3]))
+    pr.run(0.85, 2)
+    pr.run(0.85, iters)
+    N.sync()
+    t = time.perf_counter()
+    for _ in range(3):
+        pr.run(0.85, iters)
+    N.sync()
+    dt = (time.perf_counter() - t) / 3
+    out["pagerank_ms_per_step"] = round(1e3 * dt, 2)
+    out["pagerank_gteps"] = round(g.nE * iters / dt / 1e9, 2)
+    pr.close()
+    del g
+    gs = N.Graph.rmat(S, 16, 6, True, scramble=True)
+    lev, _, _ = N.bfs(gs, int(np.argmax(gs.outdeg())))
+    reached = np.nonzero(lev >= 0)[0]
+    srcs = sample_distinct(np.random.default_rng(6), reached, 64).astype(np.uint32)
+    N.bfs(gs, int(srcs[0]), want_levels=False)
+    rates = []
+    for s_ in srcs[:nbfs]:
+        N.sync()
+        t = time.perf_counter()
+        _, er, _ = N.bfs(gs, int(s_), want_levels=False)
+        N.sync()
+        rates.append((er / 2) / (time.perf_counter() - t) / 1e9)
+    out["bfs_sources"] = len(rates)
+    out["bfs_gteps_harmonic_mean"] = round(len(rates) / sum(1.0 / r for r in rates), 3)
+    N.msbfs(gs, srcs, want_levels=False)
+    N.sync()
+    t = time.perf_counter()
+    _, ems = N.msbfs(gs, srcs, want_levels=False)
+    N.sync()
+    dt = time.perf_counter() - t
+    out["msbfs_ms"] = round(1e3 * dt, 2)
+    out["msbfs_gteps"] = round((ems / 2) / dt / 1e9, 3)
+    del gs
+    return out
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -497,6 +547,8 @@ def main():
         log("bc:", extra["bc"])
         extra["graph_replay"] = bench_graph_replay(N)
         log("graph replay:", extra["graph_replay"])
+        extra["scrambled"] = bench_scrambled(N, S, spec, iters, a.bfs_sources)
+        log("scrambled labels:", extra["scrambled"])
 
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,

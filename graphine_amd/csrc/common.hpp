@@ -119,6 +119,23 @@ __host__ __device__ inline uint64_t mix64(uint64_t k) {
   return k;
 }
 
+// Vertex-label scramble of the R-MAT generator (Graph500's generator permutes
+// its labels the same way): a bijection of [0, 2^scale) keyed by the seed,
+// each step invertible modulo 2^scale (add, odd multiply, xor with a right
+// shift of itself).
+__host__ __device__ inline uint32_t rmat_scramble(int scale, uint64_t seed, uint32_t x) {
+  const uint64_t mask = (1ULL << scale) - 1;
+  const uint64_t k = mix64(seed ^ 0xA0761D6478BD642FULL);
+  uint64_t y = x;
+  y = (y + k) & mask;
+  y = (y * ((k >> 32) | 1ULL)) & mask;
+  y ^= y >> ((scale + 1) / 2);
+  y = (y * GRE_K0) & mask;
+  y ^= y >> ((scale + 2) / 3);
+  y = (y + (k >> 11)) & mask;
+  return (uint32_t)y;
+}
+
 // ---- wave helpers (64 lanes) ------------------------------------------------
 template <typename T>
 __device__ inline T wave_sum(T v) {

@@ -35,12 +35,16 @@ __device__ inline void rmat_edge(int scale, uint64_t seed, uint64_t i, uint32_t&
   }
 }
 
-__global__ void k_rmat_keys(int scale, uint64_t seed, uint64_t ngen, int sym, uint64_t* keys) {
+__global__ void k_rmat_keys(int scale, uint64_t seed, uint64_t ngen, int sym, int scramble, uint64_t* keys) {
   const uint64_t marker = ((1ULL << (2 * scale)) - 1);
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ngen;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t u, v;
     rmat_edge(scale, seed, i, u, v);
+    if (scramble) {
+      u = rmat_scramble(scale, seed, u);
+      v = rmat_scramble(scale, seed, v);
+    }
     const uint64_t k1 = (u == v) ? marker : (((uint64_t)u << scale) | v);
     if (sym) {
       const uint64_t k2 = (u == v) ? marker : (((uint64_t)v << scale) | u);
@@ -408,9 +412,11 @@ using namespace gre;
 
 extern "C" {
 
-int gre_graph_rmat(gre_graph** out, int scale, int ef, uint64_t seed, int sym) {
+int gre_graph_rmat(gre_graph** out, int scale, int ef, uint64_t seed, int flags) {
   return guard([&] {
     GRE_CHECK(out, GRE_DEV_NULL, "null out pointer");
+    GRE_CHECK(flags >= 0 && flags <= (GRE_RMAT_SYM | GRE_RMAT_SCRAMBLE), GRE_DEV_NULL, "unknown R-MAT flags");
+    const int sym = flags & GRE_RMAT_SYM;
     GRE_CHECK(scale >= 1 && scale <= 30, GRE_DEV_NULL, "scale must be in [1,30]");
     GRE_CHECK(ef >= 1 && ef <= 1024, GRE_DEV_NULL, "edge factor must be in [1,1024]");
     hipStream_t s = stream();
@@ -424,7 +430,7 @@ int gre_graph_rmat(gre_graph** out, int scale, int ef, uint64_t seed, int sym) {
       a.alloc(sizeof(uint64_t) * nk);
       b.alloc(sizeof(uint64_t) * nk);
       hipLaunchKernelGGL(k_rmat_keys, dim3(grid_cap(ngen)), dim3(kBlock), 0, s, scale, seed, ngen, sym,
-                         a.as<uint64_t>());
+                         (flags & GRE_RMAT_SCRAMBLE) ? 1 : 0, a.as<uint64_t>());
       GRE_HIP(hipGetLastError());
       sort_keys(a.as<uint64_t>(), b.as<uint64_t>(), nk, kbits, s);
       uint64_t n = unique_keys(b.as<uint64_t>(), a.as<uint64_t>(), nk, s);

@@ -198,9 +198,9 @@ class Graph:
         self.nV, self.nE = nv.value, ne.value
 
     @classmethod
-    def rmat(cls, scale, ef=16, seed=1, sym=False):
+    def rmat(cls, scale, ef=16, seed=1, sym=False, scramble=False):
         h = C.c_void_p()
-        _ck(lib().gre_graph_rmat(C.byref(h), scale, ef, seed, 1 if sym else 0))
+        _ck(lib().gre_graph_rmat(C.byref(h), scale, ef, seed, (1 if sym else 0) | (2 if scramble else 0)))
         return cls(h)
 
     @classmethod

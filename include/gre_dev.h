@@ -95,8 +95,11 @@ int gre_graph_from_csr(gre_graph** g, uint32_t nV, uint64_t nE, const uint64_t* 
                        const uint32_t* idx);
 /* Seeded R-MAT generated and built on the device: ef<<scale draws,
  * (a,b,c,d)=(0.57,0.19,0.19,0.05), self-loops and duplicates removed,
- * rows sorted; sym=1 adds the reverse of every edge. */
-int gre_graph_rmat(gre_graph** g, int scale, int ef, uint64_t seed, int sym);
+ * rows sorted.  flags: GRE_RMAT_SYM adds the reverse of every edge;
+ * GRE_RMAT_SCRAMBLE permutes the vertex labels by a seeded bijection (as
+ * Graph500's generator does), so the hubs are not the low ids. */
+enum { GRE_RMAT_SYM = 1, GRE_RMAT_SCRAMBLE = 2 };
+int gre_graph_rmat(gre_graph** g, int scale, int ef, uint64_t seed, int flags);
 int gre_graph_destroy(gre_graph* g);
 int gre_graph_info(const gre_graph* g, uint32_t* nV, uint64_t* nE);
 /* Any output may be NULL.  off/coff: nV+1 uint64; idx/cidx: nE uint32;

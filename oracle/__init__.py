@@ -55,6 +55,7 @@ def _declare(L):
     L.o_mix64.argtypes, L.o_mix64.restype = [u64], u64
     L.o_rmat_keys.argtypes, L.o_rmat_keys.restype = [C.c_int, C.c_int, u64, C.c_int, vp], sz
     L.o_rmat_raw.argtypes = [C.c_int, u64, u64, u64, vp, vp]
+    L.o_rmat_scramble_many.argtypes = [C.c_int, u64, u64, vp, vp]
     L.o_sort_unique_keys.argtypes = [vp, C.POINTER(sz)]
     L.o_csr_from_keys.argtypes = [u32, sz, vp, vp, vp]
     L.o_csr_from_edges.argtypes = [u32, sz, vp, vp, vp, vp]
@@ -120,11 +121,19 @@ def rmat_raw(scale, seed, n, first=0):
     return u, v
 
 
-def rmat_keys(scale, ef=16, seed=1, sym=False):
+def rmat_keys(scale, ef=16, seed=1, sym=False, scramble=False):
     cap = (ef << scale) * (2 if sym else 1)
     keys = np.empty(cap, np.uint64)
-    n = lib().o_rmat_keys(scale, ef, seed, 1 if sym else 0, _p(keys))
+    n = lib().o_rmat_keys(scale, ef, seed, (1 if sym else 0) | (2 if scramble else 0), _p(keys))
     return keys[:n].copy()
+
+
+def rmat_scramble(scale, seed, x):
+    """The generator's label bijection of [0, 2^scale), applied to x."""
+    x = np.ascontiguousarray(x, np.uint32)
+    y = np.empty_like(x)
+    lib().o_rmat_scramble_many(scale, seed, len(x), _p(x), _p(y))
+    return y
 
 
 def csr_from_keys(nV, keys):
@@ -134,10 +143,11 @@ def csr_from_keys(nV, keys):
     return off, idx
 
 
-def rmat_graph(scale, ef=16, seed=1, sym=False):
-    """Seeded R-MAT (a,b,c,d)=(.57,.19,.19,.05); self-loops and duplicates removed."""
+def rmat_graph(scale, ef=16, seed=1, sym=False, scramble=False):
+    """Seeded R-MAT (a,b,c,d)=(.57,.19,.19,.05); self-loops and duplicates removed;
+    scramble: vertex labels permuted by rmat_scramble."""
     nV = 1 << scale
-    keys = rmat_keys(scale, ef, seed, sym)
+    keys = rmat_keys(scale, ef, seed, sym, scramble)
     off, idx = csr_from_keys(nV, keys)
     return Graph(nV, off, idx)
 

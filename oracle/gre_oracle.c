@@ -106,13 +106,37 @@ void o_rmat_raw(int scale, uint64_t seed, uint64_t first, uint64_t n, uint32_t* 
   for (uint64_t i = 0; i < n; ++i) rmat_edge(scale, seed, first + i, &u[i], &v[i]);
 }
 
-size_t o_rmat_keys(int scale, int ef, uint64_t seed, int sym, uint64_t* keys) {
+/* Label scramble (as Graph500's generator permutes its vertex labels): a
+ * bijection of [0, 2^scale) keyed by the seed, built from steps that are each
+ * invertible modulo 2^scale (add, multiply by an odd number, xor with a right
+ * shift of itself). */
+uint32_t o_rmat_scramble(int scale, uint64_t seed, uint32_t x) {
+  const uint64_t mask = (1ULL << scale) - 1;
+  const uint64_t k = o_mix64(seed ^ 0xA0761D6478BD642FULL);
+  uint64_t y = x;
+  y = (y + k) & mask;
+  y = (y * ((k >> 32) | 1ULL)) & mask;
+  y ^= y >> ((scale + 1) / 2);
+  y = (y * 0x9E3779B97F4A7C15ULL) & mask;
+  y ^= y >> ((scale + 2) / 3);
+  y = (y + (k >> 11)) & mask;
+  return (uint32_t)y;
+}
+
+void o_rmat_scramble_many(int scale, uint64_t seed, uint64_t n, const uint32_t* x, uint32_t* y) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = o_rmat_scramble(scale, seed, x[i]);
+}
+
+/* flags: 1 = symmetrise, 2 = scramble the labels */
+size_t o_rmat_keys(int scale, int ef, uint64_t seed, int flags, uint64_t* keys) {
   const uint64_t ngen = (uint64_t)ef << scale;
+  const int sym = flags & 1;
   size_t n = 0;
   for (uint64_t i = 0; i < ngen; ++i) {
     uint32_t u, v;
     rmat_edge(scale, seed, i, &u, &v);
     if (u == v) continue;
+    if (flags & 2) { u = o_rmat_scramble(scale, seed, u); v = o_rmat_scramble(scale, seed, v); }
     keys[n++] = ((uint64_t)u << 32) | v;
     if (sym) keys[n++] = ((uint64_t)v << 32) | u;
   }

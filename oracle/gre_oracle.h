@@ -27,7 +27,9 @@ uint64_t o_mix64(uint64_t k);
 /* Generates ef<<scale edges, drops self-loops, optionally symmetrises,
  * sorts by (src,dst) and removes duplicates.  keys must hold
  * (ef<<scale)*(sym?2:1) entries; returns the unique count; key = src<<32|dst. */
-size_t o_rmat_keys(int scale, int ef, uint64_t seed, int sym, uint64_t* keys);
+size_t o_rmat_keys(int scale, int ef, uint64_t seed, int flags, uint64_t* keys);
+uint32_t o_rmat_scramble(int scale, uint64_t seed, uint32_t x);
+void o_rmat_scramble_many(int scale, uint64_t seed, uint64_t n, const uint32_t* x, uint32_t* y);
 /* Raw draws first..first+n-1 (self-loops and duplicates kept). */
 void o_rmat_raw(int scale, uint64_t seed, uint64_t first, uint64_t n, uint32_t* u, uint32_t* v);
 void o_sort_unique_keys(uint64_t* keys, size_t* n);

@@ -30,6 +30,16 @@ def test_rmat_device_equals_oracle(N, O, scale, ef, seed, sym):
     assert np.array_equal(d["outdeg"], og.outdeg)
 
 
+@pytest.mark.parametrize("sym", [False, True])
+def test_rmat_scrambled_device_equals_oracle(N, O, sym):
+    g = N.Graph.rmat(13, 16, 4, sym, scramble=True)
+    og = O.rmat_graph(13, 16, 4, sym, scramble=True)
+    d = g.download(csc=True)
+    assert g.nE == og.nE
+    assert np.array_equal(d["off"], og.off) and np.array_equal(d["idx"], og.idx)
+    assert np.array_equal(d["coff"], og.coff) and np.array_equal(d["cidx"], og.cidx)
+
+
 def test_graph_from_edges_keeps_input_order(N, O):
     rng = np.random.default_rng(3)
     n = 300

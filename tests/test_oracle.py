@@ -84,6 +84,31 @@ def test_rmat_symmetric(O):
     assert np.array_equal(g.off, g.coff) and np.array_equal(g.idx, g.cidx)
 
 
+@pytest.mark.parametrize("S", [1, 2, 3, 7, 12, 16])
+def test_rmat_scramble_is_a_bijection(O, S):
+    x = np.arange(1 << S, dtype=np.uint32)
+    y = O.rmat_scramble(S, 6, x)
+    assert np.array_equal(np.sort(y), x)
+    if S >= 7:
+        assert np.mean(y == x) < 0.05                              # it moves almost every label
+        assert not np.array_equal(O.rmat_scramble(S, 7, x), y)    # and depends on the seed
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_rmat_scrambled_graph_is_the_relabelled_graph(O, sym):
+    """The scrambled graph holds exactly the edges (p(u), p(v)) of the plain one."""
+    S = 11
+    g = O.rmat_graph(S, 8, seed=3, sym=sym)
+    h = O.rmat_graph(S, 8, seed=3, sym=sym, scramble=True)
+    src = np.repeat(np.arange(g.nV, dtype=np.uint32), g.outdeg)
+    pu, pv = O.rmat_scramble(S, 3, src), O.rmat_scramble(S, 3, g.idx)
+    want = np.unique((pu.astype(np.uint64) << np.uint64(32)) | pv.astype(np.uint64))
+    hsrc = np.repeat(np.arange(h.nV, dtype=np.uint64), h.outdeg)
+    got = (hsrc << np.uint64(32)) | h.idx.astype(np.uint64)
+    assert np.array_equal(got, want)
+    assert int(np.argmax(h.outdeg)) != int(np.argmax(g.outdeg))   # the hub left vertex 0
+
+
 def test_rmat_fixture(O):
     import os
     f = os.path.join(os.path.dirname(__file__), "golden", "rmat_s10_ef8_seed1.npz")
