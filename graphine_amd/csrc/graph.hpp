@@ -27,6 +27,12 @@ struct XPart {
   DevBuf pair_off;   // uint32 [nV+1] segments of row v: pair_list[pair_off[v] .. pair_off[v+1])
   DevBuf pair_list;  // uint32 [nseg] segment ids, by row then class-major range
   uint64_t class_start[9] = {0};  // first entry of each class; class_start[8] = E
+  // Rows relabelled (CSC partitions for the PageRank forward): partition row r
+  // is vertex rowinv[r], rowperm[v] = r, so hub rows do not cluster in the
+  // first tiles; outdeg_r / cid_r hold the vertices' out-degree and compact id
+  // in row order.  Empty when rows are the vertices themselves.
+  bool permuted = false;
+  DevBuf rowperm, rowinv, outdeg_r, cid_r;
   int K = 0;         // (class, range) keys: 8 * (1 + R/8)
   DevBuf kseg;       // uint32 [K+1] first segment of each key; kseg[K] = nseg
   struct Tiles {
@@ -95,7 +101,8 @@ struct gre_graph {
   gre::DevBuf did, idx_c;
   uint32_t ntgt = 0;
   // source-range partitions, by (tail ranges, shared sources) (built on demand, shared with
-  // handles); over the CSC by (R, H), over the CSR (compact targets) by (-R, H)
+  // handles); over the CSC by (R, H), with relabelled rows by (R | 1024, H), over the CSR (compact
+  // targets) by (-R, H)
   std::map<std::pair<int, uint32_t>, std::shared_ptr<gre::XPart>> xpart;
   // engine pull row classes by direction (built on demand)
   std::map<int, gre::PullRows> pull_rows;
@@ -126,7 +133,9 @@ void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint3
 // The source partition of g: the H hottest compact sources as one shared range
 // and the rest in R tail ranges (R in {8, 16, ..., 256}); built once and cached.
 // csc = false partitions the CSR the same way, over compact target ids.
-std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = true);
+// rows_permuted (csc only): the partition's rows are the vertices relabelled by
+// a fixed bijection (XPart::permuted); cached apart from the plain one.
+std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = true, bool rows_permuted = false);
 // Drops the cached partitions that no PageRank handle or engine holds any more.
 void prune_xparts(gre_graph* g);
 // Its tiles of te entries (cached on the partition).

@@ -196,16 +196,25 @@ struct ModeRowSum {
   __device__ void finalize(uint32_t r, double s, double&, double&) const { rowsum[r] = s; }
 };
 
+// rowperm (relabelled partition rows): x, and p_r when personalised, are
+// written in the partition's row order, x[rowperm[v]]
 template <typename TT>
 __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t* __restrict__ outdeg,
                            TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part,
-                           const uint32_t* __restrict__ cid, bool craw) {
+                           const uint32_t* __restrict__ cid, bool craw, const uint32_t* __restrict__ rowperm,
+                           TT* __restrict__ p_r) {
   __shared__ double red[8];
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const TT xv = p ? p[v] : (TT)invn;
-    x[v] = xv;
+    if (rowperm) {
+      const uint32_t r = rowperm[v];
+      x[r] = xv;
+      if (p) p_r[r] = xv;
+    } else {
+      x[v] = xv;
+    }
     const int32_t d = outdeg[v];
     if (contrib) {
       if (cid) { if (d > 0) contrib[cid[v]] = craw ? xv : xv / (TT)d; }
@@ -215,6 +224,14 @@ __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t*
   }
   block_sum2(a0, a1, red);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+// the ranks back from the partition's row order: out[v] = in[rowperm[v]]
+template <typename TT>
+__global__ void k_rows_to_vertices(uint32_t nV, const uint32_t* __restrict__ rowperm, const TT* __restrict__ in,
+                                   TT* __restrict__ out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = in[rowperm[v]];
 }
 
 template <typename TT>
@@ -337,6 +354,9 @@ void ensure_scal(gre_pr* P, int iters) {
 // order (compact) or in the partition's entry order (part).
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
+// the partitioned forward walks relabelled rows: a run keeps its ranks in the
+// partition's row order until fwd_finish
+bool rows_relabelled(const gre_pr* P) { return use_part(P) && P->xp && P->xp->permuted; }
 
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
@@ -346,10 +366,15 @@ void ensure_part_plan(gre_pr* P, int ipt) {
     P->xp.reset();
     P->wpart_xp.reset();
     P->wpart_ver = ~0ULL;
-    P->xp = graph_xpart(P->g, P->ranges, P->shared);
+    P->xp = graph_xpart(P->g, P->ranges, P->shared, true, true);
     if (!P->tuning) prune_xparts(P->g);
     P->plan_ipt_x = 0;
   }
+  // a run keeps its ranks (and the personalisation) in the partition's row
+  // order and turns them back into the other rank buffer at its end
+  const size_t xb = tsize(P->dtype) * (size_t)P->g->nV;
+  if (P->x[1].bytes < xb) P->x[1].alloc(xb);
+  if (P->p_r.bytes < xb) P->p_r.alloc(xb);
   if (P->plan_ipt_x != ipt) {
     P->buf_x.make(P->plan_x, *P->xp, ipt, tsize(P->dtype));
     // the per-row pass always uses 8 entries per thread
@@ -469,6 +494,12 @@ ModeFwd<TT> fwd_mode(gre_pr* P, double damp) {
   m.p = P->has_p ? P->p.as<TT>() : nullptr;
   m.outdeg = P->g->outdeg.as<int32_t>();
   m.cid = use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr;
+  if (rows_relabelled(P)) {
+    // the row pass walks the partition's rows: their data in row order
+    m.p = P->has_p ? P->p_r.as<TT>() : nullptr;
+    m.outdeg = P->xp->outdeg_r.as<int32_t>();
+    m.cid = P->xp->cid_r.as<uint32_t>();
+  }
   m.damp = damp;
   m.invn = 1.0 / (double)P->g->nV;
   return m;
@@ -511,9 +542,23 @@ void fwd_init(gre_pr* P, hipStream_t s, double* tmp, double* scal) {
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
                      P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(),
                      (P->weighted && !use_compact(P)) ? nullptr : P->contrib[0].as<TT>(), 1.0 / (double)nV, tmp,
-                     use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted);
+                     use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted,
+                     rows_relabelled(P) ? P->xp->rowperm.as<uint32_t>() : nullptr,
+                     rows_relabelled(P) ? P->p_r.as<TT>() : nullptr);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, scal);
   GRE_HIP(hipGetLastError());
+}
+
+// The end of a forward run: over relabelled partition rows the ranks are in
+// row order, and go back to vertex order in the other rank buffer.
+template <typename TT>
+void fwd_finish(gre_pr* P, hipStream_t s) {
+  if (!rows_relabelled(P)) return;
+  const uint32_t nV = P->g->nV;
+  hipLaunchKernelGGL(k_rows_to_vertices<TT>, dim3(gcap(nV)), dim3(kB), 0, s, nV, P->xp->rowperm.as<uint32_t>(),
+                     P->x[P->xcur].as<TT>(), P->x[P->xcur ^ 1].as<TT>());
+  GRE_HIP(hipGetLastError());
+  P->xcur ^= 1;
 }
 
 // Everything a captured forward run bakes into its kernels' arguments and
@@ -546,7 +591,11 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters, con
     k.push_back((uint64_t)P->xp->R);
     k.push_back(P->xp->H);
     k.push_back(P->xp->nseg);
+    k.push_back(ptr(P->xp->rowperm.p));
+    k.push_back(ptr(P->xp->outdeg_r.p));
+    k.push_back(ptr(P->xp->cid_r.p));
   }
+  k.push_back(ptr(P->p_r.p));
   return k;
 }
 
@@ -570,6 +619,7 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
       fwd_init<TT>(P, s, tmp, sc);
       // only the last iteration's L1 change is returned
       for (int it = 0; it < iters; ++it) fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s, it + 1 == iters);
+      fwd_finish<TT>(P, s);
       GRE_HIP(hipStreamEndCapture(s, &P->fwd_graph));
       GRE_HIP(hipGraphInstantiate(&P->fwd_exec, P->fwd_graph, nullptr, nullptr, 0));
       P->fwd_key = key;
@@ -609,6 +659,7 @@ double forward_eager(gre_pr* P, double damp, int iters, double tol, int* done, d
       if (res < tol) { ++it; break; }
     }
   }
+  fwd_finish<TT>(P, s);
   if (done) *done = it;
   if (!sync && tol <= 0.0) return 0.0;
   if (tol <= 0.0 && iters > 0) {
@@ -799,7 +850,7 @@ void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* g
   DevBuf dummy;
   dummy.alloc(sizeof(TT) * (size_t)g->nV);
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, g->nV, x, g->outdeg.as<int32_t>(), dummy.as<TT>(),
-                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr, false);
+                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr, false, (const uint32_t*)nullptr, (TT*)nullptr);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dsc);
   if (gp_host) {
     hipLaunchKernelGGL(k_grad_p<TT>, dim3(gb), dim3(kB), 0, s, g->nV, lam, dsc, damp, dummy.as<TT>());
@@ -886,6 +937,8 @@ bool fwd_plan_ready(const gre_pr* P) {
     if (!P->xp || P->xp->R != P->ranges || P->xp->H != std::min(P->shared, P->g->nsrc)) return false;
     if (P->plan_ipt_x != P->ipt) return false;
     if (P->xsum.bytes < tsize(P->dtype) * (size_t)P->xp->nseg) return false;
+    if (P->x[1].bytes < tsize(P->dtype) * (size_t)P->g->nV || P->p_r.bytes < tsize(P->dtype) * (size_t)P->g->nV)
+      return false;
     if (P->weighted && (P->wpart_xp != P->xp || P->wpart_ver != P->wver)) return false;
     return true;
   }

@@ -61,6 +61,7 @@ struct gre_pr {
   int xcur = 0;           // x[xcur] holds the current ranks (always 0 when unweighted, k == 1)
   int ccur = 0;           // contrib[ccur] holds the current x/outdeg (unweighted, k == 1)
   gre::DevBuf p;          // personalisation (nV*k), optional
+  gre::DevBuf p_r;        // the personalisation in the partition's row order (relabelled rows)
   gre::DevBuf wraw, wcsr, wcsc, rowsum;  // weights (CSR raw, CSR norm, CSC norm), FP64 row sums
   // adjoint
   gre::DevBuf lam[2], c, wl;
