@@ -196,25 +196,16 @@ struct ModeRowSum {
   __device__ void finalize(uint32_t r, double s, double&, double&) const { rowsum[r] = s; }
 };
 
-// rowperm (relabelled partition rows): x, and p_r when personalised, are
-// written in the partition's row order, x[rowperm[v]]
 template <typename TT>
 __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t* __restrict__ outdeg,
                            TT* __restrict__ x, TT* __restrict__ contrib, double invn, double* __restrict__ part,
-                           const uint32_t* __restrict__ cid, bool craw, const uint32_t* __restrict__ rowperm,
-                           TT* __restrict__ p_r) {
+                           const uint32_t* __restrict__ cid, bool craw) {
   __shared__ double red[8];
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const TT xv = p ? p[v] : (TT)invn;
-    if (rowperm) {
-      const uint32_t r = rowperm[v];
-      x[r] = xv;
-      if (p) p_r[r] = xv;
-    } else {
-      x[v] = xv;
-    }
+    x[v] = xv;
     const int32_t d = outdeg[v];
     if (contrib) {
       if (cid) { if (d > 0) contrib[cid[v]] = craw ? xv : xv / (TT)d; }
@@ -224,6 +215,34 @@ __global__ void k_fwd_init(uint32_t nV, const TT* __restrict__ p, const int32_t*
   }
   block_sum2(a0, a1, red);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+// The same over relabelled partition rows: everything in row order (p_r,
+// outdeg_r, cid_r; x written at the row), contrib by compact id as before.
+template <typename TT>
+__global__ void k_fwd_init_rows(uint32_t nV, const TT* __restrict__ p_r, const int32_t* __restrict__ outdeg_r,
+                                TT* __restrict__ x, TT* __restrict__ contrib, double invn,
+                                double* __restrict__ part, const uint32_t* __restrict__ cid_r, bool craw) {
+  __shared__ double red[8];
+  double a0 = 0.0, a1 = 0.0;
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < nV;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const TT xv = p_r ? p_r[r] : (TT)invn;
+    x[r] = xv;
+    const int32_t d = outdeg_r[r];
+    if (d > 0) contrib[cid_r[r]] = craw ? xv : xv / (TT)d;
+    if (d == 0) a0 += (double)xv;
+  }
+  block_sum2(a0, a1, red);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+// a vertex-order array into the partition's row order: out[rowperm[v]] = in[v]
+template <typename TT>
+__global__ void k_vertices_to_rows(uint32_t nV, const uint32_t* __restrict__ rowperm, const TT* __restrict__ in,
+                                   TT* __restrict__ out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x)
+    out[rowperm[v]] = in[v];
 }
 
 // the ranks back from the partition's row order: out[v] = in[rowperm[v]]
@@ -355,7 +374,7 @@ void ensure_scal(gre_pr* P, int iters) {
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
 // the partitioned forward walks relabelled rows: a run keeps its ranks in the
-// partition's row order until fwd_finish
+// partition's row order until ranks_to_vertices
 bool rows_relabelled(const gre_pr* P) { return use_part(P) && P->xp && P->xp->permuted; }
 
 // the source-range partition, its segment plan and the segment sums
@@ -539,26 +558,54 @@ void fwd_init(gre_pr* P, hipStream_t s, double* tmp, double* scal) {
   const unsigned gb = gcap(nV);
   P->xcur = 0;
   P->ccur = 0;
-  hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
-                     P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(),
-                     (P->weighted && !use_compact(P)) ? nullptr : P->contrib[0].as<TT>(), 1.0 / (double)nV, tmp,
-                     use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted,
-                     rows_relabelled(P) ? P->xp->rowperm.as<uint32_t>() : nullptr,
-                     rows_relabelled(P) ? P->p_r.as<TT>() : nullptr);
+  if (rows_relabelled(P))
+    hipLaunchKernelGGL(k_fwd_init_rows<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p_r.as<TT>() : nullptr,
+                       P->xp->outdeg_r.as<int32_t>(), P->x[0].as<TT>(), P->contrib[0].as<TT>(), 1.0 / (double)nV,
+                       tmp, P->xp->cid_r.as<uint32_t>(), P->weighted);
+  else
+    hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, nV, P->has_p ? P->p.as<TT>() : nullptr,
+                       P->g->outdeg.as<int32_t>(), P->x[0].as<TT>(),
+                       (P->weighted && !use_compact(P)) ? nullptr : P->contrib[0].as<TT>(), 1.0 / (double)nV, tmp,
+                       use_compact(P) ? P->g->cid.as<uint32_t>() : nullptr, P->weighted);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, scal);
   GRE_HIP(hipGetLastError());
 }
 
-// The end of a forward run: over relabelled partition rows the ranks are in
-// row order, and go back to vertex order in the other rank buffer.
+// Before a run over relabelled partition rows: the personalisation in row
+// order, if it changed since (outside any captured graph).
 template <typename TT>
-void fwd_finish(gre_pr* P, hipStream_t s) {
-  if (!rows_relabelled(P)) return;
+void fwd_prepare(gre_pr* P, hipStream_t s) {
+  if (!rows_relabelled(P) || !P->has_p || (P->p_r_ver == P->pver && P->p_r_xp == P->xp)) return;
   const uint32_t nV = P->g->nV;
-  hipLaunchKernelGGL(k_rows_to_vertices<TT>, dim3(gcap(nV)), dim3(kB), 0, s, nV, P->xp->rowperm.as<uint32_t>(),
+  hipLaunchKernelGGL(k_vertices_to_rows<TT>, dim3(gcap(nV)), dim3(kB), 0, s, nV, P->xp->rowperm.as<uint32_t>(),
+                     P->p.as<TT>(), P->p_r.as<TT>());
+  GRE_HIP(hipGetLastError());
+  P->p_r_ver = P->pver;
+  P->p_r_xp = P->xp;
+}
+
+// After a run: over relabelled partition rows the ranks are in row order
+// until something reads them (ranks_to_vertices).
+void fwd_done(gre_pr* P) {
+  P->x_rows = rows_relabelled(P);
+  P->rows_xp = P->x_rows ? P->xp : nullptr;
+}
+
+// The ranks back in vertex order, in the other rank buffer.
+template <typename TT>
+void ranks_to_vertices_t(gre_pr* P, hipStream_t s) {
+  const uint32_t nV = P->g->nV;
+  hipLaunchKernelGGL(k_rows_to_vertices<TT>, dim3(gcap(nV)), dim3(kB), 0, s, nV, P->rows_xp->rowperm.as<uint32_t>(),
                      P->x[P->xcur].as<TT>(), P->x[P->xcur ^ 1].as<TT>());
   GRE_HIP(hipGetLastError());
+}
+void ranks_to_vertices(gre_pr* P) {
+  if (!P->x_rows) return;
+  if (P->dtype == GRE_F64) ranks_to_vertices_t<double>(P, stream());
+  else ranks_to_vertices_t<float>(P, stream());
   P->xcur ^= 1;
+  P->x_rows = false;
+  P->rows_xp.reset();
 }
 
 // Everything a captured forward run bakes into its kernels' arguments and
@@ -611,6 +658,7 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
     // the scratch is sized (and so placed) before the key is taken
     double* tmp = static_cast<double*>(scratch(3, fwd_init_tmp_bytes(P->g->nV)));
     std::vector<uint64_t> key = fwd_graph_key(P, damp, iters, tmp);
+    fwd_prepare<TT>(P, s);
     if (!P->fwd_exec || key != P->fwd_key) {
       if (P->fwd_exec) { (void)hipGraphExecDestroy(P->fwd_exec); P->fwd_exec = nullptr; }
       if (P->fwd_graph) { (void)hipGraphDestroy(P->fwd_graph); P->fwd_graph = nullptr; }
@@ -619,7 +667,6 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
       fwd_init<TT>(P, s, tmp, sc);
       // only the last iteration's L1 change is returned
       for (int it = 0; it < iters; ++it) fwd_iter<TT>(P, damp, sc + 2 * it, sc + 2 * (it + 1), s, it + 1 == iters);
-      fwd_finish<TT>(P, s);
       GRE_HIP(hipStreamEndCapture(s, &P->fwd_graph));
       GRE_HIP(hipGraphInstantiate(&P->fwd_exec, P->fwd_graph, nullptr, nullptr, 0));
       P->fwd_key = key;
@@ -629,6 +676,7 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
     GRE_HIP(hipGraphLaunch(P->fwd_exec, s));
     P->xcur = P->fwd_post_xcur;
     P->ccur = P->fwd_post_ccur;
+    fwd_done(P);
     double h[2];
     GRE_HIP(hipMemcpyAsync(h, P->scal.as<double>() + 2 * iters, sizeof(h), hipMemcpyDeviceToHost, s));
     GRE_HIP(hipStreamSynchronize(s));
@@ -646,6 +694,7 @@ double run_forward(gre_pr* P, double damp, int iters, double tol, int* done) {
 template <typename TT>
 double forward_eager(gre_pr* P, double damp, int iters, double tol, int* done, double* tmp, double* sc, bool sync) {
   hipStream_t s = stream();
+  fwd_prepare<TT>(P, s);
   fwd_init<TT>(P, s, tmp, sc);
   int it = 0;
   double res = 0.0;
@@ -659,7 +708,7 @@ double forward_eager(gre_pr* P, double damp, int iters, double tol, int* done, d
       if (res < tol) { ++it; break; }
     }
   }
-  fwd_finish<TT>(P, s);
+  fwd_done(P);
   if (done) *done = it;
   if (!sync && tol <= 0.0) return 0.0;
   if (tol <= 0.0 && iters > 0) {
@@ -709,7 +758,8 @@ void autotune_forward(gre_pr* P, double damp) {
     explicit Tuning(gre_pr* p) : P(p) { P->tuning = true; }
     ~Tuning() { P->tuning = false; }
   } tuning_scope(P);
-  auto trial = [&](int sc, int ipt, bool prt, const std::pair<int, uint32_t>& h) {
+  // one candidate: a warm iteration, then `timed` timed ones; returns their ms
+  auto time_it = [&](int sc, int ipt, bool prt, const std::pair<int, uint32_t>& h, int timed) {
     apply_schedule(P, sc, ipt);
     if (prt) { P->ranges = h.first; P->shared = h.second; }
     ensure_plan(P, true, ipt);
@@ -717,38 +767,78 @@ void autotune_forward(gre_pr* P, double damp) {
     fwd_init<TT>(P, s, static_cast<double*>(scratch(3, fwd_init_tmp_bytes(P->g->nV))), scal);
     fwd_iter<TT>(P, damp, scal, scal + 2, s);  // warm
     GRE_HIP(hipEventRecord(a, s));
-    fwd_iter<TT>(P, damp, scal + 2, scal + 4, s);
-    fwd_iter<TT>(P, damp, scal + 4, scal + 6, s);
+    for (int i = 0; i < timed; ++i) fwd_iter<TT>(P, damp, scal + 2 + 2 * (i & 1), scal + 4 - 2 * (i & 1), s);
     GRE_HIP(hipEventRecord(b, s));
     GRE_HIP(hipEventSynchronize(b));
     float ms = 0.f;
     GRE_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  };
+  auto trial = [&](int sc, int ipt, bool prt, const std::pair<int, uint32_t>& h) {
+    const float ms = time_it(sc, ipt, prt, h, 2);
     if (ms < best_ms) {
       best_ms = ms;
       best_s = sc;
       best_i = ipt;
       if (prt) { best_r = h.first; best_h = h.second; }
     }
+    return ms;
   };
   for (int sc : scheds)
     for (int ipt : ipts) trial(sc, ipt, false, {P->ranges, P->shared});
   // partitioned schedules try (tail ranges, shared sources) pairs, one
-  // partition at a time: only the best one so far stays cached beside it
-  std::shared_ptr<XPart> best_xp;
+  // partition at a time; the two best partitions (each with its best
+  // schedule) stay cached and are timed again against each other at the end,
+  // alternately, since a few percent separate them and two iterations right
+  // after a partition build do not always rank them right
+  struct PartCand {
+    float ms = 1e30f;
+    int sc = 0, ipt = 0;
+    std::pair<int, uint32_t> h{0, 0};
+    std::shared_ptr<XPart> xp;
+  } top[2];
   if (can_part) {
     const std::pair<int, uint32_t> vars[5] = {{256, 1u << 12}, {128, 1u << 12}, {256, 0u}, {128, 1u << 17}, {64, 1u << 18}};
     for (const auto& h : vars) {
+      PartCand c;
+      c.h = h;
       for (int sc : {GRE_SCHED_PART, GRE_SCHED_PART_NT})
-        for (int ipt : {4, 8}) trial(sc, ipt, true, h);
-      if ((best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT) && best_r == h.first && best_h == h.second)
-        best_xp = P->xp;
+        for (int ipt : {4, 8}) {
+          const float ms = trial(sc, ipt, true, h);
+          if (ms < c.ms) { c.ms = ms; c.sc = sc; c.ipt = ipt; }
+        }
+      c.xp = P->xp;
+      if (c.ms < top[0].ms) { top[1] = std::move(top[0]); top[0] = std::move(c); }
+      else if (c.ms < top[1].ms) top[1] = std::move(c);
       P->xp.reset();
       P->wpart_xp.reset();
       P->wpart_ver = ~0ULL;
       P->plan_ipt_x = 0;
       prune_xparts(P->g);
     }
+    const bool part_won = best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT;
+    if (part_won && top[0].xp && top[1].xp) {
+      float tot[2] = {0.f, 0.f};
+      for (int round = 0; round < 3; ++round)
+        for (int k = 0; k < 2; ++k) {
+          P->xp = top[k].xp;
+          P->wpart_xp.reset();
+          P->wpart_ver = ~0ULL;
+          P->plan_ipt_x = 0;
+          tot[k] += time_it(top[k].sc, top[k].ipt, true, top[k].h, 4);
+        }
+      if (tot[1] < tot[0]) std::swap(top[0], top[1]);
+    }
+    if (part_won && top[0].xp) {
+      best_s = top[0].sc;
+      best_i = top[0].ipt;
+      best_r = top[0].h.first;
+      best_h = top[0].h.second;
+    }
   }
+  std::shared_ptr<XPart> best_xp =
+      (best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT) ? top[0].xp : std::shared_ptr<XPart>();
+  top[1].xp.reset();
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   apply_schedule(P, best_s, best_i);
@@ -841,6 +931,7 @@ template <typename TT>
 void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* gp_host) {
   hipStream_t s = stream();
   gre_graph* g = P->g;
+  ranks_to_vertices(P);
   const TT* x = P->x[P->xcur].as<TT>();
   const TT* lam = P->lam[P->lcur].as<TT>();
   // dangling mass of the final forward, recomputed in fixed order
@@ -850,7 +941,7 @@ void gradients(gre_pr* P, double damp, float* gW_host, float* graw_host, void* g
   DevBuf dummy;
   dummy.alloc(sizeof(TT) * (size_t)g->nV);
   hipLaunchKernelGGL(k_fwd_init<TT>, dim3(gb), dim3(kB), 0, s, g->nV, x, g->outdeg.as<int32_t>(), dummy.as<TT>(),
-                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr, false, (const uint32_t*)nullptr, (TT*)nullptr);
+                     (TT*)nullptr, 0.0, tmp, (const uint32_t*)nullptr, false);
   hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, dsc);
   if (gp_host) {
     hipLaunchKernelGGL(k_grad_p<TT>, dim3(gb), dim3(kB), 0, s, g->nV, lam, dsc, damp, dummy.as<TT>());
@@ -1126,6 +1217,7 @@ int gre_pr_set_personalization(gre_pr* P, const void* pvec) {
     GRE_HIP(hipMemcpyAsync(P->p.p, pvec, bytes, hipMemcpyHostToDevice, stream()));
     GRE_HIP(hipStreamSynchronize(stream()));
     P->has_p = true;
+    ++P->pver;
   });
 }
 
@@ -1182,6 +1274,7 @@ int gre_pr_get_ranks(gre_pr* P, void* host_out) {
   return guard([&] {
     GRE_CHECK(P && host_out, GRE_DEV_NULL, "null argument");
     const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
+    ranks_to_vertices(P);
     GRE_HIP(hipMemcpyAsync(host_out, P->x[P->xcur].p, bytes, hipMemcpyDeviceToHost, stream()));
     GRE_HIP(hipStreamSynchronize(stream()));
   });

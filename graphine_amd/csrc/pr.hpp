@@ -62,6 +62,14 @@ struct gre_pr {
   int ccur = 0;           // contrib[ccur] holds the current x/outdeg (unweighted, k == 1)
   gre::DevBuf p;          // personalisation (nV*k), optional
   gre::DevBuf p_r;        // the personalisation in the partition's row order (relabelled rows)
+  uint64_t pver = 0;      // bumped by every set_personalization
+  uint64_t p_r_ver = ~0ULL;                 // the p version p_r holds ...
+  std::shared_ptr<gre::XPart> p_r_xp;       // ... in this partition's row order
+  // A run over relabelled partition rows leaves its ranks in x[xcur] in the
+  // partition's row order (rows_xp: that partition); whatever reads the ranks
+  // first turns them back into vertex order.
+  bool x_rows = false;
+  std::shared_ptr<gre::XPart> rows_xp;
   gre::DevBuf wraw, wcsr, wcsc, rowsum;  // weights (CSR raw, CSR norm, CSC norm), FP64 row sums
   // adjoint
   gre::DevBuf lam[2], c, wl;
