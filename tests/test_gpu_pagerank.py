@@ -401,3 +401,54 @@ def test_batched_graph_replay_matches_eager(N, O, sched):
         assert np.array_equal(xe, xg)
         assert np.array_equal(le, lg)
     assert np.array_equal(out[True][0][0], out[True][1][0])
+
+
+def test_partitioned_rows_relabelled_between_runs(N, O):
+    """The partitioned forward walks its rows in a relabelled order and leaves
+    the ranks in that order until they are read: reading them twice gives the
+    same vertex-order ranks, a new personalisation reaches the next run, and
+    the gradients read after a partitioned run equal those after a tile run."""
+    scale, d = 13, 0.85
+    g = N.Graph.rmat(scale, 16, 5, False)
+    og = O.rmat_graph(scale, 16, 5, False)
+    part = N.PageRank(g, N.F64, 1, weighted=True)
+    part.set_schedule(6, 8)
+    part.set_partition(16, 1 << 10)
+    tile = N.PageRank(g, N.F64, 1, weighted=True)
+    tile.set_schedule(1, 8)
+    for h in (part, tile):
+        h.set_edge_weights(None, seed=5)
+    for seed in (5, 7):
+        p = ppr_problem(O, og, seed=seed)
+        for h in (part, tile):
+            h.set_personalization(p)
+            h.set_graphs(seed == 7)
+            h.run(d, 60)
+        x1 = part.ranks()
+        assert np.array_equal(x1, part.ranks())
+        assert rel_l1(x1, tile.ranks()) < 1e-12
+    tg = np.random.default_rng(2).choice(og.nV, 32, replace=False).astype(np.uint32)
+    for h in (part, tile):
+        h.adjoint(tg, d, 60)
+        h.run(d, 60)               # a fresh forward: its ranks are read by the gradients
+    gW, graw, gp = part.gradients(d)
+    rW, rraw, rp = tile.gradients(d)
+    assert rel_l1(gW, rW) < 1e-9 and rel_l1(graw, rraw) < 1e-9 and rel_l1(gp, rp) < 1e-9
+
+
+def test_partitioned_unweighted_f32_personalisation_changes(N, O):
+    scale = 14
+    g = N.Graph.rmat(scale, 16, 3, False)
+    og = O.rmat_graph(scale, 16, 3, False)
+    pr = N.PageRank(g, N.F32)
+    pr.set_schedule(6, 8)
+    pr.set_partition(32, 1 << 8)
+    for seed, graphs in ((3, True), (4, True), (5, False)):
+        p = ppr_problem(O, og, seed=seed)
+        pr.set_personalization(p)
+        pr.set_graphs(graphs)
+        pr.run(0.85, 30)
+        src = np.repeat(np.arange(og.nV), og.outdeg)
+        wc = (1.0 / og.outdeg[src].astype(np.float64))[og.ceid.astype(np.int64)]
+        xref, _ = O.ppr(og, wc, p, 0.85, tol=0.0, maxit=30)
+        assert rel_l1(pr.ranks(), xref) < 1e-6, seed
