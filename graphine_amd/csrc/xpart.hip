@@ -314,27 +314,46 @@ __global__ void k_xstage(const uint32_t* __restrict__ runA, const uint16_t* __re
   }
 }
 
-// The row relabelling: r = (a * v) mod nV with a coprime to nV and near
-// 0.618 nV, a bijection that sends neighbouring ids far apart (R-MAT's hub
-// rows are the low ids; spread over the tiles, they balance the segment and
-// row passes).  Measured against a mixing bijection (add / odd multiply /
-// xor-shift, walked into range) and against rows in compact source order:
-// 5.02 ms per iteration against 5.06 and 5.21 (5.13 unrelabelled).
-__global__ void k_rowperm(uint32_t nV, uint64_t a, uint32_t* __restrict__ perm, uint32_t* __restrict__ inv) {
+// The row relabelling: rows with more than thr in-edges (the heavy rows:
+// R-MAT puts them at the low ids) are spread evenly among the others, which
+// keep their order.  Heavy row j goes before light row ins(j) = floor((2j+1)
+// L / 2H) (L light and H heavy rows), so light row l lands at l + #{j :
+// ins(j) <= l} and heavy row j at ins(j) + j: a merge, so a bijection.  The
+// light rows keep their order, so the row pass's writes of x / out-degree by
+// compact source id keep the locality they had; the heavy rows no longer fill
+// the first tiles.  At R-MAT scale 26 (part_nt, 256 ranges, 4,096 shared):
+// 5.13 ms per iteration unrelabelled, 4.73 with heavy = more than 8 times the
+// mean in-degree (4.77 at 2 times, 4.78 at 32, 4.74 at 128); a multiplicative
+// bijection of all rows gave 5.03 (its scattered compact-id writes cost the
+// row pass 2.6 times the write bytes).
+constexpr uint64_t kHeavy = 8;
+__global__ void k_heavy(uint32_t nV, const uint64_t* __restrict__ coff, uint64_t thr, uint64_t* __restrict__ h) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x)
+    h[v] = coff[v + 1] - coff[v] > thr ? 1 : 0;
+}
+__global__ void k_rowperm(uint32_t nV, const uint64_t* __restrict__ h, const uint64_t* __restrict__ hrank,
+                          uint64_t H, uint32_t* __restrict__ perm, uint32_t* __restrict__ inv) {
+  const uint64_t L = nV - H;
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = (uint32_t)((a * v) % nV);
-    perm[v] = r;
+    uint64_t r = v;
+    if (H > 0 && L > 0) {
+      const uint64_t j = hrank[v];
+      if (h[v]) {
+        r = (2 * j + 1) * L / (2 * H) + j;
+      } else {
+        const uint64_t l = v - j;
+        const uint64_t n = 2 * H * (l + 1);
+        uint64_t before = 0;
+        if (n > L) {
+          before = (n - L + 2 * L - 1) / (2 * L);
+          if (before > H) before = H;
+        }
+        r = l + before;
+      }
+    }
+    perm[v] = (uint32_t)r;
     inv[r] = (uint32_t)v;
   }
-}
-
-uint64_t gcd_u64(uint64_t a, uint64_t b) {
-  while (b) {
-    const uint64_t t = a % b;
-    a = b;
-    b = t;
-  }
-  return a;
 }
 
 // cnt[r] = in-degree of row r's vertex; with the forward's row data in row order
@@ -420,10 +439,6 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc, bo
   DevBuf coff2, cidx2, orig, rb2;
   if (rows_permuted) {
     xp->permuted = true;
-    // a = an odd number near nV * 0.618, coprime to nV (a < 2^32, so a * v
-    // fits 64 bits)
-    uint64_t a = (uint64_t)((double)nV * 0.6180339887498949) | 1ULL;
-    while (gcd_u64(a, nV) != 1) a += 2;
     xp->rowperm.alloc(4 * (size_t)nV);
     xp->rowinv.alloc(4 * (size_t)nV);
     xp->outdeg_r.alloc(4 * (size_t)nV);
@@ -431,9 +446,23 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc, bo
     coff2.alloc(8 * ((size_t)nV + 1));
     cidx2.alloc(4 * E);
     orig.alloc(4 * E);
-    hipLaunchKernelGGL(k_rowperm, dim3(gcap(nV)), dim3(kB), 0, s, nV, a % std::max<uint64_t>(nV, 1),
-                       xp->rowperm.as<uint32_t>(), xp->rowinv.as<uint32_t>());
-    GRE_HIP(hipGetLastError());
+    {
+      // heavy: more than kHeavy times the mean in-degree
+      const uint64_t thr = kHeavy * std::max<uint64_t>(E / std::max<uint32_t>(nV, 1), 1);
+      DevBuf h, hr;
+      h.alloc(8 * ((size_t)nV + 1));
+      hr.alloc(8 * ((size_t)nV + 1));
+      hipLaunchKernelGGL(k_heavy, dim3(gcap(nV)), dim3(kB), 0, s, nV, g->coff.as<uint64_t>(), thr, h.as<uint64_t>());
+      GRE_HIP(hipMemsetAsync(h.as<uint64_t>() + nV, 0, 8, s));
+      scan_excl_u64(h.as<uint64_t>(), hr.as<uint64_t>(), (size_t)nV + 1, s);
+      uint64_t H = 0;
+      GRE_HIP(hipMemcpyAsync(&H, hr.as<uint64_t>() + nV, 8, hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      hipLaunchKernelGGL(k_rowperm, dim3(gcap(nV)), dim3(kB), 0, s, nV, h.as<uint64_t>(), hr.as<uint64_t>(), H,
+                         xp->rowperm.as<uint32_t>(), xp->rowinv.as<uint32_t>());
+      GRE_HIP(hipGetLastError());
+      GRE_HIP(hipStreamSynchronize(s));
+    }
     {
       DevBuf cnt;
       cnt.alloc(8 * ((size_t)nV + 1));
