@@ -319,20 +319,28 @@ def main():
     n_fix, ms_fix = N.prof_get("pull_fixup")
     n_cmb, ms_cmb = N.prof_get("part_combine")
     n_cfx, ms_cfx = N.prof_get("part_combine_fixup")
+    n_bsc, ms_bsc = N.prof_get("bin_scatter")
+    n_bac, ms_bac = N.prof_get("bin_accum")
+    n_bfi, ms_bfi = N.prof_get("bin_finish")
     N.prof_enable(False)
     V, E = g.nV, g.nE
     o, s_ = 8, 4
     bytes_iter = o * (V + 1) + (4 + s_) * E + (3 * s_ + 4) * V
-    # one PageRank iteration = the pull tile kernel, its fix-up and (partitioned
-    # schedules) the per-row combine of the range partial sums
-    avg_tile_s = ms_tile / max(n_tile, 1) / 1e3
-    avg_fix_s = ms_fix / max(n_fix, 1) / 1e3
-    avg_cmb_s = (ms_cmb / max(n_cmb, 1) + ms_cfx / max(n_cfx, 1)) / 1e3 if n_cmb else 0.0
-    iter_s = avg_tile_s + avg_fix_s + avg_cmb_s
-    achieved = bytes_iter / iter_s / 1e9
     sched, tile_items = pr.schedule()
-    kern = ("k_pull_tile + k_pull_fixup (segments), then k_pull_staged + k_pull_fixup (per row)" if n_cmb
-            else "k_pull_tile + k_pull_fixup")
+    if n_bsc:
+        # binned: the scatter into bins, the accumulate (and row update), the finishing pass
+        stages = {"scatter_avg_ms": ms_bsc / n_bsc, "accumulate_avg_ms": ms_bac / max(n_bac, 1),
+                  "finish_avg_ms": ms_bfi / max(n_bfi, 1)}
+        kern = "k_bin_scatter, k_bin_accum, k_bin_fin"
+    else:
+        # one PageRank iteration = the pull tile kernel, its fix-up and (partitioned
+        # schedules) the per-row combine of the range partial sums
+        stages = {"pull_tile_avg_ms": ms_tile / max(n_tile, 1), "fixup_avg_ms": ms_fix / max(n_fix, 1),
+                  "combine_avg_ms": (ms_cmb / max(n_cmb, 1) + ms_cfx / max(n_cfx, 1)) if n_cmb else 0.0}
+        kern = ("k_pull_tile + k_pull_fixup (segments), then k_pull_staged + k_pull_fixup (per row)" if n_cmb
+                else "k_pull_tile + k_pull_fixup")
+    iter_s = sum(stages.values()) / 1e3
+    achieved = bytes_iter / iter_s / 1e9
     spec = f"{sched}:{tile_items}"
     if sched.startswith("part"):
         spec += ":{}:{}".format(*pr.partition())
@@ -341,14 +349,13 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_schedule": traffic_sched,
                 "traffic_schedule_is_measured_schedule": traffic_sched == spec,
                 "kernel": f"{kern} (one PageRank f32 iteration)",
-                "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4),
-                "pull_tile_avg_ms": round(avg_tile_s * 1e3, 4), "fixup_avg_ms": round(avg_fix_s * 1e3, 4),
-                "combine_avg_ms": round(avg_cmb_s * 1e3, 4)}
+                "alg_bytes_per_launch": bytes_iter, "avg_iteration_kernels_ms": round(iter_s * 1e3, 4)}
+    roofline.update({k: round(v, 4) for k, v in stages.items()})
     if traffic:
         # the profiled HBM bytes of an iteration over this run's iteration time
         roofline["traffic_gbs"] = round(traffic / iter_s / 1e9, 1)
-    log(f"iteration kernels {iter_s * 1e3:.3f} ms (tile {avg_tile_s * 1e3:.3f}, fixup {avg_fix_s * 1e3:.3f}, "
-        f"combine {avg_cmb_s * 1e3:.3f}) -> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
+    log(f"iteration kernels {iter_s * 1e3:.3f} ms ({', '.join(f'{k} {v:.3f}' for k, v in stages.items())}) "
+        f"-> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
     roofline["schedule"] = (f"{sched}, {tile_items} entries/thread"
                             + (", {} tail source ranges, {} shared sources".format(*pr.partition())
                                if sched.startswith("part") else "")
@@ -378,6 +385,7 @@ def main():
         # grouped by partition: the graph keeps the partition of the last closed handle cached
         cand += [(sc, ti, r, h) for r, h in ((256, 1 << 12), (128, 1 << 12), (256, 0), (128, 1 << 17), (64, 1 << 18))
                  for sc in (5, 6) for ti in (4, 8)]
+        cand += [(8, 8, 64, 0)]
         for sc, ti, r, h in cand:
             q = N.PageRank(g, N.F32)
             q.set_schedule(sc, ti)
@@ -388,11 +396,12 @@ def main():
             q.run(0.85, 10)
             N.sync()
             ms_ = 0.0
-            for kname in ("pull_tile", "pull_fixup", "part_combine", "part_combine_fixup"):
+            for kname in ("pull_tile", "pull_fixup", "part_combine", "part_combine_fixup", "bin_scatter",
+                          "bin_accum", "bin_finish"):
                 nt_, mt_ = N.prof_get(kname)
                 ms_ += mt_ / max(nt_, 1)
             N.prof_enable(False)
-            key = f"{N.PageRank.SCHEDULES[sc]}/{ti}" + (f"/R={r}/H={h}" if sc >= 5 else "")
+            key = f"{N.PageRank.SCHEDULES[sc]}" + (f"/{ti}" if sc != 8 else "") + (f"/R={r}/H={h}" if sc in (5, 6) else "")
             sweep[key] = round(ms_, 3)
             del q
         roofline["schedule_sweep_ms"] = sweep

@@ -1,0 +1,810 @@
+// Binned forward pull (propagation blocking) of the single-vector FP32
+// PageRank iteration, x' = d (W^T x + (g.x) p) + (1 - d) p with W_uv =
+// 1/outdeg(u): the same superstep as the tile pulls, with the random accesses
+// moved into LDS.
+//
+// The gathers of a pull cost one cache request per adjacency entry, which is
+// what bounds the tile and partitioned pulls (scattered 4-byte reads run at the
+// caches' request rate, not at HBM bandwidth).  Here every HBM access is a
+// stream:
+//   scatter:    chunk w (consecutive source rows, at most kChunkRows of them
+//               and kChunkEdges entries) stages its sources' rank/out-degree in
+//               LDS, then walks its entries in (bin, source) order, reading
+//               each value from LDS and writing it to the bin array, where
+//               bin b holds the runs (w, b) of every chunk in chunk order.  A
+//               run is ~70 consecutive floats, so the writes stream too.
+//   accumulate: each bin (at most kBinRows consecutive destination rows) is
+//               summed in LDS: 64-bit fixed-point atomics (exact, so the sum
+//               does not depend on the order of the additions), then each
+//               row's update.  A bin with more than kSliceEdges entries is
+//               cut into slices whose sums meet in 64-bit integers in HBM,
+//               finished by the last pass with the rows without in-edges.
+// Rows are relabelled so that bins have equal entry counts: the vertices with
+// in-edges ranked by in-degree are dealt to the bins in snake order (round k
+// left to right when k is even, right to left otherwise), a bin's rows being
+// consecutive; the vertices without in-edges follow, those with out-edges first
+// by descending out-degree.  Ranks, personalisation and out-degrees live in
+// that order during a run (the handle turns the ranks back on first read), and
+// contributions are indexed by row, so every row writes its rank and its
+// contribution in sequence.
+#include "binned.hpp"
+#include "pull.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <vector>
+
+namespace gre {
+namespace {
+
+constexpr unsigned kB = 256;
+constexpr int kSrcBits = 14;  // source row within a chunk: kChunkRows = 2^14
+static_assert(kChunkRows == (1 << kSrcBits), "chunk rows are 2^kSrcBits");
+static_assert(kBinMaxBins <= 65536 && kChunkRows <= 65536, "a scatter word holds 16-bit bin and row fields");
+static_assert(kBinRows <= 65536, "dloc is 16 bits");
+constexpr int kScatterBlock = 1024;
+constexpr int kScatterU = 8;     // entries in flight per thread
+constexpr int kAccumBlock = 1024;
+constexpr int kAccumU = 8;
+constexpr uint32_t kBinKey = 1u << 14;  // cache key of the layout among a graph's partitions
+
+unsigned gcap(uint64_t n) {
+  uint64_t g = (n + kB - 1) / kB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+uint64_t align64(uint64_t n) { return (n + 63) & ~63ULL; }
+
+unsigned bits_of(uint64_t n) {
+  unsigned b = 1;
+  while (b < 63 && (1ULL << b) < n) ++b;
+  return b;
+}
+
+// fixed-order sum of two doubles over a block of up to 1024 threads; thread 0
+// gets the totals
+__device__ inline void block_sum2_big(double& a, double& b, double* red /*[32]*/) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { red[w] = a; red[16 + w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0.0;
+    b = 0.0;
+    for (unsigned i = 0; i < (blockDim.x >> 6); ++i) { a += red[i]; b += red[16 + i]; }
+  }
+}
+
+// ---- iteration kernels ------------------------------------------------------
+
+// Row r's update from its in-edge sum s (the same formula as the tile pulls'
+// forward mode, contribution indexed by row).
+template <bool R>
+struct BinUpdate {
+  float* contrib_out;
+  float* x;
+  const float* p_r;
+  const int32_t* outdeg_r;
+  const double* scal_in;
+  float damp;
+  float omd;   // 1 - damp
+  float invn;
+  struct Pre {
+    float pv, old;
+    int32_t d;
+  };
+  __device__ Pre pre(uint32_t r) const {
+    Pre q;
+    q.pv = p_r ? p_r[r] : invn;
+    q.old = R ? x[r] : 0.f;
+    q.d = outdeg_r[r];
+    return q;
+  }
+  __device__ void fin(uint32_t r, float s, const Pre& q, double& a0, double& a1) const {
+    const float dang = (float)scal_in[0];
+    const float y = damp * (s + dang * q.pv) + omd * q.pv;
+    x[r] = y;
+    if (q.d > 0) contrib_out[r] = y / (float)q.d;
+    else a0 += (double)y;
+    if (R) a1 += fabs((double)y - (double)q.old);
+  }
+  __device__ void apply(uint32_t r, float s, double& a0, double& a1) const { fin(r, s, pre(r), a0, a1); }
+};
+
+__device__ inline float fix_to_float(unsigned long long v) {
+  return (float)((double)(long long)v * 0x1p-58);
+}
+static_assert(kBinFix == 58, "fix_to_float and the scatter scale use 2^58");
+
+// Scatter: workgroup b runs chunk (b % 8) * per + b / 8, so each XCD walks its
+// own contiguous stretch of chunks, and consecutive chunks (whose runs of one
+// bin are adjacent in the bin array) share one L2.  Entry i of the chunk reads
+// its source's value from the LDS stage and stores it at i + rb[rank], rank =
+// the run holding i, found from the per-64-entry run-start bits and counts
+// (one wave's 64 entries share them: a uniform load).
+//
+// One counter tracks a wave's loads and stores together, so a round that
+// waited for its loads behind the previous round's stores would wait for those
+// stores too.  The rounds therefore alternate between two register sets: the
+// next round's loads are issued before this round's stores, and every store is
+// unconditional (lanes past the end read the last entry and store to the sink
+// slot binval[Eb]), so the wait for a round's loads leaves the stores that
+// follow them in flight.
+template <int U>
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
+    const float* __restrict__ contrib, const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ ch_nr,
+    const uint32_t* __restrict__ ch_e0, const uint32_t* __restrict__ ch_w0, const uint32_t* __restrict__ ch_q0,
+    const uint16_t* __restrict__ src16, const uint4* __restrict__ rmeta, const uint32_t* __restrict__ rbase,
+    uint32_t nch, uint32_t sink, float* __restrict__ binval) {
+  __shared__ float sv[kChunkRows];
+  __shared__ uint32_t rb[kBinMaxBins];
+  const uint32_t per = (nch + 7) / 8;
+  const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (w >= nch) return;
+  const uint32_t r0 = ch_r0[w], nr = ch_nr[w];
+  const uint32_t n = ch_e0[w + 1] - ch_e0[w];
+  const uint32_t w0 = ch_w0[w], q0 = ch_q0[w], nq = ch_q0[w + 1] - q0;
+  const uint16_t* __restrict__ sp = src16 + w0;
+  const uint4* __restrict__ mp = rmeta + (w0 >> 6);
+  constexpr uint32_t step = kScatterBlock * U;
+  struct Set {
+    uint32_t src[U];
+    uint4 m[U];
+  };
+  // lanes past the end take the last entry, which lies in the 64-entry block of
+  // the wave's first lane, so the block index is uniform across the wave
+  auto load = [&](Set& t, uint32_t base) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kScatterBlock + threadIdx.x;
+      const uint32_t ii = i < n ? i : n - 1;
+      t.src[j] = __builtin_nontemporal_load(sp + ii);
+      t.m[j] = mp[__builtin_amdgcn_readfirstlane(ii >> 6)];
+    }
+  };
+  auto put = [&](const Set& t, uint32_t base) {
+    uint32_t pos[U];
+    float v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kScatterBlock + threadIdx.x;
+      const uint32_t ii = i < n ? i : n - 1;
+      const uint64_t bits = ((uint64_t)t.m[j].y << 32) | t.m[j].x;
+      const uint32_t rank = t.m[j].z + (uint32_t)__popcll(bits & (~0ULL >> (63 - (ii & 63)))) - 1u;
+      pos[j] = i < n ? i + rb[rank] : sink;
+      v[j] = sv[t.src[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) binval[pos[j]] = v[j];
+  };
+  Set a, b;
+  load(a, 0);  // in flight while the stage fills
+  {
+    // the stage's loads are all issued before any is stored (lanes past the
+    // end reload the last value)
+    constexpr int FS = kChunkRows / kScatterBlock, FR = kBinMaxBins / kScatterBlock;
+    float f[FS];
+    uint32_t q[FR];
+#pragma unroll
+    for (int k = 0; k < FS; ++k) {
+      const uint32_t i = threadIdx.x + k * kScatterBlock;
+      f[k] = contrib[r0 + (i < nr ? i : nr - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < FR; ++k) {
+      const uint32_t i = threadIdx.x + k * kScatterBlock;
+      q[k] = rbase[q0 + (i < nq ? i : nq - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < FS; ++k) {
+      const uint32_t i = threadIdx.x + k * kScatterBlock;
+      if (i < nr) sv[i] = f[k];
+    }
+#pragma unroll
+    for (int k = 0; k < FR; ++k) {
+      const uint32_t i = threadIdx.x + k * kScatterBlock;
+      if (i < nq) rb[i] = q[k];
+    }
+  }
+  __syncthreads();
+  for (uint32_t base = 0;;) {
+    load(b, base + step);
+    put(a, base);
+    base += step;
+    if (base >= n) break;
+    load(a, base + step);
+    put(b, base);
+    base += step;
+    if (base >= n) break;
+  }
+}
+
+// Accumulate slice s of bin b: 64-bit fixed-point sums in LDS, then either the
+// rows' updates (the bin's only slice) or the sums added into the bin's slots
+// of gacc (finished by k_bin_fin).  (acc0, acc1) partials per slice.  Rounds
+// alternate between two register sets as in the scatter; lanes past the end
+// add zero to the first entry's row.
+template <bool R, int U>
+__global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
+    const float* __restrict__ binval, const uint16_t* __restrict__ dloc, const uint32_t* __restrict__ sl_e0,
+    const uint32_t* __restrict__ sl_bin, const uint32_t* __restrict__ sl_g, const uint32_t* __restrict__ bin_r0,
+    BinUpdate<R> up, unsigned long long* __restrict__ gacc, double* __restrict__ part) {
+  __shared__ unsigned long long acc[kBinRows];
+  __shared__ double red[32];
+  const uint32_t s = blockIdx.x;
+  const uint32_t b = sl_bin[s];
+  const uint32_t r0 = bin_r0[b], nr = bin_r0[b + 1] - r0;
+  const uint32_t e0 = sl_e0[s], n = sl_e0[s + 1] - e0;
+  const float* __restrict__ vp = binval + e0;
+  const uint16_t* __restrict__ dp = dloc + e0;
+  constexpr uint32_t step = kAccumBlock * U;
+  auto load = [&](float (&v)[U], uint32_t (&d)[U], uint32_t base) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kAccumBlock + threadIdx.x;
+      const uint32_t ii = i < n ? i : 0u;
+      v[j] = __builtin_nontemporal_load(vp + ii);
+      d[j] = __builtin_nontemporal_load(dp + ii);
+    }
+  };
+  auto add = [&](const float (&v)[U], const uint32_t (&d)[U], uint32_t base) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kAccumBlock + threadIdx.x;
+      const float x = i < n ? v[j] : 0.f;
+      atomicAdd(&acc[d[j]], (unsigned long long)__float2ll_rn(x * 0x1p58f));
+    }
+  };
+  float va[U], vb[U];
+  uint32_t da[U], db[U];
+  load(va, da, 0);  // in flight while the accumulators are cleared
+  for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) acc[i] = 0ULL;
+  __syncthreads();
+  for (uint32_t base = 0;;) {
+    load(vb, db, base + step);
+    add(va, da, base);
+    base += step;
+    if (base >= n) break;
+    load(va, da, base + step);
+    add(vb, db, base);
+    base += step;
+    if (base >= n) break;
+  }
+  __syncthreads();
+  double a0 = 0.0, a1 = 0.0;
+  const uint32_t g = sl_g[s];
+  if (g == 0xffffffffu) {
+    // every row's data is loaded before any row is updated
+    constexpr int Q = kBinRows / kAccumBlock;
+    typename BinUpdate<R>::Pre q[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const uint32_t i = threadIdx.x + k * kAccumBlock;
+      q[k] = up.pre(r0 + (i < nr ? i : 0u));
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const uint32_t i = threadIdx.x + k * kAccumBlock;
+      if (i < nr) up.fin(r0 + i, fix_to_float(acc[i]), q[k], a0, a1);
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock)
+      if (acc[i]) atomicAdd(&gacc[g + i], acc[i]);
+  }
+  block_sum2_big(a0, a1, red);
+  if (threadIdx.x == 0) { part[2 * s] = a0; part[2 * s + 1] = a1; }
+}
+
+// Finish: the rows of the bins cut into slices (their sums in gacc, which is
+// cleared for the next iteration), then the rows without in-edges (sum 0).
+template <bool R>
+__global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_r0,
+                                                const uint32_t* __restrict__ mb_g, uint32_t nmb, uint32_t nmrows,
+                                                uint32_t nd, uint32_t nV, BinUpdate<R> up,
+                                                unsigned long long* __restrict__ gacc, double* __restrict__ part) {
+  __shared__ double red[32];
+  double a0 = 0.0, a1 = 0.0;
+  const uint64_t total = (uint64_t)nmrows + (nV - nd);
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < total; q += (uint64_t)gridDim.x * blockDim.x) {
+    if (q < nmrows) {
+      uint32_t lo = 0, hi = nmb;  // largest k with mb_g[k] <= q
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (mb_g[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+      const unsigned long long v = gacc[q];
+      gacc[q] = 0ULL;
+      up.apply(mb_r0[lo] + (uint32_t)(q - mb_g[lo]), fix_to_float(v), a0, a1);
+    } else {
+      up.apply(nd + (uint32_t)(q - nmrows), 0.f, a0, a1);
+    }
+  }
+  block_sum2_big(a0, a1, red);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
+}
+
+// ---- layout build -----------------------------------------------------------
+
+// rows of the vertices with in-edges: in-degree rank i dealt to bin b in snake
+// order, row = bin_r0[b] + round
+__global__ void k_bin_destrows(const uint32_t* __restrict__ did, uint32_t nV, uint32_t B,
+                               const uint32_t* __restrict__ bin_r0, uint32_t* __restrict__ rowperm) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = did[v];
+    if (i == 0xffffffffu) continue;
+    const uint32_t k = i / B, p = i % B;
+    const uint32_t b = (k & 1u) ? B - 1 - p : p;
+    rowperm[v] = bin_r0[b] + k;
+  }
+}
+
+// sort keys of the tail: vertices without in-edges by descending out-degree,
+// the others last
+__global__ void k_bin_tailkeys(const uint64_t* __restrict__ coff, const int32_t* __restrict__ outdeg, uint32_t nV,
+                               uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    key[v] = coff[v + 1] == coff[v] ? 0x7fffffffu - (uint32_t)outdeg[v] : 0xffffffffu;
+    val[v] = (uint32_t)v;
+  }
+}
+
+__global__ void k_bin_tailrows(const uint32_t* __restrict__ order, uint32_t ntail, uint32_t nd,
+                               uint32_t* __restrict__ rowperm) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < ntail; j += (uint64_t)gridDim.x * blockDim.x)
+    rowperm[order[j]] = nd + (uint32_t)j;
+}
+
+__global__ void k_bin_rowdata(const uint32_t* __restrict__ rowperm, const int32_t* __restrict__ outdeg, uint32_t nV,
+                              uint32_t* __restrict__ rowinv, int32_t* __restrict__ outdeg_r,
+                              uint32_t* __restrict__ cid_r) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rowperm[v];
+    rowinv[r] = (uint32_t)v;
+    outdeg_r[r] = outdeg[v];
+    cid_r[r] = outdeg[v] > 0 ? r : 0xffffffffu;
+  }
+}
+
+__global__ void k_bin_rowbin(const uint32_t* __restrict__ bin_r0, uint32_t B, uint32_t nd,
+                             uint32_t* __restrict__ rowbin) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < nd; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = B;  // largest b with bin_r0[b] <= r
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (bin_r0[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    rowbin[r] = lo;
+  }
+}
+
+// Sort key of CSR entry e (u -> v): (chunk, bin of v's row, source row within
+// the chunk); value = v's row.  The row of e is found inside e's tile of the
+// CSR's row tiles.
+__global__ void k_bin_keys(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                           const uint32_t* __restrict__ rb, uint32_t te, uint64_t E,
+                           const uint32_t* __restrict__ rowperm, const int32_t* __restrict__ outdeg,
+                           const uint32_t* __restrict__ row_ch, const uint32_t* __restrict__ ch_r0,
+                           const uint32_t* __restrict__ rowbin, int bbits, uint64_t* __restrict__ key,
+                           uint32_t* __restrict__ val) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = e / te;
+    uint32_t lo = rb[t] > 0 ? rb[t] - 1 : 0, hi = rb[t + 1];
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (off[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t u = lo;
+    const uint32_t ru = rowperm[u];
+    const bool split = (uint32_t)outdeg[u] > kChunkEdges;
+    const uint32_t w = row_ch[ru] + (split ? (uint32_t)((e - off[u]) / kChunkEdges) : 0u);
+    const uint32_t sl = ru - ch_r0[w];
+    const uint32_t rv = rowperm[idx[e]];
+    key[e] = ((uint64_t)w << (bbits + kSrcBits)) | ((uint64_t)rowbin[rv] << kSrcBits) | sl;
+    val[e] = rv;
+  }
+}
+
+// scatter sources and run-start bits at each entry's padded word position,
+// and the first and one-past-last entry of every run (w, b)
+__global__ void k_bin_runs(const uint64_t* __restrict__ key, uint64_t E, int bbits, uint32_t B,
+                           const uint32_t* __restrict__ ch_e0, const uint32_t* __restrict__ ch_w0,
+                           uint16_t* __restrict__ src16, unsigned long long* __restrict__ bits,
+                           uint32_t* __restrict__ rstart, uint32_t* __restrict__ rend) {
+  const uint64_t bmask = (1ULL << bbits) - 1;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = key[i];
+    const uint64_t wb = k >> kSrcBits;
+    const uint32_t b = (uint32_t)(wb & bmask);
+    const uint64_t w = wb >> bbits;
+    const uint32_t g = ch_w0[w] + ((uint32_t)i - ch_e0[w]);
+    src16[g] = (uint16_t)(k & ((1u << kSrcBits) - 1));
+    if (i == 0 || (key[i - 1] >> kSrcBits) != wb) {
+      rstart[w * B + b] = (uint32_t)i;
+      atomicOr(&bits[g >> 6], 1ULL << (g & 63));
+    }
+    if (i + 1 == E || (key[i + 1] >> kSrcBits) != wb) rend[w * B + b] = (uint32_t)(i + 1);
+  }
+}
+
+// dloc at each entry's bin-array position; bad[0] counts positions out of
+// range, bad[1] accumulates the positions (their sum is checked on the host)
+__global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t E,
+                           uint64_t Eb, int bbits, uint32_t B, const uint32_t* __restrict__ ch_e0,
+                           const uint32_t* __restrict__ runbase, const uint32_t* __restrict__ bin_r0,
+                           uint16_t* __restrict__ dloc, unsigned long long* __restrict__ bad) {
+  const uint64_t bmask = (1ULL << bbits) - 1;
+  unsigned long long nbad = 0, psum = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t wb = key[i] >> kSrcBits;
+    const uint32_t b = (uint32_t)(wb & bmask);
+    const uint64_t w = wb >> bbits;
+    const uint32_t pos = (uint32_t)i - ch_e0[w] + runbase[w * B + b];
+    if (pos < Eb) {
+      dloc[pos] = (uint16_t)(val[i] - bin_r0[b]);
+      psum += pos;
+    } else {
+      ++nbad;
+    }
+  }
+  block_add2(nbad, bad, psum, bad + 1);
+}
+
+template <typename T>
+void copy_to_dev(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (!h.empty()) GRE_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
+  const auto mkey = std::make_pair((int)kBinKey, 0u);
+  auto it = g->xpart.find(mkey);
+  if (it != g->xpart.end()) return it->second;
+  const uint64_t E = g->nE;
+  const uint32_t nV = g->nV;
+  GRE_CHECK(E > 0 && E < 0xffffffffULL, GRE_DEV_NULL, "the binned schedule needs 0 < edges < 2^32");
+  graph_compact_targets(g);
+  const uint32_t nd = g->ntgt;
+  const uint32_t B = (nd + kBinRows - 1) / kBinRows;
+  GRE_CHECK(B >= 1 && B <= (uint32_t)kBinMaxBins, GRE_DEV_NULL,
+            "the binned schedule takes at most 4096 * 16384 vertices with in-edges");
+  hipStream_t s = stream();
+  auto xp = std::make_shared<XPart>();
+  xp->binned = true;
+  xp->permuted = true;
+  xp->R = (int)kBinKey;
+  BinPlan& bp = xp->bin;
+  bp.E = E;
+  bp.nd = nd;
+  bp.B = B;
+
+  // bins: B - rem bins get nd / B rows, rem get one more (the last, partial
+  // round of the snake deal)
+  std::vector<uint32_t> bin_r0(B + 1, 0);
+  {
+    const uint32_t full = nd / B, rem = nd % B;
+    const bool even = (full % 2) == 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      const bool extra = even ? b < rem : b >= B - rem;
+      bin_r0[b + 1] = bin_r0[b] + full + (extra ? 1u : 0u);
+    }
+  }
+  copy_to_dev(bp.bin_r0, bin_r0, s);
+
+  // rows
+  xp->rowperm.alloc(4 * (size_t)nV);
+  xp->rowinv.alloc(4 * (size_t)nV);
+  xp->outdeg_r.alloc(4 * (size_t)nV);
+  xp->cid_r.alloc(4 * (size_t)nV);
+  hipLaunchKernelGGL(k_bin_destrows, dim3(gcap(nV)), dim3(kB), 0, s, g->did.as<uint32_t>(), nV, B,
+                     bp.bin_r0.as<uint32_t>(), xp->rowperm.as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+  if (nd < nV) {
+    DevBuf k0, k1, v0, v1;
+    k0.alloc(4 * (size_t)nV);
+    k1.alloc(4 * (size_t)nV);
+    v0.alloc(4 * (size_t)nV);
+    v1.alloc(4 * (size_t)nV);
+    hipLaunchKernelGGL(k_bin_tailkeys, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(),
+                       g->outdeg.as<int32_t>(), nV, k0.as<uint32_t>(), v0.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    size_t tb = 0;
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, k0.as<uint32_t>(), k1.as<uint32_t>(), v0.as<uint32_t>(),
+                                      v1.as<uint32_t>(), (size_t)nV, 0, 32, s));
+    void* tmp = scratch(0, tb);
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, k0.as<uint32_t>(), k1.as<uint32_t>(), v0.as<uint32_t>(),
+                                      v1.as<uint32_t>(), (size_t)nV, 0, 32, s));
+    hipLaunchKernelGGL(k_bin_tailrows, dim3(gcap(nV - nd)), dim3(kB), 0, s, v1.as<uint32_t>(), nV - nd, nd,
+                       xp->rowperm.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  hipLaunchKernelGGL(k_bin_rowdata, dim3(gcap(nV)), dim3(kB), 0, s, xp->rowperm.as<uint32_t>(),
+                     g->outdeg.as<int32_t>(), nV, xp->rowinv.as<uint32_t>(), xp->outdeg_r.as<int32_t>(),
+                     xp->cid_r.as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+
+  // scatter chunks over consecutive source rows
+  std::vector<int32_t> deg(nV);
+  GRE_HIP(hipMemcpyAsync(deg.data(), xp->outdeg_r.p, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> ch_r0, ch_nr, ch_e0(1, 0), row_ch(nV, 0);
+  for (uint32_t r = 0; r < nV;) {
+    const uint32_t d = (uint32_t)deg[r];
+    if (d == 0) { ++r; continue; }
+    if (d > kChunkEdges) {
+      row_ch[r] = (uint32_t)ch_r0.size();
+      for (uint32_t done = 0; done < d; done += kChunkEdges) {
+        ch_r0.push_back(r);
+        ch_nr.push_back(1);
+        ch_e0.push_back(ch_e0.back() + std::min(kChunkEdges, d - done));
+      }
+      ++r;
+      continue;
+    }
+    const uint32_t r0 = r, w = (uint32_t)ch_r0.size();
+    uint32_t edges = 0;
+    while (r < nV && r - r0 < (uint32_t)kChunkRows) {
+      const uint32_t dr = (uint32_t)deg[r];
+      if (dr > kChunkEdges || edges + dr > kChunkEdges) break;
+      edges += dr;
+      row_ch[r] = w;
+      ++r;
+    }
+    ch_r0.push_back(r0);
+    ch_nr.push_back(r - r0);
+    ch_e0.push_back(ch_e0.back() + edges);
+  }
+  const uint32_t nch = (uint32_t)ch_r0.size();
+  GRE_CHECK(ch_e0.back() == E, GRE_DEV_FAILURE, "binned layout: chunk entries do not add up to the edges");
+  bp.nch = nch;
+  copy_to_dev(bp.ch_r0, ch_r0, s);
+  copy_to_dev(bp.ch_nr, ch_nr, s);
+  copy_to_dev(bp.ch_e0, ch_e0, s);
+  // each chunk's words start on a 64-entry boundary
+  std::vector<uint32_t> ch_w0(nch);
+  uint64_t Ew = 0;
+  for (uint32_t w = 0; w < nch; ++w) {
+    ch_w0[w] = (uint32_t)Ew;
+    Ew += align64(ch_e0[w + 1] - ch_e0[w]);
+  }
+  GRE_CHECK(Ew < 0xffffffffULL, GRE_DEV_NULL, "the binned schedule needs fewer than 2^32 padded entries");
+  bp.Ew = Ew;
+  copy_to_dev(bp.ch_w0, ch_w0, s);
+
+  // entries sorted by (chunk, bin, source row)
+  const int bbits = (int)bits_of(B);
+  const int wbits = (int)bits_of((uint64_t)nch);
+  const int kbits = wbits + bbits + kSrcBits;
+  GRE_CHECK(kbits <= 64, GRE_DEV_FAILURE, "binned layout: sort key too wide");
+  DevBuf keys0, keys1, vals0, vals1;
+  {
+    DevBuf rowch_d, rowbin;
+    copy_to_dev(rowch_d, row_ch, s);
+    rowbin.alloc(4 * (size_t)std::max<uint32_t>(nd, 1));
+    hipLaunchKernelGGL(k_bin_rowbin, dim3(gcap(nd)), dim3(kB), 0, s, bp.bin_r0.as<uint32_t>(), B, nd,
+                       rowbin.as<uint32_t>());
+    uint64_t nt = 0;
+    const uint32_t* rb = graph_tiles(g, false, 2048, &nt);
+    keys0.alloc(8 * E);
+    vals0.alloc(4 * E);
+    hipLaunchKernelGGL(k_bin_keys, dim3(gcap(E)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), rb,
+                       2048u, E, xp->rowperm.as<uint32_t>(), g->outdeg.as<int32_t>(), rowch_d.as<uint32_t>(),
+                       bp.ch_r0.as<uint32_t>(), rowbin.as<uint32_t>(), bbits, keys0.as<uint64_t>(),
+                       vals0.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  keys1.alloc(8 * E);
+  vals1.alloc(4 * E);
+  {
+    size_t tb = 0;
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint32_t>(),
+                                      vals1.as<uint32_t>(), (size_t)E, 0, (unsigned)kbits, s));
+    void* tmp = scratch(0, tb);
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint32_t>(),
+                                      vals1.as<uint32_t>(), (size_t)E, 0, (unsigned)kbits, s));
+    GRE_HIP(hipStreamSynchronize(s));
+  }
+  keys0.release();
+  vals0.release();
+
+  // scatter sources, run-start bits and run bounds
+  const size_t ntab = (size_t)nch * B;
+  const size_t nblk = Ew / 64;
+  bp.src16.alloc(2 * Ew);
+  DevBuf rstart, rend, bits;
+  rstart.alloc(4 * ntab);
+  rend.alloc(4 * ntab);
+  bits.alloc(8 * nblk);
+  GRE_HIP(hipMemsetAsync(rstart.p, 0, 4 * ntab, s));
+  GRE_HIP(hipMemsetAsync(rend.p, 0, 4 * ntab, s));
+  GRE_HIP(hipMemsetAsync(bits.p, 0, 8 * nblk, s));
+  GRE_HIP(hipMemsetAsync(bp.src16.p, 0, 2 * Ew, s));
+  hipLaunchKernelGGL(k_bin_runs, dim3(gcap(E)), dim3(kB), 0, s, keys1.as<uint64_t>(), E, bbits, B,
+                     bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(), bp.src16.as<uint16_t>(),
+                     bits.as<unsigned long long>(), rstart.as<uint32_t>(), rend.as<uint32_t>());
+  GRE_HIP(hipGetLastError());
+  std::vector<uint32_t> hs(ntab), he(ntab);
+  std::vector<uint64_t> hbits(nblk);
+  GRE_HIP(hipMemcpyAsync(hs.data(), rstart.p, 4 * ntab, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipMemcpyAsync(he.data(), rend.p, 4 * ntab, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipMemcpyAsync(hbits.data(), bits.p, 8 * nblk, hipMemcpyDeviceToHost, s));
+  GRE_HIP(hipStreamSynchronize(s));
+  rstart.release();
+  rend.release();
+  bits.release();
+  // per 64-word block: its bits and the chunk's runs started before it
+  {
+    std::vector<uint32_t> meta(4 * nblk, 0);
+    for (uint32_t w = 0; w < nch; ++w) {
+      const uint64_t k0 = ch_w0[w] / 64, k1 = k0 + align64(ch_e0[w + 1] - ch_e0[w]) / 64;
+      uint32_t before = 0;
+      for (uint64_t k = k0; k < k1; ++k) {
+        meta[4 * k] = (uint32_t)hbits[k];
+        meta[4 * k + 1] = (uint32_t)(hbits[k] >> 32);
+        meta[4 * k + 2] = before;
+        before += (uint32_t)__builtin_popcountll(hbits[k]);
+      }
+    }
+    copy_to_dev(bp.rmeta, meta, s);
+  }
+
+  // bin-array positions: bin b holds its runs in chunk order from bin_e0[b],
+  // a 64-entry boundary; bin_n[b] entries
+  std::vector<uint64_t> bin_n(B, 0), bin_e0(B + 1, 0);
+  for (uint32_t w = 0; w < nch; ++w)
+    for (uint32_t b = 0; b < B; ++b) bin_n[b] += he[(size_t)w * B + b] - hs[(size_t)w * B + b];
+  uint64_t tot = 0;
+  for (uint32_t b = 0; b < B; ++b) {
+    bin_e0[b + 1] = bin_e0[b] + align64(bin_n[b]);
+    tot += bin_n[b];
+  }
+  GRE_CHECK(tot == E, GRE_DEV_FAILURE, "binned layout: runs do not add up to the edges");
+  const uint64_t Eb = bin_e0[B];
+  GRE_CHECK(Eb < 0xffffffffULL, GRE_DEV_NULL, "the binned schedule needs fewer than 2^32 padded entries");
+  bp.Eb = Eb;
+  std::vector<uint32_t> runbase(ntab, 0);
+  {
+    std::vector<uint64_t> pos(bin_e0.begin(), bin_e0.end() - 1);
+    for (uint32_t w = 0; w < nch; ++w)
+      for (uint32_t b = 0; b < B; ++b) {
+        const size_t k = (size_t)w * B + b;
+        const uint32_t c = he[k] - hs[k];
+        if (!c) continue;
+        runbase[k] = (uint32_t)pos[b] - (hs[k] - ch_e0[w]);
+        pos[b] += c;
+      }
+  }
+  // the dense table serves the build; the scatter keeps each chunk's non-empty runs
+  DevBuf runbase_d;
+  copy_to_dev(runbase_d, runbase, s);
+  {
+    std::vector<uint32_t> q0(nch + 1, 0), rbc;
+    for (uint32_t w = 0; w < nch; ++w) {
+      for (uint32_t b = 0; b < B; ++b) {
+        const size_t k = (size_t)w * B + b;
+        if (he[k] != hs[k]) rbc.push_back(runbase[k]);
+      }
+      q0[w + 1] = (uint32_t)rbc.size();
+    }
+    copy_to_dev(bp.ch_q0, q0, s);
+    copy_to_dev(bp.rbase, rbc, s);
+  }
+
+  // accumulate slices; bins with more than kSliceEdges entries are cut (at
+  // 64-entry boundaries); a bin's last slice runs to the next bin's start
+  std::vector<uint32_t> sl_e0, sl_bin, sl_g, mb_r0, mb_g(1, 0);
+  for (uint32_t b = 0; b < B; ++b) {
+    const uint64_t n = bin_n[b];
+    const uint64_t ns = std::max<uint64_t>(1, (n + kSliceEdges - 1) / kSliceEdges);
+    uint32_t gslot = 0xffffffffu;
+    if (ns > 1) {
+      gslot = mb_g.back();
+      mb_r0.push_back(bin_r0[b]);
+      mb_g.push_back(mb_g.back() + (bin_r0[b + 1] - bin_r0[b]));
+    }
+    for (uint64_t j = 0; j < ns; ++j) {
+      sl_e0.push_back((uint32_t)(bin_e0[b] + ((j * n / ns) & ~63ULL)));
+      sl_bin.push_back(b);
+      sl_g.push_back(gslot);
+    }
+  }
+  sl_e0.push_back((uint32_t)Eb);
+  bp.nsl = (uint32_t)sl_bin.size();
+  bp.nmb = (uint32_t)mb_r0.size();
+  bp.nmrows = mb_g.back();
+  copy_to_dev(bp.sl_e0, sl_e0, s);
+  copy_to_dev(bp.sl_bin, sl_bin, s);
+  copy_to_dev(bp.sl_g, sl_g, s);
+  copy_to_dev(bp.mb_r0, mb_r0, s);
+  copy_to_dev(bp.mb_g, mb_g, s);
+
+  // destination of every entry at its bin-array position
+  bp.dloc.alloc(2 * Eb);
+  {
+    DevBuf bad;
+    bad.alloc(16);
+    GRE_HIP(hipMemsetAsync(bad.p, 0, 16, s));
+    GRE_HIP(hipMemsetAsync(bp.dloc.p, 0, 2 * Eb, s));
+    hipLaunchKernelGGL(k_bin_dloc, dim3(gcap(E)), dim3(kB), 0, s, keys1.as<uint64_t>(), vals1.as<uint32_t>(), E, Eb,
+                       bbits, B, bp.ch_e0.as<uint32_t>(), runbase_d.as<uint32_t>(), bp.bin_r0.as<uint32_t>(),
+                       bp.dloc.as<uint16_t>(), bad.as<unsigned long long>());
+    GRE_HIP(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    GRE_HIP(hipMemcpyAsync(h, bad.p, 16, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    // the positions of bin b are bin_e0[b] .. bin_e0[b] + bin_n[b] - 1
+    unsigned long long want = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      const unsigned long long a = bin_e0[b], n = bin_n[b];
+      want += n * a + (n % 2 ? n * ((n - 1) / 2) : (n / 2) * (n - 1));
+    }
+    GRE_CHECK(h[0] == 0 && h[1] == want, GRE_DEV_FAILURE, "binned layout: bin positions are not a permutation");
+  }
+  g->xpart[mkey] = xp;
+  return xp;
+}
+
+void BinBuffers::make(const XPart& xp, uint32_t nV, hipStream_t s) {
+  const BinPlan& bp = xp.bin;
+  // the gaps between bins are never scattered into and must hold zero;
+  // binval[Eb] is the scatter's sink slot
+  binval.alloc(4 * ((size_t)bp.Eb + 1));
+  GRE_HIP(hipMemsetAsync(binval.p, 0, 4 * ((size_t)bp.Eb + 1), s));
+  gacc.alloc(8 * (size_t)std::max<uint32_t>(bp.nmrows, 1));
+  GRE_HIP(hipMemsetAsync(gacc.p, 0, 8 * (size_t)std::max<uint32_t>(bp.nmrows, 1), s));
+  part.alloc(2 * sizeof(double) * bp.nsl);
+  fin_grid = gcap((uint64_t)bp.nmrows + (nV - bp.nd));
+  part2.alloc(2 * sizeof(double) * fin_grid);
+  part3.alloc(2 * sizeof(double) * kRedBlocks);
+  GRE_HIP(hipStreamSynchronize(s));
+  for_xp = &xp;
+}
+
+namespace {
+template <bool R>
+void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s) {
+  const BinPlan& bp = xp.bin;
+  BinUpdate<R> up{a.contrib_out, a.x, a.p_r, a.outdeg_r, a.scal_in, (float)a.damp, (float)(1.0 - a.damp),
+                  (float)a.invn};
+  {
+    ProfScope ps("bin_scatter");
+    const unsigned per = (bp.nch + 7) / 8;
+    hipLaunchKernelGGL((k_bin_scatter<kScatterU>), dim3(8 * per), dim3(kScatterBlock), 0, s, a.contrib_in,
+                       bp.ch_r0.as<uint32_t>(), bp.ch_nr.as<uint32_t>(), bp.ch_e0.as<uint32_t>(),
+                       bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(), bp.src16.as<uint16_t>(),
+                       bp.rmeta.as<uint4>(), bp.rbase.as<uint32_t>(), bp.nch, (uint32_t)bp.Eb, bb.binval.as<float>());
+  }
+  {
+    ProfScope ps("bin_accum");
+    hipLaunchKernelGGL((k_bin_accum<R, kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<float>(),
+                       bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(), bp.sl_bin.as<uint32_t>(),
+                       bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), up, bb.gacc.as<unsigned long long>(),
+                       bb.part.as<double>());
+  }
+  {
+    ProfScope ps("bin_finish");
+    hipLaunchKernelGGL((k_bin_fin<R>), dim3(bb.fin_grid), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
+                       bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, bp.nd, nV, up, bb.gacc.as<unsigned long long>(),
+                       bb.part2.as<double>());
+  }
+  reduce_pairs(bb.part.as<double>(), bp.nsl, bb.part2.as<double>(), bb.fin_grid, a.scal_out, bb.part3.as<double>(), s);
+  GRE_HIP(hipGetLastError());
+}
+}  // namespace
+
+void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s) {
+  if (a.resid) bin_iter_t<true>(xp, bb, a, nV, s);
+  else bin_iter_t<false>(xp, bb, a, nV, s);
+}
+
+}  // namespace gre

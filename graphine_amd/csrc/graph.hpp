@@ -17,6 +17,41 @@ namespace gre {
 // each row's list of segment ids sums them in a fixed order.
 // most (class, range) keys of a partition: 8 classes * (1 + 256/8)
 constexpr int kMaxKeys = 264;
+// Binned (propagation-blocking) layout of the forward pull; see binned.hip.
+// Rows are relabelled: rows [0, nd) have in-edges and are cut into bins of
+// consecutive rows; rows [nd, nV) have none.  The scatter pass walks chunks of
+// consecutive source rows, the accumulate pass slices of each bin's entries.
+constexpr int kBinRows = 16384;        // rows per bin (64-bit LDS accumulators)
+constexpr int kBinMaxBins = 4096;      // bins (the scatter pass keeps a run table of this many in LDS)
+constexpr int kChunkRows = 16384;      // source rows whose values a chunk stages in LDS
+constexpr uint32_t kChunkEdges = 262144;   // entries per chunk (a longer source row is cut)
+constexpr uint32_t kSliceEdges = 1u << 20;  // entries per accumulate slice (a longer bin is cut)
+struct BinPlan {
+  // Chunks start at 64-entry boundaries of the word array and bins at 64-entry
+  // boundaries of the bin array, so every wave's loads cover whole lines; the
+  // gaps are never scattered into and hold zero contributions.
+  uint64_t E = 0;      // entries
+  uint64_t Ew = 0;     // words, with each chunk's gap
+  uint64_t Eb = 0;     // bin-array slots, with each bin's gap
+  uint32_t nd = 0, B = 0, nch = 0, nsl = 0, nmb = 0, nmrows = 0;
+  DevBuf ch_r0;    // uint32 [nch] first source row of chunk w
+  DevBuf ch_nr;    // uint32 [nch] source rows chunk w stages
+  DevBuf ch_e0;    // uint32 [nch+1] entries before chunk w (chunk w has ch_e0[w+1] - ch_e0[w])
+  DevBuf ch_w0;    // uint32 [nch] first word of chunk w (a multiple of 64)
+  DevBuf src16;    // uint16 [Ew] scatter order (by bin, then source): source row - ch_r0
+  // per 64 words: (bit j set if word 64k + j starts a run: uint64 as two
+  // uint32, runs of the chunk started before word 64k, 0)
+  DevBuf rmeta;    // uint4 [Ew/64]
+  DevBuf ch_q0;    // uint32 [nch+1] first run of chunk w in rbase
+  DevBuf rbase;    // uint32 [runs] bin-array position of each run minus its chunk-local start (mod 2^32)
+  DevBuf dloc;     // uint16 [Eb] bin order: destination row - bin_r0[bin] (0 in the gaps)
+  DevBuf bin_r0;   // uint32 [B+1] first row of each bin
+  DevBuf sl_e0;    // uint32 [nsl+1] slice ranges in the bin array (a bin's last slice takes its gap)
+  DevBuf sl_bin;   // uint32 [nsl] bin of each slice
+  DevBuf sl_g;     // uint32 [nsl] ~0: the slice's bin has one slice; else the bin's first slot in the multi-slice sums
+  DevBuf mb_r0;    // uint32 [nmb] first row of each multi-slice bin
+  DevBuf mb_g;     // uint32 [nmb+1] its first slot in the multi-slice sums (mb_g[nmb] = nmrows)
+};
 struct XPart {
   int R = 0;
   uint32_t H = 0;    // sources [0, H) form one range shared by all XCDs (cut by rows)
@@ -33,6 +68,9 @@ struct XPart {
   // in row order.  Empty when rows are the vertices themselves.
   bool permuted = false;
   DevBuf rowperm, rowinv, outdeg_r, cid_r;
+  // set for the binned layout (graph_binpart); the segment fields stay empty
+  bool binned = false;
+  BinPlan bin;
   int K = 0;         // (class, range) keys: 8 * (1 + R/8)
   DevBuf kseg;       // uint32 [K+1] first segment of each key; kseg[K] = nseg
   struct Tiles {
@@ -136,6 +174,9 @@ void build_tile_rows(const uint32_t* off, uint32_t nrows, uint64_t ntiles, uint3
 // rows_permuted (csc only): the partition's rows are the vertices relabelled by
 // a fixed bijection (XPart::permuted); cached apart from the plain one.
 std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = true, bool rows_permuted = false);
+// The binned layout of g's forward pull (E < 2^32, at most kBinMaxBins bins);
+// built once and cached with the partitions.
+std::shared_ptr<XPart> graph_binpart(gre_graph* g);
 // Drops the cached partitions that no PageRank handle or engine holds any more.
 void prune_xparts(gre_graph* g);
 // Its tiles of te entries (cached on the partition).

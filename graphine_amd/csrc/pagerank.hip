@@ -373,9 +373,39 @@ void ensure_scal(gre_pr* P, int iters) {
 // order (compact) or in the partition's entry order (part).
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
-// the partitioned forward walks relabelled rows: a run keeps its ranks in the
-// partition's row order until ranks_to_vertices
-bool rows_relabelled(const gre_pr* P) { return use_part(P) && P->xp && P->xp->permuted; }
+// the binned pull: single vector, FP32, unweighted
+bool use_bin(const gre_pr* P) { return P->bin && P->k == 1; }
+// the partitioned and binned forwards walk relabelled rows: a run keeps its
+// ranks in the layout's row order until ranks_to_vertices
+bool rows_relabelled(const gre_pr* P) { return (use_part(P) || use_bin(P)) && P->xp && P->xp->permuted; }
+
+// Why the binned pull cannot run this handle, or nullptr.
+const char* bin_refusal(const gre_pr* P) {
+  if (P->k != 1) return "the binned schedule runs single-vector handles only";
+  if (P->dtype != GRE_F32) return "the binned schedule runs FP32 handles only";
+  if (P->weighted) return "the binned schedule runs unweighted handles only";
+  if (P->g->nE == 0 || P->g->nE >= 0xffffffffULL) return "the binned schedule needs 0 < edges < 2^32";
+  if (P->has_p && P->p_l1 > kBinMaxPL1)
+    return "the binned schedule needs a personalisation of L1 norm at most 1 (its fixed-point sums are bounded by it)";
+  return nullptr;
+}
+
+// the binned layout and the handle's buffers for it
+void ensure_bin_plan(gre_pr* P) {
+  if (const char* why = bin_refusal(P)) throw Error(GRE_DEV_NULL, why);
+  if (!P->xp || !P->xp->binned) {
+    P->xp.reset();
+    P->wpart_xp.reset();
+    P->wpart_ver = ~0ULL;
+    P->plan_ipt_x = 0;
+    P->xp = graph_binpart(P->g);
+    if (!P->tuning) prune_xparts(P->g);
+  }
+  const size_t xb = sizeof(float) * (size_t)P->g->nV;
+  if (P->x[1].bytes < xb) P->x[1].alloc(xb);
+  if (P->p_r.bytes < xb) P->p_r.alloc(xb);
+  if (P->bbuf.for_xp != P->xp.get()) P->bbuf.make(*P->xp, P->g->nV, stream());
+}
 
 // the source-range partition, its segment plan and the segment sums
 void ensure_part_plan(gre_pr* P, int ipt) {
@@ -466,6 +496,10 @@ void ensure_part_plan_adj(gre_pr* P, int ipt) {
 }
 
 void ensure_plan(gre_pr* P, bool csc, int ipt) {
+  if (csc && use_bin(P)) {
+    ensure_bin_plan(P);
+    return;
+  }
   if (csc && use_part(P)) {
     graph_compact_sources(P->g);
     ensure_part_plan(P, ipt);
@@ -530,6 +564,24 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
   ModeFwd<TT> m = fwd_mode<TT>(P, damp);
   m.scal_in = scal_in;
   m.resid = resid;
+  if constexpr (std::is_same<TT, float>::value) {
+    if (use_bin(P)) {
+      BinIter a;
+      a.contrib_in = P->contrib[P->ccur].as<float>();
+      a.contrib_out = P->contrib[P->ccur ^ 1].as<float>();
+      a.x = P->x[P->xcur].as<float>();
+      a.p_r = P->has_p ? P->p_r.as<float>() : nullptr;
+      a.outdeg_r = P->xp->outdeg_r.as<int32_t>();
+      a.scal_in = scal_in;
+      a.scal_out = scal_out;
+      a.damp = damp;
+      a.invn = 1.0 / (double)P->g->nV;
+      a.resid = resid;
+      bin_forward_iter(*P->xp, P->bbuf, a, P->g->nV, s);
+      P->ccur ^= 1;
+      return;
+    }
+  }
   if (use_part(P)) {
     // segment sums over XCD-local source ranges, then each row's segments
     if (P->weighted) pull_pass(P->plan_x, ModeSeg<TT, true>{m.gin, P->xsum.as<TT>(), P->wpart.as<TT>()}, nullptr, s);
@@ -634,7 +686,16 @@ std::vector<uint64_t> fwd_graph_key(const gre_pr* P, double damp, int iters, con
     k.push_back(ptr(pl->slot));
     k.push_back(pl->grid);
   }
+  k.push_back((uint64_t)P->bin);
+  k.push_back(ptr(P->bbuf.binval.p));
+  k.push_back(ptr(P->bbuf.gacc.p));
+  k.push_back(ptr(P->bbuf.part.p));
+  k.push_back(ptr(P->bbuf.part2.p));
+  k.push_back(ptr(P->bbuf.part3.p));
   if (P->xp) {
+    k.push_back((uint64_t)P->xp->binned);
+    k.push_back(ptr(P->xp->bin.src16.p));
+    k.push_back(ptr(P->xp->bin.dloc.p));
     k.push_back((uint64_t)P->xp->R);
     k.push_back(P->xp->H);
     k.push_back(P->xp->nseg);
@@ -728,6 +789,7 @@ void apply_schedule(gre_pr* P, int sc, int ipt) {
   P->nt = sc == GRE_SCHED_TILE_NT || sc == GRE_SCHED_COMPACT_NT || sc == GRE_SCHED_PART_NT;
   P->part = sc == GRE_SCHED_PART || sc == GRE_SCHED_PART_NT;
   P->compact = sc == GRE_SCHED_COMPACT || sc == GRE_SCHED_COMPACT_NT || P->part;
+  P->bin = sc == GRE_SCHED_BIN;
 }
 
 // Autotuner: times every (schedule, tile size) candidate on two forward
@@ -786,6 +848,16 @@ void autotune_forward(gre_pr* P, double damp) {
   };
   for (int sc : scheds)
     for (int ipt : ipts) trial(sc, ipt, false, {P->ranges, P->shared});
+  // the binned pull (FP32 single vector, unweighted): one layout, no tile size
+  std::shared_ptr<XPart> bin_xp;
+  {
+    graph_compact_targets(P->g);  // the vertices with in-edges, which the bins hold
+    if (!bin_refusal(P) && (uint64_t)P->g->ntgt <= (uint64_t)kBinMaxBins * kBinRows) {
+      trial(GRE_SCHED_BIN, 8, false, {P->ranges, P->shared});
+      bin_xp = P->xp;
+      P->xp.reset();
+    }
+  }
   // partitioned schedules try (tail ranges, shared sources) pairs, one
   // partition at a time; the two best partitions (each with its best
   // schedule) stay cached and are timed again against each other at the end,
@@ -837,7 +909,10 @@ void autotune_forward(gre_pr* P, double damp) {
     }
   }
   std::shared_ptr<XPart> best_xp =
-      (best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT) ? top[0].xp : std::shared_ptr<XPart>();
+      (best_s == GRE_SCHED_PART || best_s == GRE_SCHED_PART_NT) ? top[0].xp
+      : best_s == GRE_SCHED_BIN                                 ? bin_xp
+                                                                : std::shared_ptr<XPart>();
+  bin_xp.reset();
   top[1].xp.reset();
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -852,7 +927,7 @@ void autotune_forward(gre_pr* P, double damp) {
   P->wpart_xp.reset();
   P->wpart_ver = ~0ULL;
   P->plan_ipt_x = 0;
-  if (P->part) ensure_plan(P, true, best_i);
+  if (P->part || P->bin) ensure_plan(P, true, best_i);
   best_xp.reset();
   prune_xparts(P->g);
 }
@@ -1023,6 +1098,10 @@ double* noalloc_scratch() {
 // would allocate nothing and launch nothing.
 bool fwd_plan_ready(const gre_pr* P) {
   if (P->ipt == 0) return false;                      // schedule not tuned yet
+  if (use_bin(P)) {
+    if (bin_refusal(P) || !P->xp || !P->xp->binned || P->bbuf.for_xp != P->xp.get()) return false;
+    return P->x[1].bytes >= sizeof(float) * (size_t)P->g->nV && P->p_r.bytes >= sizeof(float) * (size_t)P->g->nV;
+  }
   if (use_compact(P) && !P->g->cid.p) return false;
   if (use_part(P)) {
     if (!P->xp || P->xp->R != P->ranges || P->xp->H != std::min(P->shared, P->g->nsrc)) return false;
@@ -1092,8 +1171,9 @@ int gre_pr_set_graphs(gre_pr* P, int on) {
 int gre_pr_set_schedule(gre_pr* P, int schedule, int tile_items) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_ROWS, GRE_DEV_NULL,
+    GRE_CHECK(schedule >= GRE_SCHED_AUTO && schedule <= GRE_SCHED_BIN, GRE_DEV_NULL,
               "schedule must be one of GRE_SCHED_*");
+    GRE_CHECK(schedule != GRE_SCHED_BIN || P->k == 1, GRE_DEV_NULL, "GRE_SCHED_BIN runs single-vector handles only");
     GRE_CHECK(P->k > 1 ? (schedule == GRE_SCHED_AUTO || schedule == GRE_SCHED_TILE || schedule == GRE_SCHED_ROWS)
                        : schedule != GRE_SCHED_ROWS,
               GRE_DEV_NULL, "batched handles take GRE_SCHED_AUTO, _TILE or _ROWS; GRE_SCHED_ROWS needs k > 1");
@@ -1211,10 +1291,21 @@ int gre_pr_get_edge_weights(gre_pr* P, float* w_raw, float* w_norm) {
 int gre_pr_set_personalization(gre_pr* P, const void* pvec) {
   return guard([&] {
     GRE_CHECK(P, GRE_DEV_NULL, "null handle");
-    if (!pvec) { P->has_p = false; P->p.release(); return; }
+    if (!pvec) { P->has_p = false; P->p_l1 = 1.0; P->p.release(); return; }
     const size_t bytes = tsize(P->dtype) * (size_t)P->g->nV * P->k;
     P->p.alloc(bytes);
     GRE_HIP(hipMemcpyAsync(P->p.p, pvec, bytes, hipMemcpyHostToDevice, stream()));
+    // the largest column L1 norm (the binned schedule's fixed-point sums are bounded by it)
+    double l1 = 0.0;
+    const size_t n = (size_t)P->g->nV;
+    for (int j = 0; j < P->k; ++j) {
+      double c = 0.0;
+      for (size_t v = 0; v < n; ++v)
+        c += std::fabs(P->dtype == GRE_F64 ? static_cast<const double*>(pvec)[v * P->k + j]
+                                           : (double)static_cast<const float*>(pvec)[v * P->k + j]);
+      l1 = std::max(l1, c);
+    }
+    P->p_l1 = l1;
     GRE_HIP(hipStreamSynchronize(stream()));
     P->has_p = true;
     ++P->pver;

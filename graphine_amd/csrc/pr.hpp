@@ -1,5 +1,6 @@
 // The PageRank handle shared by the single-vector and batched kernels.
 #pragma once
+#include "binned.hpp"
 #include "pull.hpp"
 
 #include <cstring>
@@ -41,6 +42,7 @@ struct gre_pr {
   bool nt = false;        // non-temporal neighbour-id stream
   bool compact = false;   // hub-sorted compact source ids (k == 1)
   bool part = false;      // source-range partitioned pull (implies compact)
+  bool bin = false;       // binned pull (single vector, FP32, unweighted)
   bool tuning = false;    // the autotuner is running: keep the candidates' partitions cached
   int ranges = 64;        // tail source ranges of the partitioned pull
   uint32_t shared = 1u << 18;  // hottest sources processed as one range by all XCDs
@@ -54,6 +56,8 @@ struct gre_pr {
   gre::PullBuffers buf_x, buf_px;
   int plan_ipt_x = 0;
   bool has_p = false;
+  double p_l1 = 1.0;      // L1 norm of the personalisation (1 when uniform)
+  gre::BinBuffers bbuf;   // binned pull: bin array, multi-slice sums, partials
   // forward state
   gre::DevBuf x[2];       // ranks (weighted: ping-pong) ; unweighted: x[0] in place
   gre::DevBuf contrib[2]; // gathered per-source values, ping-pong: x/outdeg (unweighted), or the

@@ -343,7 +343,7 @@ class PageRank:
         _ck(lib().gre_pr_create(C.byref(self.h), graph.h, dtype, k, 1 if weighted else 0))
 
     SCHEDULES = {0: "auto", 1: "tile", 2: "tile_nt", 3: "compact", 4: "compact_nt", 5: "part",
-                 6: "part_nt", 7: "rows"}
+                 6: "part_nt", 7: "rows", 8: "bin"}
 
     def set_schedule(self, schedule, tile_items=0):
         _ck(lib().gre_pr_set_schedule(self.h, schedule, tile_items))

@@ -220,10 +220,17 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
  * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
  * ROWS (one group of lanes per destination row, wavefront per vertex).
+ * BIN (single vector, FP32, unweighted, fewer than 2^32 edges, personalisation
+ * of L1 norm at most 1) is a propagation-blocking pull: chunks of source rows
+ * stage their values in LDS and stream each entry's value into per-bin runs,
+ * then each bin of destination rows sums its entries in LDS as 64-bit fixed
+ * point (exact, so the result does not depend on the order of the additions);
+ * its rows are relabelled like PART's.  The adjoint of a BIN handle runs TILE.
  * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest
  * (for batched handles, per direction: forward and adjoint). */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,
-       GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6, GRE_SCHED_ROWS = 7 };
+       GRE_SCHED_COMPACT_NT = 4, GRE_SCHED_PART = 5, GRE_SCHED_PART_NT = 6, GRE_SCHED_ROWS = 7,
+       GRE_SCHED_BIN = 8 };
 int gre_pr_set_schedule(gre_pr* p, int schedule, int tile_items);
 int gre_pr_get_schedule(gre_pr* p, int* schedule, int* tile_items);
 /* PART schedules: the shared_sources hottest compact sources (0 = none; fewer
