@@ -796,7 +796,8 @@ void apply_schedule(gre_pr* P, int sc, int ipt) {
 // iterations after a warm-up one and keeps the fastest.  Candidates: tile
 // sizes of 4/8/16 entries per thread, with and without the non-temporal id
 // stream, and for the single vector (weighted or not) the compact and the
-// partitioned schedules.
+// partitioned schedules; for the unweighted FP32 single vector also the
+// binned pull.
 template <typename TT>
 void autotune_forward(gre_pr* P, double damp) {
   hipStream_t s = stream();
