@@ -82,8 +82,9 @@ __device__ inline void block_sum2_big(double& a, double& b, double* red /*[32]*/
 // ---- iteration kernels ------------------------------------------------------
 
 // Row r's update from its in-edge sum s (the same formula as the tile pulls'
-// forward mode, contribution indexed by row).
-template <bool R>
+// forward mode, contribution indexed by row: rank / out-degree, or the rank
+// itself when the scatter multiplies by normalised edge weights, CRAW).
+template <bool R, bool CRAW>
 struct BinUpdate {
   float* contrib_out;
   float* x;
@@ -108,7 +109,7 @@ struct BinUpdate {
     const float dang = (float)scal_in[0];
     const float y = damp * (s + dang * q.pv) + omd * q.pv;
     x[r] = y;
-    if (q.d > 0) contrib_out[r] = y / (float)q.d;
+    if (q.d > 0) contrib_out[r] = CRAW ? y : y / (float)q.d;
     else a0 += (double)y;
     if (R) a1 += fabs((double)y - (double)q.old);
   }
@@ -134,12 +135,12 @@ static_assert(kBinFix == 58, "fix_to_float and the scatter scale use 2^58");
 // unconditional (lanes past the end read the last entry and store to the sink
 // slot binval[Eb]), so the wait for a round's loads leaves the stores that
 // follow them in flight.
-template <int U>
+template <int U, bool W>
 __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
     const float* __restrict__ contrib, const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ ch_nr,
     const uint32_t* __restrict__ ch_e0, const uint32_t* __restrict__ ch_w0, const uint32_t* __restrict__ ch_q0,
     const uint16_t* __restrict__ src16, const uint4* __restrict__ rmeta, const uint32_t* __restrict__ rbase,
-    uint32_t nch, uint32_t sink, float* __restrict__ binval) {
+    const float* __restrict__ wbin, uint32_t nch, uint32_t sink, float* __restrict__ binval) {
   __shared__ float sv[kChunkRows];
   __shared__ uint32_t rb[kBinMaxBins];
   const uint32_t per = (nch + 7) / 8;
@@ -149,11 +150,13 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
   const uint32_t n = ch_e0[w + 1] - ch_e0[w];
   const uint32_t w0 = ch_w0[w], q0 = ch_q0[w], nq = ch_q0[w + 1] - q0;
   const uint16_t* __restrict__ sp = src16 + w0;
+  const float* __restrict__ wq = W ? wbin + w0 : nullptr;
   const uint4* __restrict__ mp = rmeta + (w0 >> 6);
   constexpr uint32_t step = kScatterBlock * U;
   struct Set {
     uint32_t src[U];
     uint4 m[U];
+    float wt[W ? U : 1];
   };
   // lanes past the end take the last entry, which lies in the 64-entry block of
   // the wave's first lane, so the block index is uniform across the wave
@@ -163,6 +166,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
       const uint32_t i = base + j * kScatterBlock + threadIdx.x;
       const uint32_t ii = i < n ? i : n - 1;
       t.src[j] = __builtin_nontemporal_load(sp + ii);
+      if constexpr (W) t.wt[j] = __builtin_nontemporal_load(wq + ii);
       t.m[j] = mp[__builtin_amdgcn_readfirstlane(ii >> 6)];
     }
   };
@@ -176,7 +180,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
       const uint64_t bits = ((uint64_t)t.m[j].y << 32) | t.m[j].x;
       const uint32_t rank = t.m[j].z + (uint32_t)__popcll(bits & (~0ULL >> (63 - (ii & 63)))) - 1u;
       pos[j] = i < n ? i + rb[rank] : sink;
-      v[j] = sv[t.src[j]];
+      if constexpr (W) v[j] = sv[t.src[j]] * t.wt[j];
+      else v[j] = sv[t.src[j]];
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) binval[pos[j]] = v[j];
@@ -228,11 +233,11 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
 // of gacc (finished by k_bin_fin).  (acc0, acc1) partials per slice.  Rounds
 // alternate between two register sets as in the scatter; lanes past the end
 // add zero to the first entry's row.
-template <bool R, int U>
+template <class Up, int U>
 __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
     const float* __restrict__ binval, const uint16_t* __restrict__ dloc, const uint32_t* __restrict__ sl_e0,
     const uint32_t* __restrict__ sl_bin, const uint32_t* __restrict__ sl_g, const uint32_t* __restrict__ bin_r0,
-    BinUpdate<R> up, unsigned long long* __restrict__ gacc, double* __restrict__ part) {
+    Up up, unsigned long long* __restrict__ gacc, double* __restrict__ part) {
   __shared__ unsigned long long acc[kBinRows];
   __shared__ double red[32];
   const uint32_t s = blockIdx.x;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
   if (g == 0xffffffffu) {
     // every row's data is loaded before any row is updated
     constexpr int Q = kBinRows / kAccumBlock;
-    typename BinUpdate<R>::Pre q[Q];
+    typename Up::Pre q[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       const uint32_t i = threadIdx.x + k * kAccumBlock;
@@ -301,10 +306,10 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
 
 // Finish: the rows of the bins cut into slices (their sums in gacc, which is
 // cleared for the next iteration), then the rows without in-edges (sum 0).
-template <bool R>
+template <class Up>
 __global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_r0,
                                                 const uint32_t* __restrict__ mb_g, uint32_t nmb, uint32_t nmrows,
-                                                uint32_t nd, uint32_t nV, BinUpdate<R> up,
+                                                uint32_t nd, uint32_t nV, Up up,
                                                 unsigned long long* __restrict__ gacc, double* __restrict__ part) {
   __shared__ double red[32];
   double a0 = 0.0, a1 = 0.0;
@@ -391,7 +396,7 @@ __global__ void k_bin_keys(const uint64_t* __restrict__ off, const uint32_t* __r
                            const uint32_t* __restrict__ rowperm, const int32_t* __restrict__ outdeg,
                            const uint32_t* __restrict__ row_ch, const uint32_t* __restrict__ ch_r0,
                            const uint32_t* __restrict__ rowbin, int bbits, uint64_t* __restrict__ key,
-                           uint32_t* __restrict__ val) {
+                           uint64_t* __restrict__ val) {
   for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t t = e / te;
     uint32_t lo = rb[t] > 0 ? rb[t] - 1 : 0, hi = rb[t + 1];
@@ -407,7 +412,7 @@ __global__ void k_bin_keys(const uint64_t* __restrict__ off, const uint32_t* __r
     const uint32_t sl = ru - ch_r0[w];
     const uint32_t rv = rowperm[idx[e]];
     key[e] = ((uint64_t)w << (bbits + kSrcBits)) | ((uint64_t)rowbin[rv] << kSrcBits) | sl;
-    val[e] = rv;
+    val[e] = (e << 32) | rv;
   }
 }
 
@@ -433,12 +438,14 @@ __global__ void k_bin_runs(const uint64_t* __restrict__ key, uint64_t E, int bbi
   }
 }
 
-// dloc at each entry's bin-array position; bad[0] counts positions out of
-// range, bad[1] accumulates the positions (their sum is checked on the host)
-__global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t E,
+// dloc at each entry's bin-array position and the entry's CSR position at its
+// word; bad[0] counts positions out of range, bad[1] accumulates the positions
+// (their sum is checked on the host)
+__global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint64_t* __restrict__ val, uint64_t E,
                            uint64_t Eb, int bbits, uint32_t B, const uint32_t* __restrict__ ch_e0,
-                           const uint32_t* __restrict__ runbase, const uint32_t* __restrict__ bin_r0,
-                           uint16_t* __restrict__ dloc, unsigned long long* __restrict__ bad) {
+                           const uint32_t* __restrict__ ch_w0, const uint32_t* __restrict__ runbase,
+                           const uint32_t* __restrict__ bin_r0, uint16_t* __restrict__ dloc,
+                           uint32_t* __restrict__ wpos, unsigned long long* __restrict__ bad) {
   const uint64_t bmask = (1ULL << bbits) - 1;
   unsigned long long nbad = 0, psum = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -446,8 +453,9 @@ __global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint32_t* __r
     const uint32_t b = (uint32_t)(wb & bmask);
     const uint64_t w = wb >> bbits;
     const uint32_t pos = (uint32_t)i - ch_e0[w] + runbase[w * B + b];
+    wpos[ch_w0[w] + ((uint32_t)i - ch_e0[w])] = (uint32_t)(val[i] >> 32);
     if (pos < Eb) {
-      dloc[pos] = (uint16_t)(val[i] - bin_r0[b]);
+      dloc[pos] = (uint16_t)((uint32_t)val[i] - bin_r0[b]);
       psum += pos;
     } else {
       ++nbad;
@@ -595,23 +603,23 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
     uint64_t nt = 0;
     const uint32_t* rb = graph_tiles(g, false, 2048, &nt);
     keys0.alloc(8 * E);
-    vals0.alloc(4 * E);
+    vals0.alloc(8 * E);
     hipLaunchKernelGGL(k_bin_keys, dim3(gcap(E)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), rb,
                        2048u, E, xp->rowperm.as<uint32_t>(), g->outdeg.as<int32_t>(), rowch_d.as<uint32_t>(),
                        bp.ch_r0.as<uint32_t>(), rowbin.as<uint32_t>(), bbits, keys0.as<uint64_t>(),
-                       vals0.as<uint32_t>());
+                       vals0.as<uint64_t>());
     GRE_HIP(hipGetLastError());
     GRE_HIP(hipStreamSynchronize(s));
   }
   keys1.alloc(8 * E);
-  vals1.alloc(4 * E);
+  vals1.alloc(8 * E);
   {
     size_t tb = 0;
-    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint32_t>(),
-                                      vals1.as<uint32_t>(), (size_t)E, 0, (unsigned)kbits, s));
+    GRE_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint64_t>(),
+                                      vals1.as<uint64_t>(), (size_t)E, 0, (unsigned)kbits, s));
     void* tmp = scratch(0, tb);
-    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint32_t>(),
-                                      vals1.as<uint32_t>(), (size_t)E, 0, (unsigned)kbits, s));
+    GRE_HIP(rocprim::radix_sort_pairs(tmp, tb, keys0.as<uint64_t>(), keys1.as<uint64_t>(), vals0.as<uint64_t>(),
+                                      vals1.as<uint64_t>(), (size_t)E, 0, (unsigned)kbits, s));
     GRE_HIP(hipStreamSynchronize(s));
   }
   keys0.release();
@@ -735,9 +743,12 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
     bad.alloc(16);
     GRE_HIP(hipMemsetAsync(bad.p, 0, 16, s));
     GRE_HIP(hipMemsetAsync(bp.dloc.p, 0, 2 * Eb, s));
-    hipLaunchKernelGGL(k_bin_dloc, dim3(gcap(E)), dim3(kB), 0, s, keys1.as<uint64_t>(), vals1.as<uint32_t>(), E, Eb,
-                       bbits, B, bp.ch_e0.as<uint32_t>(), runbase_d.as<uint32_t>(), bp.bin_r0.as<uint32_t>(),
-                       bp.dloc.as<uint16_t>(), bad.as<unsigned long long>());
+    bp.wpos.alloc(4 * Ew);
+    GRE_HIP(hipMemsetAsync(bp.wpos.p, 0, 4 * Ew, s));
+    hipLaunchKernelGGL(k_bin_dloc, dim3(gcap(E)), dim3(kB), 0, s, keys1.as<uint64_t>(), vals1.as<uint64_t>(), E, Eb,
+                       bbits, B, bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(), runbase_d.as<uint32_t>(),
+                       bp.bin_r0.as<uint32_t>(), bp.dloc.as<uint16_t>(), bp.wpos.as<uint32_t>(),
+                       bad.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
     unsigned long long h[2] = {0, 0};
     GRE_HIP(hipMemcpyAsync(h, bad.p, 16, hipMemcpyDeviceToHost, s));
@@ -771,29 +782,30 @@ void BinBuffers::make(const XPart& xp, uint32_t nV, hipStream_t s) {
 }
 
 namespace {
-template <bool R>
+template <bool R, bool W>
 void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s) {
   const BinPlan& bp = xp.bin;
-  BinUpdate<R> up{a.contrib_out, a.x, a.p_r, a.outdeg_r, a.scal_in, (float)a.damp, (float)(1.0 - a.damp),
-                  (float)a.invn};
+  using Up = BinUpdate<R, W>;
+  Up up{a.contrib_out, a.x, a.p_r, a.outdeg_r, a.scal_in, (float)a.damp, (float)(1.0 - a.damp), (float)a.invn};
   {
     ProfScope ps("bin_scatter");
     const unsigned per = (bp.nch + 7) / 8;
-    hipLaunchKernelGGL((k_bin_scatter<kScatterU>), dim3(8 * per), dim3(kScatterBlock), 0, s, a.contrib_in,
+    hipLaunchKernelGGL((k_bin_scatter<kScatterU, W>), dim3(8 * per), dim3(kScatterBlock), 0, s, a.contrib_in,
                        bp.ch_r0.as<uint32_t>(), bp.ch_nr.as<uint32_t>(), bp.ch_e0.as<uint32_t>(),
                        bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(), bp.src16.as<uint16_t>(),
-                       bp.rmeta.as<uint4>(), bp.rbase.as<uint32_t>(), bp.nch, (uint32_t)bp.Eb, bb.binval.as<float>());
+                       bp.rmeta.as<uint4>(), bp.rbase.as<uint32_t>(), a.wbin, bp.nch, (uint32_t)bp.Eb,
+                       bb.binval.as<float>());
   }
   {
     ProfScope ps("bin_accum");
-    hipLaunchKernelGGL((k_bin_accum<R, kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<float>(),
+    hipLaunchKernelGGL((k_bin_accum<Up, kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<float>(),
                        bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(), bp.sl_bin.as<uint32_t>(),
                        bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), up, bb.gacc.as<unsigned long long>(),
                        bb.part.as<double>());
   }
   {
     ProfScope ps("bin_finish");
-    hipLaunchKernelGGL((k_bin_fin<R>), dim3(bb.fin_grid), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
+    hipLaunchKernelGGL((k_bin_fin<Up>), dim3(bb.fin_grid), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
                        bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, bp.nd, nV, up, bb.gacc.as<unsigned long long>(),
                        bb.part2.as<double>());
   }
@@ -803,8 +815,13 @@ void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_
 }  // namespace
 
 void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s) {
-  if (a.resid) bin_iter_t<true>(xp, bb, a, nV, s);
-  else bin_iter_t<false>(xp, bb, a, nV, s);
+  if (a.wbin) {
+    if (a.resid) bin_iter_t<true, true>(xp, bb, a, nV, s);
+    else bin_iter_t<false, true>(xp, bb, a, nV, s);
+  } else {
+    if (a.resid) bin_iter_t<true, false>(xp, bb, a, nV, s);
+    else bin_iter_t<false, false>(xp, bb, a, nV, s);
+  }
 }
 
 }  // namespace gre

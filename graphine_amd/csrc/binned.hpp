@@ -17,8 +17,12 @@ struct BinBuffers {
 };
 
 // One iteration's operands.  Ranks, personalisation and out-degrees are in the
-// layout's row order; contributions (rank / out-degree) are indexed by row.
+// layout's row order; contributions are indexed by row.
 struct BinIter {
+  // weighted: the normalised edge weights in word order (BinPlan::wpos), and
+  // contributions are the ranks themselves; nullptr: contributions are rank /
+  // out-degree
+  const float* wbin = nullptr;
   const float* contrib_in = nullptr;
   float* contrib_out = nullptr;
   float* x = nullptr;            // ranks, updated in place

@@ -44,6 +44,7 @@ struct BinPlan {
   DevBuf rmeta;    // uint4 [Ew/64]
   DevBuf ch_q0;    // uint32 [nch+1] first run of chunk w in rbase
   DevBuf rbase;    // uint32 [runs] bin-array position of each run minus its chunk-local start (mod 2^32)
+  DevBuf wpos;     // uint32 [Ew] CSR position of each word's entry (edge weights follow it; 0 in the gaps)
   DevBuf dloc;     // uint16 [Eb] bin order: destination row - bin_r0[bin] (0 in the gaps)
   DevBuf bin_r0;   // uint32 [B+1] first row of each bin
   DevBuf sl_e0;    // uint32 [nsl+1] slice ranges in the bin array (a bin's last slice takes its gap)

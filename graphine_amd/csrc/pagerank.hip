@@ -373,8 +373,15 @@ void ensure_scal(gre_pr* P, int iters) {
 // order (compact) or in the partition's entry order (part).
 bool use_compact(const gre_pr* P) { return P->compact && P->k == 1; }
 bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
-// the binned pull: single vector, FP32, unweighted
+// the binned pull: single vector, FP32
 bool use_bin(const gre_pr* P) { return P->bin && P->k == 1; }
+// The single-vector adjoint's schedule family: the compact and partitioned
+// forwards' own, and for the binned forward (which has no transposed layout)
+// the partitioned adjoint over the CSR with the handle's (ranges, shared).
+bool adj_part(const gre_pr* P) {
+  return use_part(P) || (use_bin(P) && P->g->nE > 0 && P->g->nE < 0xffffffffULL);
+}
+bool adj_compact(const gre_pr* P) { return use_compact(P) || use_bin(P); }
 // the partitioned and binned forwards walk relabelled rows: a run keeps its
 // ranks in the layout's row order until ranks_to_vertices
 bool rows_relabelled(const gre_pr* P) { return (use_part(P) || use_bin(P)) && P->xp && P->xp->permuted; }
@@ -383,7 +390,6 @@ bool rows_relabelled(const gre_pr* P) { return (use_part(P) || use_bin(P)) && P-
 const char* bin_refusal(const gre_pr* P) {
   if (P->k != 1) return "the binned schedule runs single-vector handles only";
   if (P->dtype != GRE_F32) return "the binned schedule runs FP32 handles only";
-  if (P->weighted) return "the binned schedule runs unweighted handles only";
   if (P->g->nE == 0 || P->g->nE >= 0xffffffffULL) return "the binned schedule needs 0 < edges < 2^32";
   if (P->has_p && P->p_l1 > kBinMaxPL1)
     return "the binned schedule needs a personalisation of L1 norm at most 1 (its fixed-point sums are bounded by it)";
@@ -405,6 +411,17 @@ void ensure_bin_plan(gre_pr* P) {
   if (P->x[1].bytes < xb) P->x[1].alloc(xb);
   if (P->p_r.bytes < xb) P->p_r.alloc(xb);
   if (P->bbuf.for_xp != P->xp.get()) P->bbuf.make(*P->xp, P->g->nV, stream());
+  // weighted: the normalised weights in the layout's word order
+  if (P->weighted && (P->wpart_xp != P->xp || P->wpart_ver != P->wver)) {
+    const uint64_t Ew = P->xp->bin.Ew;
+    hipStream_t s = stream();
+    P->wpart.alloc(sizeof(float) * Ew);
+    hipLaunchKernelGGL(k_permute32<float>, dim3(gcap(Ew)), dim3(kB), 0, s, P->wcsr.as<float>(),
+                       P->xp->bin.wpos.as<uint32_t>(), Ew, P->wpart.as<float>());
+    GRE_HIP(hipGetLastError());
+    P->wpart_xp = P->xp;
+    P->wpart_ver = P->wver;
+  }
 }
 
 // the source-range partition, its segment plan and the segment sums
@@ -505,7 +522,7 @@ void ensure_plan(gre_pr* P, bool csc, int ipt) {
     ensure_part_plan(P, ipt);
     return;
   }
-  if (!csc && use_part(P)) {
+  if (!csc && adj_part(P)) {
     graph_compact_targets(P->g);
     ensure_part_plan_adj(P, ipt);
     return;
@@ -523,7 +540,7 @@ void ensure_plan(gre_pr* P, bool csc, int ipt) {
     plan.idx = P->g->cidx_c.as<uint32_t>();
   }
   // the adjoint of the compact schedules gathers lambda by compact target id
-  if (!csc && use_compact(P)) {
+  if (!csc && adj_compact(P)) {
     graph_compact_targets(P->g);
     plan.idx = P->g->idx_c.as<uint32_t>();
   }
@@ -567,6 +584,7 @@ void fwd_iter(gre_pr* P, double damp, const double* scal_in, double* scal_out, h
   if constexpr (std::is_same<TT, float>::value) {
     if (use_bin(P)) {
       BinIter a;
+      a.wbin = P->weighted ? P->wpart.as<float>() : nullptr;
       a.contrib_in = P->contrib[P->ccur].as<float>();
       a.contrib_out = P->contrib[P->ccur ^ 1].as<float>();
       a.x = P->x[P->xcur].as<float>();
@@ -905,6 +923,9 @@ void autotune_forward(gre_pr* P, double damp) {
     if (part_won && top[0].xp) {
       best_s = top[0].sc;
       best_i = top[0].ipt;
+    }
+    // the best partition's (ranges, shared) also serve a binned forward's adjoint
+    if (top[0].xp) {
       best_r = top[0].h.first;
       best_h = top[0].h.second;
     }
@@ -941,7 +962,7 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
   double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
   ensure_plan(P, false, P->ipt);
   ensure_scal(P, iters);
-  const bool cpt = use_compact(P);
+  const bool cpt = adj_compact(P);
   if (cpt && P->lc[0].bytes < tsize(P->dtype) * (size_t)nV) {
     P->lc[0].alloc(tsize(P->dtype) * (size_t)nV);
     P->lc[1].alloc(tsize(P->dtype) * (size_t)nV);
@@ -971,7 +992,7 @@ double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
     m.scal_in = sc + 2 * it;
     m.damp = damp;
     m.invn = 1.0 / (double)nV;
-    if (use_part(P)) {
+    if (adj_part(P)) {
       // segment sums over XCD-local ranges of compact targets, then each row's segments
       if (P->weighted)
         pull_pass(P->plan_xa, ModeSeg<TT, true>{m.gin, P->asum.as<TT>(), P->wparta.as<TT>()}, nullptr, s);
@@ -1101,6 +1122,7 @@ bool fwd_plan_ready(const gre_pr* P) {
   if (P->ipt == 0) return false;                      // schedule not tuned yet
   if (use_bin(P)) {
     if (bin_refusal(P) || !P->xp || !P->xp->binned || P->bbuf.for_xp != P->xp.get()) return false;
+    if (P->weighted && (P->wpart_xp != P->xp || P->wpart_ver != P->wver)) return false;
     return P->x[1].bytes >= sizeof(float) * (size_t)P->g->nV && P->p_r.bytes >= sizeof(float) * (size_t)P->g->nV;
   }
   if (use_compact(P) && !P->g->cid.p) return false;

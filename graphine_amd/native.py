@@ -4,8 +4,10 @@ This is plumbing for tests and the benchmark: it passes numpy buffers to the
 C ABI and raises on any non-zero status.  There is no fallback: if the HIP
 library is missing or fails to load, every entry point raises.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -34,6 +36,27 @@ STATE_DTYPES = {
 }
 
 _lib = None
+
+# Handles still open when the interpreter exits are closed while the HIP
+# runtime is up (engines and PageRank handles before the graphs they use);
+# after that, finalisers do nothing.
+_live_handles = weakref.WeakSet()
+_live_graphs = weakref.WeakSet()
+_exiting = False
+
+
+def _close_all():
+    global _exiting
+    for pool in (_live_handles, _live_graphs):
+        for o in list(pool):
+            try:
+                o.close()
+            except Exception:
+                pass
+    _exiting = True
+
+
+atexit.register(_close_all)
 
 
 class GreError(RuntimeError):
@@ -192,6 +215,7 @@ class Graph:
 
     def __init__(self, handle):
         self.h = handle
+        _live_graphs.add(self)
         nv = C.c_uint32()
         ne = C.c_uint64()
         _ck(lib().gre_graph_info(self.h, C.byref(nv), C.byref(ne)))
@@ -250,6 +274,8 @@ class Graph:
             self.h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:
@@ -264,6 +290,7 @@ class Engine:
         self.prog = prog
         self.h = C.c_void_p()
         _ck(lib().gre_engine_create(C.byref(self.h), graph.h, prog))
+        _live_handles.add(self)
 
     max_ss = 1024
 
@@ -325,6 +352,8 @@ class Engine:
             self.h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:
@@ -341,6 +370,7 @@ class PageRank:
         self.np_dtype = np.float64 if dtype == F64 else np.float32
         self.h = C.c_void_p()
         _ck(lib().gre_pr_create(C.byref(self.h), graph.h, dtype, k, 1 if weighted else 0))
+        _live_handles.add(self)
 
     SCHEDULES = {0: "auto", 1: "tile", 2: "tile_nt", 3: "compact", 4: "compact_nt", 5: "part",
                  6: "part_nt", 7: "rows", 8: "bin"}
@@ -431,6 +461,8 @@ class PageRank:
             self.h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:

@@ -214,18 +214,20 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * sums in a second pull (single vector, fewer than 2^32 edges; ranges set with
  * gre_pr_set_partition; weighted handles keep a copy of the weights in the
  * partition's entry order, remade when the weights or the partition change).
- * Under COMPACT and PART the single-vector adjoint pulls over the CSR in
+ * Under COMPACT, PART and BIN the single-vector adjoint pulls over the CSR in
  * compact target ids (vertices with in-edges by descending in-degree); under
- * PART with the same ranges, partitioned over the CSR.
+ * PART and BIN, partitioned over the CSR with the handle's ranges (for BIN,
+ * those of the best partition the autotuner timed, or the defaults).
  * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
  * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
  * ROWS (one group of lanes per destination row, wavefront per vertex).
- * BIN (single vector, FP32, unweighted, fewer than 2^32 edges, personalisation
- * of L1 norm at most 1) is a propagation-blocking pull: chunks of source rows
- * stage their values in LDS and stream each entry's value into per-bin runs,
- * then each bin of destination rows sums its entries in LDS as 64-bit fixed
- * point (exact, so the result does not depend on the order of the additions);
- * its rows are relabelled like PART's.  The adjoint of a BIN handle runs TILE.
+ * BIN (single vector, FP32, fewer than 2^32 edges, personalisation of L1 norm
+ * at most 1) is a propagation-blocking pull: chunks of source rows stage their
+ * values in LDS and stream each entry's value into per-bin runs (weighted
+ * handles multiply by the normalised weights, kept in the layout's entry
+ * order), then each bin of destination rows sums its entries in LDS as 64-bit
+ * fixed point (exact, so the result does not depend on the order of the
+ * additions); its rows are relabelled like PART's.
  * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest
  * (for batched handles, per direction: forward and adjoint). */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,

@@ -247,6 +247,6 @@ def test_c5_batched_ppr_adjoint_32_vectors(N, O):
         assert rel_l1(X[:, j], sp.ranks().astype(np.float64)) < 1e-6
         sp.adjoint(tg, d, its)
         assert rel_l1(LAM[:, j], sp.adjoint_vector().astype(np.float64)) < 1e-6
-    assert sp.schedule()[0].startswith("part")
+    assert sp.schedule()[0].startswith(("bin", "part"))   # the autotuned single-vector schedules at this size
     sp.close()
     g.close()
