@@ -170,6 +170,49 @@ def test_modes_pagerank_partitioned(N, O):
         assert np.allclose(e.download(N.ARR_VERTEX), ost[2], rtol=1e-12, atol=0)
 
 
+def test_modes_pagerank_partitioned_long_rows(N, O):
+    """GRE's PageRank program on the partitioned pull where a destination has
+    1.2 M in-edges (its row spans hundreds of segment tiles) and a source has
+    300,000 out-edges; R-MAT edges around them.  Counts exact; every vertex but
+    the hub to 1e-12 of the oracle.  The hub is pinned to its exact value:
+    GRE's PageRank scatters its init value 0.15/outdeg every superstep, so the
+    hub holds 0.15 + 0.85 * the sum over its in-neighbours of 0.15/outdeg,
+    which math.fsum rounds correctly; both the device's fixed-order sum and the
+    oracle's one-by-one sum of the 1.2 M terms stay within the n*u bound of a
+    recursive sum.  A second engine gives the same bits."""
+    import math
+    base = O.rmat_graph(19, 16, 1, False)
+    n = 1 << 21
+    rsrc = np.repeat(np.arange(base.nV, dtype=np.uint32), base.outdeg)
+    star_in = np.arange(1 << 19, (1 << 19) + 1_200_000, dtype=np.uint32)
+    src = np.concatenate([rsrc, star_in, np.full(300_000, 5, np.uint32)])
+    dst = np.concatenate([base.idx, np.full(star_in.size, 7, np.uint32),
+                          np.arange(1 << 20, (1 << 20) + 300_000, dtype=np.uint32)])
+    g, og = N.Graph.from_edges(n, src, dst), O.graph_from_edges(n, src, dst)
+    hub = 7
+    assert len(og.idx) >= 1 << 22 and int(og.indeg[hub]) > 1 << 20
+    ins = src[dst == hub]
+    terms = [0.15 / float(og.outdeg[u]) for u in ins]
+    exact = 0.15 + 0.85 * math.fsum(terms)
+    bound = len(terms) * 2.0 ** -53 * exact
+    others = np.arange(n) != hub
+    for mode in (2, 0):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=8)
+        assert ss == oss == 8 and np.array_equal(counts, ocounts)
+        assert e.pull_steps() == 8
+        x = e.download(N.ARR_VERTEX)
+        assert np.allclose(x[others], ost[2][others], rtol=1e-12, atol=0)
+        assert abs(x[hub] - exact) <= bound
+        assert abs(ost[2][hub] - exact) <= bound
+        e.close()
+        e2 = N.Engine(g, N.PAGERANK)
+        e2.set_mode(mode)
+        e2.config_max_supersteps(8)
+        e2.run(N.DIR_OUT)
+        assert np.array_equal(e2.download(N.ARR_VERTEX), x)
+        e2.close()
+
+
 @pytest.mark.parametrize("mode", [2, 0])
 def test_modes_pagerank_delta_partitioned(N, O, mode):
     # PageRank_delta's first superstep has a complete frontier, so from 2^22
