@@ -8,7 +8,7 @@ namespace gre {
 
 // Per-handle working memory of the binned forward.
 struct BinBuffers {
-  DevBuf binval;           // float [E] each entry's contribution, bin order
+  DevBuf binval;           // float [Eb] each slot's group sum, bin order
   DevBuf gacc;             // int64 [nmrows] sums of the bins cut into several slices (zero between iterations)
   DevBuf part, part2, part3;
   unsigned fin_grid = 0;   // workgroups of the finishing pass

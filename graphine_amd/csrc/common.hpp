@@ -56,18 +56,30 @@ void ensure_init();
 
 // device memory owned by a handle
 struct DevBuf {
-  void* p = nullptr;
+  void* p = nullptr;     // the buffer (aligned as asked)
   size_t bytes = 0;
+  void* raw = nullptr;   // what hipMalloc returned
+  size_t held = 0;       // bytes allocated
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), raw(o.raw), held(o.held) {
+    o.p = o.raw = nullptr;
+    o.bytes = o.held = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes; raw = o.raw; held = o.held;
+      o.p = o.raw = nullptr;
+      o.bytes = o.held = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
-  void alloc(size_t n);
+  // align (a power of two; ignored up to 256, hipMalloc's own alignment): the
+  // buffer starts on that boundary, at the cost of up to align more bytes
+  void alloc(size_t n, size_t align = 0);
   void release();
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };

@@ -33,19 +33,22 @@ struct BinPlan {
   uint64_t E = 0;      // entries
   uint64_t Ew = 0;     // words, with each chunk's gap
   uint64_t Eb = 0;     // bin-array slots, with each bin's gap
+  uint64_t ngrp = 0;   // groups (one slot each): entries of one chunk to one destination, cut at 16-word rows
   uint32_t nd = 0, B = 0, nch = 0, nsl = 0, nmb = 0, nmrows = 0;
   DevBuf ch_r0;    // uint32 [nch] first source row of chunk w
   DevBuf ch_nr;    // uint32 [nch] source rows chunk w stages
   DevBuf ch_e0;    // uint32 [nch+1] entries before chunk w (chunk w has ch_e0[w+1] - ch_e0[w])
   DevBuf ch_w0;    // uint32 [nch] first word of chunk w (a multiple of 64)
-  DevBuf src16;    // uint16 [Ew] scatter order (by bin, then source): source row - ch_r0
-  // per 64 words: (bit j set if word 64k + j starts a run: uint64 as two
-  // uint32, runs of the chunk started before word 64k, 0)
-  DevBuf rmeta;    // uint4 [Ew/64]
+  // uint16 [Ew] scatter order (by bin, then destination): source row - ch_r0
+  // (14 bits) | run start << 14 | group start << 15
+  DevBuf src16;
+  // per 64 words: (runs of the chunk started before word 64k + bit 0 of the
+  // block's run starts - 1) | (groups of the chunk before word 64k) << 12
+  DevBuf rcnt;     // uint32 [Ew/64]
   DevBuf ch_q0;    // uint32 [nch+1] first run of chunk w in rbase
-  DevBuf rbase;    // uint32 [runs] bin-array position of each run minus its chunk-local start (mod 2^32)
+  DevBuf rbase;    // uint32 [runs] first slot of each run minus the chunk's groups before it (mod 2^32)
   DevBuf wpos;     // uint32 [Ew] CSR position of each word's entry (edge weights follow it; 0 in the gaps)
-  DevBuf dloc;     // uint16 [Eb] bin order: destination row - bin_r0[bin] (0 in the gaps)
+  DevBuf dloc;     // uint16 [Eb] each slot's destination row - bin_r0[bin] (0 in the gaps)
   DevBuf bin_r0;   // uint32 [B+1] first row of each bin
   DevBuf sl_e0;    // uint32 [nsl+1] slice ranges in the bin array (a bin's last slice takes its gap)
   DevBuf sl_bin;   // uint32 [nsl] bin of each slice

@@ -60,27 +60,32 @@ hipStream_t stream() {
   return g_stream;
 }
 
-void DevBuf::alloc(size_t n) {
+void DevBuf::alloc(size_t n, size_t align) {
   release();
   if (n == 0) n = 8;
   ensure_init();
-  hipError_t e = hipMalloc(&p, n);
+  const size_t pad = align > 256 ? align : 0;
+  hipError_t e = hipMalloc(&raw, n + pad);
   if (e != hipSuccess) {
-    p = nullptr;
+    raw = nullptr;
     (void)hipGetLastError();
-    throw Error(GRE_DEV_NOBUFFER, "hipMalloc(" + std::to_string(n) + ") failed: " + hipGetErrorString(e));
+    throw Error(GRE_DEV_NOBUFFER, "hipMalloc(" + std::to_string(n + pad) + ") failed: " + hipGetErrorString(e));
   }
+  p = pad ? reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(raw) + pad - 1) / pad * pad) : raw;
   bytes = n;
-  g_bytes += n;
+  held = n + pad;
+  g_bytes += held;
 }
 
 void DevBuf::release() {
-  if (p) {
-    (void)hipFree(p);
-    g_bytes -= bytes;
+  if (raw) {
+    (void)hipFree(raw);
+    g_bytes -= held;
   }
+  raw = nullptr;
   p = nullptr;
   bytes = 0;
+  held = 0;
 }
 
 void* scratch(int slot, size_t bytes) {

@@ -223,11 +223,12 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * ROWS (one group of lanes per destination row, wavefront per vertex).
  * BIN (single vector, FP32, fewer than 2^32 edges, personalisation of L1 norm
  * at most 1) is a propagation-blocking pull: chunks of source rows stage their
- * values in LDS and stream each entry's value into per-bin runs (weighted
- * handles multiply by the normalised weights, kept in the layout's entry
- * order), then each bin of destination rows sums its entries in LDS as 64-bit
- * fixed point (exact, so the result does not depend on the order of the
- * additions); its rows are relabelled like PART's.
+ * values in LDS, sum each destination row's entries of the chunk (at most 16
+ * at a time, in a fixed order) and stream the sums into per-bin runs
+ * (weighted handles multiply by the normalised weights, kept in the layout's
+ * entry order), then each bin of destination rows adds its sums in LDS as
+ * 64-bit fixed point (exact, so the result does not depend on the order of
+ * the additions); its rows are relabelled like PART's.
  * GRE_SCHED_AUTO times every candidate on the first run and keeps the fastest
  * (for batched handles, per direction: forward and adjoint). */
 enum { GRE_SCHED_AUTO = 0, GRE_SCHED_TILE = 1, GRE_SCHED_TILE_NT = 2, GRE_SCHED_COMPACT = 3,

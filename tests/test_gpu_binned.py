@@ -1,10 +1,12 @@
 """GPU parity of the binned PageRank pull (GRE_SCHED_BIN, FP32, single vector)
 through the C ABI, against the FP64 oracle.
 
-The binned pull sums each row's in-edges as 64-bit fixed point (exact, then
-one rounding to FP32), so it is deterministic: handles, captured replays and
-eager runs give the same bits.  Tolerance against the FP64 oracle: 1e-6
-relative L1 (the bar for FP32 PageRank).
+The binned pull sums the entries of one chunk to one destination (at most 16
+consecutive ones) in FP64 and rounds the sum once to FP32, then sums each
+row's partial sums as 64-bit fixed point (exact, then one rounding to FP32),
+so it is deterministic: handles, captured replays and eager runs give the same
+bits.  Tolerance against the FP64 oracle: 1e-6 relative L1 (the bar for FP32
+PageRank).
 """
 import ctypes as C
 
@@ -57,8 +59,8 @@ def test_binned_scrambled_labels(N, O):
 
 def test_binned_edge_cases(N, O):
     """A source whose out-row is cut into three scatter chunks (300,000 entries),
-    a destination whose 1.2 M in-edges make its bin take several accumulate
-    slices (their sums meet in 64-bit integers in HBM), isolated vertices,
+    a destination with 1.2 M in-edges (runs of 16,384 entries to one row, each
+    cut into 1,024 groups of 16), isolated vertices,
     vertices with only in- or only out-edges, and a 2-cycle."""
     n_in, n_out = 1_200_000, 300_000
     n = n_in + 10
@@ -80,6 +82,30 @@ def test_binned_edge_cases(N, O):
         assert rel_l1(x, ref) < 1e-6
         # vertex-level: the hub destination, a leaf of the out-star, an isolated vertex
         for v in (0, 1, 2, n_out + 1, n - 1, n - 2):
+            assert abs(float(x[v]) - ref[v]) <= 1e-6 * ref[v] + 1e-12, v
+
+
+def test_binned_multi_slice_bin(N, O):
+    """One bin of 16,384 destinations fed by 2,000,000 out-degree-1 sources,
+    source s -> s mod 16,384: the sources are staged in vertex order, so every
+    chunk of 16,384 of them reaches each destination once and no entries
+    share a slot; the bin's 2,000,000 slots take two accumulate slices, whose
+    sums meet in 64-bit integers in HBM."""
+    nd, ns = 16384, 2_000_000
+    n = nd + ns
+    src = np.arange(nd, n, dtype=np.uint32)
+    dst = (np.arange(ns, dtype=np.uint64) % nd).astype(np.uint32)
+    g = N.Graph.from_edges(n, src, dst)
+    og = O.graph_from_edges(n, src, dst)
+    ref = O.pagerank(og, 0.85, 20)
+    for graphs in (True, False):
+        pr = binned(N, g)
+        pr.set_graphs(graphs)
+        it, _ = pr.run(0.85, 20)
+        assert it == 20
+        x = pr.ranks()
+        assert rel_l1(x, ref) < 1e-6
+        for v in (0, 1, 5000, nd - 1, nd, n - 1):
             assert abs(float(x[v]) - ref[v]) <= 1e-6 * ref[v] + 1e-12, v
 
 
