@@ -33,6 +33,7 @@
 #include "pull.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -49,6 +50,12 @@ static_assert(kBinMaxBins <= 65536 && kChunkRows <= 65536, "a scatter word holds
 static_assert(kBinRows <= 65536, "dloc is 16 bits");
 constexpr int kScatterBlock = 1024;
 constexpr int kScatterU = 8;     // entries in flight per thread
+// the engine's FP64 scatter: 4 entries in flight per thread measured faster
+// than 8 (3.50 against 3.79-3.88 ms at R-MAT scale 26), and chunks of 16,384
+// rows (a 128 KB stage, one workgroup per CU) faster than 8,192 (two per CU,
+// runs half as long: 3.80-4.52 ms)
+constexpr int kEngScatterU = 4;
+constexpr int kEngChunkRows = 16384;
 constexpr int kAccumBlock = 1024;
 constexpr int kAccumU = 8;
 constexpr uint32_t kBinKey = 1u << 14;  // cache key of the layout among a graph's partitions
@@ -66,6 +73,12 @@ unsigned bits_of(uint64_t n) {
   unsigned b = 1;
   while (b < 63 && (1ULL << b) < n) ++b;
   return b;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
 }
 
 // fixed-order sum of two doubles over a block of up to 1024 threads; thread 0
@@ -136,6 +149,7 @@ static_assert(kBinFix == 58, "fix_to_float and the scatter scale use 2^58");
 // the chunk before the block + bit 0 of its run starts - 1) | (groups of the
 // chunk before the block) << 12.
 constexpr int kGroupLanes = 16;   // a DPP row
+constexpr uint32_t kTrashBlocks = 256;  // FP64 scatter: workgroups (mod this) with trash slots of their own
 constexpr int kRunBits = 12;      // runs of a chunk <= kBinMaxBins
 constexpr uint32_t kWordRun = 1u << 14, kWordGroup = 1u << 15;
 static_assert(kSrcBits == 14, "a scatter word holds a 14-bit source and two flags");
@@ -156,6 +170,29 @@ __device__ inline float group_step(float s, uint64_t m) {
 // lanes max(head, l - 2D + 1) .. l of its group; groups lie in one row, so
 // four steps finish them.
 __device__ inline float group_scan(float s, uint64_t c) {
+  s = group_step<1>(s, c);
+  c &= c << 1;
+  s = group_step<2>(s, c);
+  c &= c << 2;
+  s = group_step<4>(s, c);
+  c &= c << 4;
+  return group_step<8>(s, c);
+}
+
+// the same for FP64 values (the engine's sums): two 32-bit DPP moves per shift
+template <int D>
+__device__ inline double row_shr64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x110 | D, 0xf, 0xf, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x110 | D, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int D>
+__device__ inline double group_step(double s, uint64_t m) {
+  const double y = row_shr64<D>(s);
+  return __builtin_amdgcn_inverse_ballot_w64(m) ? s + y : s;
+}
+__device__ inline double group_scan(double s, uint64_t c) {
   s = group_step<1>(s, c);
   c &= c << 1;
   s = group_step<2>(s, c);
@@ -189,13 +226,14 @@ extern "C" __device__ void gre_buffer_store_idx(float v, __amdgpu_buffer_rsrc_t 
 // record, which the hardware drops.  A resource's byte offsets are 32-bit, so
 // a bin array of 2^30 slots or more (BIG) is stored through four resources of
 // 2^30 slots each, one store per resource.
-template <int U, bool W, bool BIG>
+template <int U, bool W, bool BIG, typename T = float, int CR = kChunkRows>
 __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
-    const float* __restrict__ contrib, const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ ch_nr,
+    const T* __restrict__ contrib, const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ ch_nr,
     const uint32_t* __restrict__ ch_e0, const uint32_t* __restrict__ ch_w0, const uint32_t* __restrict__ ch_q0,
     const uint16_t* __restrict__ src16, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ rbase,
-    const float* __restrict__ wbin, uint32_t nch, uint32_t nslots, float* __restrict__ binval) {
-  __shared__ float sv[kChunkRows];
+    const float* __restrict__ wbin, uint32_t nch, uint32_t nslots, T* __restrict__ binval, T* __restrict__ trash) {
+  constexpr bool F64 = std::is_same<T, double>::value;
+  __shared__ T sv[CR];
   __shared__ uint32_t rb[kBinMaxBins];
   const uint32_t per = (nch + 7) / 8;
   const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -206,15 +244,25 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
   const uint16_t* __restrict__ sp = src16 + w0;
   const float* __restrict__ wq = W ? wbin + w0 : nullptr;
   const uint32_t* __restrict__ cp = rcnt + (w0 >> 6);
-  // BIG: the bin array is split in four windows of 2^30 slots, one resource each
+  // FP32: stores through buffer resources; BIG: the bin array is split in four
+  // windows of 2^30 slots, one resource each.  FP64 (whose slots pass 4 GiB
+  // of offsets sooner): plain stores, and lanes with nothing to write store to
+  // this wave's own trash slots (2 U x 64 values), so no store is
+  // conditional either
   constexpr int NW = BIG ? 4 : 1;
   __amdgpu_buffer_rsrc_t out[NW];
+  if constexpr (!F64) {
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    const uint64_t lo = (uint64_t)k << 30;
-    const uint32_t nrec = nslots > lo ? (uint32_t)std::min<uint64_t>(nslots - lo, 1ULL << 30) : 0u;
-    out[k] = __builtin_amdgcn_make_buffer_rsrc(binval + lo, (short)4, (int)nrec, 0x00020000);
+    for (int k = 0; k < NW; ++k) {
+      const uint64_t lo = (uint64_t)k << 30;
+      const uint32_t nrec = nslots > lo ? (uint32_t)std::min<uint64_t>(nslots - lo, 1ULL << 30) : 0u;
+      out[k] = __builtin_amdgcn_make_buffer_rsrc(binval + lo, (short)4, (int)nrec, 0x00020000);
+    }
   }
+  T* __restrict__ tw =
+      F64 ? trash + ((size_t)(blockIdx.x & (kTrashBlocks - 1)) * (kScatterBlock / 64) + (threadIdx.x >> 6)) * (2 * U * 64)
+          : nullptr;
+  const uint32_t lane = threadIdx.x & 63u;
   constexpr uint32_t step = kScatterBlock * U;
   struct Set {
     uint16_t wd[U];  // widened only where used, so that no use sits next to the load
@@ -243,13 +291,15 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
       const uint64_t valid = __builtin_amdgcn_ballot_w64(i < n);
       const uint64_t gs = __builtin_amdgcn_ballot_w64(wd & kWordGroup) | ~valid;
       const uint64_t rs = __builtin_amdgcn_ballot_w64(wd & kWordRun) & valid;
-      float x = sv[wd & (kWordRun - 1)];
+      T x = sv[wd & (kWordRun - 1)];
       if constexpr (W) x *= t.wt[j];
-      const float v = group_scan(x, ~gs);
+      const T v = group_scan(x, ~gs);
       const uint32_t rank = (t.c[j] & ((1u << kRunBits) - 1)) + lanes_below(rs >> 1);
       const uint32_t at = (t.c[j] >> kRunBits) + lanes_below(gs >> 1) + rb[rank];
       const bool tail = __builtin_amdgcn_inverse_ballot_w64(((gs >> 1) | (1ULL << 63)) & valid);
-      if constexpr (BIG) {
+      if constexpr (F64) {
+        *(tail ? binval + at : tw + lane) = v;
+      } else if constexpr (BIG) {
 #pragma unroll
         for (int k = 0; k < NW; ++k)
           gre_buffer_store_idx(v, out[k], tail && (at >> 30) == (uint32_t)k ? (int)(at & ((1u << 30) - 1)) : -1, 0, 0,
@@ -266,8 +316,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
     // the stage's loads are all issued before any is stored (lanes past the
     // end reload the last value); the stores fill both arrays whole, with no
     // branch
-    constexpr int FS = kChunkRows / kScatterBlock, FR = kBinMaxBins / kScatterBlock;
-    float f[FS];
+    constexpr int FS = CR / kScatterBlock, FR = kBinMaxBins / kScatterBlock;
+    T f[FS];
     uint32_t q[FR];
 #pragma unroll
     for (int k = 0; k < FS; ++k) {
@@ -295,7 +345,10 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
   Set a, b;
   load(a, 0);
 #pragma unroll
-  for (int j = 0; j < 2 * U; ++j) gre_buffer_store_idx(0.f, out[0], -1 - j, 0, 0, 0);  // distinct, or they merge
+  for (int j = 0; j < 2 * U; ++j) {  // distinct, or they merge
+    if constexpr (F64) tw[64 * j + lane] = 0.0;
+    else gre_buffer_store_idx(0.f, out[0], -1 - j, 0, 0, 0);
+  }
   for (uint32_t r = 0; r + 1 < rounds; r += 2) {
     load(b, (r + 1) * step);
     __builtin_amdgcn_sched_barrier(0);
@@ -412,6 +465,162 @@ __global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
 }
 
+// ---- the engine's FP64 sum pull ----------------------------------------------
+//
+// Each value v of a slot is added into its row as two 64-bit integers: hi =
+// round(v 2^K) and lo = round((v 2^K - hi) 2^L) (both scalings exact, and the
+// difference exact), so a row's sums are exact integers, independent of the
+// order of the additions; the row's sum is hi 2^-K + lo 2^-(K+L), within
+// n 2^-(K+L+1) of the exact sum of its n values.  K and L come from the
+// largest |value| m and the largest in-degree n_max: with 2^e >= m n_max and
+// 2^f > n_max, K = 61 - e and L = 61 - f keep |sum hi| < 2^62 and
+// |sum lo| < 2^61.
+__device__ inline void eng_scales(const unsigned long long* __restrict__ smax_bits, uint32_t max_in, int& K, int& L) {
+  const double m = __longlong_as_double((long long)*smax_bits);
+  int f = 0;
+  frexp((double)max_in, &f);
+  L = 61 - f;
+  if (!(m > 0.0)) {
+    K = 0;
+    return;
+  }
+  int e = 0;
+  frexp(m * (double)max_in, &e);
+  K = 61 - e;
+}
+
+__device__ inline void eng_split(double x, int K, int L, long long& hi, long long& lo) {
+  const double xs = ldexp(x, K);
+  const double h = rint(xs);
+  hi = (long long)h;
+  lo = (long long)rint(ldexp(xs - h, L));
+}
+
+__device__ inline double eng_join(unsigned long long hi, unsigned long long lo, int K, int L) {
+  return ldexp((double)(long long)hi, -K) + ldexp((double)(long long)lo, -(K + L));
+}
+
+// Sc[cid[v]] = S[v] for the vertices with out-edges, and the largest |S[v]|
+// among them (as the bits of a non-negative double: their order is the
+// values')
+__global__ void k_beng_compact(const double* __restrict__ S, const uint32_t* __restrict__ cid, uint32_t nV,
+                               double* __restrict__ Sc, unsigned long long* __restrict__ smax_bits) {
+  unsigned long long m = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = cid[v];
+    if (k < nV) {
+      const double x = S[v];
+      Sc[k] = x;
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(x));
+      m = b > m ? b : m;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(smax_bits, m);
+}
+
+// Accumulate slice s of bin b of the engine's layout (rows = the vertices with
+// in-edges in vertex order): the two-word sums in LDS, then C[v] += the row's
+// sum (the bin's only slice) or the sums added into gacc (finished by
+// k_beng_fin).  Rounds alternate between two register sets as in the FP32
+// accumulate; lanes past the end add zero to the first slot's row.
+template <int U>
+__global__ __launch_bounds__(kAccumBlock) void k_beng_accum(
+    const double* __restrict__ binval, const uint16_t* __restrict__ dloc, const uint32_t* __restrict__ sl_e0,
+    const uint32_t* __restrict__ sl_bin, const uint32_t* __restrict__ sl_g, const uint32_t* __restrict__ bin_r0,
+    const uint32_t* __restrict__ rowv, const unsigned long long* __restrict__ smax_bits, uint32_t max_in,
+    double* __restrict__ C, unsigned long long* __restrict__ gacc) {
+  __shared__ unsigned long long ahi[kEngBinRows], alo[kEngBinRows];
+  const uint32_t s = blockIdx.x;
+  const uint32_t b = sl_bin[s];
+  const uint32_t r0 = bin_r0[b], nr = bin_r0[b + 1] - r0;
+  const uint32_t e0 = sl_e0[s], n = sl_e0[s + 1] - e0;
+  const double* __restrict__ vp = binval + e0;
+  const uint16_t* __restrict__ dp = dloc + e0;
+  int K, L;
+  eng_scales(smax_bits, max_in, K, L);
+  constexpr uint32_t step = kAccumBlock * U;
+  auto load = [&](double (&v)[U], uint32_t (&d)[U], uint32_t base) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kAccumBlock + threadIdx.x;
+      const uint32_t ii = i < n ? i : 0u;
+      v[j] = __builtin_nontemporal_load(vp + ii);
+      d[j] = __builtin_nontemporal_load(dp + ii);
+    }
+  };
+  auto add = [&](const double (&v)[U], const uint32_t (&d)[U], uint32_t base) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = base + j * kAccumBlock + threadIdx.x;
+      long long hi, lo;
+      eng_split(i < n ? v[j] : 0.0, K, L, hi, lo);
+      atomicAdd(&ahi[d[j]], (unsigned long long)hi);
+      atomicAdd(&alo[d[j]], (unsigned long long)lo);
+    }
+  };
+  double va[U], vb[U];
+  uint32_t da[U], db[U];
+  load(va, da, 0);  // in flight while the accumulators are cleared
+  for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) {
+    ahi[i] = 0ULL;
+    alo[i] = 0ULL;
+  }
+  __syncthreads();
+  for (uint32_t base = 0;;) {
+    load(vb, db, base + step);
+    __builtin_amdgcn_sched_barrier(0);
+    add(va, da, base);
+    base += step;
+    if (base >= n) break;
+    load(va, da, base + step);
+    __builtin_amdgcn_sched_barrier(0);
+    add(vb, db, base);
+    base += step;
+    if (base >= n) break;
+  }
+  __syncthreads();
+  const uint32_t g = sl_g[s];
+  if (g == 0xffffffffu) {
+    for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) {
+      const uint32_t v = rowv[r0 + i];
+      C[v] = C[v] + eng_join(ahi[i], alo[i], K, L);
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) {
+      if (ahi[i]) atomicAdd(&gacc[2 * (size_t)(g + i)], ahi[i]);
+      if (alo[i]) atomicAdd(&gacc[2 * (size_t)(g + i) + 1], alo[i]);
+    }
+  }
+}
+
+// the rows of the engine layout's bins cut into slices (their two-word sums in
+// gacc, cleared for the next superstep)
+__global__ void k_beng_fin(const uint32_t* __restrict__ mb_r0, const uint32_t* __restrict__ mb_g, uint32_t nmb,
+                           uint32_t nmrows, const uint32_t* __restrict__ rowv,
+                           const unsigned long long* __restrict__ smax_bits, uint32_t max_in, double* __restrict__ C,
+                           unsigned long long* __restrict__ gacc) {
+  int K, L;
+  eng_scales(smax_bits, max_in, K, L);
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nmrows; q += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nmb;  // largest k with mb_g[k] <= q
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (mb_g[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    const unsigned long long h = gacc[2 * q], l = gacc[2 * q + 1];
+    gacc[2 * q] = 0ULL;
+    gacc[2 * q + 1] = 0ULL;
+    const uint32_t v = rowv[mb_r0[lo] + (uint32_t)(q - mb_g[lo])];
+    C[v] = C[v] + eng_join(h, l, K, L);
+  }
+}
+
 // ---- layout build -----------------------------------------------------------
 
 // rows of the vertices with in-edges: in-degree rank i dealt to bin b in snake
@@ -468,14 +677,16 @@ __global__ void k_bin_rowbin(const uint32_t* __restrict__ bin_r0, uint32_t B, ui
 }
 
 // Sort key of CSR entry e (u -> v): (chunk, bin of v's row, v's row within
-// the bin); value = (e, source row within the chunk).  The row of e is found
-// inside e's tile of the CSR's row tiles.
+// the bin); value = (e, u's staging index within the chunk).  srcmap gives a
+// vertex's staging index (the order the chunks cut), dstmap its row.  The row
+// of e is found inside e's tile of the CSR's row tiles.
 __global__ void k_bin_keys(const uint64_t* __restrict__ off, const uint32_t* __restrict__ idx,
                            const uint32_t* __restrict__ rb, uint32_t te, uint64_t E,
-                           const uint32_t* __restrict__ rowperm, const int32_t* __restrict__ outdeg,
-                           const uint32_t* __restrict__ row_ch, const uint32_t* __restrict__ ch_r0,
-                           const uint32_t* __restrict__ rowbin, const uint32_t* __restrict__ bin_r0, int bbits,
-                           uint64_t* __restrict__ key, uint64_t* __restrict__ val) {
+                           const uint32_t* __restrict__ srcmap, const uint32_t* __restrict__ dstmap,
+                           const int32_t* __restrict__ outdeg, const uint32_t* __restrict__ row_ch,
+                           const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ rowbin,
+                           const uint32_t* __restrict__ bin_r0, int bbits, uint64_t* __restrict__ key,
+                           uint64_t* __restrict__ val) {
   for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t t = e / te;
     uint32_t lo = rb[t] > 0 ? rb[t] - 1 : 0, hi = rb[t + 1];
@@ -485,11 +696,11 @@ __global__ void k_bin_keys(const uint64_t* __restrict__ off, const uint32_t* __r
       else hi = mid;
     }
     const uint32_t u = lo;
-    const uint32_t ru = rowperm[u];
+    const uint32_t ru = srcmap[u];
     const bool split = (uint32_t)outdeg[u] > kChunkEdges;
     const uint32_t w = row_ch[ru] + (split ? (uint32_t)((e - off[u]) / kChunkEdges) : 0u);
     const uint32_t sl = ru - ch_r0[w];
-    const uint32_t rv = rowperm[idx[e]];
+    const uint32_t rv = dstmap[idx[e]];
     const uint32_t b = rowbin[rv];
     key[e] = ((uint64_t)w << (bbits + kDstBits)) | ((uint64_t)b << kDstBits) | (rv - bin_r0[b]);
     val[e] = (e << 32) | sl;
@@ -540,7 +751,7 @@ __global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint64_t* __r
     const uint32_t b = (uint32_t)(wb & bmask);
     const uint64_t w = wb >> bbits;
     const uint32_t g = ch_w0[w] + ((uint32_t)i - ch_e0[w]);
-    wpos[g] = (uint32_t)(val[i] >> 32);
+    if (wpos) wpos[g] = (uint32_t)(val[i] >> 32);
     const unsigned long long gb = gbits[g >> 6];
     if (!((gb >> (g & 63)) & 1ULL)) continue;
     const uint32_t c = (rcnt[g >> 6] >> kRunBits) + (uint32_t)__popcll(gb & ((1ULL << (g & 63)) - 1));
@@ -555,6 +766,40 @@ __global__ void k_bin_dloc(const uint64_t* __restrict__ key, const uint64_t* __r
   block_add2(nbad, bad, psum, bad + 1);
 }
 
+// engine layout: flag[v] = v has in-edges; then rows in vertex order
+__global__ void k_beng_flags(const uint64_t* __restrict__ coff, uint32_t nV, uint32_t* __restrict__ flag,
+                             uint32_t* __restrict__ max_in) {
+  uint32_t m = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = coff[v + 1] - coff[v];
+    flag[v] = d > 0 ? 1u : 0u;
+    m = d > m ? (uint32_t)d : m;
+  }
+  m = wave_max_u32(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(max_in, m);
+}
+
+__global__ void k_beng_rows(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, uint32_t nV,
+                            uint32_t* __restrict__ dmap, uint32_t* __restrict__ rowv) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    if (flag[v]) {
+      dmap[v] = pos[v];
+      rowv[pos[v]] = (uint32_t)v;
+    } else {
+      dmap[v] = 0xffffffffu;
+    }
+  }
+}
+
+// out-degree by compact source id
+__global__ void k_beng_degc(const uint32_t* __restrict__ cid, const int32_t* __restrict__ outdeg, uint32_t nV,
+                            int32_t* __restrict__ degc) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nV; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = cid[v];
+    if (k < nV) degc[k] = outdeg[v];
+  }
+}
+
 template <typename T>
 void copy_to_dev(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
   d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
@@ -563,42 +808,13 @@ void copy_to_dev(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
 
 }  // namespace
 
-std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
-  const auto mkey = std::make_pair((int)kBinKey, 0u);
-  auto it = g->xpart.find(mkey);
-  if (it != g->xpart.end()) return it->second;
-  const uint64_t E = g->nE;
-  const uint32_t nV = g->nV;
-  GRE_CHECK(E > 0 && E < 0xffffffffULL, GRE_DEV_NULL, "the binned schedule needs 0 < edges < 2^32");
-  graph_compact_targets(g);
-  const uint32_t nd = g->ntgt;
-  const uint32_t B = (nd + kBinRows - 1) / kBinRows;
-  GRE_CHECK(B >= 1 && B <= (uint32_t)kBinMaxBins, GRE_DEV_NULL,
-            "the binned schedule takes at most 4096 * 16384 vertices with in-edges");
-  hipStream_t s = stream();
-  auto xp = std::make_shared<XPart>();
-  xp->binned = true;
-  xp->permuted = true;
-  xp->R = (int)kBinKey;
-  BinPlan& bp = xp->bin;
-  bp.E = E;
-  bp.nd = nd;
-  bp.B = B;
+namespace {
 
-  // bins: B - rem bins get nd / B rows, rem get one more (the last, partial
-  // round of the snake deal)
-  std::vector<uint32_t> bin_r0(B + 1, 0);
-  {
-    const uint32_t full = nd / B, rem = nd % B;
-    const bool even = (full % 2) == 0;
-    for (uint32_t b = 0; b < B; ++b) {
-      const bool extra = even ? b < rem : b >= B - rem;
-      bin_r0[b + 1] = bin_r0[b] + full + (extra ? 1u : 0u);
-    }
-  }
-  copy_to_dev(bp.bin_r0, bin_r0, s);
-
-  // rows
+// rows of the PageRank handles' layout: the vertices with in-edges dealt to
+// the bins by in-degree in snake order, then the others (with out-edges
+// first, by descending out-degree); rowinv, out-degree and compact id by row
+void build_rows_snake(gre_graph* g, XPart* xp, const BinPlan& bp, uint32_t B, hipStream_t s) {
+  const uint32_t nV = g->nV, nd = bp.nd;
   xp->rowperm.alloc(4 * (size_t)nV);
   xp->rowinv.alloc(4 * (size_t)nV);
   xp->outdeg_r.alloc(4 * (size_t)nV);
@@ -631,12 +847,106 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
                      xp->cid_r.as<uint32_t>());
   GRE_HIP(hipGetLastError());
 
-  // scatter chunks over consecutive source rows
-  std::vector<int32_t> deg(nV);
-  GRE_HIP(hipMemcpyAsync(deg.data(), xp->outdeg_r.p, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
-  GRE_HIP(hipStreamSynchronize(s));
-  std::vector<uint32_t> ch_r0, ch_nr, ch_e0(1, 0), row_ch(nV, 0);
-  for (uint32_t r = 0; r < nV;) {
+}
+
+// The binned layout.  eng = false: the PageRank handles' (bins of kBinRows
+// rows dealt by in-degree in snake order, one row order for sources and
+// destinations, ranks and contributions kept in it).  eng = true: the engine's
+// FP64 sum pull (bins of kEngBinRows rows: the vertices with in-edges in
+// vertex order, so each bin's results go to ascending vertices; sources staged
+// in compact source order, hub-sorted).
+std::shared_ptr<XPart> build_binpart(gre_graph* g, bool eng) {
+  const auto mkey = std::make_pair((int)kBinKey, eng ? 1u : 0u);
+  auto it = g->xpart.find(mkey);
+  if (it != g->xpart.end()) return it->second;
+  const uint64_t E = g->nE;
+  const uint32_t nV = g->nV;
+  GRE_CHECK(E > 0 && E < 0xffffffffULL, GRE_DEV_NULL, "the binned schedule needs 0 < edges < 2^32");
+  graph_compact_targets(g);
+  if (eng) graph_compact_sources(g);
+  const uint32_t nd = g->ntgt;
+  const uint32_t brows = eng ? (uint32_t)kEngBinRows : (uint32_t)kBinRows;
+  const uint32_t B = (nd + brows - 1) / brows;
+  GRE_CHECK(B >= 1 && B <= (uint32_t)kBinMaxBins, GRE_DEV_NULL,
+            "the binned layout takes at most 4096 bins of vertices with in-edges");
+  hipStream_t s = stream();
+  auto xp = std::make_shared<XPart>();
+  xp->binned = true;
+  xp->permuted = !eng;
+  xp->R = (int)kBinKey;
+  BinPlan& bp = xp->bin;
+  bp.E = E;
+  bp.nd = nd;
+  bp.B = B;
+  bp.bin_rows = brows;
+
+  std::vector<uint32_t> bin_r0(B + 1, 0);
+  if (eng) {
+    for (uint32_t b = 0; b < B; ++b) bin_r0[b + 1] = std::min(nd, (b + 1) * brows);
+  } else {
+    // B - rem bins get nd / B rows, rem get one more (the last, partial round
+    // of the snake deal)
+    const uint32_t full = nd / B, rem = nd % B;
+    const bool even = (full % 2) == 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      const bool extra = even ? b < rem : b >= B - rem;
+      bin_r0[b + 1] = bin_r0[b] + full + (extra ? 1u : 0u);
+    }
+  }
+  copy_to_dev(bp.bin_r0, bin_r0, s);
+
+  // staging index and row of every vertex, and the staging order's degrees
+  const uint32_t* srcmap = nullptr;
+  const uint32_t* dstmap = nullptr;
+  uint32_t nS = nV;
+  std::vector<int32_t> deg;
+  DevBuf dmap;
+  if (eng) {
+    DevBuf flag, pos, mx;
+    flag.alloc(4 * (size_t)nV);
+    pos.alloc(4 * (size_t)nV);
+    mx.alloc(4);
+    GRE_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+    hipLaunchKernelGGL(k_beng_flags, dim3(gcap(nV)), dim3(kB), 0, s, g->coff.as<uint64_t>(), nV, flag.as<uint32_t>(),
+                       mx.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    size_t tb = 0;
+    GRE_HIP(rocprim::exclusive_scan(nullptr, tb, flag.as<uint32_t>(), pos.as<uint32_t>(), 0u, (size_t)nV,
+                                    rocprim::plus<uint32_t>(), s));
+    void* tmp = scratch(0, tb);
+    GRE_HIP(rocprim::exclusive_scan(tmp, tb, flag.as<uint32_t>(), pos.as<uint32_t>(), 0u, (size_t)nV,
+                                    rocprim::plus<uint32_t>(), s));
+    dmap.alloc(4 * (size_t)nV);
+    bp.rowv.alloc(4 * (size_t)std::max<uint32_t>(nd, 1));
+    hipLaunchKernelGGL(k_beng_rows, dim3(gcap(nV)), dim3(kB), 0, s, flag.as<uint32_t>(), pos.as<uint32_t>(), nV,
+                       dmap.as<uint32_t>(), bp.rowv.as<uint32_t>());
+    GRE_HIP(hipGetLastError());
+    nS = g->nsrc;
+    DevBuf degc;
+    degc.alloc(4 * (size_t)std::max<uint32_t>(nS, 1));
+    hipLaunchKernelGGL(k_beng_degc, dim3(gcap(nV)), dim3(kB), 0, s, g->cid.as<uint32_t>(), g->outdeg.as<int32_t>(),
+                       nV, degc.as<int32_t>());
+    GRE_HIP(hipGetLastError());
+    deg.resize(nS);
+    GRE_HIP(hipMemcpyAsync(deg.data(), degc.p, 4 * (size_t)nS, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipMemcpyAsync(&bp.max_in, mx.p, 4, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    srcmap = g->cid.as<uint32_t>();
+    dstmap = dmap.as<uint32_t>();
+  } else {
+    build_rows_snake(g, xp.get(), bp, B, s);
+    deg.resize(nV);
+    GRE_HIP(hipMemcpyAsync(deg.data(), xp->outdeg_r.p, 4 * (size_t)nV, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    srcmap = xp->rowperm.as<uint32_t>();
+    dstmap = xp->rowperm.as<uint32_t>();
+  }
+
+  // scatter chunks over consecutive staging indices
+  const uint32_t crows = eng ? (uint32_t)kEngChunkRows : (uint32_t)kChunkRows;
+  bp.chunk_rows = crows;
+  std::vector<uint32_t> ch_r0, ch_nr, ch_e0(1, 0), row_ch(nS, 0);
+  for (uint32_t r = 0; r < nS;) {
     const uint32_t d = (uint32_t)deg[r];
     if (d == 0) { ++r; continue; }
     if (d > kChunkEdges) {
@@ -651,7 +961,7 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
     }
     const uint32_t r0 = r, w = (uint32_t)ch_r0.size();
     uint32_t edges = 0;
-    while (r < nV && r - r0 < (uint32_t)kChunkRows) {
+    while (r < nS && r - r0 < crows) {
       const uint32_t dr = (uint32_t)deg[r];
       if (dr > kChunkEdges || edges + dr > kChunkEdges) break;
       edges += dr;
@@ -696,7 +1006,7 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
     keys0.alloc(8 * E);
     vals0.alloc(8 * E);
     hipLaunchKernelGGL(k_bin_keys, dim3(gcap(E)), dim3(kB), 0, s, g->off.as<uint64_t>(), g->idx.as<uint32_t>(), rb,
-                       2048u, E, xp->rowperm.as<uint32_t>(), g->outdeg.as<int32_t>(), rowch_d.as<uint32_t>(),
+                       2048u, E, srcmap, dstmap, g->outdeg.as<int32_t>(), rowch_d.as<uint32_t>(),
                        bp.ch_r0.as<uint32_t>(), rowbin.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), bbits,
                        keys0.as<uint64_t>(), vals0.as<uint64_t>());
     GRE_HIP(hipGetLastError());
@@ -868,12 +1178,14 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
     bad.alloc(16);
     GRE_HIP(hipMemsetAsync(bad.p, 0, 16, s));
     GRE_HIP(hipMemsetAsync(bp.dloc.p, 0, 2 * Eb, s));
-    bp.wpos.alloc(4 * Ew);
-    GRE_HIP(hipMemsetAsync(bp.wpos.p, 0, 4 * Ew, s));
+    if (!eng) {  // the engine's programs carry no edge weights
+      bp.wpos.alloc(4 * Ew);
+      GRE_HIP(hipMemsetAsync(bp.wpos.p, 0, 4 * Ew, s));
+    }
     hipLaunchKernelGGL(k_bin_dloc, dim3(gcap(E)), dim3(kB), 0, s, keys1.as<uint64_t>(), vals1.as<uint64_t>(), E, Eb,
                        bbits, B, bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(), runbase_d.as<uint32_t>(),
                        gbits.as<unsigned long long>(), bp.rcnt.as<uint32_t>(), bp.dloc.as<uint16_t>(),
-                       bp.wpos.as<uint32_t>(), bad.as<unsigned long long>());
+                       eng ? nullptr : bp.wpos.as<uint32_t>(), bad.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
     unsigned long long h[2] = {0, 0};
     GRE_HIP(hipMemcpyAsync(h, bad.p, 16, hipMemcpyDeviceToHost, s));
@@ -889,6 +1201,11 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g) {
   g->xpart[mkey] = xp;
   return xp;
 }
+
+}  // namespace
+
+std::shared_ptr<XPart> graph_binpart(gre_graph* g) { return build_binpart(g, false); }
+std::shared_ptr<XPart> graph_binpart_engine(gre_graph* g) { return build_binpart(g, true); }
 
 void BinBuffers::make(const XPart& xp, uint32_t nV, hipStream_t s) {
   const BinPlan& bp = xp.bin;
@@ -932,7 +1249,7 @@ void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_
                        bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(), bp.src16.as<uint16_t>(),
                        bp.rcnt.as<uint32_t>(), bp.rbase.as<uint32_t>(), a.wbin, bp.nch,
                        (uint32_t)bp.Eb,
-                       bb.binval.as<float>());
+                       bb.binval.as<float>(), (float*)nullptr);
   }
   {
     ProfScope ps("bin_accum");
@@ -951,6 +1268,62 @@ void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_
   GRE_HIP(hipGetLastError());
 }
 }  // namespace
+
+void EngBinBuffers::make(const XPart& xp, uint32_t nsrc, hipStream_t s) {
+  const BinPlan& bp = xp.bin;
+  const size_t nb = 8 * (size_t)bp.Eb;
+  size_t al = 0;  // aligned as the FP32 bin array
+  if (nb >= (64u << 20)) {
+    al = 64u << 20;
+    while (al < (1u << 30) && 2 * al <= nb) al *= 2;
+  }
+  binval.alloc(nb, al);
+  GRE_HIP(hipMemsetAsync(binval.p, 0, nb, s));
+  trash.alloc(sizeof(double) * (size_t)kTrashBlocks * (kScatterBlock / 64) * (2 * kEngScatterU * 64));
+  gacc.alloc(16 * (size_t)std::max<uint32_t>(bp.nmrows, 1));
+  GRE_HIP(hipMemsetAsync(gacc.p, 0, 16 * (size_t)std::max<uint32_t>(bp.nmrows, 1), s));
+  Sc.alloc(sizeof(double) * (size_t)std::max<uint32_t>(nsrc, 1));
+  smax.alloc(8);
+  GRE_HIP(hipStreamSynchronize(s));
+  for_xp = &xp;
+}
+
+void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, const uint32_t* cid, uint32_t nV,
+                    bool fresh_s, double* C, hipStream_t s) {
+  const BinPlan& bp = xp.bin;
+  if (fresh_s) {
+    GRE_HIP(hipMemsetAsync(bb.smax.p, 0, 8, s));
+    hipLaunchKernelGGL(k_beng_compact, dim3(gcap(nV)), dim3(kB), 0, s, S, cid, nV, bb.Sc.as<double>(),
+                       bb.smax.as<unsigned long long>());
+    GRE_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("beng_scatter");
+    const unsigned per = (bp.nch + 7) / 8;
+    hipLaunchKernelGGL((k_bin_scatter<kEngScatterU, false, false, double, kEngChunkRows>), dim3(8 * per),
+                       dim3(kScatterBlock), 0, s,
+                       (const double*)bb.Sc.as<double>(), bp.ch_r0.as<uint32_t>(), bp.ch_nr.as<uint32_t>(),
+                       bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(),
+                       bp.src16.as<uint16_t>(), bp.rcnt.as<uint32_t>(), bp.rbase.as<uint32_t>(), (const float*)nullptr,
+                       bp.nch, (uint32_t)bp.Eb, bb.binval.as<double>(), bb.trash.as<double>());
+    GRE_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("beng_accum");
+    hipLaunchKernelGGL((k_beng_accum<kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<double>(),
+                       bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(), bp.sl_bin.as<uint32_t>(),
+                       bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), bp.rowv.as<uint32_t>(),
+                       bb.smax.as<unsigned long long>(), bp.max_in, C, bb.gacc.as<unsigned long long>());
+    GRE_HIP(hipGetLastError());
+  }
+  if (bp.nmrows) {
+    ProfScope ps("beng_finish");
+    hipLaunchKernelGGL(k_beng_fin, dim3(gcap(bp.nmrows)), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
+                       bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, bp.rowv.as<uint32_t>(),
+                       bb.smax.as<unsigned long long>(), bp.max_in, C, bb.gacc.as<unsigned long long>());
+    GRE_HIP(hipGetLastError());
+  }
+}
 
 void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s) {
   if (a.wbin) {

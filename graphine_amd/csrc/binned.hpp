@@ -36,6 +36,24 @@ struct BinIter {
 
 void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s);
 
+// The engine's FP64 sum pull over the binned layout of graph_binpart_engine
+// (sources staged in compact source order, rows = the vertices with in-edges
+// in vertex order): C[v] += the sum of S over v's in-neighbours, as exact
+// two-word fixed-point sums, for every vertex v with in-edges.
+struct EngBinBuffers {
+  DevBuf binval;   // double [Eb] each slot's group sum, bin order
+  DevBuf trash;    // the scatter's slots for lanes with nothing to write
+  DevBuf gacc;     // int64 [2 nmrows] two-word sums of the bins cut into slices
+  DevBuf Sc;       // double [nsrc] the scatter data in compact source order
+  DevBuf smax;     // the largest |S| (bits of a double), which sets the fixed-point scales
+  const XPart* for_xp = nullptr;
+  void make(const XPart& xp, uint32_t nsrc, hipStream_t s);
+};
+// fresh_s: S changed since the last call (its compact copy and largest value
+// are remade)
+void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, const uint32_t* cid, uint32_t nV,
+                    bool fresh_s, double* C, hipStream_t s);
+
 // Fixed-point scale of the accumulate pass: contributions are added as
 // round(c * 2^kBinFix) in 64-bit integers, so every sum is exact and
 // independent of the order of the additions.  |sum| must stay below

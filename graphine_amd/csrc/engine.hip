@@ -19,6 +19,7 @@
 // atomicMin, first-writer CAS), which plays the role of GRE's 8192 striped
 // spin-locks; apply + halt + count are one thread-per-vertex kernel that
 // rebuilds the bitmap words with wave ballots; rebuild = block scan + write.
+#include "binned.hpp"
 #include "graph.hpp"
 #include "pull.hpp"
 
@@ -960,6 +961,10 @@ struct gre_engine {
   gre::PullPlan seg_plan, row_plan;
   gre::PullBuffers seg_buf, row_buf;
   gre::DevBuf Sc, xsum;
+  // the binned FP64 sum pull (GRE's PageRank, complete frontier, large graphs)
+  std::shared_ptr<gre::XPart> bxp;
+  gre::EngBinBuffers bbuf;
+  bool bsc_valid = false;      // bbuf holds the current scatter data's compact copy
 };
 
 namespace gre {
@@ -1115,6 +1120,7 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
   hipStream_t s = stream();
   e->dir = dir;
   e->sc_valid = false;  // init writes the scatter data
+  e->bsc_valid = false;
   const double t0 = now();
   // run(): superstep != 0 -> currSet.clear(), nextSet.clear() (Engine.hpp:182-193)
   GRE_HIP(hipMemsetAsync(e->curr.p, 0, e->curr.bytes, s));
@@ -1158,6 +1164,13 @@ void engine_begin(gre_engine* e, int dir, const uint32_t* srcs, int64_t nsrc) {
 constexpr int kEngRanges = 256;
 constexpr uint32_t kEngShared = 1u << 12;
 bool engine_part_ok(const gre_graph* g) { return g->nE >= (1ull << 22) && g->nE < 0xffffffffULL; }
+// The binned FP64 pull (GRE's PageRank): the same edge range, and at most
+// kBinMaxBins bins of kEngBinRows vertices with in-edges.
+bool engine_bin_ok(gre_graph* g) {
+  if (!engine_part_ok(g)) return false;
+  graph_compact_targets(g);
+  return (uint64_t)g->ntgt <= (uint64_t)kBinMaxBins * kEngBinRows;
+}
 
 int64_t engine_step(gre_engine* e) {
   GRE_CHECK(e->begun, GRE_DEV_NULL, "call gre_engine_begin first");
@@ -1197,7 +1210,30 @@ int64_t engine_step(gre_engine* e) {
   bool fused = false;   // the compute phase applied in place (kFuseApply)
   const bool fsum = e->prog == GRE_PROG_PAGERANK || e->prog == GRE_PROG_PAGERANK_DELTA ||
                     e->prog == GRE_PROG_BC_BACKWARD;
-  if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
+  if (pull && e->prog == GRE_PROG_PAGERANK && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_bin_ok(e->g)) {
+    // GRE's PageRank with every vertex active: the binned pull.  The sources
+    // are staged in compact order (their copy made once per run: GRE's
+    // PageRank scatters its init value every superstep, PageRank.hpp:38-47),
+    // each destination's sum is exact fixed point (binned.hip)
+    gre_graph* g = e->g;
+    if (!e->bxp) {
+      graph_compact_sources(g);
+      e->bxp = graph_binpart_engine(g);
+    }
+    if (e->bbuf.for_xp != e->bxp.get()) e->bbuf.make(*e->bxp, g->nsrc, s);
+    const uint32_t nvpad = e->nwords * 32;
+    if (!e->rows_mark.p) {
+      e->rows_mark.alloc(4 * (size_t)e->nwords);
+      hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->nV, nvpad,
+                         e->rows_mark.as<uint32_t>());
+      GRE_HIP(hipGetLastError());
+    }
+    GRE_HIP(hipMemcpyAsync(e->next.p, e->rows_mark.p, 4 * (size_t)e->nwords, hipMemcpyDeviceToDevice, s));
+    bin_engine_sum(*e->bxp, e->bbuf, (const double*)c.S, g->cid.as<uint32_t>(), g->nV, !e->bsc_valid,
+                   (double*)c.C, s);
+    e->bsc_valid = true;
+    ++e->pull_steps;
+  } else if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
     gre_graph* g = e->g;
     if (!e->xp) {
       graph_compact_sources(g);
@@ -1404,6 +1440,7 @@ int gre_engine_upload(gre_engine* e, int which, const void* host) {
     GRE_HIP(hipMemcpyAsync(b.p, host, e->es[which] * (size_t)e->g->nV, hipMemcpyHostToDevice, stream()));
     GRE_HIP(hipStreamSynchronize(stream()));
     e->sc_valid = false;  // the scatter data may have changed
+    e->bsc_valid = false;
   });
 }
 

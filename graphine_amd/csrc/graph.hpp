@@ -23,6 +23,7 @@ constexpr int kMaxKeys = 264;
 // consecutive source rows, the accumulate pass slices of each bin's entries.
 constexpr int kBinRows = 16384;        // rows per bin (64-bit LDS accumulators)
 constexpr int kBinMaxBins = 4096;      // bins (the scatter pass keeps a run table of this many in LDS)
+constexpr int kEngBinRows = 8192;      // rows per bin of the engine's FP64 layout (two 64-bit LDS accumulators each)
 constexpr int kChunkRows = 16384;      // source rows whose values a chunk stages in LDS
 constexpr uint32_t kChunkEdges = 262144;   // entries per chunk (a longer source row is cut)
 constexpr uint32_t kSliceEdges = 1u << 20;  // entries per accumulate slice (a longer bin is cut)
@@ -34,6 +35,10 @@ struct BinPlan {
   uint64_t Ew = 0;     // words, with each chunk's gap
   uint64_t Eb = 0;     // bin-array slots, with each bin's gap
   uint64_t ngrp = 0;   // groups (one slot each): entries of one chunk to one destination, cut at 16-word rows
+  uint32_t bin_rows = 0;  // rows per bin (kBinRows, or kEngBinRows for the engine's layout)
+  uint32_t chunk_rows = 0;  // source rows per scatter chunk
+  uint32_t max_in = 0;    // the engine's layout: largest in-degree
+  DevBuf rowv;     // the engine's layout: uint32 [nd] vertex of each row (ascending)
   uint32_t nd = 0, B = 0, nch = 0, nsl = 0, nmb = 0, nmrows = 0;
   DevBuf ch_r0;    // uint32 [nch] first source row of chunk w
   DevBuf ch_nr;    // uint32 [nch] source rows chunk w stages
@@ -181,6 +186,9 @@ std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc = t
 // The binned layout of g's forward pull (E < 2^32, at most kBinMaxBins bins);
 // built once and cached with the partitions.
 std::shared_ptr<XPart> graph_binpart(gre_graph* g);
+// The binned layout of the engine's FP64 sum pull (binned.hip), cached on the
+// graph like the other partitions.
+std::shared_ptr<XPart> graph_binpart_engine(gre_graph* g);
 // Drops the cached partitions that no PageRank handle or engine holds any more.
 void prune_xparts(gre_graph* g);
 // Its tiles of te entries (cached on the partition).

@@ -159,8 +159,8 @@ def test_partitioned_pagerank_scatter_upload_between_steps(N):
 
 
 def test_modes_pagerank_partitioned(N, O):
-    # from 2^22 edges the complete-frontier pull runs over the XCD source
-    # partition (segment sums, then each row's segments)
+    # from 2^22 edges GRE's PageRank takes the binned FP64 pull (its
+    # complete-frontier supersteps: exact two-word fixed-point row sums)
     g, og = N.Graph.rmat(19, 16, 1, False), O.rmat_graph(19, 16, 1, False)
     assert len(og.idx) >= 1 << 22
     for mode in (2, 0):
@@ -171,15 +171,17 @@ def test_modes_pagerank_partitioned(N, O):
 
 
 def test_modes_pagerank_partitioned_long_rows(N, O):
-    """GRE's PageRank program on the partitioned pull where a destination has
-    1.2 M in-edges (its row spans hundreds of segment tiles) and a source has
-    300,000 out-edges; R-MAT edges around them.  Counts exact; every vertex but
-    the hub to 1e-12 of the oracle.  The hub is pinned to its exact value:
-    GRE's PageRank scatters its init value 0.15/outdeg every superstep, so the
-    hub holds 0.15 + 0.85 * the sum over its in-neighbours of 0.15/outdeg,
-    which math.fsum rounds correctly; both the device's fixed-order sum and the
-    oracle's one-by-one sum of the 1.2 M terms stay within the n*u bound of a
-    recursive sum.  A second engine gives the same bits."""
+    """GRE's PageRank program on the engine's large-graph pull (the binned FP64
+    pull) where a destination has 1.2 M in-edges (73 scatter chunks of 16,384
+    out-degree-1 sources each write 1,024 group sums into its bin) and a source
+    has 300,000 out-edges (its out-row cut into two chunks); R-MAT edges around
+    them.  Counts exact; every vertex but the hub to 1e-12 of the oracle.  The
+    hub is pinned to its exact value: GRE's PageRank scatters its init value
+    0.15/outdeg every superstep, so the hub holds 0.15 + 0.85 * the sum over
+    its in-neighbours of 0.15/outdeg, which math.fsum rounds correctly; both
+    the device's sum and the oracle's one-by-one sum of the 1.2 M terms stay
+    within the n*u bound of a recursive sum.  A second engine gives the same
+    bits."""
     import math
     base = O.rmat_graph(19, 16, 1, False)
     n = 1 << 21
@@ -208,6 +210,45 @@ def test_modes_pagerank_partitioned_long_rows(N, O):
         e2 = N.Engine(g, N.PAGERANK)
         e2.set_mode(mode)
         e2.config_max_supersteps(8)
+        e2.run(N.DIR_OUT)
+        assert np.array_equal(e2.download(N.ARR_VERTEX), x)
+        e2.close()
+
+
+def test_engine_pagerank_binned_multi_slice(N, O):
+    """GRE's PageRank through the engine's binned FP64 pull where both bins of
+    destinations take several accumulate slices: 4,500,000 out-degree-1
+    sources, source s -> s mod 16,384.  The sources are staged in compact
+    order (all of out-degree 1, so in vertex order) and every chunk of 16,384
+    of them reaches each destination once, so no two entries share a slot:
+    each of the two 8,192-row bins holds 2.25 M slots, cut into three slices
+    whose two-word sums meet in 64-bit integers in HBM.  Counts exact, ranks to
+    1e-12 of the oracle, every destination within the n*u bound of its exact
+    value 0.15 + 0.85 * 0.15 * indegree (math.fsum), and a second engine gives
+    the same bits."""
+    import math
+    nd, ns = 16384, 4_500_000
+    n = nd + ns
+    src = np.arange(nd, n, dtype=np.uint32)
+    dst = (np.arange(ns, dtype=np.uint64) % nd).astype(np.uint32)
+    g, og = N.Graph.from_edges(n, src, dst), O.graph_from_edges(n, src, dst)
+    assert len(og.idx) >= 1 << 22
+    indeg = og.indeg[:nd].astype(np.int64)
+    exact = np.array([0.15 + 0.85 * math.fsum([0.15] * int(k)) for k in np.unique(indeg)])
+    ex_of = dict(zip(np.unique(indeg).tolist(), exact.tolist()))
+    for mode in (2, 0):
+        e, ss, counts, oss, ocounts, ost = run_mode(N, O, g, og, N.PAGERANK, N.DIR_OUT, mode, max_ss=6)
+        assert ss == oss == 6 and np.array_equal(counts, ocounts)
+        assert e.pull_steps() == 6
+        x = e.download(N.ARR_VERTEX)
+        assert np.allclose(x, ost[2], rtol=1e-12, atol=0)
+        for v in (0, 1, 8191, 8192, nd - 1):
+            k = int(indeg[v])
+            assert abs(x[v] - ex_of[k]) <= k * 2.0 ** -53 * ex_of[k], v
+        e.close()
+        e2 = N.Engine(g, N.PAGERANK)
+        e2.set_mode(mode)
+        e2.config_max_supersteps(6)
         e2.run(N.DIR_OUT)
         assert np.array_equal(e2.download(N.ARR_VERTEX), x)
         e2.close()
