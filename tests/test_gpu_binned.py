@@ -294,3 +294,30 @@ def test_binned_weighted_adjoint_gradients(N, O):
     assert rel_l1(gW, rW.astype(np.float64)) < 1e-5
     assert rel_l1(graw, rraw.astype(np.float64)) < 1e-5
     assert rel_l1(gp, rp.astype(np.float64)) < 1e-5
+
+
+def test_binned_bin_array_past_2_30_slots(N):
+    """A bin array of more than 2^30 slots (directed R-MAT scale 26, edge factor
+    24: about 1.6 G entries, 1.3 G slots): the scatter stores through four buffer resources
+    of 2^30 slots each (a resource's byte offsets are 32-bit).  No oracle runs
+    at this size; the ranks are checked against the partitioned schedule's on
+    the same graph (both FP32, different summation orders: 1e-6 relative L1)
+    and against the invariants of the iteration (sum 1, every rank at least
+    (1 - d) / n)."""
+    g = N.Graph.rmat(26, 24, 6, False)
+    assert g.nE > 1_500_000_000  # over 2^30 slots even if a third of the entries shared one
+    pr = binned(N, g)
+    it, _ = pr.run(0.85, 10)
+    assert it == 10
+    x = pr.ranks().astype(np.float64)
+    pr.close()
+    part = N.PageRank(g, N.F32)
+    part.set_schedule(5, 8)
+    part.run(0.85, 10)
+    y = part.ranks().astype(np.float64)
+    part.close()
+    g.close()
+    n = x.size
+    assert abs(x.sum() - 1.0) < 1e-4
+    assert x.min() >= 0.15 / n * (1 - 1e-6)
+    assert rel_l1(x, y) < 1e-6
