@@ -58,6 +58,7 @@ constexpr int kEngScatterU = 4;
 constexpr int kEngChunkRows = 16384;
 constexpr int kAccumBlock = 1024;
 constexpr int kAccumU = 8;
+constexpr uint32_t kTailRows = 16384;  // rows without in-edges per accumulate workgroup after the slices
 constexpr uint32_t kBinKey = 1u << 14;  // cache key of the layout among a graph's partitions
 
 unsigned gcap(uint64_t n) {
@@ -369,10 +370,33 @@ template <class Up, int U>
 __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
     const float* __restrict__ binval, const uint16_t* __restrict__ dloc, const uint32_t* __restrict__ sl_e0,
     const uint32_t* __restrict__ sl_bin, const uint32_t* __restrict__ sl_g, const uint32_t* __restrict__ bin_r0,
-    Up up, unsigned long long* __restrict__ gacc, double* __restrict__ part) {
+    uint32_t nsl, uint32_t nd, uint32_t nV, Up up, unsigned long long* __restrict__ gacc,
+    double* __restrict__ part) {
   __shared__ unsigned long long acc[kBinRows];
   __shared__ double red[32];
   const uint32_t s = blockIdx.x;
+  if (s >= nsl) {
+    // the workgroups after the slices update kTailRows rows without in-edges
+    // each (sum 0; they need nothing from the slices, and run in the CUs the
+    // last slices leave idle)
+    constexpr int Q = kTailRows / kAccumBlock;
+    const uint32_t t0 = nd + (s - nsl) * kTailRows;
+    typename Up::Pre q[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const uint32_t r = t0 + threadIdx.x + k * kAccumBlock;
+      q[k] = up.pre(r < nV ? r : nd);
+    }
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const uint32_t r = t0 + threadIdx.x + k * kAccumBlock;
+      if (r < nV) up.fin(r, 0.f, q[k], a0, a1);
+    }
+    block_sum2_big(a0, a1, red);
+    if (threadIdx.x == 0) { part[2 * s] = a0; part[2 * s + 1] = a1; }
+    return;
+  }
   const uint32_t b = sl_bin[s];
   const uint32_t r0 = bin_r0[b], nr = bin_r0[b + 1] - r0;
   const uint32_t e0 = sl_e0[s], n = sl_e0[s + 1] - e0;
@@ -437,29 +461,24 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
 }
 
 // Finish: the rows of the bins cut into slices (their sums in gacc, which is
-// cleared for the next iteration), then the rows without in-edges (sum 0).
+// cleared for the next iteration).
 template <class Up>
 __global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_r0,
                                                 const uint32_t* __restrict__ mb_g, uint32_t nmb, uint32_t nmrows,
-                                                uint32_t nd, uint32_t nV, Up up,
-                                                unsigned long long* __restrict__ gacc, double* __restrict__ part) {
+                                                Up up, unsigned long long* __restrict__ gacc,
+                                                double* __restrict__ part) {
   __shared__ double red[32];
   double a0 = 0.0, a1 = 0.0;
-  const uint64_t total = (uint64_t)nmrows + (nV - nd);
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < total; q += (uint64_t)gridDim.x * blockDim.x) {
-    if (q < nmrows) {
-      uint32_t lo = 0, hi = nmb;  // largest k with mb_g[k] <= q
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (mb_g[mid] <= q) lo = mid;
-        else hi = mid;
-      }
-      const unsigned long long v = gacc[q];
-      gacc[q] = 0ULL;
-      up.apply(mb_r0[lo] + (uint32_t)(q - mb_g[lo]), fix_to_float(v), a0, a1);
-    } else {
-      up.apply(nd + (uint32_t)(q - nmrows), 0.f, a0, a1);
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nmrows; q += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nmb;  // largest k with mb_g[k] <= q
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (mb_g[mid] <= q) lo = mid;
+      else hi = mid;
     }
+    const unsigned long long v = gacc[q];
+    gacc[q] = 0ULL;
+    up.apply(mb_r0[lo] + (uint32_t)(q - mb_g[lo]), fix_to_float(v), a0, a1);
   }
   block_sum2_big(a0, a1, red);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
@@ -1226,8 +1245,9 @@ void BinBuffers::make(const XPart& xp, uint32_t nV, hipStream_t s) {
   GRE_HIP(hipMemsetAsync(binval.p, 0, nb, s));
   gacc.alloc(8 * (size_t)std::max<uint32_t>(bp.nmrows, 1));
   GRE_HIP(hipMemsetAsync(gacc.p, 0, 8 * (size_t)std::max<uint32_t>(bp.nmrows, 1), s));
-  part.alloc(2 * sizeof(double) * bp.nsl);
-  fin_grid = gcap((uint64_t)bp.nmrows + (nV - bp.nd));
+  tail_blocks = (uint32_t)(((uint64_t)(nV - bp.nd) + kTailRows - 1) / kTailRows);
+  part.alloc(2 * sizeof(double) * ((size_t)bp.nsl + tail_blocks));
+  fin_grid = gcap(std::max<uint32_t>(bp.nmrows, 1));
   part2.alloc(2 * sizeof(double) * fin_grid);
   part3.alloc(2 * sizeof(double) * kRedBlocks);
   GRE_HIP(hipStreamSynchronize(s));
@@ -1253,18 +1273,19 @@ void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_
   }
   {
     ProfScope ps("bin_accum");
-    hipLaunchKernelGGL((k_bin_accum<Up, kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<float>(),
-                       bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(), bp.sl_bin.as<uint32_t>(),
-                       bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), up, bb.gacc.as<unsigned long long>(),
-                       bb.part.as<double>());
+    hipLaunchKernelGGL((k_bin_accum<Up, kAccumU>), dim3(bp.nsl + bb.tail_blocks), dim3(kAccumBlock), 0, s,
+                       bb.binval.as<float>(), bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(),
+                       bp.sl_bin.as<uint32_t>(), bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), bp.nsl, bp.nd,
+                       nV, up, bb.gacc.as<unsigned long long>(), bb.part.as<double>());
   }
   {
     ProfScope ps("bin_finish");
     hipLaunchKernelGGL((k_bin_fin<Up>), dim3(bb.fin_grid), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
-                       bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, bp.nd, nV, up, bb.gacc.as<unsigned long long>(),
+                       bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, up, bb.gacc.as<unsigned long long>(),
                        bb.part2.as<double>());
   }
-  reduce_pairs(bb.part.as<double>(), bp.nsl, bb.part2.as<double>(), bb.fin_grid, a.scal_out, bb.part3.as<double>(), s);
+  reduce_pairs(bb.part.as<double>(), bp.nsl + bb.tail_blocks, bb.part2.as<double>(), bb.fin_grid, a.scal_out,
+               bb.part3.as<double>(), s);
   GRE_HIP(hipGetLastError());
 }
 }  // namespace

@@ -12,6 +12,7 @@ struct BinBuffers {
   DevBuf gacc;             // int64 [nmrows] sums of the bins cut into several slices (zero between iterations)
   DevBuf part, part2, part3;
   unsigned fin_grid = 0;   // workgroups of the finishing pass
+  uint32_t tail_blocks = 0;  // accumulate workgroups after the slices, for the rows without in-edges
   const XPart* for_xp = nullptr;  // the layout these were made for
   void make(const XPart& xp, uint32_t nV, hipStream_t s);
 };
