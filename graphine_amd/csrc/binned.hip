@@ -210,9 +210,10 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
 extern "C" __device__ void gre_buffer_store_idx(float v, __amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
                                                 int soffset, int aux) __asm("llvm.amdgcn.struct.ptr.buffer.store.f32");
 
-// Scatter: workgroup b runs chunk (b % 8) * per + b / 8, so each XCD walks its
-// own contiguous stretch of chunks, and consecutive chunks (whose runs of one
-// bin are adjacent in the bin array) share one L2.  Entry i of the chunk reads
+// Scatter: workgroup b runs chunk xs[b % 8] + b / 8 (workgroups go to the XCDs
+// in turn), so each XCD walks its own contiguous stretch of chunks, and
+// consecutive chunks (whose runs of one bin are adjacent in the bin array)
+// share one L2.  Entry i of the chunk reads
 // its source's value from the LDS stage; the last entry of each group writes
 // the group's sum to slot c + rb[rank], c = the chunk's groups before it and
 // rank = its run, counted from the wave's flags (ballots) and its block's
@@ -232,13 +233,14 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(
     const T* __restrict__ contrib, const uint32_t* __restrict__ ch_r0, const uint32_t* __restrict__ ch_nr,
     const uint32_t* __restrict__ ch_e0, const uint32_t* __restrict__ ch_w0, const uint32_t* __restrict__ ch_q0,
     const uint16_t* __restrict__ src16, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ rbase,
-    const float* __restrict__ wbin, uint32_t nch, uint32_t nslots, T* __restrict__ binval, T* __restrict__ trash) {
+    const float* __restrict__ wbin, XcdStretches xs, uint32_t nslots, T* __restrict__ binval,
+    T* __restrict__ trash) {
   constexpr bool F64 = std::is_same<T, double>::value;
   __shared__ T sv[CR];
   __shared__ uint32_t rb[kBinMaxBins];
-  const uint32_t per = (nch + 7) / 8;
-  const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  if (w >= nch) return;
+  const uint32_t x = blockIdx.x & 7u;
+  const uint32_t w = xs.s[x] + (blockIdx.x >> 3);
+  if (w >= xs.s[x + 1]) return;
   const uint32_t r0 = ch_r0[w], nr = ch_nr[w];
   const uint32_t n = ch_e0[w + 1] - ch_e0[w];
   const uint32_t w0 = ch_w0[w], q0 = ch_q0[w], nq = ch_q0[w + 1] - q0;
@@ -994,6 +996,18 @@ std::shared_ptr<XPart> build_binpart(gre_graph* g, bool eng) {
   const uint32_t nch = (uint32_t)ch_r0.size();
   GRE_CHECK(ch_e0.back() == E, GRE_DEV_FAILURE, "binned layout: chunk entries do not add up to the edges");
   bp.nch = nch;
+  // the scatter's XCD stretches: XCD x runs chunks [xs[x], xs[x+1]) in order,
+  // cut at equal shares of the entries (equal chunk counts gave the stretch
+  // of the last, low-degree sources 0.4 of a share and the others 1.09)
+  bp.xs.s[0] = 0;
+  for (uint32_t x = 1, w = 0; x < 8; ++x) {
+    const uint64_t want = (uint64_t)E * x / 8;
+    while (w < nch && ch_e0[w] < want) ++w;
+    bp.xs.s[x] = std::max(w, bp.xs.s[x - 1]);
+  }
+  bp.xs.s[8] = nch;
+  bp.xs_max = 0;
+  for (uint32_t x = 0; x < 8; ++x) bp.xs_max = std::max(bp.xs_max, bp.xs.s[x + 1] - bp.xs.s[x]);
   copy_to_dev(bp.ch_r0, ch_r0, s);
   copy_to_dev(bp.ch_nr, ch_nr, s);
   copy_to_dev(bp.ch_e0, ch_e0, s);
@@ -1262,12 +1276,11 @@ void bin_iter_t(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_
   Up up{a.contrib_out, a.x, a.p_r, a.outdeg_r, a.scal_in, (float)a.damp, (float)(1.0 - a.damp), (float)a.invn};
   {
     ProfScope ps("bin_scatter");
-    const unsigned per = (bp.nch + 7) / 8;
     auto kern = bp.Eb >= (1ULL << 30) ? k_bin_scatter<kScatterU, W, true> : k_bin_scatter<kScatterU, W, false>;
-    hipLaunchKernelGGL(kern, dim3(8 * per), dim3(kScatterBlock), 0, s, a.contrib_in,
+    hipLaunchKernelGGL(kern, dim3(8 * bp.xs_max), dim3(kScatterBlock), 0, s, a.contrib_in,
                        bp.ch_r0.as<uint32_t>(), bp.ch_nr.as<uint32_t>(), bp.ch_e0.as<uint32_t>(),
                        bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(), bp.src16.as<uint16_t>(),
-                       bp.rcnt.as<uint32_t>(), bp.rbase.as<uint32_t>(), a.wbin, bp.nch,
+                       bp.rcnt.as<uint32_t>(), bp.rbase.as<uint32_t>(), a.wbin, bp.xs,
                        (uint32_t)bp.Eb,
                        bb.binval.as<float>(), (float*)nullptr);
   }
@@ -1320,13 +1333,12 @@ void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, c
   }
   {
     ProfScope ps("beng_scatter");
-    const unsigned per = (bp.nch + 7) / 8;
-    hipLaunchKernelGGL((k_bin_scatter<kEngScatterU, false, false, double, kEngChunkRows>), dim3(8 * per),
+    hipLaunchKernelGGL((k_bin_scatter<kEngScatterU, false, false, double, kEngChunkRows>), dim3(8 * bp.xs_max),
                        dim3(kScatterBlock), 0, s,
                        (const double*)bb.Sc.as<double>(), bp.ch_r0.as<uint32_t>(), bp.ch_nr.as<uint32_t>(),
                        bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(), bp.ch_q0.as<uint32_t>(),
                        bp.src16.as<uint16_t>(), bp.rcnt.as<uint32_t>(), bp.rbase.as<uint32_t>(), (const float*)nullptr,
-                       bp.nch, (uint32_t)bp.Eb, bb.binval.as<double>(), bb.trash.as<double>());
+                       bp.xs, (uint32_t)bp.Eb, bb.binval.as<double>(), bb.trash.as<double>());
     GRE_HIP(hipGetLastError());
   }
   {

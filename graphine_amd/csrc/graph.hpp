@@ -27,6 +27,10 @@ constexpr int kEngBinRows = 8192;      // rows per bin of the engine's FP64 layo
 constexpr int kChunkRows = 16384;      // source rows whose values a chunk stages in LDS
 constexpr uint32_t kChunkEdges = 262144;   // entries per chunk (a longer source row is cut)
 constexpr uint32_t kSliceEdges = 1u << 20;  // entries per accumulate slice (a longer bin is cut)
+// the scatter's chunk stretches, one per XCD: XCD x runs chunks [s[x], s[x+1])
+struct XcdStretches {
+  uint32_t s[9];
+};
 struct BinPlan {
   // Chunks start at 64-entry boundaries of the word array and bins at 64-entry
   // boundaries of the bin array, so every wave's loads cover whole lines; the
@@ -40,6 +44,8 @@ struct BinPlan {
   uint32_t max_in = 0;    // the engine's layout: largest in-degree
   DevBuf rowv;     // the engine's layout: uint32 [nd] vertex of each row (ascending)
   uint32_t nd = 0, B = 0, nch = 0, nsl = 0, nmb = 0, nmrows = 0;
+  XcdStretches xs{};    // chunk stretches of the XCDs, cut at equal entry counts
+  uint32_t xs_max = 0;  // the longest stretch (the scatter's grid is 8 of these)
   DevBuf ch_r0;    // uint32 [nch] first source row of chunk w
   DevBuf ch_nr;    // uint32 [nch] source rows chunk w stages
   DevBuf ch_e0;    // uint32 [nch+1] entries before chunk w (chunk w has ch_e0[w+1] - ch_e0[w])
