@@ -99,6 +99,26 @@ __device__ inline void block_sum2_big(double& a, double& b, double* red /*[32]*/
 
 // ---- iteration kernels ------------------------------------------------------
 
+__device__ inline float fix_to_float(unsigned long long v) {
+  return (float)((double)(long long)v * 0x1p-58);
+}
+static_assert(kBinFix == 58, "fix_to_float and the scatter scale use 2^58");
+
+// The accumulate's fixed-point conversions.  The forward: the fixed scale
+// 2^58 (its sums are bounded by the ranks' L1 norm).
+struct Fix58 {
+  __device__ long long to(float x) const { return __float2ll_rn(x * 0x1p58f); }
+  __device__ float from(unsigned long long v) const { return fix_to_float(v); }
+};
+// A scale 2^T for one launch: with every row's sum of |values| below 2^e,
+// T = 61 - e keeps each value, each partial sum and each row's sum below 2^62
+// in magnitude, whatever the order of the additions.
+struct FixDyn {
+  double sc, isc;  // 2^T, 2^-T
+  __device__ long long to(float x) const { return __double2ll_rn((double)x * sc); }
+  __device__ float from(unsigned long long v) const { return (float)((double)(long long)v * isc); }
+};
+
 // Row r's update from its in-edge sum s (the same formula as the tile pulls'
 // forward mode, contribution indexed by row: rank / out-degree, or the rank
 // itself when the scatter multiplies by normalised edge weights, CRAW).
@@ -132,12 +152,59 @@ struct BinUpdate {
     if (R) a1 += fabs((double)y - (double)q.old);
   }
   __device__ void apply(uint32_t r, float s, double& a0, double& a1) const { fin(r, s, pre(r), a0, a1); }
+  __device__ Fix58 fix() const { return {}; }
 };
 
-__device__ inline float fix_to_float(unsigned long long v) {
-  return (float)((double)(long long)v * 0x1p-58);
-}
-static_assert(kBinFix == 58, "fix_to_float and the scatter scale use 2^58");
+// Row u's adjoint update over the binned layout of the reversed graph (its
+// rows are u's out-edges' heads: the same formula as the tile pulls' adjoint
+// mode).  s = the sum of lambda_v over u's out-edges (u -> v), times the
+// normalised weight w_uv (W); unweighted sums are divided by u's out-degree d:
+//   (W lambda)_u = s,  lambda'_u = c_u + damp (s + [d == 0] p.lambda).
+// The fixed-point scale follows max |lambda| (the bits in *mx_in) times F, a
+// bound on sum_v |w_uv| over every row (the largest out-degree, times the
+// largest |w_uv| when weighted).
+template <bool W>
+struct BinAdjUpdate {
+  float* lout;
+  float* wl;
+  const float* lin;
+  const float* c_r;
+  const float* p_r;
+  const int32_t* deg_r;     // out-degree of each row's vertex
+  const double* scal_in;    // [0] = p.lambda of lin
+  const uint32_t* mx_in;    // bits of max |lin|
+  double F;
+  float damp;
+  float invn;
+  struct Pre {
+    float cu, old, pv;
+    int32_t d;
+  };
+  __device__ FixDyn fix() const {
+    const double S = (double)__uint_as_float(*mx_in) * F;
+    int e = 0;
+    if (S > 0.0 && S < 0x1p1000) frexp(S, &e);  // S < 2^e
+    return FixDyn{ldexp(1.0, 61 - e), ldexp(1.0, e - 61)};
+  }
+  __device__ Pre pre(uint32_t r) const {
+    Pre q;
+    q.cu = c_r[r];
+    q.old = lin[r];
+    q.pv = p_r ? p_r[r] : invn;
+    q.d = deg_r[r];
+    return q;
+  }
+  __device__ void fin(uint32_t r, float s, const Pre& q, double& a0, double& a1) const {
+    if (!W && q.d > 0) s = s / (float)q.d;
+    wl[r] = s;
+    const float y = q.cu + damp * (s + (q.d == 0 ? (float)scal_in[0] : 0.f));
+    lout[r] = y;
+    a0 += (double)q.pv * (double)y;
+    a1 += fabs((double)y - (double)q.old);
+  }
+  __device__ void apply(uint32_t r, float s, double& a0, double& a1) const { fin(r, s, pre(r), a0, a1); }
+};
+
 
 // Lane l of a wave holds entry 64k + l of its chunk.  Entries of one run that
 // go to the same destination are consecutive (a run is sorted by destination)
@@ -399,6 +466,7 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
     if (threadIdx.x == 0) { part[2 * s] = a0; part[2 * s + 1] = a1; }
     return;
   }
+  const auto fx = up.fix();
   const uint32_t b = sl_bin[s];
   const uint32_t r0 = bin_r0[b], nr = bin_r0[b + 1] - r0;
   const uint32_t e0 = sl_e0[s], n = sl_e0[s + 1] - e0;
@@ -419,7 +487,7 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
     for (int j = 0; j < U; ++j) {
       const uint32_t i = base + j * kAccumBlock + threadIdx.x;
       const float x = i < n ? v[j] : 0.f;
-      atomicAdd(&acc[d[j]], (unsigned long long)__float2ll_rn(x * 0x1p58f));
+      atomicAdd(&acc[d[j]], (unsigned long long)fx.to(x));
     }
   };
   float va[U], vb[U];
@@ -452,7 +520,7 @@ __global__ __launch_bounds__(kAccumBlock) void k_bin_accum(
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       const uint32_t i = threadIdx.x + k * kAccumBlock;
-      if (i < nr) up.fin(r0 + i, fix_to_float(acc[i]), q[k], a0, a1);
+      if (i < nr) up.fin(r0 + i, fx.from(acc[i]), q[k], a0, a1);
     }
   } else {
     for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock)
@@ -470,6 +538,7 @@ __global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_
                                                 Up up, unsigned long long* __restrict__ gacc,
                                                 double* __restrict__ part) {
   __shared__ double red[32];
+  const auto fx = up.fix();
   double a0 = 0.0, a1 = 0.0;
   for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nmrows; q += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t lo = 0, hi = nmb;  // largest k with mb_g[k] <= q
@@ -480,7 +549,7 @@ __global__ __launch_bounds__(kB) void k_bin_fin(const uint32_t* __restrict__ mb_
     }
     const unsigned long long v = gacc[q];
     gacc[q] = 0ULL;
-    up.apply(mb_r0[lo] + (uint32_t)(q - mb_g[lo]), fix_to_float(v), a0, a1);
+    up.apply(mb_r0[lo] + (uint32_t)(q - mb_g[lo]), fx.from(v), a0, a1);
   }
   block_sum2_big(a0, a1, red);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a0; part[2 * blockIdx.x + 1] = a1; }
@@ -1366,6 +1435,89 @@ void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, u
     if (a.resid) bin_iter_t<true, false>(xp, bb, a, nV, s);
     else bin_iter_t<false, false>(xp, bb, a, nV, s);
   }
+}
+
+// ---- the binned adjoint -------------------------------------------------------
+
+// *mx = the larger of *mx and the bits of max |x[i]| (the bits of non-negative
+// floats order as their values); ABS = false: of max x[i].  One atomic per
+// workgroup, on a grid of at most kMaxGrid (atomics on one address serialise:
+// one per wave over 67 M values took 2.9 ms)
+constexpr unsigned kMaxGrid = 1024;
+template <bool ABS>
+__global__ __launch_bounds__(kB) void k_bin_max(const uint32_t* __restrict__ x, uint64_t n,
+                                                uint32_t* __restrict__ mx) {
+  __shared__ uint32_t wm[kB / 64];
+  uint32_t m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = ABS ? x[i] & 0x7fffffffu : x[i];
+    m = b > m ? b : m;
+  }
+  m = wave_max_u32(m);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned k = 1; k < kB / 64; ++k) m = wm[k] > m ? wm[k] : m;
+    if (m) atomicMax(mx, m);
+  }
+}
+
+__global__ void k_bin_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+void bin_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_zero_u32, dim3(gcap(n)), dim3(kB), 0, s, p, n);
+  GRE_HIP(hipGetLastError());
+}
+void bin_absmax(const float* x, uint64_t n, uint32_t* mx, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_max<true>, dim3(std::min(gcap(n), kMaxGrid)), dim3(kB), 0, s,
+                     reinterpret_cast<const uint32_t*>(x), n, mx);
+  GRE_HIP(hipGetLastError());
+}
+void bin_maxu(const uint32_t* x, uint64_t n, uint32_t* mx, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_max<false>, dim3(std::min(gcap(n), kMaxGrid)), dim3(kB), 0, s, x, n, mx);
+  GRE_HIP(hipGetLastError());
+}
+
+namespace {
+template <bool W>
+void bin_adj_t(const XPart& xp, const BinBuffers& bb, const BinAdjIter& a, uint32_t nV, hipStream_t s) {
+  const BinPlan& bp = xp.bin;
+  using Up = BinAdjUpdate<W>;
+  Up up{a.lout, a.wl, a.lin, a.c_r, a.p_r, a.deg_r, a.scal_in, a.mx_in, a.F, (float)a.damp, (float)a.invn};
+  {
+    ProfScope ps("bin_adj_scatter");
+    auto kern = bp.Eb >= (1ULL << 30) ? k_bin_scatter<kScatterU, W, true> : k_bin_scatter<kScatterU, W, false>;
+    hipLaunchKernelGGL(kern, dim3(8 * bp.xs_max), dim3(kScatterBlock), 0, s, a.lin, bp.ch_r0.as<uint32_t>(),
+                       bp.ch_nr.as<uint32_t>(), bp.ch_e0.as<uint32_t>(), bp.ch_w0.as<uint32_t>(),
+                       bp.ch_q0.as<uint32_t>(), bp.src16.as<uint16_t>(), bp.rcnt.as<uint32_t>(),
+                       bp.rbase.as<uint32_t>(), a.wbin, bp.xs, (uint32_t)bp.Eb, bb.binval.as<float>(),
+                       (float*)nullptr);
+  }
+  {
+    ProfScope ps("bin_adj_accum");
+    hipLaunchKernelGGL((k_bin_accum<Up, kAccumU>), dim3(bp.nsl + bb.tail_blocks), dim3(kAccumBlock), 0, s,
+                       bb.binval.as<float>(), bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(),
+                       bp.sl_bin.as<uint32_t>(), bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), bp.nsl, bp.nd,
+                       nV, up, bb.gacc.as<unsigned long long>(), bb.part.as<double>());
+  }
+  {
+    ProfScope ps("bin_adj_finish");
+    hipLaunchKernelGGL((k_bin_fin<Up>), dim3(bb.fin_grid), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
+                       bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, up, bb.gacc.as<unsigned long long>(),
+                       bb.part2.as<double>());
+  }
+  reduce_pairs(bb.part.as<double>(), bp.nsl + bb.tail_blocks, bb.part2.as<double>(), bb.fin_grid, a.scal_out,
+               bb.part3.as<double>(), s);
+  GRE_HIP(hipGetLastError());
+  bin_absmax(a.lout, nV, a.mx_out, s);
+}
+}  // namespace
+
+void bin_adjoint_iter(const XPart& xp, const BinBuffers& bb, const BinAdjIter& a, uint32_t nV, hipStream_t s) {
+  if (a.wbin) bin_adj_t<true>(xp, bb, a, nV, s);
+  else bin_adj_t<false>(xp, bb, a, nV, s);
 }
 
 }  // namespace gre

@@ -37,6 +37,31 @@ struct BinIter {
 
 void bin_forward_iter(const XPart& xp, const BinBuffers& bb, const BinIter& a, uint32_t nV, hipStream_t s);
 
+// One iteration of the single-vector FP32 adjoint over the binned layout of the
+// reversed graph (graph_reversed): rows are the reversed graph's, every vector
+// in its row order, lambda's scatter reads lin itself.  Buffers as the forward's
+// (BinBuffers made for that layout).
+struct BinAdjIter {
+  const float* wbin = nullptr;   // weighted: normalised weights in word order; nullptr: unweighted
+  const float* lin = nullptr;    // lambda read
+  float* lout = nullptr;         // lambda written
+  float* wl = nullptr;           // (W lambda) written
+  const float* c_r = nullptr;
+  const float* p_r = nullptr;    // personalisation, or nullptr (uniform)
+  const int32_t* deg_r = nullptr;  // out-degree (of the graph, not the reversed one)
+  const double* scal_in = nullptr;  // [0] = p.lambda of lin
+  double* scal_out = nullptr;       // (p.lambda, L1 change) of lout
+  const uint32_t* mx_in = nullptr;  // bits of max |lin|
+  uint32_t* mx_out = nullptr;       // max |lout| is folded in (zero it first)
+  double F = 1.0;                   // bound on sum_v |w_uv| over every row u
+  double damp = 0.85, invn = 0.0;
+};
+void bin_adjoint_iter(const XPart& xp, const BinBuffers& bb, const BinAdjIter& a, uint32_t nV, hipStream_t s);
+// *mx = max(*mx, bits of max |x[i]|); max over uint32 values; zero fill
+void bin_absmax(const float* x, uint64_t n, uint32_t* mx, hipStream_t s);
+void bin_maxu(const uint32_t* x, uint64_t n, uint32_t* mx, hipStream_t s);
+void bin_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
 // The engine's FP64 sum pull over the binned layout of graph_binpart_engine
 // (sources staged in compact source order, rows = the vertices with in-edges
 // in vertex order): C[v] += the sum of S over v's in-neighbours, as exact

@@ -81,6 +81,12 @@ struct DevBuf {
   // buffer starts on that boundary, at the cost of up to align more bytes
   void alloc(size_t n, size_t align = 0);
   void release();
+  // a view of o's memory: never freed through this buffer, valid while o is
+  void view(const DevBuf& o) {
+    release();
+    p = o.p;
+    bytes = o.bytes;
+  }
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 

@@ -381,6 +381,25 @@ void graph_compact_targets(gre_graph* g) {
   g->ntgt = compact_ids(g, false, g->idx.as<uint32_t>(), g->did, g->idx_c);
 }
 
+gre_graph* graph_reversed(gre_graph* g) {
+  if (g->rev) return g->rev.get();
+  hipStream_t s = stream();
+  auto r = std::make_unique<gre_graph>();
+  r->nV = g->nV;
+  r->nE = g->nE;
+  r->off.view(g->coff);
+  r->idx.view(g->cidx);
+  r->coff.view(g->off);
+  r->cidx.view(g->idx);
+  r->outdeg.alloc(sizeof(int32_t) * (size_t)g->nV);
+  hipLaunchKernelGGL(k_outdeg, dim3(grid_cap(g->nV)), dim3(kBlock), 0, s, g->coff.as<uint64_t>(), g->nV,
+                     r->outdeg.as<int32_t>());
+  GRE_HIP(hipGetLastError());
+  GRE_HIP(hipStreamSynchronize(s));
+  g->rev = std::move(r);
+  return g->rev.get();
+}
+
 const uint32_t* graph_tiles(gre_graph* g, bool csc, uint32_t te, uint64_t* ntiles) {
   uint64_t nt = (g->nE + te - 1) / te;
   if (nt == 0) nt = 1;

@@ -165,6 +165,9 @@ struct gre_graph {
   // caller sets them)
   uint64_t ms_long = 0, ms_chunk = 0;
   uint64_t bc_flong = 0, bc_fchunk = 0;
+  // the reversed graph (built on demand): its CSR and CSC are views of this
+  // graph's CSC and CSR, its out-degrees are this graph's in-degrees
+  std::unique_ptr<gre_graph> rev;
 };
 
 namespace gre {
@@ -195,7 +198,10 @@ std::shared_ptr<XPart> graph_binpart(gre_graph* g);
 // The binned layout of the engine's FP64 sum pull (binned.hip), cached on the
 // graph like the other partitions.
 std::shared_ptr<XPart> graph_binpart_engine(gre_graph* g);
-// Drops the cached partitions that no PageRank handle or engine holds any more.
+// The reversed graph of g (cached on g; the views stay valid while g lives).
+gre_graph* graph_reversed(gre_graph* g);
+// Drops the cached partitions that no PageRank handle or engine holds any more
+// (the reversed graph's too).
 void prune_xparts(gre_graph* g);
 // Its tiles of te entries (cached on the partition).
 const XPart::Tiles& xpart_tiles(XPart& xp, uint32_t te, hipStream_t s);

@@ -382,6 +382,14 @@ bool adj_part(const gre_pr* P) {
   return use_part(P) || (use_bin(P) && P->g->nE > 0 && P->g->nE < 0xffffffffULL);
 }
 bool adj_compact(const gre_pr* P) { return use_compact(P) || use_bin(P); }
+// The binned forward's adjoint, FP32: the binned layout built over the reversed
+// graph, when its rows (the vertices with out-edges) fit the bins
+bool adj_bin(gre_pr* P) {
+  if (!use_bin(P) || P->dtype != GRE_F32 || P->g->nE == 0 || P->g->nE >= 0xffffffffULL) return false;
+  gre_graph* r = graph_reversed(P->g);
+  graph_compact_targets(r);
+  return (uint64_t)r->ntgt <= (uint64_t)kBinMaxBins * kBinRows;
+}
 // the partitioned and binned forwards walk relabelled rows: a run keeps its
 // ranks in the layout's row order until ranks_to_vertices
 bool rows_relabelled(const gre_pr* P) { return (use_part(P) || use_bin(P)) && P->xp && P->xp->permuted; }
@@ -954,8 +962,144 @@ void autotune_forward(gre_pr* P, double damp) {
   prune_xparts(P->g);
 }
 
+// the binned adjoint's layout, buffers, out-degrees and weights in its order,
+// and the bounds of its fixed-point sums
+void ensure_bin_adj_plan(gre_pr* P, int iters) {
+  hipStream_t s = stream();
+  gre_graph* g = P->g;
+  const uint32_t nV = g->nV;
+  if (!P->xpb) {
+    P->xpb = graph_binpart(graph_reversed(g));
+    P->wbinb_xp.reset();
+    P->wbinb_ver = ~0ULL;
+    P->maxdeg_b = 0.0;
+  }
+  const XPart& xp = *P->xpb;
+  if (P->bbufb.for_xp != &xp) P->bbufb.make(xp, nV, s);
+  const size_t xb = sizeof(float) * (size_t)nV;
+  if (P->cb_r.bytes < xb) {
+    P->cb_r.alloc(xb);
+    P->pb_r.alloc(xb);
+    P->degb_r.alloc(xb);
+    P->wlb_r.alloc(xb);
+  }
+  if (P->mxb.bytes < 4 * ((size_t)iters + 1)) P->mxb.alloc(4 * ((size_t)iters + 1));
+  if (P->maxdeg_b == 0.0) {
+    hipLaunchKernelGGL(k_permute32<int32_t>, dim3(gcap(nV)), dim3(kB), 0, s, g->outdeg.as<int32_t>(),
+                       xp.rowinv.as<uint32_t>(), (uint64_t)nV, P->degb_r.as<int32_t>());
+    GRE_HIP(hipGetLastError());
+    uint32_t* m = static_cast<uint32_t*>(scratch(4, 16));
+    bin_zero_u32(m, 1, s);
+    bin_maxu(g->outdeg.as<uint32_t>(), nV, m, s);
+    uint32_t h = 0;
+    GRE_HIP(hipMemcpyAsync(&h, m, 4, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    P->maxdeg_b = std::max(1.0, (double)h);
+  }
+  if (P->weighted && (P->wbinb_xp != P->xpb || P->wbinb_ver != P->wver)) {
+    // the normalised weights of the reversed graph's CSR order are the CSC's
+    const uint64_t Ew = xp.bin.Ew;
+    P->wbinb.alloc(sizeof(float) * Ew);
+    hipLaunchKernelGGL(k_permute32<float>, dim3(gcap(Ew)), dim3(kB), 0, s, P->wcsc.as<float>(),
+                       xp.bin.wpos.as<uint32_t>(), Ew, P->wbinb.as<float>());
+    GRE_HIP(hipGetLastError());
+    uint32_t* m = static_cast<uint32_t*>(scratch(4, 16));
+    bin_zero_u32(m, 1, s);
+    bin_absmax(P->wcsc.as<float>(), P->g->nE, m, s);
+    uint32_t h = 0;
+    GRE_HIP(hipMemcpyAsync(&h, m, 4, hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    float w = 0.f;
+    std::memcpy(&w, &h, 4);
+    P->wmax_b = (double)w;
+    P->wbinb_xp = P->xpb;
+    P->wbinb_ver = P->wver;
+  }
+}
+
+// The single-vector FP32 adjoint over the binned layout of the reversed graph:
+// lambda, c, p and (W lambda) kept in its row order, turned back into vertex
+// order at the end.  Each iteration's fixed-point scale follows max |lambda|
+// of the iteration before (measured on the device).
+double run_adjoint_bin(gre_pr* P, double damp, int iters, double tol, int* done) {
+  hipStream_t s = stream();
+  gre_graph* g = P->g;
+  const uint32_t nV = g->nV;
+  const unsigned gb = gcap(nV);
+  ensure_bin_adj_plan(P, iters);
+  ensure_scal(P, iters);
+  double* tmp = static_cast<double*>(scratch(3, sizeof(double) * 2 * gb));
+  const XPart& xp = *P->xpb;
+  const uint32_t* rinv = xp.rowinv.as<uint32_t>();
+  const uint32_t* rperm = xp.rowperm.as<uint32_t>();
+  hipLaunchKernelGGL(k_permute32<float>, dim3(gb), dim3(kB), 0, s, P->c.as<float>(), rinv, (uint64_t)nV,
+                     P->cb_r.as<float>());
+  if (P->has_p)
+    hipLaunchKernelGGL(k_permute32<float>, dim3(gb), dim3(kB), 0, s, P->p.as<float>(), rinv, (uint64_t)nV,
+                       P->pb_r.as<float>());
+  uint32_t* mx = P->mxb.as<uint32_t>();
+  bin_zero_u32(mx, (uint32_t)iters + 1, s);
+  double* sc = P->scal.as<double>();
+  P->lcur = 0;
+  hipLaunchKernelGGL(k_adj_init<float>, dim3(gb), dim3(kB), 0, s, nV, P->cb_r.as<float>(),
+                     P->has_p ? P->pb_r.as<float>() : nullptr, P->lam[0].as<float>(), 1.0 / (double)nV, tmp,
+                     (const uint32_t*)nullptr, (float*)nullptr);
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, tmp, (uint64_t)gb, tmp, (uint64_t)0, sc);
+  GRE_HIP(hipGetLastError());
+  bin_absmax(P->lam[0].as<float>(), nV, mx, s);
+  const double F = P->maxdeg_b * (P->weighted ? P->wmax_b : 1.0);
+  int it = 0;
+  double res = 0.0;
+  for (it = 0; it < iters; ++it) {
+    BinAdjIter a;
+    a.wbin = P->weighted ? P->wbinb.as<float>() : nullptr;
+    a.lin = P->lam[P->lcur].as<float>();
+    a.lout = P->lam[P->lcur ^ 1].as<float>();
+    a.wl = P->wlb_r.as<float>();
+    a.c_r = P->cb_r.as<float>();
+    a.p_r = P->has_p ? P->pb_r.as<float>() : nullptr;
+    a.deg_r = P->degb_r.as<int32_t>();
+    a.scal_in = sc + 2 * it;
+    a.scal_out = sc + 2 * (it + 1);
+    a.mx_in = mx + it;
+    a.mx_out = mx + it + 1;
+    a.F = F;
+    a.damp = damp;
+    a.invn = 1.0 / (double)nV;
+    bin_adjoint_iter(xp, P->bbufb, a, nV, s);
+    P->lcur ^= 1;
+    if (tol > 0.0) {
+      double h[2];
+      GRE_HIP(hipMemcpyAsync(h, sc + 2 * (it + 1), sizeof(h), hipMemcpyDeviceToHost, s));
+      GRE_HIP(hipStreamSynchronize(s));
+      res = h[1];
+      if (res < tol) { ++it; break; }
+    }
+  }
+  if (tol <= 0.0 && iters > 0) {
+    double h[2];
+    GRE_HIP(hipMemcpyAsync(h, sc + 2 * iters, sizeof(h), hipMemcpyDeviceToHost, s));
+    GRE_HIP(hipStreamSynchronize(s));
+    res = h[1];
+  }
+  // back to vertex order
+  hipLaunchKernelGGL(k_permute32<float>, dim3(gb), dim3(kB), 0, s, P->lam[P->lcur].as<float>(), rperm,
+                     (uint64_t)nV, P->lam[P->lcur ^ 1].as<float>());
+  P->lcur ^= 1;
+  if (it > 0)
+    hipLaunchKernelGGL(k_permute32<float>, dim3(gb), dim3(kB), 0, s, P->wlb_r.as<float>(), rperm, (uint64_t)nV,
+                       P->wl.as<float>());
+  GRE_HIP(hipGetLastError());
+  GRE_HIP(hipStreamSynchronize(s));
+  if (done) *done = it;
+  return res;
+}
+
 template <typename TT>
 double run_adjoint(gre_pr* P, double damp, int iters, double tol, int* done) {
+  if constexpr (std::is_same<TT, float>::value) {
+    if (adj_bin(P)) return run_adjoint_bin(P, damp, iters, tol, done);
+  }
   hipStream_t s = stream();
   const uint32_t nV = P->g->nV;
   const unsigned gb = gcap(nV);

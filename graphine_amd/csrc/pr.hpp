@@ -87,6 +87,15 @@ struct gre_pr {
   gre::PullPlan plan_xa, plan_pxa;
   gre::PullBuffers buf_xa, buf_pxa;
   int plan_ipt_xa = 0;
+  // adjoint, binned schedule: the binned layout of the reversed graph, its
+  // buffers, c, p, out-degrees and (W lambda) in its row order, the weights in
+  // its word order, max |lambda| per iteration, and the bound of the row sums
+  std::shared_ptr<gre::XPart> xpb;
+  gre::BinBuffers bbufb;
+  gre::DevBuf cb_r, pb_r, degb_r, wlb_r, wbinb, mxb;
+  std::shared_ptr<gre::XPart> wbinb_xp;
+  uint64_t wbinb_ver = ~0ULL;
+  double maxdeg_b = 0.0, wmax_b = 0.0;
   int lcur = 0;
   bool have_adj = false;
   bool have_fwd = false;

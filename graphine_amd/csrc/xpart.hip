@@ -408,6 +408,7 @@ void prune_xparts(gre_graph* g) {
     if (it->second.use_count() == 1) it = g->xpart.erase(it);
     else ++it;
   }
+  if (g->rev) prune_xparts(g->rev.get());
 }
 
 std::shared_ptr<XPart> graph_xpart(gre_graph* g, int R, uint32_t H, bool csc, bool rows_permuted) {
