@@ -376,8 +376,9 @@ bool use_part(const gre_pr* P) { return P->part && use_compact(P); }
 // the binned pull: single vector, FP32
 bool use_bin(const gre_pr* P) { return P->bin && P->k == 1; }
 // The single-vector adjoint's schedule family: the compact and partitioned
-// forwards' own, and for the binned forward (which has no transposed layout)
-// the partitioned adjoint over the CSR with the handle's (ranges, shared).
+// forwards' own; the binned forward's adjoint runs its layout over the
+// reversed graph (adj_bin), else the partitioned adjoint over the CSR with the
+// handle's (ranges, shared).
 bool adj_part(const gre_pr* P) {
   return use_part(P) || (use_bin(P) && P->g->nE > 0 && P->g->nE < 0xffffffffULL);
 }

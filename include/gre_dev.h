@@ -214,10 +214,13 @@ int gre_pr_get_adjoint(gre_pr* p, void* host_out);
  * sums in a second pull (single vector, fewer than 2^32 edges; ranges set with
  * gre_pr_set_partition; weighted handles keep a copy of the weights in the
  * partition's entry order, remade when the weights or the partition change).
- * Under COMPACT, PART and BIN the single-vector adjoint pulls over the CSR in
+ * Under COMPACT and PART the single-vector adjoint pulls over the CSR in
  * compact target ids (vertices with in-edges by descending in-degree); under
- * PART and BIN, partitioned over the CSR with the handle's ranges (for BIN,
- * those of the best partition the autotuner timed, or the defaults).
+ * PART, partitioned over the CSR with the handle's ranges.  Under BIN it runs
+ * the binned pull over the reversed graph (fixed-point scale set per iteration
+ * from max |lambda|); a graph with more than 4096 x 16384 vertices with
+ * out-edges falls back to PART's adjoint with the ranges of the best
+ * partition the autotuner timed, or the defaults.
  * Batched handles (k > 1) have two schedules: TILE (groups of lanes walk
  * fixed tiles of entries, rows split across tiles are joined by a fix-up) and
  * ROWS (one group of lanes per destination row, wavefront per vertex).

@@ -184,9 +184,9 @@ def test_binned_noalloc_equals_run(N):
 
 
 def test_binned_then_adjoint_and_gradients(N, O):
-    """The adjoint of a binned handle runs on the partitioned pull over the CSR;
-    the gradients read the forward's ranks (kept in the binned row order until
-    read) correctly."""
+    """The adjoint of a binned handle runs the binned pull over the reversed
+    graph and agrees with the tile schedule's; the gradients read the forward's
+    ranks (kept in the binned row order until read) correctly."""
     scale, d = 13, 0.85
     g = N.Graph.rmat(scale, 16, 4, False)
     bh = binned(N, g)
@@ -269,8 +269,9 @@ def test_binned_weighted_ppr_matches_oracle(N, O):
 
 
 def test_binned_weighted_adjoint_gradients(N, O):
-    """After a weighted binned forward the adjoint (partitioned pull over the
-    CSR) and the gradients read the forward's ranks in vertex order."""
+    """After a weighted binned forward the adjoint (binned pull over the
+    reversed graph, weights in its word order) and the gradients read the
+    forward's ranks in vertex order."""
     scale, d = 13, 0.85
     g = N.Graph.rmat(scale, 16, 5, False)
     og = O.rmat_graph(scale, 16, 5, False)
