@@ -24,6 +24,7 @@ GTEPS = stored edges x iterations / time.  The same JSON line carries:
   graph_replay  20 PageRank iterations on scale 16, eager vs hipGraph replay.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26] [--schedule auto]
+                       [--no-extra | --ppr-only] [--no-cpu] [--no-sweep]
 """
 import argparse
 import json
@@ -54,6 +55,7 @@ def parse():
     ap.add_argument("--bfs-sources", type=int, default=16)
     ap.add_argument("--ppr-k", type=int, default=32)
     ap.add_argument("--no-extra", action="store_true", help="headline PageRank only")
+    ap.add_argument("--ppr-only", action="store_true", help="of the extra lines, the PPR comparison only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-sweep", action="store_true", help="skip the pull-schedule sweep")
@@ -507,57 +509,58 @@ def main():
         }
         log("ppr:", extra["ppr"])
         del pr
-        extra["engine_pagerank"] = bench_engine_pr(N, g, iter_s * 1e3)
-        log("engine pagerank:", extra["engine_pagerank"])
-        del g
-        # ---------------- BFS on the symmetrised graph ----------------------
-        t0 = time.time()
-        gs = N.Graph.rmat(S, 16, 6, True)
-        log(f"symmetrised R-MAT scale {S}: V={gs.nV} E={gs.nE} built in {time.time() - t0:.1f}s")
-        lev, _, _ = N.bfs(gs, int(np.argmax(gs.outdeg())))
-        reached = np.nonzero(lev >= 0)[0]
-        # the plain BFS runs from the first bfs_sources of 64 sources; the
-        # 64-bit-mask BFS runs from all 64 in one launch
-        srcs = sample_distinct(rng, reached, max(64, a.bfs_sources)).astype(np.uint32)
-        N.bfs(gs, int(srcs[0]), want_levels=False)
-        rates, plain_ms = [], []
-        for s in srcs[:a.bfs_sources]:
-            N.sync()
-            t1 = time.perf_counter()
-            _, er, _ = N.bfs(gs, int(s), want_levels=False)
-            N.sync()
-            dt_ = time.perf_counter() - t1
-            rates.append((er / 2) / dt_ / 1e9)
-            plain_ms.append(1e3 * dt_)
-        hm = len(rates) / sum(1.0 / r for r in rates)
-        extra["bfs"] = {"sources": len(rates), "graph": f"R-MAT scale {S} symmetrised, seed 6",
-                        "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
-                        "gteps_max": round(max(rates), 3)}
-        # the first 64-source call sizes the working arrays and times the
-        # long-row chunk candidates on the graph; the mean of three later ones is reported
-        N.msbfs(gs, srcs[:64], want_levels=False)
-        ms_runs = []
-        for _ in range(3):
-            N.sync()
-            t1 = time.perf_counter()
-            _, ems = N.msbfs(gs, srcs[:64], want_levels=False)
-            N.sync()
-            ms_runs.append(time.perf_counter() - t1)
-        t_ms = sum(ms_runs) / len(ms_runs)
-        extra["msbfs"] = {"sources": int(min(64, len(srcs))), "gteps": round((ems / 2) / t_ms / 1e9, 3),
-                          "ms": round(1e3 * t_ms, 2), "ms_runs": [round(1e3 * x, 2) for x in ms_runs],
-                          "long_row_chunks": list(N.traversal_chunks(gs, N.CHUNKS_MSBFS)),
-                          "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
-        log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
-        extra["engine_bfs"] = bench_engine_bfs(N, gs, srcs[:4], plain_ms[:4])
-        log("engine bfs:", extra["engine_bfs"])
-        del gs
-        extra["bc"] = bench_bc(N, rng)
-        log("bc:", extra["bc"])
-        extra["graph_replay"] = bench_graph_replay(N)
-        log("graph replay:", extra["graph_replay"])
-        extra["scrambled"] = bench_scrambled(N, S, spec, iters, a.bfs_sources)
-        log("scrambled labels:", extra["scrambled"])
+        if not a.ppr_only:
+            extra["engine_pagerank"] = bench_engine_pr(N, g, iter_s * 1e3)
+            log("engine pagerank:", extra["engine_pagerank"])
+            del g
+            # ---------------- BFS on the symmetrised graph ----------------------
+            t0 = time.time()
+            gs = N.Graph.rmat(S, 16, 6, True)
+            log(f"symmetrised R-MAT scale {S}: V={gs.nV} E={gs.nE} built in {time.time() - t0:.1f}s")
+            lev, _, _ = N.bfs(gs, int(np.argmax(gs.outdeg())))
+            reached = np.nonzero(lev >= 0)[0]
+            # the plain BFS runs from the first bfs_sources of 64 sources; the
+            # 64-bit-mask BFS runs from all 64 in one launch
+            srcs = sample_distinct(rng, reached, max(64, a.bfs_sources)).astype(np.uint32)
+            N.bfs(gs, int(srcs[0]), want_levels=False)
+            rates, plain_ms = [], []
+            for s in srcs[:a.bfs_sources]:
+                N.sync()
+                t1 = time.perf_counter()
+                _, er, _ = N.bfs(gs, int(s), want_levels=False)
+                N.sync()
+                dt_ = time.perf_counter() - t1
+                rates.append((er / 2) / dt_ / 1e9)
+                plain_ms.append(1e3 * dt_)
+            hm = len(rates) / sum(1.0 / r for r in rates)
+            extra["bfs"] = {"sources": len(rates), "graph": f"R-MAT scale {S} symmetrised, seed 6",
+                            "gteps_harmonic_mean": round(hm, 3), "gteps_min": round(min(rates), 3),
+                            "gteps_max": round(max(rates), 3)}
+            # the first 64-source call sizes the working arrays and times the
+            # long-row chunk candidates on the graph; the mean of three later ones is reported
+            N.msbfs(gs, srcs[:64], want_levels=False)
+            ms_runs = []
+            for _ in range(3):
+                N.sync()
+                t1 = time.perf_counter()
+                _, ems = N.msbfs(gs, srcs[:64], want_levels=False)
+                N.sync()
+                ms_runs.append(time.perf_counter() - t1)
+            t_ms = sum(ms_runs) / len(ms_runs)
+            extra["msbfs"] = {"sources": int(min(64, len(srcs))), "gteps": round((ems / 2) / t_ms / 1e9, 3),
+                              "ms": round(1e3 * t_ms, 2), "ms_runs": [round(1e3 * x, 2) for x in ms_runs],
+                              "long_row_chunks": list(N.traversal_chunks(gs, N.CHUNKS_MSBFS)),
+                              "vs_plain_bfs": round(((ems / 2) / t_ms / 1e9) / hm, 2)}
+            log("bfs:", extra["bfs"], "msbfs:", extra["msbfs"])
+            extra["engine_bfs"] = bench_engine_bfs(N, gs, srcs[:4], plain_ms[:4])
+            log("engine bfs:", extra["engine_bfs"])
+            del gs
+            extra["bc"] = bench_bc(N, rng)
+            log("bc:", extra["bc"])
+            extra["graph_replay"] = bench_graph_replay(N)
+            log("graph replay:", extra["graph_replay"])
+            extra["scrambled"] = bench_scrambled(N, S, spec, iters, a.bfs_sources)
+            log("scrambled labels:", extra["scrambled"])
 
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,

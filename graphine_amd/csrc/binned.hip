@@ -1449,9 +1449,22 @@ __global__ __launch_bounds__(kB) void k_bin_max(const uint32_t* __restrict__ x, 
                                                 uint32_t* __restrict__ mx) {
   __shared__ uint32_t wm[kB / 64];
   uint32_t m = 0;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t b = ABS ? x[i] & 0x7fffffffu : x[i];
-    m = b > m ? b : m;
+  const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  // eight loads in flight per thread (one at a time read 268 MB at 2 TB/s)
+  for (; i + 7 * st < n; i += 8 * st) {
+    uint32_t b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = x[i + u * st];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t v = ABS ? b[u] & 0x7fffffffu : b[u];
+      m = v > m ? v : m;
+    }
+  }
+  for (; i < n; i += st) {
+    const uint32_t v = ABS ? x[i] & 0x7fffffffu : x[i];
+    m = v > m ? v : m;
   }
   m = wave_max_u32(m);
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
