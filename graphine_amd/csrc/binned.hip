@@ -613,17 +613,29 @@ __global__ void k_beng_compact(const double* __restrict__ S, const uint32_t* __r
   if ((threadIdx.x & 63) == 0 && m) atomicMax(smax_bits, m);
 }
 
+// Row v's result: C[v] += its sum, or, with a fused update (f.X set), the
+// update applied in place: X[v] = f.a + f.b (C[v] + sum), C[v] = 0.
+__device__ inline void eng_row_done(double* __restrict__ C, const EngFuse& f, uint32_t v, double sum) {
+  const double cv = C[v] + sum;
+  if (f.X) {
+    f.X[v] = f.a + f.b * cv;
+    C[v] = 0.0;
+  } else {
+    C[v] = cv;
+  }
+}
+
 // Accumulate slice s of bin b of the engine's layout (rows = the vertices with
-// in-edges in vertex order): the two-word sums in LDS, then C[v] += the row's
-// sum (the bin's only slice) or the sums added into gacc (finished by
-// k_beng_fin).  Rounds alternate between two register sets as in the FP32
+// in-edges in vertex order): the two-word sums in LDS, then row v's result
+// (eng_row_done; the bin's only slice) or the sums added into gacc (finished
+// by k_beng_fin).  Rounds alternate between two register sets as in the FP32
 // accumulate; lanes past the end add zero to the first slot's row.
 template <int U>
 __global__ __launch_bounds__(kAccumBlock) void k_beng_accum(
     const double* __restrict__ binval, const uint16_t* __restrict__ dloc, const uint32_t* __restrict__ sl_e0,
     const uint32_t* __restrict__ sl_bin, const uint32_t* __restrict__ sl_g, const uint32_t* __restrict__ bin_r0,
     const uint32_t* __restrict__ rowv, const unsigned long long* __restrict__ smax_bits, uint32_t max_in,
-    double* __restrict__ C, unsigned long long* __restrict__ gacc) {
+    double* __restrict__ C, EngFuse f, unsigned long long* __restrict__ gacc) {
   __shared__ unsigned long long ahi[kEngBinRows], alo[kEngBinRows];
   const uint32_t s = blockIdx.x;
   const uint32_t b = sl_bin[s];
@@ -676,10 +688,8 @@ __global__ __launch_bounds__(kAccumBlock) void k_beng_accum(
   __syncthreads();
   const uint32_t g = sl_g[s];
   if (g == 0xffffffffu) {
-    for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) {
-      const uint32_t v = rowv[r0 + i];
-      C[v] = C[v] + eng_join(ahi[i], alo[i], K, L);
-    }
+    for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock)
+      eng_row_done(C, f, rowv[r0 + i], eng_join(ahi[i], alo[i], K, L));
   } else {
     for (uint32_t i = threadIdx.x; i < nr; i += kAccumBlock) {
       if (ahi[i]) atomicAdd(&gacc[2 * (size_t)(g + i)], ahi[i]);
@@ -693,7 +703,7 @@ __global__ __launch_bounds__(kAccumBlock) void k_beng_accum(
 __global__ void k_beng_fin(const uint32_t* __restrict__ mb_r0, const uint32_t* __restrict__ mb_g, uint32_t nmb,
                            uint32_t nmrows, const uint32_t* __restrict__ rowv,
                            const unsigned long long* __restrict__ smax_bits, uint32_t max_in, double* __restrict__ C,
-                           unsigned long long* __restrict__ gacc) {
+                           EngFuse f, unsigned long long* __restrict__ gacc) {
   int K, L;
   eng_scales(smax_bits, max_in, K, L);
   for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nmrows; q += (uint64_t)gridDim.x * blockDim.x) {
@@ -706,8 +716,7 @@ __global__ void k_beng_fin(const uint32_t* __restrict__ mb_r0, const uint32_t* _
     const unsigned long long h = gacc[2 * q], l = gacc[2 * q + 1];
     gacc[2 * q] = 0ULL;
     gacc[2 * q + 1] = 0ULL;
-    const uint32_t v = rowv[mb_r0[lo] + (uint32_t)(q - mb_g[lo])];
-    C[v] = C[v] + eng_join(h, l, K, L);
+    eng_row_done(C, f, rowv[mb_r0[lo] + (uint32_t)(q - mb_g[lo])], eng_join(h, l, K, L));
   }
 }
 
@@ -1392,7 +1401,7 @@ void EngBinBuffers::make(const XPart& xp, uint32_t nsrc, hipStream_t s) {
 }
 
 void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, const uint32_t* cid, uint32_t nV,
-                    bool fresh_s, double* C, hipStream_t s) {
+                    bool fresh_s, double* C, const EngFuse& f, hipStream_t s) {
   const BinPlan& bp = xp.bin;
   if (fresh_s) {
     GRE_HIP(hipMemsetAsync(bb.smax.p, 0, 8, s));
@@ -1415,14 +1424,14 @@ void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, c
     hipLaunchKernelGGL((k_beng_accum<kAccumU>), dim3(bp.nsl), dim3(kAccumBlock), 0, s, bb.binval.as<double>(),
                        bp.dloc.as<uint16_t>(), bp.sl_e0.as<uint32_t>(), bp.sl_bin.as<uint32_t>(),
                        bp.sl_g.as<uint32_t>(), bp.bin_r0.as<uint32_t>(), bp.rowv.as<uint32_t>(),
-                       bb.smax.as<unsigned long long>(), bp.max_in, C, bb.gacc.as<unsigned long long>());
+                       bb.smax.as<unsigned long long>(), bp.max_in, C, f, bb.gacc.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
   }
   if (bp.nmrows) {
     ProfScope ps("beng_finish");
     hipLaunchKernelGGL(k_beng_fin, dim3(gcap(bp.nmrows)), dim3(kB), 0, s, bp.mb_r0.as<uint32_t>(),
                        bp.mb_g.as<uint32_t>(), bp.nmb, bp.nmrows, bp.rowv.as<uint32_t>(),
-                       bb.smax.as<unsigned long long>(), bp.max_in, C, bb.gacc.as<unsigned long long>());
+                       bb.smax.as<unsigned long long>(), bp.max_in, C, f, bb.gacc.as<unsigned long long>());
     GRE_HIP(hipGetLastError());
   }
 }

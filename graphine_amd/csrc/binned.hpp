@@ -75,10 +75,16 @@ struct EngBinBuffers {
   const XPart* for_xp = nullptr;
   void make(const XPart& xp, uint32_t nsrc, hipStream_t s);
 };
+// An update fused into the sum pull: for every vertex v with in-edges,
+// X[v] = a + b (C[v] + sum), then C[v] = 0.  X = nullptr: C[v] += sum only.
+struct EngFuse {
+  double* X = nullptr;
+  double a = 0.0, b = 0.0;
+};
 // fresh_s: S changed since the last call (its compact copy and largest value
 // are remade)
 void bin_engine_sum(const XPart& xp, const EngBinBuffers& bb, const double* S, const uint32_t* cid, uint32_t nV,
-                    bool fresh_s, double* C, hipStream_t s);
+                    bool fresh_s, double* C, const EngFuse& f, hipStream_t s);
 
 // Fixed-point scale of the accumulate pass: contributions are added as
 // round(c * 2^kBinFix) in 64-bit integers, so every sum is exact and

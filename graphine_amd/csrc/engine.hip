@@ -65,9 +65,10 @@ template <> struct Prog<GRE_PROG_PAGERANK> {            // PageRank.hpp:18-126
     atomic_add_f64((double*)c.C + v, ((const double*)c.S)[u]);
     return true;
   }
+  static constexpr double kResetProb = 0.15;
   __device__ static bool apply(const EngCtx& c, uint32_t v) {
     double* pr = (double*)c.X; double* cm = (double*)c.C;
-    pr[v] = 0.15 + (1.0 - 0.15) * cm[v];
+    pr[v] = kResetProb + (1.0 - kResetProb) * cm[v];
     cm[v] = 0.0;
     return false;
   }
@@ -1221,17 +1222,18 @@ int64_t engine_step(gre_engine* e) {
       e->bxp = graph_binpart_engine(g);
     }
     if (e->bbuf.for_xp != e->bxp.get()) e->bbuf.make(*e->bxp, g->nsrc, s);
-    const uint32_t nvpad = e->nwords * 32;
-    if (!e->rows_mark.p) {
-      e->rows_mark.alloc(4 * (size_t)e->nwords);
-      hipLaunchKernelGGL(k_mark_rows, dim3(gcap(nvpad)), dim3(kB), 0, s, g->coff.as<uint64_t>(), g->nV, nvpad,
-                         e->rows_mark.as<uint32_t>());
-      GRE_HIP(hipGetLastError());
-    }
-    GRE_HIP(hipMemcpyAsync(e->next.p, e->rows_mark.p, 4 * (size_t)e->nwords, hipMemcpyDeviceToDevice, s));
+    // apply() fused into the pull: every vertex with in-edges gets a message
+    // and runs apply(), which activates none (PageRank.hpp:70-74), so next
+    // stays clear
+    using PR = Prog<GRE_PROG_PAGERANK>;
+    EngFuse f;
+    f.X = (double*)c.X;
+    f.a = PR::kResetProb;
+    f.b = 1.0 - PR::kResetProb;
     bin_engine_sum(*e->bxp, e->bbuf, (const double*)c.S, g->cid.as<uint32_t>(), g->nV, !e->bsc_valid,
-                   (double*)c.C, s);
+                   (double*)c.C, f, s);
     e->bsc_valid = true;
+    fused = true;
     ++e->pull_steps;
   } else if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
     gre_graph* g = e->g;
