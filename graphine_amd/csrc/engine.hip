@@ -1209,6 +1209,7 @@ int64_t engine_step(gre_engine* e) {
   const bool pull = e->qedges > 0 && (e->mode == GRE_ENGINE_PULL || (e->mode == GRE_ENGINE_AUTO && auto_pull));
   e->pulling = pull;
   bool fused = false;   // the compute phase applied in place (kFuseApply)
+  bool settled = false;  // ... and nothing can activate or halt: the active set and its counts stand
   const bool fsum = e->prog == GRE_PROG_PAGERANK || e->prog == GRE_PROG_PAGERANK_DELTA ||
                     e->prog == GRE_PROG_BC_BACKWARD;
   if (pull && e->prog == GRE_PROG_PAGERANK && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_bin_ok(e->g)) {
@@ -1234,6 +1235,7 @@ int64_t engine_step(gre_engine* e) {
                    (double*)c.C, f, s);
     e->bsc_valid = true;
     fused = true;
+    settled = true;   // PageRank never halts (kHalt = false) and its apply activates nothing
     ++e->pull_steps;
   } else if (pull && fsum && e->dir == GRE_DIR_OUT && e->qedges == epull && engine_part_ok(e->g)) {
     gre_graph* g = e->g;
@@ -1323,7 +1325,15 @@ int64_t engine_step(gre_engine* e) {
     GRE_HIP(hipGetLastError());
   }
   GRE_HIP(hipEventRecord(e->ev[1], s));
-  {
+  if (settled) {
+    // currSet after the apply equals currSet before it, so the block counts,
+    // the totals and the queue of the last rebuild still describe it
+    GRE_HIP(hipEventRecord(e->ev[2], s));
+    hipError_t q;
+    while ((q = hipEventQuery(e->ev[2])) == hipErrorNotReady) {
+    }
+    GRE_HIP(q);
+  } else {
     ProfScope ps("apply");
     dispatch(e->prog, [&](auto P) {
       if (fused)
@@ -1333,9 +1343,10 @@ int64_t engine_step(gre_engine* e) {
         hipLaunchKernelGGL(k_apply<decltype(P)::value>, dim3(e->nb), dim3(kB), 0, s, c, e->next.as<uint32_t>(),
                            e->curr.as<uint32_t>(), e->g->nV, a, e->bcnt.as<uint32_t>(), e->bdeg.as<uint64_t>());
     });
+    GRE_HIP(hipGetLastError());
+    rebuild_queue(e, a, s, e->ev[2]);   // waits for the stream
   }
   GRE_HIP(hipGetLastError());
-  rebuild_queue(e, a, s, e->ev[2]);   // waits for the stream
   e->seen_edges += e->qedges;
   float ms_c = 0.f, ms_a = 0.f;
   GRE_HIP(hipEventElapsedTime(&ms_c, e->ev[0], e->ev[1]));
