@@ -143,3 +143,24 @@ def test_binned_adjoint_multi_slice_bin(N, O):
     assert rel_l1(lam, lam_ref) < 2e-6
     for v in (0, 1, 5000, nd - 1, nd, n - 1):
         assert abs(float(lam[v]) - lam_ref[v]) <= 1e-5 * abs(lam_ref[v]) + 1e-6, v
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_binned_adjoint_scales_exactly(N, weighted):
+    """The adjoint is linear in c, and every step (FP32 group sums, the
+    fixed-point scale taken from max |lambda|, the integer sums, the division,
+    the update, the FP64 p.lambda) commutes with a power-of-two factor while
+    no value leaves the normal FP32 range: lambda(2^k c) = 2^k lambda(c) bit
+    for bit, for k = 60 and -60.  A fixed-point scale that did not follow
+    |lambda| would overflow at 2^60 or lose every bit at 2^-60."""
+    g = N.Graph.rmat(14, 16, 2, False)
+    c = np.random.default_rng(7).standard_normal(g.nV).astype(np.float32)
+    h = handle(N, g, BIN, weighted, None)
+    h.run(D, 5)
+    h.adjoint_cotangent(c, D, 20)
+    lam = h.adjoint_vector()
+    assert np.abs(lam).max() > 0
+    for k in (60, -60):
+        f = np.float32(2.0 ** k)
+        h.adjoint_cotangent(c * f, D, 20)
+        assert np.array_equal(h.adjoint_vector(), lam * f), k
